@@ -1,0 +1,217 @@
+"""Headline benchmark: closest-hit + any-hit shadow rays at 2.5 M triangles.
+
+Workload (one step = one frame of the shadow any-hit configuration):
+  mesh   UV sphere 1250 x 1000 quads = 2,500,000 FP32 triangles, radius 1,
+         seeded random radial displacement +-2 % (generated on the device),
+         Box4 BVH built on the device once (inputs resident before timing)
+  step   1920x1080 pinhole camera rays (60 deg vertical FOV, every pixel hits)
+         -> closest-hit trace -> 16 shadow rays per pixel towards 16 lights
+         uniform in the box 3x the mesh bounds (tmax = light distance)
+         -> any-hit trace -> per-pixel 16-bit occlusion masks
+  value  (2,073,600 + 33,177,600) rays / step time, Mrays/s
+Also reported: device build time at 4,096 triangles (fused single-workgroup
+path) and at 2.5 M, p50 / p99 of the per-frame latency, the roofline of the
+dominant kernel (the shadow any-hit traversal), and the CPU traversal of the
+same tree and rays on the host cores.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+import torch
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "rdna2-accel-raster_amd"))
+import rtbvh as rt  # noqa: E402
+
+METRIC = "Mrays/s any-hit+closest-hit @2.5M tris; BVH build µs @4K tris; p99 req latency"
+HBM_PEAK_GBS = 8000.0
+W, H, NLIGHTS = 1920, 1080, 16
+NU, NV, SEED = 1250, 1000, 7
+EPS = 1e-4
+
+
+def percentile(v, q):
+    """common/vk_timer.cpp:106-107 convention: sorted[floor(q (n - 1))]."""
+    s = sorted(v)
+    return s[int(q * (len(s) - 1))]
+
+
+def build_time_4k(iters=200, warm=20):
+    """Median device build time of the 4,096-triangle deforming grid
+    (generate_dynamic_grid_mesh layout), fused single-workgroup kernel."""
+    xyz = rt.gen_dynamic_grid(4096, 0.0, 2.0)
+    bvh = rt.GpuBVH4.build(xyz, None)
+    s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    times = []
+    for i in range(warm + iters):
+        s.record()
+        rt.call("rt_gpu_build", rt.stream_handle(), rt.ptr(xyz), None, rt.u32(4096),
+                rt.ptr(bvh.nodes), rt.u32(bvh.node_count), rt.ptr(bvh.tris), None, None, None,
+                rt.C.c_size_t(0))
+        e.record()
+        e.synchronize()
+        if i >= warm:
+            times.append(s.elapsed_time(e) * 1000.0)
+    return percentile(times, 0.5), percentile(times, 0.99)
+
+
+class Frame:
+    def __init__(self):
+        self.pos, self.idx = rt.gen_sphere(NU, NV, 1.0, 0.02, SEED)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        self.bvh = rt.GpuBVH4.build(self.pos, self.idx)
+        torch.cuda.synchronize()
+        self.build_ms = (time.perf_counter() - t0) * 1000.0
+        lo = self.pos.min(0).values.cpu().numpy().astype(np.float64)
+        hi = self.pos.max(0).values.cpu().numpy().astype(np.float64)
+        c, ext = (lo + hi) / 2, (hi - lo)
+        lights = rt.lights_in_box(c - 1.5 * ext, c + 1.5 * ext, NLIGHTS, 1337)
+        self.lights = torch.tensor(lights, dtype=torch.float32, device="cuda")
+        self.npix = W * H
+        self.primary = rt.Rays.empty(self.npix)
+        self.shadow = rt.Rays.empty(self.npix * NLIGHTS)
+        self.prim = torch.empty(self.npix, dtype=torch.int32, device="cuda")
+        self.t = torch.empty(self.npix, dtype=torch.float32, device="cuda")
+        self.bits = torch.empty((self.npix * NLIGHTS + 31) // 32, dtype=torch.int32, device="cuda")
+        self.masks = torch.empty(self.npix, dtype=torch.int16, device="cuda")
+        self.ev = None
+
+    def step(self, events=None):
+        rt.gen_camera_rays(W, H, (0.0, 0.0, 1.25), (0.0, 0.0, 0.0), (0.0, 1.0, 0.0), 60.0,
+                           0.0, 1e30, out=self.primary)
+        self.bvh.intersect(self.primary, out=(self.prim, self.t))
+        rt.gen_shadow_rays(self.pos, self.idx, self.primary, self.prim, self.t, self.lights, EPS,
+                           out=self.shadow)
+        if events:
+            events[0].record()
+        self.bvh.occluded(self.shadow, out=self.bits)
+        if events:
+            events[1].record()
+        rt.pack_light_masks(self.bits, self.npix, NLIGHTS, out=self.masks)
+
+    @property
+    def rays_per_step(self):
+        return self.npix + self.npix * NLIGHTS
+
+
+def algorithmic_bytes_shadow(frame):
+    """Bytes one shadow any-hit launch must move at minimum: the whole tree in
+    the RADV-equivalent size (192 + 128 boxes + 64 N) plus 32 B per SoA ray
+    and 1/8 B of occlusion bit per ray."""
+    _, _, tb, as_size = rt.tree_layout(frame.bvh.tri_count)
+    r = frame.npix * NLIGHTS
+    return as_size + r * 32 + r / 8.0
+
+
+def cpu_baseline(frame, threads):
+    """The CPU oracle's BVH4 traversal on the host cores over the same tree and
+    a bounded sample of the same rays: all primary rays (closest-hit) + the
+    shadow rays of the first 2 lights (any-hit)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    nodes = frame.bvh.nodes.cpu().numpy()[: frame.bvh.node_count * 128].view(O.NODE4)
+    tris = frame.bvh.tris.cpu().numpy()[: frame.bvh.tri_count * 64].view(O.TRI64)
+    prim_rays = O.Rays.stack(frame.primary.data.cpu().numpy())
+    nsh = frame.npix * 2
+    sh_rays = O.Rays.stack(frame.shadow.data[:, :nsh].cpu().numpy())
+    t0 = time.perf_counter()
+    O.trace_bvh4(nodes, tris, prim_rays, mode=O.CLOSEST, threads=threads)
+    O.trace_bvh4(nodes, tris, sh_rays, mode=O.ANY, threads=threads)
+    dt = time.perf_counter() - t0
+    n = len(prim_rays) + len(sh_rays)
+    return {"value": round(n / dt / 1e6, 3), "unit": "Mrays/s", "cores": threads, "kind": "port",
+            "sample": f"{len(prim_rays)} primary closest-hit + {len(sh_rays)} shadow any-hit rays "
+                      f"(2 of 16 lights) of the same frame and tree; {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); the per-GPU shard "
+              "is the one-GPU workload and there is no multi-GPU path in this build",
+              file=sys.stderr)
+        sys.exit(2)
+    torch.cuda.set_device(0)
+    rt.lib()
+
+    b4k_p50, b4k_p99 = build_time_4k()
+    f = Frame()
+    for _ in range(a.warmup):
+        f.step()
+    torch.cuda.synchronize()
+
+    step_ms, kern_ms = [], []
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+           for _ in range(a.steps)]
+    fs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+          for _ in range(a.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(a.steps):
+        fs[i][0].record()
+        f.step(events=evs[i])
+        fs[i][1].record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    for i in range(a.steps):
+        step_ms.append(fs[i][0].elapsed_time(fs[i][1]))
+        kern_ms.append(evs[i][0].elapsed_time(evs[i][1]))
+
+    ms_per_step = wall * 1000.0 / a.steps
+    value = f.rays_per_step / (ms_per_step * 1e-3) / 1e6
+    kern_avg = sum(kern_ms) / len(kern_ms)
+    ab = algorithmic_bytes_shadow(f)
+    achieved = ab / (kern_avg * 1e-3) / 1e9
+    traffic = None
+    tp = os.path.join(ROOT, "profiles", "traffic_shadow_anyhit_2p5m.json")
+    if os.path.exists(tp):
+        with open(tp) as fh:
+            traffic = json.load(fh).get("hbm_bytes_per_launch")
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "shadow any-hit: 2.5M-tri displaced sphere, 1920x1080 primary "
+                               "closest-hit + 16 shadow rays/pixel",
+                   "tris": f.bvh.tri_count, "width": W, "height": H, "lights": NLIGHTS,
+                   "rays_per_step": f.rays_per_step},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": traffic, "kernel": "k_trace any-hit (shadow)",
+                     "kernel_ms": round(kern_avg, 4), "algorithmic_bytes": int(ab)},
+        "build_us_4k_p50": round(b4k_p50, 2),
+        "build_us_4k_p99": round(b4k_p99, 2),
+        "build_ms_2p5m_host_timed": round(f.build_ms, 3),
+        "frame_latency_ms_p50": round(percentile(step_ms, 0.5), 4),
+        "frame_latency_ms_p99": round(percentile(step_ms, 0.99), 4),
+    }
+    if not a.no_cpu:
+        threads = min(16, os.cpu_count() or 1)
+        out["cpu_baseline"] = cpu_baseline(f, threads)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

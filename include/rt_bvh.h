@@ -1,0 +1,182 @@
+/*
+ * rt_bvh.h -- C ABI of the MI355X (gfx950) BVH build-and-traverse library
+ * (librtbvh.so).  Plain pointers and sizes only; every device pointer is
+ * caller-owned HBM, every call is asynchronous on the given HIP stream and
+ * returns an rt_status (no exceptions cross this boundary).
+ *
+ * Reference interfaces replaced (paths relative to the reference checkout):
+ *   common/software_bvh.hpp:11-22,178-187  node / triangle layouts  -> rt_node4, rt_tri64
+ *   common/software_bvh.hpp:35,193          SoftwareBVH::build / SoftwareBVH4::fromBVH2
+ *                                           -> rt_gpu_build (device build of the same
+ *                                              Box4 node format); a host-built
+ *                                              SoftwareBVH4 node array is byte-identical
+ *                                              to rt_node4 and its GpuTriangle array
+ *                                              converts with rt_convert_tris48
+ *   shaders/fast_bvh_builder.comp:74-83,91  FastBvhPushConstants + vkCmdDispatch(1,1,1)
+ *                                           -> rt_gpu_build / rt_gpu_build_radv
+ *   include/radv_bvh.hpp:193-215            compute_tree_layout -> rt_tree_layout
+ *   benchmarks/exp0_probe/shaders/software_bvh_bvh4.comp:22-46,60-196
+ *                                           SSBO traversal ABI -> rt_trace
+ *   benchmarks/exp0_probe/shaders/software_bvh.comp:5-15
+ *                                           RayInput / HitResult -> rt_rays_soa / rt_hits_soa
+ *   shaders/rq_ao.comp, shaders/tiled_lighting_ra.comp:136-155
+ *                                           inline any-hit ray queries -> rt_trace(RT_ANY)
+ */
+#ifndef RT_BVH_H
+#define RT_BVH_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum rt_status {
+    RT_OK = 0,
+    RT_ERR_ARG = 1,       /* null pointer, size mismatch, bad flag */
+    RT_ERR_CAPACITY = 2,  /* an output or scratch buffer is too small */
+    RT_ERR_HIP = 3,       /* a HIP runtime call failed (see rt_last_error) */
+    RT_ERR_NOMEM = 4,     /* the library could not allocate its own state */
+    RT_ERR_UNSUPPORTED = 5
+} rt_status;
+
+typedef void* rt_stream; /* hipStream_t; NULL = the null stream */
+
+/* Box4 node, byte layout of GpuBvh4Node (software_bvh.hpp:178-187): per-child
+ * SoA bounds, child[k] / count[k]:
+ *   count > 0  leaf range of `count` rt_tri64 records starting at ~child
+ *   count = -1 inner node index child
+ *   count = 0  empty slot                                                  */
+typedef struct rt_node4 {
+    float minX[4], maxX[4], minY[4], maxY[4], minZ[4], maxZ[4];
+    int32_t child[4];
+    int32_t count[4];
+} rt_node4;
+
+/* 64-B triangle leaf, byte layout of RadvBvhTriangleNode
+ * (radv_bvh.hpp:142-152): v0,v1,v2, original triangle index in triangle_id,
+ * geometry_id_and_flags = 0x80000000 (opaque), id = 9.                     */
+typedef struct rt_tri64 {
+    float v[9];
+    uint32_t reserved[3];
+    uint32_t triangle_id;
+    uint32_t geometry_id_and_flags;
+    uint32_t reserved2;
+    uint32_t id;
+} rt_tri64;
+
+/* A device-resident BVH.  depth = node levels on the longest path (sizes the
+ * per-lane traversal stack: at most 3 * depth + 1 entries are ever live).   */
+typedef struct rt_bvh_dev {
+    const rt_node4* nodes;
+    const rt_tri64* tris;
+    uint32_t node_count;
+    uint32_t tri_count;
+    uint32_t depth;
+    uint32_t reserved;
+} rt_bvh_dev;
+
+/* Structure-of-arrays rays: 8 float arrays of n entries (32 B per ray). */
+typedef struct rt_rays_soa {
+    const float *ox, *oy, *oz, *tmin, *dx, *dy, *dz, *tmax;
+} rt_rays_soa;
+
+/* Writable view of the same (for the ray generators). */
+typedef struct rt_rays_out {
+    float *ox, *oy, *oz, *tmin, *dx, *dy, *dz, *tmax;
+} rt_rays_out;
+
+/* Closest-hit results: prim = original triangle index (0xFFFFFFFF = miss),
+ * t = hit distance (the ray's tmax on a miss).                              */
+typedef struct rt_hits_soa {
+    uint32_t* prim;
+    float* t;
+} rt_hits_soa;
+
+#define RT_MISS 0xFFFFFFFFu
+
+/* rt_trace flags */
+#define RT_CLOSEST 0x0u  /* closest hit in [tmin, tmax) */
+#define RT_ANY 0x1u      /* occlusion: any hit in [tmin, tmax), first-hit terminate */
+#define RT_COUNTERS 0x100u /* also write per-ray box / triangle test counts */
+
+/* ---- library ------------------------------------------------------------ */
+const char* rt_version(void);
+const char* rt_last_error(void); /* text of the last RT_ERR_HIP on this thread */
+rt_status rt_device_sync(void);
+
+/* ---- layout (include/radv_bvh.hpp:193-215 compute_tree_layout) ----------
+ * counts/offsets need room for 32 levels.  total_as_size = 192 + 128 boxes
+ * + 64 n, the byte size of the RADV image rt_gpu_build_radv writes.        */
+rt_status rt_tree_layout(uint32_t n, uint32_t* counts, uint32_t* offsets, uint32_t* levels,
+                         uint32_t* total_boxes, uint64_t* total_as_size);
+
+/* ---- device build (replaces fast_bvh_builder.comp / SoftwareBVH::build) --
+ * Input: positions (float3 per vertex) + indices (uint3 per triangle), or
+ * d_idx = NULL for 9 floats per triangle (fast_bvh_builder.comp:120-122).
+ * Output: the implicit complete Box4 tree over triangles sorted by 30-bit
+ * Morton code of their centroid (ties by triangle index): node_count =
+ * total_boxes of rt_tree_layout(n) rt_node4 records, root first, and n
+ * rt_tri64 leaves in sorted order.  d_sorted_ids (optional) receives the
+ * sorted triangle indices, d_codes (optional) the sorted Morton codes.
+ * rt_build_scratch_size gives the scratch bytes rt_gpu_build needs.        */
+rt_status rt_build_scratch_size(uint32_t n, size_t* scratch_bytes);
+rt_status rt_gpu_build(rt_stream stream, const float* d_pos, const uint32_t* d_idx, uint32_t n,
+                       rt_node4* d_nodes, uint32_t node_capacity, rt_tri64* d_tris,
+                       uint32_t* d_sorted_ids, uint32_t* d_codes, void* d_scratch,
+                       size_t scratch_bytes);
+/* Refit for a deforming mesh: same topology (leaf order taken from
+ * d_tris[i].triangle_id), new positions.                                    */
+rt_status rt_gpu_refit(rt_stream stream, const float* d_pos, const uint32_t* d_idx, uint32_t n,
+                       rt_node4* d_nodes, uint32_t node_count, rt_tri64* d_tris,
+                       void* d_scratch, size_t scratch_bytes);
+
+/* GpuTriangle records (48 B, software_bvh.hpp:18-22, leaf order) plus the
+ * leaf-order original indices -> rt_tri64 leaves (software_bvh.hpp:66-84
+ * drops the index; d_order restores it, NULL = identity).                  */
+rt_status rt_convert_tris48(rt_stream stream, const float* d_tris48, const uint32_t* d_order,
+                            uint32_t n, rt_tri64* d_out);
+
+/* ---- trace (replaces software_bvh_bvh4.comp and the inline ray queries) --
+ * RT_CLOSEST: hits->prim / hits->t per ray (same traversal order and
+ *   predicates as software_bvh_bvh4.comp:111-190, so IDs and t are
+ *   bit-identical to the CPU path on the same tree and rays).
+ * RT_ANY: occl_bits bit r (word r/32, bit r%32) = ray r is occluded;
+ *   occl_bits needs ceil(n/32) words.
+ * RT_COUNTERS: box_tests[r] / tri_tests[r] (may be NULL otherwise).        */
+rt_status rt_trace(rt_stream stream, const rt_bvh_dev* bvh, const rt_rays_soa* rays, uint32_t n,
+                   uint32_t flags, rt_hits_soa* hits, uint32_t* occl_bits, uint32_t* box_tests,
+                   uint32_t* tri_tests);
+
+/* ---- generators (synthetic inputs made on the device) -------------------- */
+rt_status rt_gen_sphere(rt_stream stream, uint32_t nu, uint32_t nv, float radius, float disp,
+                        uint32_t seed, float* d_pos, uint32_t* d_idx);
+rt_status rt_gen_wave_grid(rt_stream stream, uint32_t nx, uint32_t ny, float extent, float t,
+                           float amp, float fx, float fy, float* d_pos, uint32_t* d_idx);
+rt_status rt_gen_heightfield(rt_stream stream, uint32_t nx, uint32_t ny, float extent, float amp,
+                             uint32_t seed, float* d_pos, uint32_t* d_idx);
+rt_status rt_gen_dynamic_grid(rt_stream stream, uint32_t n, float t, float extent, float* d_xyz9);
+rt_status rt_gen_camera_rays(rt_stream stream, uint32_t w, uint32_t h, const float eye[3],
+                             const float look[3], const float up[3], float vfov_deg, float tmin,
+                             float tmax, const rt_rays_out* out);
+rt_status rt_gen_shadow_rays(rt_stream stream, const float* d_pos, const uint32_t* d_idx,
+                             const rt_rays_soa* primary, const uint32_t* d_prim,
+                             const float* d_t, uint32_t npix, const float* d_lights,
+                             uint32_t nlights, float eps, const rt_rays_out* out);
+rt_status rt_gen_ao_rays(rt_stream stream, const float* d_pos, const uint32_t* d_idx,
+                         const rt_rays_soa* primary, const uint32_t* d_prim, const float* d_t,
+                         uint32_t w, uint32_t h, uint32_t frame, float offset, float tmin,
+                         float tmax, const rt_rays_out* out);
+rt_status rt_gen_random_rays(rt_stream stream, uint32_t count, uint32_t seed, const float lo[3],
+                             const float hi[3], float tmax, const rt_rays_out* out);
+
+/* Per-pixel light masks from light-major occlusion bits
+ * (bit l*npix + p -> mask[p] bit l), nlights <= 16.                         */
+rt_status rt_pack_light_masks(rt_stream stream, const uint32_t* d_bits, uint32_t npix,
+                              uint32_t nlights, uint16_t* d_masks);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* RT_BVH_H */

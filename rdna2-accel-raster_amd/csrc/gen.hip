@@ -1,0 +1,411 @@
+// gen.hip -- synthetic inputs made on the device (meshes, ray batches) and
+// the small per-element kernels either side of traversal (GpuTriangle leaf
+// conversion, per-pixel light masks).  Elementwise and HBM-bound: one thread
+// per output element, SoA stores coalesced across the wave.
+#include <cmath>
+
+#include "rt_internal.h"
+
+namespace rt {
+namespace {
+
+constexpr int kB = 256;
+
+__device__ __forceinline__ void normalize3(float& x, float& y, float& z) {
+    float l = sqrtf(x * x + y * y + z * z);
+    x = x / l;
+    y = y / l;
+    z = z / l;
+}
+
+__device__ __forceinline__ void put_ray(const rt_rays_out& o, size_t r, float ox, float oy,
+                                        float oz, float tn, float dx, float dy, float dz,
+                                        float tx) {
+    o.ox[r] = ox; o.oy[r] = oy; o.oz[r] = oz; o.tmin[r] = tn;
+    o.dx[r] = dx; o.dy[r] = dy; o.dz[r] = dz; o.tmax[r] = tx;
+}
+
+// UV sphere, vertex (i, j): theta = pi j / nv, phi = 2 pi i / nu, radius
+// r (1 + disp (2u - 1)); pole rows share the radius of their first vertex.
+__global__ void k_sphere_verts(uint32_t nu, uint32_t nv, float radius, float disp, uint32_t seed,
+                               float* __restrict__ pos) {
+    const uint32_t v = blockIdx.x * kB + threadIdx.x;
+    if (v >= (nv + 1) * nu) return;
+    const uint32_t j = v / nu, i = v % nu;
+    const float kPi = 3.14159265358979f;
+    const float theta = kPi * (float)j / (float)nv;
+    const float st = sinf(theta), ct = cosf(theta);
+    const uint32_t hv = (j == 0 || j == nv) ? j * nu : v;
+    const float phi = 2.0f * kPi * (float)i / (float)nu;
+    const float r = radius * (1.0f + disp * (2.0f * hash01(seed, hv) - 1.0f));
+    pos[3 * (size_t)v + 0] = r * (st * cosf(phi));
+    pos[3 * (size_t)v + 1] = r * (st * sinf(phi));
+    pos[3 * (size_t)v + 2] = r * ct;
+}
+
+__global__ void k_sphere_idx(uint32_t nu, uint32_t nv, uint32_t* __restrict__ idx) {
+    const uint32_t q = blockIdx.x * kB + threadIdx.x;
+    if (q >= nu * nv) return;
+    const uint32_t j = q / nu, i = q % nu;
+    const uint32_t i1 = (i + 1 == nu) ? 0 : i + 1;
+    const uint32_t a = j * nu + i, b = j * nu + i1, c = (j + 1) * nu + i, d = (j + 1) * nu + i1;
+    uint32_t* o = idx + 6 * (size_t)q;
+    o[0] = a; o[1] = c; o[2] = b;
+    o[3] = b; o[4] = c; o[5] = d;
+}
+
+// Grid topology shared by the wave grid and the heightfield: quad (c, r),
+// tri 1 = v00 v10 v01, tri 2 = v10 v11 v01 (include/radv_bvh.hpp:257-279).
+__global__ void k_grid_idx(uint32_t nx, uint32_t ny, uint32_t* __restrict__ idx) {
+    const uint32_t q = blockIdx.x * kB + threadIdx.x;
+    if (q >= nx * ny) return;
+    const uint32_t r = q / nx, c = q % nx;
+    const uint32_t v00 = r * (nx + 1) + c, v10 = v00 + 1, v01 = v00 + nx + 1, v11 = v01 + 1;
+    uint32_t* o = idx + 6 * (size_t)q;
+    o[0] = v00; o[1] = v10; o[2] = v01;
+    o[3] = v10; o[4] = v11; o[5] = v01;
+}
+
+__global__ void k_wave_verts(uint32_t nx, uint32_t ny, float e, float t, float amp, float fx,
+                             float fy, float* __restrict__ pos) {
+    const uint32_t v = blockIdx.x * kB + threadIdx.x;
+    if (v >= (nx + 1) * (ny + 1)) return;
+    const uint32_t r = v / (nx + 1), c = v % (nx + 1);
+    const float dx = (2.0f * e) / (float)nx, dy = (2.0f * e) / (float)ny;
+    const float x = -e + (float)c * dx, y = -e + (float)r * dy;
+    pos[3 * (size_t)v + 0] = x;
+    pos[3 * (size_t)v + 1] = y;
+    pos[3 * (size_t)v + 2] = amp * sinf(fx * x + t) * cosf(fy * y);
+}
+
+__global__ void k_height_verts(uint32_t nx, uint32_t ny, float e, float amp, uint32_t seed,
+                               float* __restrict__ pos) {
+    const uint32_t v = blockIdx.x * kB + threadIdx.x;
+    if (v >= (nx + 1) * (ny + 1)) return;
+    const uint32_t r = v / (nx + 1), c = v % (nx + 1);
+    const float dx = (2.0f * e) / (float)nx, dy = (2.0f * e) / (float)ny;
+    pos[3 * (size_t)v + 0] = -e + (float)c * dx;
+    pos[3 * (size_t)v + 1] = -e + (float)r * dy;
+    pos[3 * (size_t)v + 2] = amp * (2.0f * hash01(seed, v) - 1.0f);
+}
+
+// include/radv_bvh.hpp:224-283 generate_dynamic_grid_mesh, one quad per
+// thread (cols / rows chosen on the host exactly as the reference does).
+__global__ void k_dynamic_grid(uint32_t cols, uint32_t rows, float t, float extent,
+                               float* __restrict__ xyz) {
+    const uint32_t q = blockIdx.x * kB + threadIdx.x;
+    if (q >= cols * rows) return;
+    const uint32_t r = q / cols, c = q % cols;
+    const float dx = (2.0f * extent) / (float)cols, dy = (2.0f * extent) / (float)rows;
+    const float x0 = -extent + (float)c * dx, x1 = x0 + dx;
+    const float y0 = -extent + (float)r * dy, y1 = y0 + dy;
+    auto z = [t](float x, float y) {
+        return 5.0f + 0.2f * sinf(x * 2.0f + t) * cosf(y * 2.0f + t);
+    };
+    const float z00 = z(x0, y0), z10 = z(x1, y0), z01 = z(x0, y1), z11 = z(x1, y1);
+    float* o = xyz + 18 * (size_t)q;
+    const float tv[18] = {x0, y0, z00, x1, y0, z10, x0, y1, z01,
+                          x1, y0, z10, x1, y1, z11, x0, y1, z01};
+#pragma unroll
+    for (int k = 0; k < 18; ++k) o[k] = tv[k];
+}
+
+struct CamBasis {
+    float eye[3], F[3], R[3], U[3];
+    float sxs, sys, tmin, tmax;
+};
+
+__global__ void k_camera(uint32_t w, uint32_t h, CamBasis cb, rt_rays_out o) {
+    const uint32_t i = blockIdx.x * kB + threadIdx.x;
+    if (i >= w * h) return;
+    const uint32_t x = i % w, y = i / w;
+    const float sx = ((float)x + 0.5f) * cb.sxs - 1.0f;
+    const float sy = 1.0f - ((float)y + 0.5f) * cb.sys;
+    float d0 = cb.F[0] + sx * cb.R[0] + sy * cb.U[0];
+    float d1 = cb.F[1] + sx * cb.R[1] + sy * cb.U[1];
+    float d2 = cb.F[2] + sx * cb.R[2] + sy * cb.U[2];
+    normalize3(d0, d1, d2);
+    put_ray(o, i, cb.eye[0], cb.eye[1], cb.eye[2], cb.tmin, d0, d1, d2, cb.tmax);
+}
+
+__device__ __forceinline__ void face_normal(const float* __restrict__ pos,
+                                            const uint32_t* __restrict__ idx, uint32_t prim,
+                                            float ddx, float ddy, float ddz, float n[3]) {
+    const float* p0 = pos + 3 * (size_t)idx[3 * (size_t)prim + 0];
+    const float* p1 = pos + 3 * (size_t)idx[3 * (size_t)prim + 1];
+    const float* p2 = pos + 3 * (size_t)idx[3 * (size_t)prim + 2];
+    const float e1x = p1[0] - p0[0], e1y = p1[1] - p0[1], e1z = p1[2] - p0[2];
+    const float e2x = p2[0] - p0[0], e2y = p2[1] - p0[1], e2z = p2[2] - p0[2];
+    float nx = e1y * e2z - e1z * e2y;
+    float ny = e1z * e2x - e1x * e2z;
+    float nz = e1x * e2y - e1y * e2x;
+    normalize3(nx, ny, nz);
+    if (nx * ddx + ny * ddy + nz * ddz > 0.0f) {
+        nx = -nx;
+        ny = -ny;
+        nz = -nz;
+    }
+    n[0] = nx; n[1] = ny; n[2] = nz;
+}
+
+// Shadow rays (role of tiled_lighting_ra.comp:136-155): origin P + eps N,
+// towards each light, [eps, distance).  Light-major: ray l * npix + p.
+__global__ void k_shadow(const float* __restrict__ pos, const uint32_t* __restrict__ idx,
+                         rt_rays_soa pr, const uint32_t* __restrict__ prim,
+                         const float* __restrict__ tt, uint32_t npix,
+                         const float* __restrict__ lights, uint32_t nl, float eps,
+                         rt_rays_out o) {
+    const uint32_t p = blockIdx.x * kB + threadIdx.x;
+    if (p >= npix) return;
+    const uint32_t hp = prim[p];
+    if (hp == RT_MISS) {
+        for (uint32_t l = 0; l < nl; ++l)
+            put_ray(o, (size_t)l * npix + p, 0.f, 0.f, 0.f, 1.f, 0.f, 0.f, 1.f, 0.f);
+        return;
+    }
+    const float dx = pr.dx[p], dy = pr.dy[p], dz = pr.dz[p];
+    float n[3];
+    face_normal(pos, idx, hp, dx, dy, dz, n);
+    const float t = tt[p];
+    const float Px = pr.ox[p] + t * dx, Py = pr.oy[p] + t * dy, Pz = pr.oz[p] + t * dz;
+    const float Ox = Px + eps * n[0], Oy = Py + eps * n[1], Oz = Pz + eps * n[2];
+    for (uint32_t l = 0; l < nl; ++l) {
+        const float Lx = lights[3 * l + 0] - Ox;
+        const float Ly = lights[3 * l + 1] - Oy;
+        const float Lz = lights[3 * l + 2] - Oz;
+        const float dist = sqrtf(Lx * Lx + Ly * Ly + Lz * Lz);
+        put_ray(o, (size_t)l * npix + p, Ox, Oy, Oz, eps, Lx / dist, Ly / dist, Lz / dist, dist);
+    }
+}
+
+// shaders/rq_ao.comp:18-25 hash
+__device__ __forceinline__ uint32_t ao_hash(uint32_t x) {
+    x += (x << 10u); x ^= (x >> 6u); x += (x << 3u); x ^= (x >> 11u); x += (x << 15u);
+    return x;
+}
+
+// One cosine-weighted hemisphere ray per pixel (rq_ao.comp:32-45,70-87).
+__global__ void k_ao(const float* __restrict__ pos, const uint32_t* __restrict__ idx,
+                     rt_rays_soa pr, const uint32_t* __restrict__ prim,
+                     const float* __restrict__ tt, uint32_t w, uint32_t h, uint32_t frame,
+                     float offset, float tmin, float tmax, rt_rays_out o) {
+    const uint32_t p = blockIdx.x * kB + threadIdx.x;
+    if (p >= w * h) return;
+    const uint32_t hp = prim[p];
+    if (hp == RT_MISS) {
+        put_ray(o, p, 0.f, 0.f, 0.f, 1.f, 0.f, 0.f, 1.f, 0.f);
+        return;
+    }
+    const float dx = pr.dx[p], dy = pr.dy[p], dz = pr.dz[p];
+    float n[3];
+    face_normal(pos, idx, hp, dx, dy, dz, n);
+    uint32_t seed = p + 5544u + frame * w * h;
+    seed = ao_hash(seed);
+    const float u1 = (float)seed / 4294967295.0f;
+    seed = ao_hash(seed);
+    const float u2 = (float)seed / 4294967295.0f;
+    const float r = sqrtf(u1);
+    const float th = 6.2831853f * u2;
+    const float lx = r * cosf(th), ly = r * sinf(th);
+    const float lz = sqrtf(fmaxf(0.0f, 1.0f - u1));
+    float upx = 0.0f, upy = 0.0f, upz = 1.0f;
+    if (!(fabsf(n[2]) < 0.999f)) {
+        upx = 1.0f;
+        upz = 0.0f;
+    }
+    float tx0 = upy * n[2] - upz * n[1];
+    float ty0 = upz * n[0] - upx * n[2];
+    float tz0 = upx * n[1] - upy * n[0];
+    normalize3(tx0, ty0, tz0);
+    const float bx = n[1] * tz0 - n[2] * ty0;
+    const float by = n[2] * tx0 - n[0] * tz0;
+    const float bz = n[0] * ty0 - n[1] * tx0;
+    float d0 = tx0 * lx + bx * ly + n[0] * lz;
+    float d1 = ty0 * lx + by * ly + n[1] * lz;
+    float d2 = tz0 * lx + bz * ly + n[2] * lz;
+    normalize3(d0, d1, d2);
+    const float t = tt[p];
+    const float Px = pr.ox[p] + t * dx, Py = pr.oy[p] + t * dy, Pz = pr.oz[p] + t * dz;
+    put_ray(o, p, Px + offset * n[0], Py + offset * n[1], Pz + offset * n[2], tmin, d0, d1, d2,
+            tmax);
+}
+
+__global__ void k_random_rays(uint32_t count, uint32_t seed, float lx, float ly, float lz,
+                              float hx, float hy, float hz, float tmax, rt_rays_out o) {
+    const uint32_t i = blockIdx.x * kB + threadIdx.x;
+    if (i >= count) return;
+    const float u0 = hash01(seed, i * 5 + 0), u1 = hash01(seed, i * 5 + 1);
+    const float u2 = hash01(seed, i * 5 + 2), u3 = hash01(seed, i * 5 + 3);
+    const float u4 = hash01(seed, i * 5 + 4);
+    const float z = 1.0f - 2.0f * u3;
+    const float r = sqrtf(fmaxf(0.0f, 1.0f - z * z));
+    const float phi = 6.2831853f * u4;
+    put_ray(o, i, lx + u0 * (hx - lx), ly + u1 * (hy - ly), lz + u2 * (hz - lz), 0.0f,
+            r * cosf(phi), r * sinf(phi), z, tmax);
+}
+
+__global__ void k_convert48(const float* __restrict__ t48, const uint32_t* __restrict__ order,
+                            uint32_t n, rt_tri64* __restrict__ out) {
+    const uint32_t i = blockIdx.x * kB + threadIdx.x;
+    if (i >= n) return;
+    const float4* s = reinterpret_cast<const float4*>(t48) + 3 * (size_t)i;
+    const float4 a = s[0], b = s[1], c = s[2];
+    float4* d = reinterpret_cast<float4*>(out + i);
+    d[0] = make_float4(a.x, a.y, a.z, b.x);
+    d[1] = make_float4(b.y, b.z, c.x, c.y);
+    d[2] = make_float4(c.z, 0.f, 0.f, 0.f);
+    reinterpret_cast<uint4*>(d)[3] = make_uint4(order ? order[i] : i, 0x80000000u, 0u, 9u);
+}
+
+__global__ void k_pack_masks(const uint32_t* __restrict__ bits, uint32_t npix, uint32_t nl,
+                             uint16_t* __restrict__ masks) {
+    const uint32_t p = blockIdx.x * kB + threadIdx.x;
+    if (p >= npix) return;
+    uint32_t m = 0;
+    for (uint32_t l = 0; l < nl; ++l) {
+        const uint64_t r = (uint64_t)l * npix + p;
+        m |= ((bits[r >> 5] >> (r & 31u)) & 1u) << l;
+    }
+    masks[p] = (uint16_t)m;
+}
+
+}  // namespace
+}  // namespace rt
+
+using namespace rt;
+
+extern "C" rt_status rt_gen_sphere(rt_stream stream, uint32_t nu, uint32_t nv, float radius,
+                                   float disp, uint32_t seed, float* d_pos, uint32_t* d_idx) {
+    if (!d_pos || !d_idx || nu < 3 || nv < 2) return RT_ERR_ARG;
+    if ((uint64_t)nu * nv * 2 > 0xFFFFFFFFull) return RT_ERR_CAPACITY;
+    hipStream_t s = as_stream(stream);
+    hipLaunchKernelGGL(k_sphere_verts, dim3(ceil_div((uint64_t)(nv + 1) * nu, kB)), dim3(kB), 0, s,
+                       nu, nv, radius, disp, seed, d_pos);
+    hipLaunchKernelGGL(k_sphere_idx, dim3(ceil_div((uint64_t)nu * nv, kB)), dim3(kB), 0, s, nu, nv,
+                       d_idx);
+    return launch_status("rt_gen_sphere");
+}
+
+extern "C" rt_status rt_gen_wave_grid(rt_stream stream, uint32_t nx, uint32_t ny, float extent,
+                                      float t, float amp, float fx, float fy, float* d_pos,
+                                      uint32_t* d_idx) {
+    if (!d_pos || !d_idx || nx == 0 || ny == 0) return RT_ERR_ARG;
+    hipStream_t s = as_stream(stream);
+    hipLaunchKernelGGL(k_wave_verts, dim3(ceil_div((uint64_t)(nx + 1) * (ny + 1), kB)), dim3(kB),
+                       0, s, nx, ny, extent, t, amp, fx, fy, d_pos);
+    hipLaunchKernelGGL(k_grid_idx, dim3(ceil_div((uint64_t)nx * ny, kB)), dim3(kB), 0, s, nx, ny,
+                       d_idx);
+    return launch_status("rt_gen_wave_grid");
+}
+
+extern "C" rt_status rt_gen_heightfield(rt_stream stream, uint32_t nx, uint32_t ny, float extent,
+                                        float amp, uint32_t seed, float* d_pos, uint32_t* d_idx) {
+    if (!d_pos || !d_idx || nx == 0 || ny == 0) return RT_ERR_ARG;
+    hipStream_t s = as_stream(stream);
+    hipLaunchKernelGGL(k_height_verts, dim3(ceil_div((uint64_t)(nx + 1) * (ny + 1), kB)),
+                       dim3(kB), 0, s, nx, ny, extent, amp, seed, d_pos);
+    hipLaunchKernelGGL(k_grid_idx, dim3(ceil_div((uint64_t)nx * ny, kB)), dim3(kB), 0, s, nx, ny,
+                       d_idx);
+    return launch_status("rt_gen_heightfield");
+}
+
+extern "C" rt_status rt_gen_dynamic_grid(rt_stream stream, uint32_t n, float t, float extent,
+                                         float* d_xyz9) {
+    if (!d_xyz9 || n < 2) return RT_ERR_ARG;
+    // include/radv_bvh.hpp:228-233 grid shape
+    const uint32_t quads = n / 2;
+    uint32_t cols = (uint32_t)std::sqrt((double)quads);
+    while (cols > 1 && quads % cols != 0) cols--;
+    const uint32_t rows = quads / cols;
+    hipLaunchKernelGGL(k_dynamic_grid, dim3(ceil_div(quads, kB)), dim3(kB), 0, as_stream(stream),
+                       cols, rows, t, extent, d_xyz9);
+    return launch_status("rt_gen_dynamic_grid");
+}
+
+extern "C" rt_status rt_gen_camera_rays(rt_stream stream, uint32_t w, uint32_t h,
+                                        const float eye[3], const float look[3],
+                                        const float up[3], float vfov_deg, float tmin,
+                                        float tmax, const rt_rays_out* out) {
+    if (!eye || !look || !up || !out || w == 0 || h == 0) return RT_ERR_ARG;
+    // Basis in double, rounded to float once.
+    double f[3] = {(double)look[0] - eye[0], (double)look[1] - eye[1], (double)look[2] - eye[2]};
+    double fl = std::sqrt(f[0] * f[0] + f[1] * f[1] + f[2] * f[2]);
+    for (double& c : f) c /= fl;
+    double r[3] = {f[1] * up[2] - f[2] * up[1], f[2] * up[0] - f[0] * up[2],
+                   f[0] * up[1] - f[1] * up[0]};
+    double rl = std::sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+    if (!(fl > 0.0) || !(rl > 0.0)) return RT_ERR_ARG;
+    for (double& c : r) c /= rl;
+    double u[3] = {r[1] * f[2] - r[2] * f[1], r[2] * f[0] - r[0] * f[2],
+                   r[0] * f[1] - r[1] * f[0]};
+    double th = std::tan(vfov_deg * 3.14159265358979323846 / 360.0);
+    double aspect = (double)w / (double)h;
+    CamBasis cb;
+    for (int c = 0; c < 3; ++c) {
+        cb.eye[c] = eye[c];
+        cb.F[c] = (float)f[c];
+        cb.R[c] = (float)(r[c] * th * aspect);
+        cb.U[c] = (float)(u[c] * th);
+    }
+    cb.sxs = 2.0f / (float)w;
+    cb.sys = 2.0f / (float)h;
+    cb.tmin = tmin;
+    cb.tmax = tmax;
+    hipLaunchKernelGGL(k_camera, dim3(ceil_div((uint64_t)w * h, kB)), dim3(kB), 0,
+                       as_stream(stream), w, h, cb, *out);
+    return launch_status("rt_gen_camera_rays");
+}
+
+extern "C" rt_status rt_gen_shadow_rays(rt_stream stream, const float* d_pos,
+                                        const uint32_t* d_idx, const rt_rays_soa* primary,
+                                        const uint32_t* d_prim, const float* d_t, uint32_t npix,
+                                        const float* d_lights, uint32_t nlights, float eps,
+                                        const rt_rays_out* out) {
+    if (!d_pos || !d_idx || !primary || !d_prim || !d_t || !d_lights || !out || nlights == 0)
+        return RT_ERR_ARG;
+    if (npix == 0) return RT_OK;
+    if ((uint64_t)npix * nlights > 0xFFFFFFFFull) return RT_ERR_CAPACITY;
+    hipLaunchKernelGGL(k_shadow, dim3(ceil_div(npix, kB)), dim3(kB), 0, as_stream(stream), d_pos,
+                       d_idx, *primary, d_prim, d_t, npix, d_lights, nlights, eps, *out);
+    return launch_status("rt_gen_shadow_rays");
+}
+
+extern "C" rt_status rt_gen_ao_rays(rt_stream stream, const float* d_pos, const uint32_t* d_idx,
+                                    const rt_rays_soa* primary, const uint32_t* d_prim,
+                                    const float* d_t, uint32_t w, uint32_t h, uint32_t frame,
+                                    float offset, float tmin, float tmax,
+                                    const rt_rays_out* out) {
+    if (!d_pos || !d_idx || !primary || !d_prim || !d_t || !out) return RT_ERR_ARG;
+    if ((uint64_t)w * h == 0) return RT_OK;
+    hipLaunchKernelGGL(k_ao, dim3(ceil_div((uint64_t)w * h, kB)), dim3(kB), 0, as_stream(stream),
+                       d_pos, d_idx, *primary, d_prim, d_t, w, h, frame, offset, tmin, tmax, *out);
+    return launch_status("rt_gen_ao_rays");
+}
+
+extern "C" rt_status rt_gen_random_rays(rt_stream stream, uint32_t count, uint32_t seed,
+                                        const float lo[3], const float hi[3], float tmax,
+                                        const rt_rays_out* out) {
+    if (!lo || !hi || !out) return RT_ERR_ARG;
+    if (count == 0) return RT_OK;
+    hipLaunchKernelGGL(k_random_rays, dim3(ceil_div(count, kB)), dim3(kB), 0, as_stream(stream),
+                       count, seed, lo[0], lo[1], lo[2], hi[0], hi[1], hi[2], tmax, *out);
+    return launch_status("rt_gen_random_rays");
+}
+
+extern "C" rt_status rt_convert_tris48(rt_stream stream, const float* d_tris48,
+                                       const uint32_t* d_order, uint32_t n, rt_tri64* d_out) {
+    if (n == 0) return RT_OK;
+    if (!d_tris48 || !d_out) return RT_ERR_ARG;
+    hipLaunchKernelGGL(k_convert48, dim3(ceil_div(n, kB)), dim3(kB), 0, as_stream(stream),
+                       d_tris48, d_order, n, d_out);
+    return launch_status("rt_convert_tris48");
+}
+
+extern "C" rt_status rt_pack_light_masks(rt_stream stream, const uint32_t* d_bits, uint32_t npix,
+                                         uint32_t nlights, uint16_t* d_masks) {
+    if (!d_bits || !d_masks || nlights == 0 || nlights > 16) return RT_ERR_ARG;
+    if (npix == 0) return RT_OK;
+    hipLaunchKernelGGL(k_pack_masks, dim3(ceil_div(npix, kB)), dim3(kB), 0, as_stream(stream),
+                       d_bits, npix, nlights, d_masks);
+    return launch_status("rt_pack_light_masks");
+}

@@ -1,0 +1,105 @@
+// rt_internal.h -- device helpers and host error plumbing shared by the
+// librtbvh.so translation units.  gfx950 only; every file that includes this
+// is compiled with -ffp-contract=off so float expressions round once per
+// operator in the order written (the CPU path rounds the same way).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+
+#include "rt_bvh.h"
+
+#pragma clang fp contract(off)
+
+namespace rt {
+
+// ---------------------------------------------------------------- host -----
+void set_error(const char* what, hipError_t e);
+rt_status launch_status(const char* what);
+
+#define RT_TRY_HIP(call)                                   \
+    do {                                                   \
+        hipError_t _e = (call);                            \
+        if (_e != hipSuccess) {                            \
+            ::rt::set_error(#call, _e);                    \
+            return RT_ERR_HIP;                             \
+        }                                                  \
+    } while (0)
+
+inline hipStream_t as_stream(rt_stream s) { return reinterpret_cast<hipStream_t>(s); }
+
+inline uint32_t ceil_div(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
+
+// include/radv_bvh.hpp:193-215 compute_tree_layout (no 8-level cap).
+struct TreeLayout {
+    uint32_t levels = 0;
+    uint32_t total_boxes = 0;
+    uint64_t total_as_size = 0;
+    uint32_t counts[32] = {0};
+    uint32_t offsets[32] = {0};
+};
+TreeLayout tree_layout(uint32_t n);
+
+// --------------------------------------------------------------- device ----
+
+// software_bvh_bvh4.comp:50-54 pcg_hash.
+__host__ __device__ inline uint32_t pcg_hash(uint32_t v) {
+    uint32_t state = v * 747796405u + 2891336453u;
+    uint32_t word = ((state >> ((state >> 28u) + 4u)) ^ state) * 277803737u;
+    return (word >> 22u) ^ word;
+}
+
+// Uniform in [0,1) with 24 random bits (exact conversion and scaling).
+__host__ __device__ inline float hash01(uint32_t seed, uint32_t i) {
+    uint32_t h = pcg_hash(i * 0x9E3779B9u + seed);
+    return (float)(h >> 8) * (1.0f / 16777216.0f);
+}
+
+// Moller-Trumbore with the predicates of software_bvh_bvh4.comp:166-182:
+// |det| < 1e-7 rejects, u,v inclusive in [0,1], u+v <= 1.
+__device__ __forceinline__ bool mt_hit(float v0x, float v0y, float v0z, float v1x, float v1y,
+                                       float v1z, float v2x, float v2y, float v2z, float ox,
+                                       float oy, float oz, float dx, float dy, float dz,
+                                       float* tout) {
+    float e1x = v1x - v0x, e1y = v1y - v0y, e1z = v1z - v0z;
+    float e2x = v2x - v0x, e2y = v2y - v0y, e2z = v2z - v0z;
+    float hx = dy * e2z - dz * e2y;
+    float hy = dz * e2x - dx * e2z;
+    float hz = dx * e2y - dy * e2x;
+    float a = e1x * hx + e1y * hy + e1z * hz;
+    if (fabsf(a) < 1e-7f) return false;
+    float f = 1.0f / a;
+    float sx = ox - v0x, sy = oy - v0y, sz = oz - v0z;
+    float u = f * (sx * hx + sy * hy + sz * hz);
+    if (u < 0.0f || u > 1.0f) return false;
+    float qx = sy * e1z - sz * e1y;
+    float qy = sz * e1x - sx * e1z;
+    float qz = sx * e1y - sy * e1x;
+    float v = f * (dx * qx + dy * qy + dz * qz);
+    if (v < 0.0f || u + v > 1.0f) return false;
+    *tout = f * (e2x * qx + e2y * qy + e2z * qz);
+    return true;
+}
+
+// Float <-> order-preserving uint (for atomicMin / atomicMax on bounds).
+__device__ __forceinline__ uint32_t f2o(float f) {
+    uint32_t b = __float_as_uint(f);
+    return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+__device__ __forceinline__ float o2f(uint32_t o) {
+    return __uint_as_float((o & 0x80000000u) ? (o & 0x7fffffffu) : ~o);
+}
+
+// Spread the low 10 bits of v to every third bit.
+__host__ __device__ inline uint32_t expand_bits10(uint32_t v) {
+    v &= 0x3FFu;
+    v = (v * 0x00010001u) & 0xFF0000FFu;
+    v = (v * 0x00000101u) & 0x0F00F00Fu;
+    v = (v * 0x00000011u) & 0xC30C30C3u;
+    v = (v * 0x00000005u) & 0x49249249u;
+    return v;
+}
+
+}  // namespace rt

@@ -1,0 +1,425 @@
+"""Python host side of librtbvh.so (ctypes over the C ABI in include/rt_bvh.h).
+
+Mirrors the reference's software_bvh interface for this path:
+  SoftwareBVH::build / SoftwareBVH4::fromBVH2 (common/software_bvh.hpp:35,193)
+      -> GpuBVH4.build (device Morton/Box4 build) and GpuBVH4.from_bvh4 (upload
+         of a host-built SoftwareBVH4 tree)
+  software_bvh_bvh4.comp traversal and the inline ray queries
+      -> GpuBVH4.intersect / GpuBVH4.occluded (rt_trace)
+PyTorch supplies device memory and the HIP stream; all compute is the HIP
+library.  There is no CPU fallback: without librtbvh.so, or without a GPU,
+every entry point raises.
+"""
+import ctypes as C
+import math
+import os
+
+import torch
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "librtbvh.so")
+MISS = 0xFFFFFFFF
+
+RT_CLOSEST, RT_ANY, RT_COUNTERS = 0x0, 0x1, 0x100
+_STATUS = {0: "RT_OK", 1: "RT_ERR_ARG", 2: "RT_ERR_CAPACITY", 3: "RT_ERR_HIP",
+           4: "RT_ERR_NOMEM", 5: "RT_ERR_UNSUPPORTED"}
+
+NODE_BYTES = 128  # rt_node4 == GpuBvh4Node
+TRI_BYTES = 64    # rt_tri64 == RadvBvhTriangleNode
+
+
+class RtError(RuntimeError):
+    def __init__(self, fn, code, detail=""):
+        self.code = code
+        super().__init__(f"{fn} -> {_STATUS.get(code, code)} {detail}".strip())
+
+
+vp = C.c_void_p
+u32 = C.c_uint32
+f32 = C.c_float
+
+
+class rt_bvh_dev(C.Structure):
+    _fields_ = [("nodes", vp), ("tris", vp), ("node_count", u32), ("tri_count", u32),
+                ("depth", u32), ("reserved", u32)]
+
+
+class rt_rays_soa(C.Structure):
+    _fields_ = [(n, vp) for n in ("ox", "oy", "oz", "tmin", "dx", "dy", "dz", "tmax")]
+
+
+class rt_rays_out(C.Structure):
+    _fields_ = [(n, vp) for n in ("ox", "oy", "oz", "tmin", "dx", "dy", "dz", "tmax")]
+
+
+class rt_hits_soa(C.Structure):
+    _fields_ = [("prim", vp), ("t", vp)]
+
+
+_lib = None
+
+# name -> argtypes (restype is rt_status = int for all but the string getters)
+_SIGS = {
+    "rt_tree_layout": [u32, vp, vp, vp, vp, vp],
+    "rt_build_scratch_size": [u32, vp],
+    "rt_gpu_build": [vp, vp, vp, u32, vp, u32, vp, vp, vp, vp, C.c_size_t],
+    "rt_gpu_refit": [vp, vp, vp, u32, vp, u32, vp, vp, C.c_size_t],
+    "rt_convert_tris48": [vp, vp, vp, u32, vp],
+    "rt_trace": [vp, vp, vp, u32, u32, vp, vp, vp, vp],
+    "rt_gen_sphere": [vp, u32, u32, f32, f32, u32, vp, vp],
+    "rt_gen_wave_grid": [vp, u32, u32, f32, f32, f32, f32, f32, vp, vp],
+    "rt_gen_heightfield": [vp, u32, u32, f32, f32, u32, vp, vp],
+    "rt_gen_dynamic_grid": [vp, u32, f32, f32, vp],
+    "rt_gen_camera_rays": [vp, u32, u32, vp, vp, vp, f32, f32, f32, vp],
+    "rt_gen_shadow_rays": [vp, vp, vp, vp, vp, vp, u32, vp, u32, f32, vp],
+    "rt_gen_ao_rays": [vp, vp, vp, vp, vp, vp, u32, u32, u32, f32, f32, f32, vp],
+    "rt_gen_random_rays": [vp, u32, u32, vp, vp, f32, vp],
+    "rt_pack_light_masks": [vp, vp, u32, u32, vp],
+    "rt_device_sync": [],
+}
+
+
+def lib():
+    """Load librtbvh.so; raises when it is missing (no fallback path)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"librtbvh.so not built ({LIB_PATH}); run __graft_entry__.build()")
+        L = C.CDLL(LIB_PATH)
+        for name, args in _SIGS.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = C.c_int
+        L.rt_version.restype = C.c_char_p
+        L.rt_last_error.restype = C.c_char_p
+        _lib = L
+    return _lib
+
+
+def exported_symbols():
+    return list(_SIGS) + ["rt_version", "rt_last_error"]
+
+
+def _check(fn, code):
+    if code != 0:
+        detail = lib().rt_last_error().decode() if code == 3 else ""
+        raise RtError(fn, code, detail)
+
+
+def call(name, *args):
+    _check(name, getattr(lib(), name)(*args))
+
+
+def _dev():
+    if not torch.cuda.is_available():
+        raise RuntimeError("rtbvh needs a GPU (torch.cuda.is_available() is False)")
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+def stream_handle(stream=None):
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return vp(s.cuda_stream)
+
+
+def ptr(t):
+    if t is None:
+        return None
+    if not t.is_cuda:
+        raise ValueError("expected a device tensor")
+    return vp(t.data_ptr())
+
+
+def _f3(v):
+    return (C.c_float * 3)(*[float(x) for x in v])
+
+
+# ------------------------------------------------------------------ layout --
+def tree_layout(n):
+    counts = (u32 * 32)()
+    offs = (u32 * 32)()
+    L, tb = u32(0), u32(0)
+    sz = C.c_uint64(0)
+    call("rt_tree_layout", u32(n), counts, offs, C.byref(L), C.byref(tb), C.byref(sz))
+    return list(counts[:L.value]), list(offs[:L.value]), tb.value, sz.value
+
+
+def build_scratch_size(n):
+    s = C.c_size_t(0)
+    call("rt_build_scratch_size", u32(n), C.byref(s))
+    return s.value
+
+
+# --------------------------------------------------------------------- rays --
+class Rays:
+    """SoA ray batch on the device: one (8, n) float32 tensor, rows
+    ox, oy, oz, tmin, dx, dy, dz, tmax (32 B per ray)."""
+    FIELDS = ("ox", "oy", "oz", "tmin", "dx", "dy", "dz", "tmax")
+
+    def __init__(self, data):
+        assert data.dim() == 2 and data.shape[0] == 8 and data.dtype == torch.float32
+        self.data = data.contiguous()
+
+    @staticmethod
+    def empty(n, device=None):
+        return Rays(torch.empty((8, n), dtype=torch.float32, device=device or _dev()))
+
+    @staticmethod
+    def from_numpy(a, device=None):
+        return Rays(torch.as_tensor(a, dtype=torch.float32).to(device or _dev()))
+
+    def __len__(self):
+        return self.data.shape[1]
+
+    def soa(self):
+        p = self.data.data_ptr()
+        stride = self.data.stride(0) * 4
+        return rt_rays_soa(*[vp(p + i * stride) for i in range(8)])
+
+    def out(self):
+        p = self.data.data_ptr()
+        stride = self.data.stride(0) * 4
+        return rt_rays_out(*[vp(p + i * stride) for i in range(8)])
+
+
+# ---------------------------------------------------------------- the BVH --
+class GpuBVH4:
+    """Device-resident Box4 BVH: rt_node4 nodes (GpuBvh4Node layout) and
+    rt_tri64 leaves (RadvBvhTriangleNode layout, carrying triangle IDs)."""
+
+    def __init__(self, nodes, tris, node_count, tri_count, depth):
+        self.nodes = nodes            # uint8 tensor, node_count * 128 B
+        self.tris = tris              # uint8 tensor, tri_count * 64 B
+        self.node_count = node_count
+        self.tri_count = tri_count
+        self.depth = depth
+        self.sorted_ids = None
+        self.codes = None
+        self._scratch = None
+
+    # -- build ---------------------------------------------------------------
+    @staticmethod
+    def build(pos, idx=None, stream=None, keep_order=False, scratch=None):
+        """Device build (rt_gpu_build): Morton-sorted implicit Box4 tree.
+
+        pos: float32 (V, 3) device tensor; idx: uint32/int32 (T, 3) or None
+        for 9 floats per triangle (pos then has shape (T, 9) or (3T, 3))."""
+        dev = pos.device
+        n = idx.shape[0] if idx is not None else pos.numel() // 9
+        counts, _, tb, _ = tree_layout(n)
+        nodes = torch.empty(max(tb, 1) * NODE_BYTES, dtype=torch.uint8, device=dev)
+        tris = torch.empty(max(n, 1) * TRI_BYTES, dtype=torch.uint8, device=dev)
+        bvh = GpuBVH4(nodes, tris, tb, n, len(counts) if n else 0)
+        ids = codes = None
+        if keep_order:
+            ids = torch.empty(max(n, 1), dtype=torch.int32, device=dev)
+            codes = torch.empty(max(n, 1), dtype=torch.int32, device=dev)
+        need = build_scratch_size(n)
+        if scratch is None or scratch.numel() < need:
+            scratch = torch.empty(max(need, 256), dtype=torch.uint8, device=dev)
+        bvh._scratch = scratch
+        call("rt_gpu_build", stream_handle(stream), ptr(pos), ptr(idx), u32(n), ptr(nodes), u32(tb),
+             ptr(tris), ptr(ids), ptr(codes), ptr(scratch), C.c_size_t(scratch.numel()))
+        bvh.sorted_ids, bvh.codes = ids, codes
+        bvh.pos, bvh.idx = pos, idx
+        return bvh
+
+    def refit(self, pos, idx=None, stream=None):
+        """Same topology, new vertex positions (rt_gpu_refit)."""
+        need = build_scratch_size(self.tri_count)
+        if self._scratch is None or self._scratch.numel() < need:
+            self._scratch = torch.empty(max(need, 256), dtype=torch.uint8, device=pos.device)
+        call("rt_gpu_refit", stream_handle(stream), ptr(pos), ptr(idx), u32(self.tri_count),
+             ptr(self.nodes), u32(self.node_count), ptr(self.tris), ptr(self._scratch),
+             C.c_size_t(self._scratch.numel()))
+        self.pos, self.idx = pos, idx
+        return self
+
+    @staticmethod
+    def from_bvh4(nodes4, tris48=None, order=None, tris64=None, depth=None, device=None):
+        """Upload a host-built SoftwareBVH4 (numpy structured/bytes arrays).
+
+        nodes4: GpuBvh4Node records (128 B each).  Either tris64 (64-B leaf
+        records) or tris48 (GpuTriangle, 48 B) plus order (leaf-order original
+        triangle indices), which rt_convert_tris48 turns into 64-B leaves."""
+        import numpy as np
+        dev = device or _dev()
+        nb = np.ascontiguousarray(nodes4).view(np.uint8).reshape(-1)
+        ncount = nb.size // NODE_BYTES
+        nodes = torch.from_numpy(nb.copy()).to(dev) if nb.size else torch.empty(
+            NODE_BYTES, dtype=torch.uint8, device=dev)
+        if tris64 is not None:
+            tb = np.ascontiguousarray(tris64).view(np.uint8).reshape(-1)
+            n = tb.size // TRI_BYTES
+            tris = torch.from_numpy(tb.copy()).to(dev)
+        else:
+            t48 = torch.from_numpy(np.ascontiguousarray(tris48, dtype=np.float32).reshape(-1)).to(dev)
+            n = t48.numel() // 12
+            tris = torch.empty(max(n, 1) * TRI_BYTES, dtype=torch.uint8, device=dev)
+            o = None
+            if order is not None:
+                o = torch.from_numpy(np.ascontiguousarray(order, dtype=np.uint32).view(np.int32)).to(dev)
+            call("rt_convert_tris48", stream_handle(), ptr(t48), ptr(o), u32(n), ptr(tris))
+            torch.cuda.current_stream().synchronize()
+        if depth is None:
+            depth = _depth_host(np.ascontiguousarray(nodes4))
+        return GpuBVH4(nodes, tris, ncount, n, depth)
+
+    # -- trace -----------------------------------------------------------------
+    def dev(self):
+        return rt_bvh_dev(ptr(self.nodes), ptr(self.tris), u32(self.node_count),
+                          u32(self.tri_count), u32(self.depth), u32(0))
+
+    def intersect(self, rays, counters=False, stream=None, out=None):
+        """Closest hit per ray: (prim uint32-as-int32, t float32[, box, tri])."""
+        n = len(rays)
+        dev = rays.data.device
+        prim = out[0] if out else torch.empty(n, dtype=torch.int32, device=dev)
+        t = out[1] if out else torch.empty(n, dtype=torch.float32, device=dev)
+        box = tri = None
+        flags = RT_CLOSEST
+        if counters:
+            box = torch.empty(n, dtype=torch.int32, device=dev)
+            tri = torch.empty(n, dtype=torch.int32, device=dev)
+            flags |= RT_COUNTERS
+        hits = rt_hits_soa(ptr(prim), ptr(t))
+        b = self.dev()
+        r = rays.soa()
+        call("rt_trace", stream_handle(stream), C.byref(b), C.byref(r), u32(n), u32(flags),
+             C.byref(hits), None, ptr(box), ptr(tri))
+        return (prim, t, box, tri) if counters else (prim, t)
+
+    def occluded(self, rays, stream=None, out=None, counters=False):
+        """Any-hit occlusion bits: int32 tensor of ceil(n/32) words."""
+        n = len(rays)
+        dev = rays.data.device
+        bits = out if out is not None else torch.empty(max(1, (n + 31) // 32), dtype=torch.int32,
+                                                       device=dev)
+        box = tri = None
+        flags = RT_ANY
+        if counters:
+            box = torch.empty(n, dtype=torch.int32, device=dev)
+            tri = torch.empty(n, dtype=torch.int32, device=dev)
+            flags |= RT_COUNTERS
+        b = self.dev()
+        r = rays.soa()
+        call("rt_trace", stream_handle(stream), C.byref(b), C.byref(r), u32(n), u32(flags), None,
+             ptr(bits), ptr(box), ptr(tri))
+        return (bits, box, tri) if counters else bits
+
+
+def _depth_host(nodes4):
+    """Node levels on the longest path of a host tree (numpy structured)."""
+    import numpy as np
+    if len(nodes4) == 0:
+        return 0
+    a = nodes4.view(np.uint8).reshape(-1, NODE_BYTES)
+    child = a[:, 96:112].copy().view(np.int32)
+    count = a[:, 112:128].copy().view(np.int32)
+    depth = 0
+    stack = [(0, 1)]
+    while stack:
+        nd, d = stack.pop()
+        depth = max(depth, d)
+        for k in range(4):
+            if count[nd, k] < 0:
+                stack.append((int(child[nd, k]), d + 1))
+    return depth
+
+
+# --------------------------------------------------------------- generators --
+def gen_sphere(nu, nv, radius=1.0, disp=0.02, seed=7, stream=None):
+    dev = _dev()
+    pos = torch.empty(((nv + 1) * nu, 3), dtype=torch.float32, device=dev)
+    idx = torch.empty((2 * nu * nv, 3), dtype=torch.int32, device=dev)
+    call("rt_gen_sphere", stream_handle(stream), u32(nu), u32(nv), f32(radius), f32(disp),
+         u32(seed), ptr(pos), ptr(idx))
+    return pos, idx
+
+
+def gen_wave_grid(nx, ny, extent=1.0, t=0.0, amp=0.1, fx=4.0, fy=4.0, stream=None):
+    dev = _dev()
+    pos = torch.empty(((nx + 1) * (ny + 1), 3), dtype=torch.float32, device=dev)
+    idx = torch.empty((2 * nx * ny, 3), dtype=torch.int32, device=dev)
+    call("rt_gen_wave_grid", stream_handle(stream), u32(nx), u32(ny), f32(extent), f32(t),
+         f32(amp), f32(fx), f32(fy), ptr(pos), ptr(idx))
+    return pos, idx
+
+
+def gen_heightfield(nx, ny, extent=1.0, amp=0.05, seed=11, stream=None):
+    dev = _dev()
+    pos = torch.empty(((nx + 1) * (ny + 1), 3), dtype=torch.float32, device=dev)
+    idx = torch.empty((2 * nx * ny, 3), dtype=torch.int32, device=dev)
+    call("rt_gen_heightfield", stream_handle(stream), u32(nx), u32(ny), f32(extent), f32(amp),
+         u32(seed), ptr(pos), ptr(idx))
+    return pos, idx
+
+
+def gen_dynamic_grid(n, t=0.0, extent=2.0, stream=None, out=None):
+    xyz = out if out is not None else torch.empty((n, 9), dtype=torch.float32, device=_dev())
+    call("rt_gen_dynamic_grid", stream_handle(stream), u32(n), f32(t), f32(extent), ptr(xyz))
+    return xyz
+
+
+def gen_camera_rays(w, h, eye, look, up=(0.0, 0.0, 1.0), vfov=60.0, tmin=0.0, tmax=1e30,
+                    stream=None, out=None):
+    rays = out if out is not None else Rays.empty(w * h)
+    o = rays.out()
+    call("rt_gen_camera_rays", stream_handle(stream), u32(w), u32(h), _f3(eye), _f3(look), _f3(up),
+         f32(vfov), f32(tmin), f32(tmax), C.byref(o))
+    return rays
+
+
+def gen_shadow_rays(pos, idx, primary, prim, t, lights, eps, stream=None, out=None):
+    npix = len(primary)
+    nl = lights.shape[0]
+    rays = out if out is not None else Rays.empty(npix * nl)
+    p = primary.soa()
+    o = rays.out()
+    call("rt_gen_shadow_rays", stream_handle(stream), ptr(pos), ptr(idx), C.byref(p), ptr(prim),
+         ptr(t), u32(npix), ptr(lights), u32(nl), f32(eps), C.byref(o))
+    return rays
+
+
+def gen_ao_rays(pos, idx, primary, prim, t, w, h, frame=0, offset=0.003, tmin=0.001, tmax=0.5,
+                stream=None, out=None):
+    rays = out if out is not None else Rays.empty(w * h)
+    p = primary.soa()
+    o = rays.out()
+    call("rt_gen_ao_rays", stream_handle(stream), ptr(pos), ptr(idx), C.byref(p), ptr(prim), ptr(t),
+         u32(w), u32(h), u32(frame), f32(offset), f32(tmin), f32(tmax), C.byref(o))
+    return rays
+
+
+def gen_random_rays(count, seed, lo, hi, tmax=1e30, stream=None, out=None):
+    rays = out if out is not None else Rays.empty(count)
+    o = rays.out()
+    call("rt_gen_random_rays", stream_handle(stream), u32(count), u32(seed), _f3(lo), _f3(hi),
+         f32(tmax), C.byref(o))
+    return rays
+
+
+def pack_light_masks(bits, npix, nlights, stream=None, out=None):
+    masks = out if out is not None else torch.empty(npix, dtype=torch.int16, device=bits.device)
+    call("rt_pack_light_masks", stream_handle(stream), ptr(bits), u32(npix), u32(nlights),
+         ptr(masks))
+    return masks
+
+
+def lights_in_box(lo, hi, count=16, seed=1337):
+    """count light positions uniform in the box [lo, hi) from a seeded hash
+    (host-side; the same values feed the CPU check)."""
+    out = []
+    for i in range(count):
+        p = []
+        for a in range(3):
+            h = _pcg(((3 * i + a) * 0x9E3779B9 + seed) & 0xFFFFFFFF)
+            u = (h >> 8) * (1.0 / 16777216.0)
+            p.append(lo[a] + u * (hi[a] - lo[a]))
+        out.append(p)
+    return out
+
+
+def _pcg(v):
+    state = (v * 747796405 + 2891336453) & 0xFFFFFFFF
+    word = (((state >> ((state >> 28) + 4)) ^ state) * 277803737) & 0xFFFFFFFF
+    return ((word >> 22) ^ word) & 0xFFFFFFFF
