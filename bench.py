@@ -44,14 +44,15 @@ def build_time_4k(iters=200, warm=20):
     """Median device build time of the 4,096-triangle deforming grid
     (generate_dynamic_grid_mesh layout), fused single-workgroup kernel."""
     xyz = rt.gen_dynamic_grid(4096, 0.0, 2.0)
-    bvh = rt.GpuBVH4.build(xyz, None)
+    bvh = rt.GpuBVH4.build(xyz, None, algorithm="implicit")
+    cfg = rt.build_cfg("implicit")
     s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     times = []
     for i in range(warm + iters):
         s.record()
         rt.call("rt_gpu_build", rt.stream_handle(), rt.ptr(xyz), None, rt.u32(4096),
-                rt.ptr(bvh.nodes), rt.u32(bvh.node_count), rt.ptr(bvh.tris), None, None, None,
-                rt.C.c_size_t(0))
+                rt.C.byref(cfg), rt.ptr(bvh.nodes), rt.u32(bvh.node_count), rt.ptr(bvh.tris),
+                rt.ptr(bvh.info), None, None, None, rt.C.c_size_t(0))
         e.record()
         e.synchronize()
         if i >= warm:
@@ -60,13 +61,20 @@ def build_time_4k(iters=200, warm=20):
 
 
 class Frame:
+    builder = "radix"
+
     def __init__(self):
         self.pos, self.idx = rt.gen_sphere(NU, NV, 1.0, 0.02, SEED)
         torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        self.bvh = rt.GpuBVH4.build(self.pos, self.idx)
+        rt.GpuBVH4.build(self.pos, self.idx, algorithm=self.builder)  # warm the build path
         torch.cuda.synchronize()
-        self.build_ms = (time.perf_counter() - t0) * 1000.0
+        s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        s.record()
+        self.bvh = rt.GpuBVH4.build(self.pos, self.idx, algorithm=self.builder, sync=False)
+        e.record()
+        e.synchronize()
+        self.build_ms = s.elapsed_time(e)
+        self.bvh.node_count, self.bvh.depth = [int(v) for v in self.bvh.info[:2].cpu().tolist()]
         lo = self.pos.min(0).values.cpu().numpy().astype(np.float64)
         hi = self.pos.max(0).values.cpu().numpy().astype(np.float64)
         c, ext = (lo + hi) / 2, (hi - lo)
@@ -135,7 +143,12 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--schedule", choices=["persistent", "simple"], default="persistent",
+                    help="traversal schedule (persistent lane-refill waves or one ray/thread)")
+    ap.add_argument("--builder", choices=["radix", "implicit"], default="radix")
     a = ap.parse_args()
+    rt.DEFAULT_PERSISTENT = a.schedule == "persistent"
+    Frame.builder = a.builder
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); the per-GPU shard "
               "is the one-GPU workload and there is no multi-GPU path in this build",
@@ -194,14 +207,17 @@ def main():
         "config": {"workload": "shadow any-hit: 2.5M-tri displaced sphere, 1920x1080 primary "
                                "closest-hit + 16 shadow rays/pixel",
                    "tris": f.bvh.tri_count, "width": W, "height": H, "lights": NLIGHTS,
-                   "rays_per_step": f.rays_per_step},
+                   "rays_per_step": f.rays_per_step, "builder": f.builder,
+                   "schedule": a.schedule, "bvh_nodes": f.bvh.node_count,
+                   "bvh_depth": f.bvh.depth},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": traffic, "kernel": "k_trace any-hit (shadow)",
+                     "traffic": traffic, "kernel": "k_trace_persist any-hit (shadow)"
+                     if a.schedule == "persistent" else "k_trace any-hit (shadow)",
                      "kernel_ms": round(kern_avg, 4), "algorithmic_bytes": int(ab)},
         "build_us_4k_p50": round(b4k_p50, 2),
         "build_us_4k_p99": round(b4k_p99, 2),
-        "build_ms_2p5m_host_timed": round(f.build_ms, 3),
+        "build_ms_2p5m": round(f.build_ms, 3),
         "frame_latency_ms_p50": round(percentile(step_ms, 0.5), 4),
         "frame_latency_ms_p99": round(percentile(step_ms, 0.99), 4),
     }

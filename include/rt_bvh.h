@@ -67,7 +67,8 @@ typedef struct rt_tri64 {
 } rt_tri64;
 
 /* A device-resident BVH.  depth = node levels on the longest path (sizes the
- * per-lane traversal stack: at most 3 * depth + 1 entries are ever live).   */
+ * per-lane traversal stack: at most 3 * depth + 1 entries are ever live);
+ * 0 = unknown (any depth up to 64, the bound for device-built trees).      */
 typedef struct rt_bvh_dev {
     const rt_node4* nodes;
     const rt_tri64* tris;
@@ -100,6 +101,8 @@ typedef struct rt_hits_soa {
 #define RT_CLOSEST 0x0u  /* closest hit in [tmin, tmax) */
 #define RT_ANY 0x1u      /* occlusion: any hit in [tmin, tmax), first-hit terminate */
 #define RT_COUNTERS 0x100u /* also write per-ray box / triangle test counts */
+#define RT_TRACE_PERSISTENT 0x200u /* persistent waves refilling finished lanes;
+                                      same results, different schedule */
 
 /* ---- library ------------------------------------------------------------ */
 const char* rt_version(void);
@@ -115,19 +118,45 @@ rt_status rt_tree_layout(uint32_t n, uint32_t* counts, uint32_t* offsets, uint32
 /* ---- device build (replaces fast_bvh_builder.comp / SoftwareBVH::build) --
  * Input: positions (float3 per vertex) + indices (uint3 per triangle), or
  * d_idx = NULL for 9 floats per triangle (fast_bvh_builder.comp:120-122).
- * Output: the implicit complete Box4 tree over triangles sorted by 30-bit
- * Morton code of their centroid (ties by triangle index): node_count =
- * total_boxes of rt_tree_layout(n) rt_node4 records, root first, and n
- * rt_tri64 leaves in sorted order.  d_sorted_ids (optional) receives the
- * sorted triangle indices, d_codes (optional) the sorted Morton codes.
- * rt_build_scratch_size gives the scratch bytes rt_gpu_build needs.        */
-rt_status rt_build_scratch_size(uint32_t n, size_t* scratch_bytes);
+ * Triangles are sorted by the 30-bit Morton code of their centroid
+ * (centroid (p0 + p1 + p2) / 3 as software_bvh.hpp:53, ties by triangle
+ * index) and the n rt_tri64 leaves are written in that order.  The tree:
+ *   RT_BUILD_IMPLICIT  the fast_bvh_builder layout (radv_bvh.hpp:193-215):
+ *                      complete 4-ary tree, leaf boxes of 4 consecutive
+ *                      triangles, root first; node count = total_boxes.
+ *   RT_BUILD_RADIX     binary radix tree over the codes (ranges of at most
+ *                      max_leaf_prims triangles are leaves) collapsed with
+ *                      SoftwareBVH4::collapse's rule (software_bvh.hpp:202-291),
+ *                      pre-order numbering; at most max(1, n - 1) nodes.
+ *   RT_BUILD_AUTO      IMPLICIT for n <= 4096 (one fused workgroup), else RADIX.
+ * d_info (device memory) receives the node count and depth of the tree.
+ * d_sorted_ids / d_codes (optional) receive the sorted triangle indices and
+ * codes.  rt_build_requirements gives the node capacity and scratch bytes.  */
+#define RT_BUILD_AUTO 0u
+#define RT_BUILD_RADIX 1u
+#define RT_BUILD_IMPLICIT 2u
+
+typedef struct rt_build_cfg {
+    uint32_t algorithm;      /* RT_BUILD_AUTO / RT_BUILD_RADIX / RT_BUILD_IMPLICIT */
+    uint32_t max_leaf_prims; /* RADIX leaf size, 1..16; 0 = 4 (maxPrimsPerLeaf) */
+    uint32_t reserved[2];
+} rt_build_cfg;
+
+typedef struct rt_build_info {
+    uint32_t node_count;
+    uint32_t depth;
+    uint32_t reserved[2];
+} rt_build_info;
+
+rt_status rt_build_requirements(uint32_t n, const rt_build_cfg* cfg, uint32_t* node_capacity,
+                                size_t* scratch_bytes);
 rt_status rt_gpu_build(rt_stream stream, const float* d_pos, const uint32_t* d_idx, uint32_t n,
-                       rt_node4* d_nodes, uint32_t node_capacity, rt_tri64* d_tris,
-                       uint32_t* d_sorted_ids, uint32_t* d_codes, void* d_scratch,
-                       size_t scratch_bytes);
-/* Refit for a deforming mesh: same topology (leaf order taken from
- * d_tris[i].triangle_id), new positions.                                    */
+                       const rt_build_cfg* cfg, rt_node4* d_nodes, uint32_t node_capacity,
+                       rt_tri64* d_tris, rt_build_info* d_info, uint32_t* d_sorted_ids,
+                       uint32_t* d_codes, void* d_scratch, size_t scratch_bytes);
+/* Refit of an RT_BUILD_IMPLICIT tree for a deforming mesh: same topology
+ * (leaf order taken from d_tris[i].triangle_id), new positions.  Scratch as
+ * rt_build_requirements reports for RT_BUILD_IMPLICIT.                      */
 rt_status rt_gpu_refit(rt_stream stream, const float* d_pos, const uint32_t* d_idx, uint32_t n,
                        rt_node4* d_nodes, uint32_t node_count, rt_tri64* d_tris,
                        void* d_scratch, size_t scratch_bytes);

@@ -489,9 +489,12 @@ static inline bool mt_hit(const or_tri64& tri, const float o[3], const float d[3
     return true;
 }
 
-// Closest-hit (mode 0) or any-hit (mode 1) BVH4 traversal.  Node order,
-// child sort and pruning follow software_bvh_bvh4.comp:111-190; ray tmin/tmax
-// come from the ray (the shader hard-codes 0.01 / 1000).  stack_limit 24
+// Closest-hit (mode 0) or any-hit (mode 1) BVH4 traversal.  Closest hit:
+// node order, child sort and pruning follow software_bvh_bvh4.comp:111-190.
+// Any hit (the TerminateOnFirstHit queries of rq_ao.comp:78-90 /
+// tiled_lighting_ra.comp:138-150) visits hit children in slot order and stops
+// at the first accepted triangle.  Ray tmin/tmax come from the ray (the
+// shader hard-codes 0.01 / 1000).  stack_limit 24
 // reproduces the shader's silent drop on overflow (:185-187); 0 = unbounded.
 // out_prim = triangle_id of the accepted leaf record (0xFFFFFFFF on a miss),
 // out_t = hit t (tmax on a miss).
@@ -528,6 +531,31 @@ void or_trace_bvh4(const or_bvh4_node* nodes, uint32_t node_count, const or_tri6
                     float tminZ = std::min(t0z, t1z), tmaxZ = std::max(t0z, t1z);
                     tEntry[k] = std::max(std::max(tminX, tminY), std::max(tminZ, rayTMin));
                     tExit[k] = std::min(std::min(tmaxX, tmaxY), std::min(tmaxZ, hitT));
+                }
+                if (mode == OR_ANY) {
+                    // any hit: children in slot order, stop at the first
+                    // accepted triangle (the occlusion bit is order-free)
+                    for (int k = 0; k < 4 && !done; ++k) {
+                        if (n.count[k] == 0 || !(tExit[k] >= tEntry[k])) continue;
+                        if (n.count[k] > 0) {
+                            uint32_t first = (uint32_t)~n.child[k];
+                            uint32_t cnt = (uint32_t)n.count[k];
+                            triTests += cnt;
+                            for (uint32_t p = 0; p < cnt; ++p) {
+                                float t;
+                                if (!mt_hit(tris[first + p], o, d, &t)) continue;
+                                if (t >= rayTMin && t < hitT) {
+                                    hitT = t;
+                                    hitPrim = tris[first + p].triangle_id;
+                                    done = true;
+                                    break;
+                                }
+                            }
+                        } else if (stack_limit == 0 || ptr < stack_limit) {
+                            stack[ptr++] = n.child[k];
+                        }
+                    }
+                    continue;
                 }
                 int slot[4];
                 float st[4];
@@ -1014,6 +1042,116 @@ int or_lbvh_build(const float* pos, const uint32_t* idx, uint32_t n, or_bvh4_nod
     if (sorted_ids) std::memcpy(sorted_ids, vals.data(), n * sizeof(uint32_t));
     if (codes_sorted) std::memcpy(codes_sorted, keys.data(), n * sizeof(uint32_t));
     *node_count = tb;
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Radix-tree LBVH: the binary radix tree over the Morton-sorted triangles
+// (split of a sorted range where its highest differing key bit changes;
+// equal keys split by the highest differing bit of the sorted position),
+// ranges of <= max_prims triangles become leaves, and the tree is collapsed
+// to Box4 with SoftwareBVH4::collapse (software_bvh.hpp:202-291) unchanged.
+// ---------------------------------------------------------------------------
+
+// Highest differing bit of positions i and j in the sorted sequence: key
+// bits first, then 32 + the position bits (Karras 2012, delta with ties).
+static inline int delta_key(const uint32_t* k, int64_t i, int64_t j) {
+    uint32_t x = k[i] ^ k[j];
+    if (x) return __builtin_clz(x);
+    return 32 + __builtin_clz((uint32_t)(i ^ j));
+}
+
+// Split position of sorted range [f, l] (f < l): the last position whose
+// prefix with f is longer than the prefix shared by the whole range.
+static uint32_t find_split(const uint32_t* k, uint32_t f, uint32_t l) {
+    const int common = delta_key(k, f, l);
+    uint32_t split = f;
+    uint32_t step = l - f;
+    do {
+        step = (step + 1) >> 1;
+        uint32_t ns = split + step;
+        if (ns < l && delta_key(k, f, ns) > common) split = ns;
+    } while (step > 1);
+    return split;
+}
+
+struct KarrasBuilder {
+    const uint32_t* keys;
+    uint32_t max_prims;
+    std::vector<or_bvh2_node> nodes;
+    const float* box;  // per sorted position: min xyz, max xyz
+
+    int32_t build(uint32_t f, uint32_t l) {
+        int32_t id = (int32_t)nodes.size();
+        nodes.emplace_back();
+        float mn[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, mx[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+        if (l - f + 1 <= max_prims) {
+            for (uint32_t i = f; i <= l; ++i)
+                for (int a = 0; a < 3; ++a) {
+                    mn[a] = std::min(mn[a], box[i * 6 + a]);
+                    mx[a] = std::max(mx[a], box[i * 6 + 3 + a]);
+                }
+            nodes[id].leftChild = (int32_t)~f;
+            nodes[id].rightChild = (int32_t)(l - f + 1);
+        } else {
+            uint32_t s = find_split(keys, f, l);
+            int32_t a = build(f, s);
+            int32_t b = build(s + 1, l);
+            for (int c = 0; c < 3; ++c) {
+                mn[c] = std::min(nodes[a].aabbMin[c], nodes[b].aabbMin[c]);
+                mx[c] = std::max(nodes[a].aabbMax[c], nodes[b].aabbMax[c]);
+            }
+            nodes[id].leftChild = a;
+            nodes[id].rightChild = b;
+        }
+        for (int c = 0; c < 3; ++c) {
+            nodes[id].aabbMin[c] = mn[c];
+            nodes[id].aabbMax[c] = mx[c];
+        }
+        return id;
+    }
+};
+
+// Full radix-tree build: Morton codes, stable sort, radix tree with
+// max_prims leaves (BVH2, pre-order), collapse to Box4, 64-B leaves in sorted
+// order.  Returns 1 when a capacity is too small.
+int or_rlbvh_build(const float* pos, const uint32_t* idx, uint32_t n, uint32_t max_prims,
+                   or_bvh2_node* n2_out, uint32_t cap2, uint32_t* count2, or_bvh4_node* n4_out,
+                   uint32_t cap4, uint32_t* count4, or_tri64* tris, uint32_t* sorted_ids,
+                   uint32_t* codes_sorted) {
+    *count2 = 0;
+    *count4 = 0;
+    if (n == 0) return 0;
+    float b[6];
+    or_centroid_bounds(pos, idx, n, b);
+    std::vector<uint32_t> keys(n), vals(n);
+    or_morton30(pos, idx, n, b, keys.data());
+    for (uint32_t i = 0; i < n; ++i) vals[i] = i;
+    or_sort_pairs(keys.data(), vals.data(), n);
+    std::vector<float> box((size_t)n * 6);
+    for (uint32_t i = 0; i < n; ++i) {
+        const float *p0, *p1, *p2;
+        tri_vertices(pos, idx, vals[i], &p0, &p1, &p2);
+        or_tri64& r = tris[i];
+        std::memset(&r, 0, sizeof r);
+        for (int c = 0; c < 3; ++c) {
+            r.v[c] = p0[c]; r.v[3 + c] = p1[c]; r.v[6 + c] = p2[c];
+            box[i * 6 + c] = fmin3(p0[c], p1[c], p2[c]) + 0.0f;
+            box[i * 6 + 3 + c] = fmax3(p0[c], p1[c], p2[c]) + 0.0f;
+        }
+        r.triangle_id = vals[i];
+        r.geometry_id_and_flags = 0x80000000u;
+        r.id = 9u;
+    }
+    KarrasBuilder kb{keys.data(), max_prims, {}, box.data()};
+    kb.nodes.reserve(2 * (size_t)n);
+    kb.build(0, n - 1);
+    if (kb.nodes.size() > cap2) return 1;
+    std::memcpy(n2_out, kb.nodes.data(), kb.nodes.size() * sizeof(or_bvh2_node));
+    *count2 = (uint32_t)kb.nodes.size();
+    if (or_bvh4_collapse(n2_out, *count2, n4_out, cap4, count4)) return 1;
+    if (sorted_ids) std::memcpy(sorted_ids, vals.data(), n * sizeof(uint32_t));
+    if (codes_sorted) std::memcpy(codes_sorted, keys.data(), n * sizeof(uint32_t));
     return 0;
 }
 

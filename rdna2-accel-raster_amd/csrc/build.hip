@@ -343,14 +343,6 @@ __global__ __launch_bounds__(kSortBlock) void k_scatter(const uint32_t* __restri
     }
 }
 
-__device__ __forceinline__ void write_slot(rt_node4* nd, int c, const float bb[6], int32_t child,
-                                           int32_t count) {
-    nd->minX[c] = bb[0]; nd->minY[c] = bb[1]; nd->minZ[c] = bb[2];
-    nd->maxX[c] = bb[3]; nd->maxY[c] = bb[4]; nd->maxZ[c] = bb[5];
-    nd->child[c] = child;
-    nd->count[c] = count;
-}
-
 // Build one node in registers and store it as 8 x 16 B.
 struct NodeReg {
     float4 mnx, mxx, mny, mxy, mnz, mxz;
@@ -513,10 +505,14 @@ template <bool REFIT>
 __global__ __launch_bounds__(kSmallBlock) void k_build_small(
     const float* __restrict__ pos, const uint32_t* __restrict__ idx, uint32_t n,
     rt_node4* __restrict__ nodes, rt_tri64* __restrict__ tris, uint32_t* __restrict__ ids_out,
-    uint32_t* __restrict__ codes_out, LevelArgs L) {
+    uint32_t* __restrict__ codes_out, LevelArgs L, rt_build_info* info) {
     __shared__ SmallLds S;
     const uint32_t tid = threadIdx.x, lane = tid & 63u, w = tid >> 6;
     int cur = 0;
+    if (info && tid == 0) {
+        info->node_count = L.offsets[L.levels - 1] + L.counts[L.levels - 1];
+        info->depth = L.levels;
+    }
     if (!REFIT) {
         // centroids of triangles p = w*256 + i*64 + lane (warp-striped order)
         float cen[kSmallItems][3];
@@ -606,6 +602,242 @@ __global__ __launch_bounds__(kSmallBlock) void k_build_small(
     }
 }
 
+// ---------------------------------------------------------------------------
+// Radix-tree path: binary radix tree over the sorted codes (Karras 2012,
+// equal codes ordered by sorted position), ranges of <= maxp triangles are
+// leaves, collapsed to Box4 with the greedy largest-surface-area expansion of
+// software_bvh.hpp:225-247 and numbered in the pre-order of
+// software_bvh.hpp:249-277.
+//
+// Node ids: internal node i in [0, m), leaf (sorted position) p -> m + p,
+// m = n - 1.  Internal node i covers sorted range [rf[i], rl[i]].
+// ---------------------------------------------------------------------------
+struct RadixArrays {
+    const uint32_t* keys;  // sorted codes
+    float* box;            // (2n - 1) x 6: min xyz, max xyz
+    uint32_t* rf;          // m
+    uint32_t* rl;          // m
+    int32_t* left;         // m
+    int32_t* right;        // m
+    int32_t* parent;       // 2n - 1
+    uint32_t* counter;     // m
+    int4* cand;            // m, -1 = empty
+    uint32_t* size4;       // m: Box4 nodes in the subtree rooted here
+    uint32_t n, m, maxp;
+};
+
+// Highest differing bit of sorted positions i and j: key bits, then 32 +
+// position bits; -1 outside [0, n).
+__device__ __forceinline__ int rdelta(const uint32_t* __restrict__ k, uint32_t n, int64_t i,
+                                      int64_t j) {
+    if (j < 0 || j >= (int64_t)n) return -1;
+    const uint32_t x = k[i] ^ k[j];
+    if (x) return __clz(x);
+    return 32 + __clz((uint32_t)(i ^ j));
+}
+
+__device__ __forceinline__ uint32_t nfirst(const RadixArrays& A, int32_t u) {
+    return (uint32_t)u < A.m ? A.rf[u] : (uint32_t)u - A.m;
+}
+__device__ __forceinline__ uint32_t nlast(const RadixArrays& A, int32_t u) {
+    return (uint32_t)u < A.m ? A.rl[u] : (uint32_t)u - A.m;
+}
+__device__ __forceinline__ bool ninner(const RadixArrays& A, int32_t u) {
+    return (uint32_t)u < A.m && A.rl[u] - A.rf[u] + 1 > A.maxp;
+}
+
+// Sorted position p: 64-B leaf record and its box (canonical zeros).
+__global__ __launch_bounds__(256) void k_rt_leaves(const float* __restrict__ pos,
+                                                   const uint32_t* __restrict__ idx, uint32_t n,
+                                                   const uint32_t* __restrict__ sorted,
+                                                   rt_tri64* __restrict__ tris,
+                                                   float* __restrict__ box, uint32_t m) {
+    const uint32_t p = blockIdx.x * 256 + threadIdx.x;
+    if (p >= n) return;
+    const uint32_t id = sorted[p];
+    float v[9];
+    tri_verts(pos, idx, id, v);
+    float4* tp = reinterpret_cast<float4*>(tris + p);
+    tp[0] = make_float4(v[0], v[1], v[2], v[3]);
+    tp[1] = make_float4(v[4], v[5], v[6], v[7]);
+    tp[2] = make_float4(v[8], 0.f, 0.f, 0.f);
+    reinterpret_cast<uint4*>(tp)[3] = make_uint4(id, 0x80000000u, 0u, 9u);
+    float* b = box + 6 * (size_t)(m + p);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        b[a] = fminf(v[a], fminf(v[3 + a], v[6 + a])) + 0.0f;
+        b[3 + a] = fmaxf(v[a], fmaxf(v[3 + a], v[6 + a])) + 0.0f;
+    }
+}
+
+// Internal node i (Karras 2012, Alg. 1 with the position tie-break).
+__global__ __launch_bounds__(256) void k_rt_radix(RadixArrays A, rt_build_info* info) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= A.m) return;
+    const uint32_t* k = A.keys;
+    const uint32_t n = A.n;
+    const int64_t ii = i;
+    const int d = (rdelta(k, n, ii, ii + 1) - rdelta(k, n, ii, ii - 1)) >= 0 ? 1 : -1;
+    const int dmin = rdelta(k, n, ii, ii - d);
+    int64_t lmax = 2;
+    while (rdelta(k, n, ii, ii + lmax * d) > dmin) lmax *= 2;
+    int64_t l = 0;
+    for (int64_t t = lmax / 2; t >= 1; t /= 2)
+        if (rdelta(k, n, ii, ii + (l + t) * d) > dmin) l += t;
+    const int64_t j = ii + l * d;
+    const int dnode = rdelta(k, n, ii, j);
+    int64_t s = 0, t = l;
+    do {
+        t = (t + 1) >> 1;
+        if (rdelta(k, n, ii, ii + (s + t) * d) > dnode) s += t;
+    } while (t > 1);
+    const int64_t g = ii + s * d + (d < 0 ? -1 : 0);
+    const int64_t lo = ii < j ? ii : j, hi = ii < j ? j : ii;
+    const int32_t L = (lo == g) ? (int32_t)(A.m + g) : (int32_t)g;
+    const int32_t R = (hi == g + 1) ? (int32_t)(A.m + g + 1) : (int32_t)(g + 1);
+    A.rf[i] = (uint32_t)lo;
+    A.rl[i] = (uint32_t)hi;
+    A.left[i] = L;
+    A.right[i] = R;
+    A.parent[L] = (int32_t)i;
+    A.parent[R] = (int32_t)i;
+    A.counter[i] = 0;
+    if (i == 0) {
+        A.parent[0] = -1;
+        info->node_count = 0;
+        info->depth = 0;
+    }
+}
+
+__device__ __forceinline__ float box_sa(const float* b) {
+    // software_bvh.hpp:231-234
+    const float dx = b[3] - b[0], dy = b[4] - b[1], dz = b[5] - b[2];
+    return 2.0f * (dx * dy + dy * dz + dz * dx);
+}
+
+// Leaf p climbs towards the root; at each internal node the second thread
+// to arrive (acquire-release counter) finishes the node: box = union of the
+// children, the greedy Box4 candidates and the Box4 subtree size.
+__global__ __launch_bounds__(256) void k_rt_bottom_up(RadixArrays A) {
+    const uint32_t p = blockIdx.x * 256 + threadIdx.x;
+    if (p >= A.n) return;
+    int32_t x = A.parent[A.m + p];
+    while (x >= 0) {
+        const uint32_t old = __hip_atomic_fetch_add(A.counter + x, 1u, __ATOMIC_ACQ_REL,
+                                                    __HIP_MEMORY_SCOPE_AGENT);
+        if (old == 0) return;  // the sibling subtree is still in flight
+        const int32_t L = A.left[x], R = A.right[x];
+        float* bx = A.box + 6 * (size_t)x;
+        const float* bl = A.box + 6 * (size_t)L;
+        const float* br = A.box + 6 * (size_t)R;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            bx[a] = fminf(bl[a], br[a]);
+            bx[3 + a] = fmaxf(bl[3 + a], br[3 + a]);
+        }
+        if (A.rl[x] - A.rf[x] + 1 > A.maxp) {
+            int32_t c[4] = {L, R, -1, -1};
+            int nc = 2;
+            while (nc < 4) {  // software_bvh.hpp:225-247
+                int best = -1;
+                float bestSa = -1.0f;
+                for (int q = 0; q < nc; ++q) {
+                    if (!ninner(A, c[q])) continue;
+                    const float sa = box_sa(A.box + 6 * (size_t)c[q]);
+                    if (sa > bestSa) {
+                        bestSa = sa;
+                        best = q;
+                    }
+                }
+                if (best < 0) break;
+                const int32_t e = c[best];
+                c[best] = A.left[e];
+                c[nc++] = A.right[e];
+            }
+            uint32_t sz = 1;
+            for (int q = 0; q < nc; ++q)
+                if (ninner(A, c[q])) sz += A.size4[c[q]];
+            A.cand[x] = make_int4(c[0], c[1], c[2], c[3]);
+            A.size4[x] = sz;
+        }
+        x = A.parent[x];
+    }
+}
+
+// Internal node x with more than maxp triangles: search down from the root
+// through the candidates; if x is a Box4 node, its pre-order index is the
+// parent's index + 1 + the sizes of the inner candidates before it.
+__global__ __launch_bounds__(256) void k_rt_emit(RadixArrays A, rt_node4* __restrict__ nodes,
+                                                 rt_build_info* info) {
+    const uint32_t x = blockIdx.x * 256 + threadIdx.x;
+    if (x >= A.m) return;
+    const uint32_t xf = A.rf[x], xl = A.rl[x];
+    if (xl - xf + 1 <= A.maxp) return;
+    int32_t cur = 0;
+    uint32_t idx = 0, level = 1;
+    while (cur != (int32_t)x) {
+        const int4 cv = A.cand[cur];
+        const int32_t cs[4] = {cv.x, cv.y, cv.z, cv.w};
+        uint32_t acc = idx + 1;
+        int32_t next = -1;
+        for (int q = 0; q < 4 && cs[q] >= 0; ++q) {
+            const int32_t c = cs[q];
+            const bool inner = ninner(A, c);
+            if (nfirst(A, c) <= xf && xl <= nlast(A, c)) {
+                if (inner) next = c;
+                break;
+            }
+            if (inner) acc += A.size4[c];
+        }
+        if (next < 0) return;  // x was expanded into its parent's Box4 node
+        cur = next;
+        idx = acc;
+        ++level;
+    }
+    const int4 cv = A.cand[x];
+    const int32_t cs[4] = {cv.x, cv.y, cv.z, cv.w};
+    NodeReg nr;
+    node_clear(nr);
+    uint32_t acc = idx + 1;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int32_t c = cs[q];
+        if (c < 0) continue;
+        float bb[6];
+#pragma unroll
+        for (int a = 0; a < 6; ++a) bb[a] = A.box[6 * (size_t)c + a];
+        if (ninner(A, c)) {
+            node_set(nr, q, bb, (int32_t)acc, -1);
+            acc += A.size4[c];
+        } else {
+            const uint32_t f = nfirst(A, c);
+            node_set(nr, q, bb, (int32_t)~f, (int32_t)(nlast(A, c) - f + 1));
+        }
+    }
+    node_store(nodes + idx, nr);
+    atomicMax(&info->depth, level);
+    if (x == 0) info->node_count = A.size4[0];
+}
+
+// n <= maxp: the whole mesh is one leaf range; a root with one slot
+// (software_bvh.hpp:205-219).
+__global__ void k_rt_single(const float* __restrict__ box, uint32_t n, uint32_t m,
+                            rt_node4* __restrict__ nodes, rt_build_info* info) {
+    if (threadIdx.x != 0) return;
+    float bb[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+    for (uint32_t p = 0; p < n; ++p)
+        for (int a = 0; a < 3; ++a) {
+            bb[a] = fminf(bb[a], box[6 * (size_t)(m + p) + a]);
+            bb[3 + a] = fmaxf(bb[3 + a], box[6 * (size_t)(m + p) + 3 + a]);
+        }
+    NodeReg nr;
+    node_clear(nr);
+    node_set(nr, 0, bb, (int32_t)~0u, (int32_t)n);
+    node_store(nodes, nr);
+    info->node_count = 1;
+    info->depth = 1;
+}
+
 LevelArgs level_args(const TreeLayout& t) {
     LevelArgs a;
     for (int i = 0; i < 32; ++i) {
@@ -618,15 +850,18 @@ LevelArgs level_args(const TreeLayout& t) {
 
 struct ScratchPlan {
     size_t bounds = 0, keys0 = 0, vals0 = 0, keys1 = 0, vals1 = 0, hist = 0, totals = 0,
-           boxes = 0, total = 0;
+           boxes = 0, rbox = 0, rf = 0, rl = 0, left = 0, right = 0, parent = 0, counter = 0,
+           cand = 0, size4 = 0, total = 0;
     uint32_t tiles = 0;
 };
 
-ScratchPlan plan_scratch(uint32_t n) {
+// Scratch of one build: sort buffers, then the implicit level boxes or the
+// radix-tree arrays.
+ScratchPlan plan_scratch(uint32_t n, uint32_t algo) {
     ScratchPlan p;
-    const TreeLayout t = tree_layout(n);
     auto al = [](size_t x) { return (x + 255) & ~size_t(255); };
     size_t off = 0;
+    const size_t m = n ? n - 1 : 0, u = m + n;
     p.tiles = ceil_div(n, kTile);
     p.bounds = off; off += al(8 * 4);
     p.keys0 = off; off += al((size_t)n * 4);
@@ -635,9 +870,106 @@ ScratchPlan plan_scratch(uint32_t n) {
     p.vals1 = off; off += al((size_t)n * 4);
     p.hist = off; off += al((size_t)256 * p.tiles * 4);
     p.totals = off; off += al(256 * 4);
-    p.boxes = off; off += al((size_t)t.total_boxes * 24);
+    if (algo == RT_BUILD_IMPLICIT) {
+        p.boxes = off; off += al((size_t)tree_layout(n).total_boxes * 24);
+    } else {
+        p.rbox = off; off += al(u * 24);
+        p.rf = off; off += al(m * 4);
+        p.rl = off; off += al(m * 4);
+        p.left = off; off += al(m * 4);
+        p.right = off; off += al(m * 4);
+        p.parent = off; off += al(u * 4);
+        p.counter = off; off += al(m * 4);
+        p.cand = off; off += al(m * 16);
+        p.size4 = off; off += al(m * 4);
+    }
     p.total = off;
     return p;
+}
+
+uint32_t resolve_algo(const rt_build_cfg* cfg, uint32_t n) {
+    const uint32_t a = cfg ? cfg->algorithm : RT_BUILD_AUTO;
+    if (a == RT_BUILD_AUTO) return n <= kSmallMax ? RT_BUILD_IMPLICIT : RT_BUILD_RADIX;
+    return a;
+}
+
+uint32_t resolve_maxp(const rt_build_cfg* cfg) {
+    return (cfg && cfg->max_leaf_prims) ? cfg->max_leaf_prims : 4u;
+}
+
+uint32_t max_nodes(uint32_t n, uint32_t algo) {
+    if (algo == RT_BUILD_IMPLICIT) return tree_layout(n).total_boxes;
+    return n > 1 ? n - 1 : 1;
+}
+
+__global__ void k_set_info(rt_build_info* info, uint32_t nodes, uint32_t depth) {
+    if (threadIdx.x == 0) {
+        info->node_count = nodes;
+        info->depth = depth;
+    }
+}
+
+// Bounds -> Morton -> 4-pass radix sort; sorted codes in k0, ids in v0.
+rt_status morton_sort(hipStream_t s, const float* pos, const uint32_t* idx, uint32_t n,
+                      const ScratchPlan& p, char* base, uint32_t** keys, uint32_t** vals) {
+    uint32_t* ob = reinterpret_cast<uint32_t*>(base + p.bounds);
+    uint32_t* k0 = reinterpret_cast<uint32_t*>(base + p.keys0);
+    uint32_t* v0 = reinterpret_cast<uint32_t*>(base + p.vals0);
+    uint32_t* k1 = reinterpret_cast<uint32_t*>(base + p.keys1);
+    uint32_t* v1 = reinterpret_cast<uint32_t*>(base + p.vals1);
+    uint32_t* hist = reinterpret_cast<uint32_t*>(base + p.hist);
+    uint32_t* totals = reinterpret_cast<uint32_t*>(base + p.totals);
+    hipLaunchKernelGGL(k_init_bounds, dim3(1), dim3(64), 0, s, ob);
+    const uint32_t bgrid = std::min<uint32_t>(ceil_div(n, 256), 2048);
+    hipLaunchKernelGGL(k_bounds, dim3(bgrid), dim3(256), 0, s, pos, idx, n, ob);
+    hipLaunchKernelGGL(k_morton, dim3(ceil_div(n, 256)), dim3(256), 0, s, pos, idx, n, ob, k0, v0);
+    rt_status st = launch_status("k_morton");
+    if (st != RT_OK) return st;
+    uint32_t *ki = k0, *vi = v0, *ko = k1, *vo = v1;
+    for (uint32_t shift = 0; shift < 32; shift += 8) {
+        hipLaunchKernelGGL(k_hist, dim3(p.tiles), dim3(kSortBlock), 0, s, ki, n, shift, p.tiles,
+                           hist);
+        hipLaunchKernelGGL(k_digit_scan, dim3(256), dim3(1024), 0, s, hist, p.tiles, totals);
+        hipLaunchKernelGGL(k_scatter, dim3(p.tiles), dim3(kSortBlock), 0, s, ki, vi, ko, vo, n,
+                           shift, p.tiles, hist, totals);
+        st = launch_status("k_scatter");
+        if (st != RT_OK) return st;
+        std::swap(ki, ko);
+        std::swap(vi, vo);
+    }
+    *keys = k0;  // four passes: back in the first buffers
+    *vals = v0;
+    return RT_OK;
+}
+
+rt_status build_radix(hipStream_t s, const float* pos, const uint32_t* idx, uint32_t n,
+                      uint32_t maxp, const ScratchPlan& p, char* base, const uint32_t* keys,
+                      const uint32_t* vals, rt_node4* nodes, rt_tri64* tris,
+                      rt_build_info* info) {
+    RadixArrays A;
+    A.keys = keys;
+    A.box = reinterpret_cast<float*>(base + p.rbox);
+    A.rf = reinterpret_cast<uint32_t*>(base + p.rf);
+    A.rl = reinterpret_cast<uint32_t*>(base + p.rl);
+    A.left = reinterpret_cast<int32_t*>(base + p.left);
+    A.right = reinterpret_cast<int32_t*>(base + p.right);
+    A.parent = reinterpret_cast<int32_t*>(base + p.parent);
+    A.counter = reinterpret_cast<uint32_t*>(base + p.counter);
+    A.cand = reinterpret_cast<int4*>(base + p.cand);
+    A.size4 = reinterpret_cast<uint32_t*>(base + p.size4);
+    A.n = n;
+    A.m = n - 1;
+    A.maxp = maxp;
+    hipLaunchKernelGGL(k_rt_leaves, dim3(ceil_div(n, 256)), dim3(256), 0, s, pos, idx, n, vals,
+                       tris, A.box, A.m);
+    if (n <= maxp) {
+        hipLaunchKernelGGL(k_rt_single, dim3(1), dim3(64), 0, s, A.box, n, A.m, nodes, info);
+        return launch_status("k_rt_single");
+    }
+    hipLaunchKernelGGL(k_rt_radix, dim3(ceil_div(A.m, 256)), dim3(256), 0, s, A, info);
+    hipLaunchKernelGGL(k_rt_bottom_up, dim3(ceil_div(n, 256)), dim3(256), 0, s, A);
+    hipLaunchKernelGGL(k_rt_emit, dim3(ceil_div(A.m, 256)), dim3(256), 0, s, A, nodes, info);
+    return launch_status("k_rt_emit");
 }
 
 rt_status emit_levels(hipStream_t s, const float* pos, const uint32_t* idx, uint32_t n,
@@ -688,62 +1020,59 @@ extern "C" rt_status rt_tree_layout(uint32_t n, uint32_t* counts, uint32_t* offs
     return RT_OK;
 }
 
-extern "C" rt_status rt_build_scratch_size(uint32_t n, size_t* scratch_bytes) {
-    if (!scratch_bytes) return RT_ERR_ARG;
-    *scratch_bytes = plan_scratch(n).total;
+extern "C" rt_status rt_build_requirements(uint32_t n, const rt_build_cfg* cfg,
+                                           uint32_t* node_capacity, size_t* scratch_bytes) {
+    const uint32_t algo = resolve_algo(cfg, n);
+    if (algo != RT_BUILD_IMPLICIT && algo != RT_BUILD_RADIX) return RT_ERR_ARG;
+    if (node_capacity) *node_capacity = max_nodes(n, algo);
+    if (scratch_bytes) {
+        const bool fused = algo == RT_BUILD_IMPLICIT && n <= kSmallMax;
+        *scratch_bytes = fused ? 0 : plan_scratch(n, algo).total;
+    }
     return RT_OK;
 }
 
 extern "C" rt_status rt_gpu_build(rt_stream stream, const float* d_pos, const uint32_t* d_idx,
-                                  uint32_t n, rt_node4* d_nodes, uint32_t node_capacity,
-                                  rt_tri64* d_tris, uint32_t* d_sorted_ids, uint32_t* d_codes,
-                                  void* d_scratch, size_t scratch_bytes) {
-    if (n == 0) return RT_OK;
+                                  uint32_t n, const rt_build_cfg* cfg, rt_node4* d_nodes,
+                                  uint32_t node_capacity, rt_tri64* d_tris, rt_build_info* d_info,
+                                  uint32_t* d_sorted_ids, uint32_t* d_codes, void* d_scratch,
+                                  size_t scratch_bytes) {
+    if (!d_info) return RT_ERR_ARG;
+    hipStream_t s = as_stream(stream);
+    if (n == 0) {
+        hipLaunchKernelGGL(k_set_info, dim3(1), dim3(64), 0, s, d_info, 0u, 0u);
+        return launch_status("k_set_info");
+    }
     if (!d_pos || !d_nodes || !d_tris) return RT_ERR_ARG;
     if (n > (1u << 30)) return RT_ERR_CAPACITY;
+    const uint32_t algo = resolve_algo(cfg, n);
+    const uint32_t maxp = resolve_maxp(cfg);
+    if (algo != RT_BUILD_IMPLICIT && algo != RT_BUILD_RADIX) return RT_ERR_ARG;
+    if (maxp < 1 || maxp > 16) return RT_ERR_ARG;
+    if (node_capacity < max_nodes(n, algo)) return RT_ERR_CAPACITY;
     const TreeLayout t = tree_layout(n);
-    if (node_capacity < t.total_boxes) return RT_ERR_CAPACITY;
-    hipStream_t s = as_stream(stream);
-    if (n <= kSmallMax) {
+    if (algo == RT_BUILD_IMPLICIT && n <= kSmallMax) {
         hipLaunchKernelGGL((k_build_small<false>), dim3(1), dim3(kSmallBlock), 0, s, d_pos, d_idx,
-                           n, d_nodes, d_tris, d_sorted_ids, d_codes, level_args(t));
+                           n, d_nodes, d_tris, d_sorted_ids, d_codes, level_args(t), d_info);
         return launch_status("k_build_small");
     }
-    const ScratchPlan p = plan_scratch(n);
+    const ScratchPlan p = plan_scratch(n, algo);
     if (!d_scratch || scratch_bytes < p.total) return RT_ERR_CAPACITY;
     char* base = static_cast<char*>(d_scratch);
-    uint32_t* ob = reinterpret_cast<uint32_t*>(base + p.bounds);
-    uint32_t* k0 = reinterpret_cast<uint32_t*>(base + p.keys0);
-    uint32_t* v0 = reinterpret_cast<uint32_t*>(base + p.vals0);
-    uint32_t* k1 = reinterpret_cast<uint32_t*>(base + p.keys1);
-    uint32_t* v1 = reinterpret_cast<uint32_t*>(base + p.vals1);
-    uint32_t* hist = reinterpret_cast<uint32_t*>(base + p.hist);
-    uint32_t* totals = reinterpret_cast<uint32_t*>(base + p.totals);
-    float* gboxes = reinterpret_cast<float*>(base + p.boxes);
-
-    hipLaunchKernelGGL(k_init_bounds, dim3(1), dim3(64), 0, s, ob);
-    const uint32_t bgrid = std::min<uint32_t>(ceil_div(n, 256), 2048);
-    hipLaunchKernelGGL(k_bounds, dim3(bgrid), dim3(256), 0, s, d_pos, d_idx, n, ob);
-    hipLaunchKernelGGL(k_morton, dim3(ceil_div(n, 256)), dim3(256), 0, s, d_pos, d_idx, n, ob, k0,
-                       v0);
-    rt_status st = launch_status("k_morton");
+    uint32_t *keys = nullptr, *vals = nullptr;
+    rt_status st = morton_sort(s, d_pos, d_idx, n, p, base, &keys, &vals);
     if (st != RT_OK) return st;
-    uint32_t *ki = k0, *vi = v0, *ko = k1, *vo = v1;
-    for (uint32_t shift = 0; shift < 32; shift += 8) {
-        hipLaunchKernelGGL(k_hist, dim3(p.tiles), dim3(kSortBlock), 0, s, ki, n, shift, p.tiles,
-                           hist);
-        hipLaunchKernelGGL(k_digit_scan, dim3(256), dim3(1024), 0, s, hist, p.tiles, totals);
-        hipLaunchKernelGGL(k_scatter, dim3(p.tiles), dim3(kSortBlock), 0, s, ki, vi, ko, vo, n,
-                           shift, p.tiles, hist, totals);
-        st = launch_status("k_scatter");
-        if (st != RT_OK) return st;
-        std::swap(ki, ko);
-        std::swap(vi, vo);
-    }
-    // four passes: sorted data is back in k0 / v0
-    if (d_sorted_ids) RT_TRY_HIP(hipMemcpyAsync(d_sorted_ids, v0, (size_t)n * 4, hipMemcpyDeviceToDevice, s));
-    if (d_codes) RT_TRY_HIP(hipMemcpyAsync(d_codes, k0, (size_t)n * 4, hipMemcpyDeviceToDevice, s));
-    return emit_levels(s, d_pos, d_idx, n, v0, d_nodes, d_tris, gboxes, t, false);
+    if (d_sorted_ids)
+        RT_TRY_HIP(hipMemcpyAsync(d_sorted_ids, vals, (size_t)n * 4, hipMemcpyDeviceToDevice, s));
+    if (d_codes)
+        RT_TRY_HIP(hipMemcpyAsync(d_codes, keys, (size_t)n * 4, hipMemcpyDeviceToDevice, s));
+    if (algo == RT_BUILD_RADIX)
+        return build_radix(s, d_pos, d_idx, n, maxp, p, base, keys, vals, d_nodes, d_tris, d_info);
+    float* gboxes = reinterpret_cast<float*>(base + p.boxes);
+    st = emit_levels(s, d_pos, d_idx, n, vals, d_nodes, d_tris, gboxes, t, false);
+    if (st != RT_OK) return st;
+    hipLaunchKernelGGL(k_set_info, dim3(1), dim3(64), 0, s, d_info, t.total_boxes, t.levels);
+    return launch_status("k_set_info");
 }
 
 extern "C" rt_status rt_gpu_refit(rt_stream stream, const float* d_pos, const uint32_t* d_idx,
@@ -756,10 +1085,10 @@ extern "C" rt_status rt_gpu_refit(rt_stream stream, const float* d_pos, const ui
     hipStream_t s = as_stream(stream);
     if (n <= kSmallMax) {
         hipLaunchKernelGGL((k_build_small<true>), dim3(1), dim3(kSmallBlock), 0, s, d_pos, d_idx,
-                           n, d_nodes, d_tris, nullptr, nullptr, level_args(t));
+                           n, d_nodes, d_tris, nullptr, nullptr, level_args(t), nullptr);
         return launch_status("k_build_small");
     }
-    const ScratchPlan p = plan_scratch(n);
+    const ScratchPlan p = plan_scratch(n, RT_BUILD_IMPLICIT);
     if (!d_scratch || scratch_bytes < p.total) return RT_ERR_CAPACITY;
     float* gboxes = reinterpret_cast<float*>(static_cast<char*>(d_scratch) + p.boxes);
     return emit_levels(s, d_pos, d_idx, n, nullptr, d_nodes, d_tris, gboxes, t, true);

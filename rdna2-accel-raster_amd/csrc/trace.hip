@@ -1,22 +1,28 @@
 // trace.hip -- Box4 BVH traversal on gfx950: closest-hit and any-hit
 // (occlusion) over rt_node4 / rt_tri64 trees, SoA rays.
 //
-// Follows benchmarks/exp0_probe/shaders/software_bvh_bvh4.comp:96-190: the
-// same inverse-direction guard, slab test, child filter, child order (the
-// shader's exchange sort by descending entry t, far children pushed first),
-// leaf-first processing and Moller-Trumbore predicate, so on the same tree
-// and rays the hit ID, t and box/triangle counters equal the CPU path bit
-// for bit.  Unlike the shader the per-lane stack never drops entries: its
-// depth is sized from the tree (3 * depth + 1 entries bound the live stack).
+// Closest hit follows benchmarks/exp0_probe/shaders/software_bvh_bvh4.comp:
+// 96-190: the same inverse-direction guard, slab test, child filter, child
+// order (the shader's exchange sort by descending entry t, far children
+// pushed first), leaf-first processing and Moller-Trumbore predicate, so on
+// the same tree and rays the hit ID, t and box/triangle counters equal the
+// CPU path bit for bit.  Any hit (the role of the TerminateOnFirstHit ray
+// queries in rq_ao.comp / tiled_lighting_ra.comp) visits children in slot
+// order and stops at the first accepted triangle; its occlusion bit does not
+// depend on the order.
 //
-// Stack: LDS, entry-major ([S][block]) so lanes at the same depth hit
-// distinct banks; trees deeper than the LDS variants use a private array.
+// Per-lane stack: the top kLdsStack entries live in LDS (entry-major, so
+// lanes at the same depth hit distinct banks; 5 KB per wave keeps 8 waves per
+// SIMD resident), deeper entries go to a private array.  Nothing is ever
+// dropped (the shader drops on overflow, :185-187): 3 * depth + 1 entries
+// bound the live stack and the variant is chosen from the tree depth.
 #include "rt_internal.h"
 
 namespace rt {
 namespace {
 
 constexpr int kBlock = 256;
+constexpr int kLdsStack = 20;
 
 struct Child {
     float key;
@@ -33,7 +39,44 @@ __device__ __forceinline__ void cx(Child& a, Child& b) {
     }
 }
 
-template <int S, bool ANY, bool COUNT, bool PRIVATE>
+template <int PRIV>
+struct Stack {
+    int32_t* lds;  // this lane's column: lds[i * kBlock]
+    int32_t priv[PRIV];
+    int ptr = 0;
+    __device__ __forceinline__ void push(int32_t v) {
+        if (ptr < kLdsStack) lds[ptr * kBlock] = v;
+        else priv[ptr - kLdsStack] = v;
+        ++ptr;
+    }
+    __device__ __forceinline__ int32_t pop() {
+        --ptr;
+        return ptr < kLdsStack ? lds[ptr * kBlock] : priv[ptr - kLdsStack];
+    }
+};
+
+// Leaf range [first, first + cnt): Moller-Trumbore on each 64-B record.
+template <bool ANY>
+__device__ __forceinline__ bool test_leaf(const rt_tri64* __restrict__ tris, uint32_t first,
+                                          uint32_t cnt, float ox, float oy, float oz, float dx,
+                                          float dy, float dz, float tmin, float& hitT,
+                                          uint32_t& hitPrim) {
+    for (uint32_t q = 0; q < cnt; ++q) {
+        const float4* tp = reinterpret_cast<const float4*>(tris + first + q);
+        const float4 a = tp[0], b = tp[1], c = tp[2];
+        float t;
+        if (!mt_hit(a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w, c.x, ox, oy, oz, dx, dy, dz, &t))
+            continue;
+        if (t >= tmin && t < hitT) {
+            hitT = t;
+            hitPrim = tris[first + q].triangle_id;
+            if (ANY) return true;
+        }
+    }
+    return false;
+}
+
+template <int PRIV, bool ANY, bool COUNT>
 __global__ __launch_bounds__(kBlock) void k_trace(const rt_node4* __restrict__ nodes,
                                                   const rt_tri64* __restrict__ tris,
                                                   uint32_t node_count, rt_rays_soa rays,
@@ -42,8 +85,7 @@ __global__ __launch_bounds__(kBlock) void k_trace(const rt_node4* __restrict__ n
                                                   uint32_t* __restrict__ bits_out,
                                                   uint32_t* __restrict__ box_out,
                                                   uint32_t* __restrict__ tri_out) {
-    __shared__ int32_t lds_stack[PRIVATE ? 1 : S * kBlock];
-    int32_t priv_stack[PRIVATE ? S : 1];
+    __shared__ int32_t lds_stack[kLdsStack * kBlock];
     const uint32_t tid = threadIdx.x;
     const uint32_t r = blockIdx.x * kBlock + tid;
     const bool valid = r < n;
@@ -64,20 +106,12 @@ __global__ __launch_bounds__(kBlock) void k_trace(const rt_node4* __restrict__ n
     uint32_t boxTests = 0, triTests = 0;
     bool done = false;
 
-    auto st_at = [&](int i) -> int32_t& {
-        if constexpr (PRIVATE) return priv_stack[i];
-        else return lds_stack[i * kBlock + tid];
-    };
+    Stack<PRIV> st;
+    st.lds = lds_stack + tid;
+    if (valid && node_count > 0) st.push(0);
 
-    int ptr = 0;
-    if (valid && node_count > 0) {
-        st_at(0) = 0;
-        ptr = 1;
-    }
-
-    while (ptr > 0) {
-        --ptr;
-        const int32_t ni = st_at(ptr);
+    while (st.ptr > 0) {
+        const int32_t ni = st.pop();
         const float4* np = reinterpret_cast<const float4*>(nodes + ni);
         const float4 mnx = np[0], mxx = np[1], mny = np[2], mxy = np[3], mnz = np[4], mxz = np[5];
         const int4 ch = reinterpret_cast<const int4*>(np)[6];
@@ -93,61 +127,66 @@ __global__ __launch_bounds__(kBlock) void k_trace(const rt_node4* __restrict__ n
         const int32_t chv[4] = {ch.x, ch.y, ch.z, ch.w};
         const int32_t cnv[4] = {cn.x, cn.y, cn.z, cn.w};
 
-        Child e[4];
+        float te[4];
+        bool hit[4];
 #pragma unroll
-        for (int p = 0; p < 4; ++p) e[p] = Child{-INFINITY, 0, 0};
-        int c = 0;
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {  // :116-144 slab test + filter, kept in slot order
+        for (int k = 0; k < 4; ++k) {  // :116-144 slab test + filter
             float t0x = (bminx[k] - ox) * ix, t1x = (bmaxx[k] - ox) * ix;
             float t0y = (bminy[k] - oy) * iy, t1y = (bmaxy[k] - oy) * iy;
             float t0z = (bminz[k] - oz) * iz, t1z = (bmaxz[k] - oz) * iz;
             float tminX = fminf(t0x, t1x), tmaxX = fmaxf(t0x, t1x);
             float tminY = fminf(t0y, t1y), tmaxY = fmaxf(t0y, t1y);
             float tminZ = fminf(t0z, t1z), tmaxZ = fmaxf(t0z, t1z);
-            float te = fmaxf(fmaxf(tminX, tminY), fmaxf(tminZ, tmin));
+            te[k] = fmaxf(fmaxf(tminX, tminY), fmaxf(tminZ, tmin));
             float tx = fminf(fminf(tmaxX, tmaxY), fminf(tmaxZ, hitT));
-            bool h = cnv[k] != 0 && tx >= te;
+            hit[k] = cnv[k] != 0 && tx >= te[k];
+        }
+
+        if (ANY) {
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                if (!hit[k]) continue;
+                if (cnv[k] > 0) {
+                    if (COUNT) triTests += (uint32_t)cnv[k];
+                    if (test_leaf<true>(tris, (uint32_t)(~chv[k]), (uint32_t)cnv[k], ox, oy, oz,
+                                        dx, dy, dz, tmin, hitT, hitPrim)) {
+                        done = true;
+                        break;
+                    }
+                } else {
+                    st.push(chv[k]);
+                }
+            }
+            if (done) break;
+            continue;
+        }
+
+        // closest hit: compact hit children in slot order, then :149-156
+        Child e[4];
+#pragma unroll
+        for (int p = 0; p < 4; ++p) e[p] = Child{-INFINITY, 0, 0};
+        int c = 0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
 #pragma unroll
             for (int p = 0; p < 4; ++p)
-                if (h && p == c) e[p] = Child{te, chv[k], cnv[k]};
-            c += h ? 1 : 0;
+                if (hit[k] && p == c) e[p] = Child{te[k], chv[k], cnv[k]};
+            c += hit[k] ? 1 : 0;
         }
         if (c == 0) continue;
-        // :149-156 exchange sort; padding (-inf) never moves
         cx(e[0], e[1]); cx(e[0], e[2]); cx(e[0], e[3]);
         cx(e[1], e[2]); cx(e[1], e[3]);
         cx(e[2], e[3]);
-
 #pragma unroll
-        for (int p = 0; p < 4; ++p) {  // :158-189
-            if (done) break;
+        for (int p = 0; p < 4; ++p) {  // :158-189; padding (count 0) is skipped
             if (e[p].count > 0) {
-                const uint32_t first = (uint32_t)(~e[p].child);
-                const uint32_t cnt = (uint32_t)e[p].count;
-                if (COUNT) triTests += cnt;
-                for (uint32_t q = 0; q < cnt; ++q) {
-                    const float4* tp = reinterpret_cast<const float4*>(tris + first + q);
-                    const float4 a = tp[0], b = tp[1], cc = tp[2];
-                    float t;
-                    if (!mt_hit(a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w, cc.x, ox, oy, oz, dx, dy,
-                                dz, &t))
-                        continue;
-                    if (t >= tmin && t < hitT) {
-                        hitT = t;
-                        hitPrim = tris[first + q].triangle_id;
-                        if (ANY) {
-                            done = true;
-                            break;
-                        }
-                    }
-                }
+                if (COUNT) triTests += (uint32_t)e[p].count;
+                test_leaf<false>(tris, (uint32_t)(~e[p].child), (uint32_t)e[p].count, ox, oy, oz,
+                                 dx, dy, dz, tmin, hitT, hitPrim);
             } else if (e[p].count < 0) {
-                st_at(ptr) = e[p].child;
-                ++ptr;
+                st.push(e[p].child);
             }
         }
-        if (ANY && done) break;
     }
 
     if (ANY) {
@@ -167,17 +206,202 @@ __global__ __launch_bounds__(kBlock) void k_trace(const rt_node4* __restrict__ n
     }
 }
 
-template <int S, bool PRIVATE>
-rt_status launch_s(hipStream_t s, const rt_bvh_dev* b, const rt_rays_soa* rays, uint32_t n,
+// Persistent form ("while-while" with lane refill): each wave owns the
+// contiguous rays [w * per_wave, (w + 1) * per_wave) and, whenever at least
+// kRefill lanes have finished, hands the next rays of its range to the idle
+// lanes (ballot + mbcnt compaction), so a wave stays full while its slowest
+// rays are still traversing.  Any-hit bits are set with atomicOr (zeroed by
+// the launcher first).  Same per-ray arithmetic and order as k_trace.
+constexpr int kRefill = 16;
+
+template <int PRIV, bool ANY, bool COUNT>
+__global__ __launch_bounds__(kBlock) void k_trace_persist(
+    const rt_node4* __restrict__ nodes, const rt_tri64* __restrict__ tris, uint32_t node_count,
+    rt_rays_soa rays, uint32_t n, uint32_t per_wave, uint32_t* __restrict__ prim_out,
+    float* __restrict__ t_out, uint32_t* __restrict__ bits_out, uint32_t* __restrict__ box_out,
+    uint32_t* __restrict__ tri_out) {
+    __shared__ int32_t lds_stack[kLdsStack * kBlock];
+    const uint32_t tid = threadIdx.x, lane = tid & 63u;
+    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64u - lane));
+    const uint32_t wave = (blockIdx.x * kBlock + tid) >> 6;
+    const uint64_t wbeg = (uint64_t)wave * per_wave;
+    uint32_t cur = (uint32_t)min<uint64_t>(wbeg, n);
+    const uint32_t end = (uint32_t)min<uint64_t>(wbeg + per_wave, n);
+
+    bool active = false;
+    uint32_t r = 0;
+    float ox = 0.f, oy = 0.f, oz = 0.f, tmin = 1.f, dx = 0.f, dy = 0.f, dz = 1.f;
+    float ix = 0.f, iy = 0.f, iz = 0.f, hitT = 0.f;
+    uint32_t hitPrim = RT_MISS, boxTests = 0, triTests = 0;
+    Stack<PRIV> st;
+    st.lds = lds_stack + tid;
+
+    for (;;) {
+        const uint64_t idle = __ballot(!active);
+        if (idle == ~0ull && cur >= end) break;
+        if (cur < end && (__popcll(idle) >= kRefill || idle == ~0ull)) {
+            if (!active) {
+                const uint32_t rr = cur + __popcll(idle & lt);
+                if (rr < end) {
+                    r = rr;
+                    ox = rays.ox[r]; oy = rays.oy[r]; oz = rays.oz[r]; tmin = rays.tmin[r];
+                    dx = rays.dx[r]; dy = rays.dy[r]; dz = rays.dz[r]; hitT = rays.tmax[r];
+                    ix = fabsf(dx) > 1e-6f ? 1.0f / dx : 1e6f;
+                    iy = fabsf(dy) > 1e-6f ? 1.0f / dy : 1e6f;
+                    iz = fabsf(dz) > 1e-6f ? 1.0f / dz : 1e6f;
+                    hitPrim = RT_MISS;
+                    boxTests = triTests = 0;
+                    st.ptr = 0;
+                    if (node_count > 0) st.push(0);
+                    active = true;
+                }
+            }
+            cur = (uint32_t)min<uint64_t>((uint64_t)cur + __popcll(idle), end);
+        }
+        if (!active) continue;
+
+        bool finished = st.ptr == 0;
+        if (!finished) {
+            const int32_t ni = st.pop();
+            const float4* np = reinterpret_cast<const float4*>(nodes + ni);
+            const float4 mnx = np[0], mxx = np[1], mny = np[2], mxy = np[3], mnz = np[4],
+                         mxz = np[5];
+            const int4 ch = reinterpret_cast<const int4*>(np)[6];
+            const int4 cn = reinterpret_cast<const int4*>(np)[7];
+            if (COUNT) boxTests += 4;
+            const float bminx[4] = {mnx.x, mnx.y, mnx.z, mnx.w};
+            const float bmaxx[4] = {mxx.x, mxx.y, mxx.z, mxx.w};
+            const float bminy[4] = {mny.x, mny.y, mny.z, mny.w};
+            const float bmaxy[4] = {mxy.x, mxy.y, mxy.z, mxy.w};
+            const float bminz[4] = {mnz.x, mnz.y, mnz.z, mnz.w};
+            const float bmaxz[4] = {mxz.x, mxz.y, mxz.z, mxz.w};
+            const int32_t chv[4] = {ch.x, ch.y, ch.z, ch.w};
+            const int32_t cnv[4] = {cn.x, cn.y, cn.z, cn.w};
+            float te[4];
+            bool hit[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                float t0x = (bminx[k] - ox) * ix, t1x = (bmaxx[k] - ox) * ix;
+                float t0y = (bminy[k] - oy) * iy, t1y = (bmaxy[k] - oy) * iy;
+                float t0z = (bminz[k] - oz) * iz, t1z = (bmaxz[k] - oz) * iz;
+                float tminX = fminf(t0x, t1x), tmaxX = fmaxf(t0x, t1x);
+                float tminY = fminf(t0y, t1y), tmaxY = fmaxf(t0y, t1y);
+                float tminZ = fminf(t0z, t1z), tmaxZ = fmaxf(t0z, t1z);
+                te[k] = fmaxf(fmaxf(tminX, tminY), fmaxf(tminZ, tmin));
+                float tx = fminf(fminf(tmaxX, tmaxY), fminf(tmaxZ, hitT));
+                hit[k] = cnv[k] != 0 && tx >= te[k];
+            }
+            if (ANY) {
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    if (!hit[k] || finished) continue;
+                    if (cnv[k] > 0) {
+                        if (COUNT) triTests += (uint32_t)cnv[k];
+                        finished = test_leaf<true>(tris, (uint32_t)(~chv[k]), (uint32_t)cnv[k],
+                                                   ox, oy, oz, dx, dy, dz, tmin, hitT, hitPrim);
+                    } else {
+                        st.push(chv[k]);
+                    }
+                }
+            } else {
+                Child e[4];
+#pragma unroll
+                for (int p = 0; p < 4; ++p) e[p] = Child{-INFINITY, 0, 0};
+                int c = 0;
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+#pragma unroll
+                    for (int p = 0; p < 4; ++p)
+                        if (hit[k] && p == c) e[p] = Child{te[k], chv[k], cnv[k]};
+                    c += hit[k] ? 1 : 0;
+                }
+                if (c > 0) {
+                    cx(e[0], e[1]); cx(e[0], e[2]); cx(e[0], e[3]);
+                    cx(e[1], e[2]); cx(e[1], e[3]);
+                    cx(e[2], e[3]);
+#pragma unroll
+                    for (int p = 0; p < 4; ++p) {
+                        if (e[p].count > 0) {
+                            if (COUNT) triTests += (uint32_t)e[p].count;
+                            test_leaf<false>(tris, (uint32_t)(~e[p].child), (uint32_t)e[p].count,
+                                             ox, oy, oz, dx, dy, dz, tmin, hitT, hitPrim);
+                        } else if (e[p].count < 0) {
+                            st.push(e[p].child);
+                        }
+                    }
+                }
+            }
+            if (st.ptr == 0) finished = true;
+        }
+        if (finished) {
+            if (ANY) {
+                if (hitPrim != RT_MISS) atomicOr(bits_out + (r >> 5), 1u << (r & 31u));
+                if (prim_out) prim_out[r] = hitPrim;
+            } else {
+                prim_out[r] = hitPrim;
+                t_out[r] = hitT;
+            }
+            if (COUNT) {
+                box_out[r] = boxTests;
+                tri_out[r] = triTests;
+            }
+            active = false;
+        }
+    }
+}
+
+__global__ void k_zero_words(uint32_t* __restrict__ p, uint32_t nwords) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i < nwords) p[i] = 0u;
+}
+
+// Resident blocks of kernel f across the device (occupancy x CUs).
+template <class F>
+uint32_t resident_blocks(F f) {
+    int dev = 0, cus = 0, per_cu = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        cus <= 0)
+        cus = 256;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, kBlock, 0) != hipSuccess ||
+        per_cu <= 0)
+        per_cu = 4;
+    return (uint32_t)(cus * per_cu);
+}
+
+template <int PRIV>
+rt_status launch_v(hipStream_t s, const rt_bvh_dev* b, const rt_rays_soa* rays, uint32_t n,
                    uint32_t flags, rt_hits_soa* hits, uint32_t* bits, uint32_t* box,
                    uint32_t* tri) {
     const bool any = flags & RT_ANY;
     const bool cnt = flags & RT_COUNTERS;
-    dim3 grid(ceil_div(n, kBlock)), block(kBlock);
     uint32_t* prim = hits ? hits->prim : nullptr;
     float* t = hits ? hits->t : nullptr;
+    if (flags & RT_TRACE_PERSISTENT) {
+        if (any) {
+            const uint32_t words = ceil_div(n, 32);
+            hipLaunchKernelGGL(k_zero_words, dim3(ceil_div(words, 256)), dim3(256), 0, s, bits,
+                               words);
+        }
+#define RT_P(A, C)                                                                            \
+    do {                                                                                      \
+        static const uint32_t res = resident_blocks(k_trace_persist<PRIV, A, C>);             \
+        const uint32_t blocks = std::min<uint32_t>(ceil_div(n, kBlock), res);                 \
+        const uint32_t per_wave = ceil_div(ceil_div(n, blocks * (kBlock / 64)), 64) * 64;     \
+        hipLaunchKernelGGL((k_trace_persist<PRIV, A, C>), dim3(blocks), dim3(kBlock), 0, s,   \
+                           b->nodes, b->tris, b->node_count, *rays, n, per_wave, prim, t,     \
+                           bits, box, tri);                                                   \
+    } while (0)
+        if (any && cnt) RT_P(true, true);
+        else if (any) RT_P(true, false);
+        else if (cnt) RT_P(false, true);
+        else RT_P(false, false);
+#undef RT_P
+        return launch_status("k_trace_persist");
+    }
+    dim3 grid(ceil_div(n, kBlock)), block(kBlock);
 #define RT_L(A, C)                                                                            \
-    hipLaunchKernelGGL((k_trace<S, A, C, PRIVATE>), grid, block, 0, s, b->nodes, b->tris,     \
+    hipLaunchKernelGGL((k_trace<PRIV, A, C>), grid, block, 0, s, b->nodes, b->tris,           \
                        b->node_count, *rays, n, prim, t, bits, box, tri)
     if (any && cnt) RT_L(true, true);
     else if (any) RT_L(true, false);
@@ -196,7 +420,7 @@ extern "C" rt_status rt_trace(rt_stream stream, const rt_bvh_dev* bvh, const rt_
                               uint32_t n, uint32_t flags, rt_hits_soa* hits, uint32_t* occl_bits,
                               uint32_t* box_tests, uint32_t* tri_tests) {
     if (!bvh || !rays) return RT_ERR_ARG;
-    if (flags & ~(RT_ANY | RT_COUNTERS)) return RT_ERR_ARG;
+    if (flags & ~(RT_ANY | RT_COUNTERS | RT_TRACE_PERSISTENT)) return RT_ERR_ARG;
     if (n == 0) return RT_OK;
     if (!rays->ox || !rays->oy || !rays->oz || !rays->tmin || !rays->dx || !rays->dy ||
         !rays->dz || !rays->tmax)
@@ -206,13 +430,13 @@ extern "C" rt_status rt_trace(rt_stream stream, const rt_bvh_dev* bvh, const rt_
     if (!any && (!hits || !hits->prim || !hits->t)) return RT_ERR_ARG;
     if ((flags & RT_COUNTERS) && (!box_tests || !tri_tests)) return RT_ERR_ARG;
     if (bvh->node_count > 0 && (!bvh->nodes || !bvh->tris)) return RT_ERR_ARG;
-    const uint64_t need = 3ull * bvh->depth + 1;
+    // live stack <= 3 * depth + 1 entries; depth 0 = unknown (<= kMaxDepth)
+    const uint32_t depth = bvh->depth ? bvh->depth : kMaxDepth;
+    if (depth > kMaxDepth) return RT_ERR_CAPACITY;
+    const uint32_t need = 3u * depth + 1u;
     hipStream_t s = as_stream(stream);
-    if (need <= 16) return launch_s<16, false>(s, bvh, rays, n, flags, hits, occl_bits, box_tests, tri_tests);
-    if (need <= 24) return launch_s<24, false>(s, bvh, rays, n, flags, hits, occl_bits, box_tests, tri_tests);
-    if (need <= 32) return launch_s<32, false>(s, bvh, rays, n, flags, hits, occl_bits, box_tests, tri_tests);
-    if (need <= 48) return launch_s<48, false>(s, bvh, rays, n, flags, hits, occl_bits, box_tests, tri_tests);
-    if (need <= 64) return launch_s<64, false>(s, bvh, rays, n, flags, hits, occl_bits, box_tests, tri_tests);
-    if (need <= 256) return launch_s<256, true>(s, bvh, rays, n, flags, hits, occl_bits, box_tests, tri_tests);
-    return RT_ERR_CAPACITY;
+    if (need <= kLdsStack + 44)
+        return launch_v<44>(s, bvh, rays, n, flags, hits, occl_bits, box_tests, tri_tests);
+    return launch_v<3 * kMaxDepth + 1 - kLdsStack>(s, bvh, rays, n, flags, hits, occl_bits,
+                                                   box_tests, tri_tests);
 }

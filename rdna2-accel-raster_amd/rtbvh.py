@@ -20,7 +20,16 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "librtbvh.so")
 MISS = 0xFFFFFFFF
 
-RT_CLOSEST, RT_ANY, RT_COUNTERS = 0x0, 0x1, 0x100
+RT_CLOSEST, RT_ANY, RT_COUNTERS, RT_TRACE_PERSISTENT = 0x0, 0x1, 0x100, 0x200
+
+# Default traversal schedule (persistent lane-refill waves or one ray per
+# thread); per call override with persistent=True/False.
+DEFAULT_PERSISTENT = True
+
+
+def _sched(persistent):
+    p = DEFAULT_PERSISTENT if persistent is None else persistent
+    return RT_TRACE_PERSISTENT if p else 0
 _STATUS = {0: "RT_OK", 1: "RT_ERR_ARG", 2: "RT_ERR_CAPACITY", 3: "RT_ERR_HIP",
            4: "RT_ERR_NOMEM", 5: "RT_ERR_UNSUPPORTED"}
 
@@ -56,13 +65,20 @@ class rt_hits_soa(C.Structure):
     _fields_ = [("prim", vp), ("t", vp)]
 
 
+class rt_build_cfg(C.Structure):
+    _fields_ = [("algorithm", u32), ("max_leaf_prims", u32), ("reserved", u32 * 2)]
+
+
+BUILD_ALGOS = {"auto": 0, "radix": 1, "implicit": 2}
+
+
 _lib = None
 
 # name -> argtypes (restype is rt_status = int for all but the string getters)
 _SIGS = {
     "rt_tree_layout": [u32, vp, vp, vp, vp, vp],
-    "rt_build_scratch_size": [u32, vp],
-    "rt_gpu_build": [vp, vp, vp, u32, vp, u32, vp, vp, vp, vp, C.c_size_t],
+    "rt_build_requirements": [u32, vp, vp, vp],
+    "rt_gpu_build": [vp, vp, vp, u32, vp, vp, u32, vp, vp, vp, vp, vp, C.c_size_t],
     "rt_gpu_refit": [vp, vp, vp, u32, vp, u32, vp, vp, C.c_size_t],
     "rt_convert_tris48": [vp, vp, vp, u32, vp],
     "rt_trace": [vp, vp, vp, u32, u32, vp, vp, vp, vp],
@@ -143,10 +159,17 @@ def tree_layout(n):
     return list(counts[:L.value]), list(offs[:L.value]), tb.value, sz.value
 
 
-def build_scratch_size(n):
+def build_cfg(algorithm="auto", max_leaf_prims=4):
+    return rt_build_cfg(BUILD_ALGOS[algorithm], max_leaf_prims, (u32 * 2)(0, 0))
+
+
+def build_requirements(n, cfg=None):
+    """(node capacity, scratch bytes) of rt_gpu_build for n triangles."""
+    cap = u32(0)
     s = C.c_size_t(0)
-    call("rt_build_scratch_size", u32(n), C.byref(s))
-    return s.value
+    call("rt_build_requirements", u32(n), C.byref(cfg) if cfg else None, C.byref(cap),
+         C.byref(s))
+    return cap.value, s.value
 
 
 # --------------------------------------------------------------------- rays --
@@ -198,34 +221,44 @@ class GpuBVH4:
 
     # -- build ---------------------------------------------------------------
     @staticmethod
-    def build(pos, idx=None, stream=None, keep_order=False, scratch=None):
-        """Device build (rt_gpu_build): Morton-sorted implicit Box4 tree.
+    def build(pos, idx=None, algorithm="auto", max_leaf_prims=4, stream=None, keep_order=False,
+              scratch=None, sync=True):
+        """Device build (rt_gpu_build) of a Morton-sorted Box4 tree.
 
         pos: float32 (V, 3) device tensor; idx: uint32/int32 (T, 3) or None
-        for 9 floats per triangle (pos then has shape (T, 9) or (3T, 3))."""
+        for 9 floats per triangle (pos then has shape (T, 9) or (3T, 3)).
+        algorithm: "auto" | "radix" | "implicit" (see rt_bvh.h).  With
+        sync=True the node count and depth are read back from the device;
+        otherwise node_count is the capacity and depth 0 (unknown)."""
         dev = pos.device
         n = idx.shape[0] if idx is not None else pos.numel() // 9
-        counts, _, tb, _ = tree_layout(n)
-        nodes = torch.empty(max(tb, 1) * NODE_BYTES, dtype=torch.uint8, device=dev)
+        cfg = build_cfg(algorithm, max_leaf_prims)
+        cap, need = build_requirements(n, cfg)
+        nodes = torch.empty(max(cap, 1) * NODE_BYTES, dtype=torch.uint8, device=dev)
         tris = torch.empty(max(n, 1) * TRI_BYTES, dtype=torch.uint8, device=dev)
-        bvh = GpuBVH4(nodes, tris, tb, n, len(counts) if n else 0)
+        info = torch.zeros(4, dtype=torch.int32, device=dev)
+        bvh = GpuBVH4(nodes, tris, cap, n, 0)
+        bvh.algorithm = algorithm
         ids = codes = None
         if keep_order:
             ids = torch.empty(max(n, 1), dtype=torch.int32, device=dev)
             codes = torch.empty(max(n, 1), dtype=torch.int32, device=dev)
-        need = build_scratch_size(n)
         if scratch is None or scratch.numel() < need:
             scratch = torch.empty(max(need, 256), dtype=torch.uint8, device=dev)
         bvh._scratch = scratch
-        call("rt_gpu_build", stream_handle(stream), ptr(pos), ptr(idx), u32(n), ptr(nodes), u32(tb),
-             ptr(tris), ptr(ids), ptr(codes), ptr(scratch), C.c_size_t(scratch.numel()))
+        call("rt_gpu_build", stream_handle(stream), ptr(pos), ptr(idx), u32(n), C.byref(cfg),
+             ptr(nodes), u32(cap), ptr(tris), ptr(info), ptr(ids), ptr(codes), ptr(scratch),
+             C.c_size_t(scratch.numel()))
+        bvh.info = info
+        if sync:
+            bvh.node_count, bvh.depth = [int(v) for v in info[:2].cpu().tolist()]
         bvh.sorted_ids, bvh.codes = ids, codes
         bvh.pos, bvh.idx = pos, idx
         return bvh
 
     def refit(self, pos, idx=None, stream=None):
-        """Same topology, new vertex positions (rt_gpu_refit)."""
-        need = build_scratch_size(self.tri_count)
+        """Same topology, new vertex positions (rt_gpu_refit, implicit trees)."""
+        need = build_requirements(self.tri_count, build_cfg("implicit"))[1]
         if self._scratch is None or self._scratch.numel() < need:
             self._scratch = torch.empty(max(need, 256), dtype=torch.uint8, device=pos.device)
         call("rt_gpu_refit", stream_handle(stream), ptr(pos), ptr(idx), u32(self.tri_count),
@@ -269,14 +302,14 @@ class GpuBVH4:
         return rt_bvh_dev(ptr(self.nodes), ptr(self.tris), u32(self.node_count),
                           u32(self.tri_count), u32(self.depth), u32(0))
 
-    def intersect(self, rays, counters=False, stream=None, out=None):
+    def intersect(self, rays, counters=False, stream=None, out=None, persistent=None):
         """Closest hit per ray: (prim uint32-as-int32, t float32[, box, tri])."""
         n = len(rays)
         dev = rays.data.device
         prim = out[0] if out else torch.empty(n, dtype=torch.int32, device=dev)
         t = out[1] if out else torch.empty(n, dtype=torch.float32, device=dev)
         box = tri = None
-        flags = RT_CLOSEST
+        flags = RT_CLOSEST | _sched(persistent)
         if counters:
             box = torch.empty(n, dtype=torch.int32, device=dev)
             tri = torch.empty(n, dtype=torch.int32, device=dev)
@@ -288,14 +321,14 @@ class GpuBVH4:
              C.byref(hits), None, ptr(box), ptr(tri))
         return (prim, t, box, tri) if counters else (prim, t)
 
-    def occluded(self, rays, stream=None, out=None, counters=False):
+    def occluded(self, rays, stream=None, out=None, counters=False, persistent=None):
         """Any-hit occlusion bits: int32 tensor of ceil(n/32) words."""
         n = len(rays)
         dev = rays.data.device
         bits = out if out is not None else torch.empty(max(1, (n + 31) // 32), dtype=torch.int32,
                                                        device=dev)
         box = tri = None
-        flags = RT_ANY
+        flags = RT_ANY | _sched(persistent)
         if counters:
             box = torch.empty(n, dtype=torch.int32, device=dev)
             tri = torch.empty(n, dtype=torch.int32, device=dev)

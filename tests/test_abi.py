@@ -1,0 +1,87 @@
+"""The C ABI without a GPU: librtbvh.so loads, exports every entry point
+include/rt_bvh.h declares, answers the host-only queries, and rejects bad
+arguments before touching the device."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "rt_bvh.h")
+
+
+def _declared():
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"^(?:rt_status|const char\*)\s+(rt_\w+)\(", text, re.M)))
+
+
+def test_header_declares_expected_surface():
+    names = _declared()
+    for must in ("rt_gpu_build", "rt_trace", "rt_tree_layout", "rt_build_requirements",
+                 "rt_gpu_refit", "rt_convert_tris48"):
+        assert must in names
+
+
+def test_library_exports_every_declared_symbol():
+    import rtbvh
+    lib = rtbvh.lib()
+    missing = [n for n in _declared() if not hasattr(lib, n)]
+    assert not missing, missing
+    assert set(rtbvh.exported_symbols()) >= set(_declared()) - {"rt_version", "rt_last_error"}
+    assert lib.rt_version().startswith(b"rtbvh")
+
+
+def test_host_tree_layout_matches_oracle(O):
+    import rtbvh
+    for n in (1, 5, 4096, 4097, 2_500_000):
+        c, o, tb, sz = rtbvh.tree_layout(n)
+        c2, o2, tb2, sz2 = O.tree_layout(n)
+        assert c == list(c2) and o == list(o2) and tb == tb2 and sz == sz2
+
+
+def test_build_requirements():
+    import rtbvh
+    cap, scratch = rtbvh.build_requirements(4096, rtbvh.build_cfg("implicit"))
+    assert cap == 1365 and scratch == 0  # fused single-workgroup path
+    cap, scratch = rtbvh.build_requirements(2_500_000, rtbvh.build_cfg("radix"))
+    assert cap == 2_499_999 and scratch > 2_500_000 * 60
+    cap, _ = rtbvh.build_requirements(10, rtbvh.build_cfg("auto"))
+    assert cap == rtbvh.tree_layout(10)[2]
+    with pytest.raises(rtbvh.RtError):
+        rtbvh.build_requirements(10, rtbvh.rt_build_cfg(7, 4, (C.c_uint32 * 2)(0, 0)))
+
+
+def test_argument_errors_before_device_work():
+    import rtbvh
+    lib = rtbvh.lib()
+    # null bvh / rays / info pointers and unknown flags are RT_ERR_ARG
+    assert lib.rt_trace(None, None, None, 10, 0, None, None, None, None) == 1
+    b = rtbvh.rt_bvh_dev()
+    r = rtbvh.rt_rays_soa()
+    assert lib.rt_trace(None, C.byref(b), C.byref(r), 10, 0, None, None, None, None) == 1
+    assert lib.rt_trace(None, C.byref(b), C.byref(r), 10, 0x8000, None, None, None, None) == 1
+    assert lib.rt_trace(None, C.byref(b), C.byref(r), 0, 0, None, None, None, None) == 0
+    b.depth = 65  # deeper than any tree the kernels accept
+    fake = C.c_void_p(16)
+    r2 = rtbvh.rt_rays_soa(*([fake] * 8))
+    h = rtbvh.rt_hits_soa(fake, fake)
+    assert lib.rt_trace(None, C.byref(b), C.byref(r2), 4, 0, C.byref(h), None, None, None) == 2
+    cfg = rtbvh.build_cfg("radix")
+    assert lib.rt_gpu_build(None, None, None, 10, C.byref(cfg), None, 0, None, None, None, None,
+                            None, 0) == 1
+    assert lib.rt_gen_sphere(None, 2, 2, 1.0, 0.0, 0, None, None) == 1
+    assert lib.rt_pack_light_masks(None, fake, 10, 17, fake) == 1
+
+
+def test_struct_layouts():
+    import rtbvh
+    assert C.sizeof(rtbvh.rt_bvh_dev) == 32
+    assert C.sizeof(rtbvh.rt_rays_soa) == 64
+    assert C.sizeof(rtbvh.rt_build_cfg) == 16
+    import oracle
+    assert oracle.NODE4.itemsize == rtbvh.NODE_BYTES and oracle.TRI64.itemsize == rtbvh.TRI_BYTES
+    # rt_tri64 = RadvBvhTriangleNode: triangle_id at byte 48 (radv_bvh.hpp:146)
+    assert oracle.TRI64.fields["triangle_id"][1] == 48
+    assert oracle.NODE4.fields["child"][1] == 96 and oracle.NODE4.fields["count"][1] == 112

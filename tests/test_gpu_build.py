@@ -1,6 +1,7 @@
-"""GPU build parity: rt_gpu_build (Morton-30 + LDS radix sort + implicit
-Box4 emit + bottom-up boxes) against the oracle's LBVH restatement.  Sorted
-order and Morton codes bit-exact; node and leaf records byte-identical."""
+"""GPU build parity: rt_gpu_build (Morton-30 + LDS radix sort, then the
+implicit fast_bvh_builder layout or the radix tree collapsed with
+SoftwareBVH4's rule) against the oracle's restatements.  Sorted order and
+Morton codes bit-exact; node and leaf records byte-identical."""
 import numpy as np
 import pytest
 import torch
@@ -10,13 +11,18 @@ from conftest import u32, to_np
 pytestmark = pytest.mark.gpu
 
 
-def _cmp_build(rt, O, pos, idx):
+def _cmp_build(rt, O, pos, idx, algorithm="implicit", maxp=4):
     n = idx.shape[0] if idx is not None else pos.size // 9
-    nodes_c, tris_c, ids_c, codes_c = O.lbvh_build(pos, idx)
+    if algorithm == "implicit":
+        nodes_c, tris_c, ids_c, codes_c = O.lbvh_build(pos, idx)
+    else:
+        nodes_c, tris_c, ids_c, codes_c = O.rlbvh_build(pos, idx, max_prims=maxp)
     dpos = torch.from_numpy(np.ascontiguousarray(pos, np.float32)).cuda()
     didx = None if idx is None else torch.from_numpy(
         np.ascontiguousarray(idx, np.uint32).view(np.int32)).cuda()
-    bvh = rt.GpuBVH4.build(dpos, didx, keep_order=True)
+    bvh = rt.GpuBVH4.build(dpos, didx, algorithm=algorithm, max_leaf_prims=maxp, keep_order=True)
+    if algorithm == "radix":
+        assert bvh.depth == O.bvh4_depth(nodes_c)
     torch.cuda.synchronize()
     np.testing.assert_array_equal(u32(bvh.codes)[:n], codes_c)
     np.testing.assert_array_equal(u32(bvh.sorted_ids)[:n], ids_c)
@@ -55,15 +61,34 @@ def test_large_path_terrain_ties(rt, O):
     _cmp_build(rt, O, pos, idx)
 
 
+@pytest.mark.parametrize("n", [1, 2, 4, 5, 6, 100, 4096, 4097, 60_000, 250_000])
+@pytest.mark.parametrize("maxp", [1, 4])
+def test_radix_path_indexed(rt, O, n, maxp):
+    nu = 50
+    nv = max(2, n // (2 * nu) + 1)
+    pos, idx = O.sphere(nu, nv, 1.0, 0.02, 9)
+    _cmp_build(rt, O, pos, idx[:n], "radix", maxp)
+
+
+def test_radix_path_duplicate_codes(rt, O):
+    """Terrain grid: long runs of equal Morton codes exercise the position
+    tie-break of the radix tree."""
+    pos, idx = O.terrain(100_000)
+    _cmp_build(rt, O, pos, idx, "radix", 4)
+    xyz, _ = O.dynamic_grid(20_000, 0.0, 2.0)
+    xyz = np.concatenate([xyz, xyz])  # every triangle twice: equal codes
+    _cmp_build(rt, O, xyz, None, "radix", 2)
+
+
 def test_build_then_trace_matches_brute_force(rt, O):
     pos, idx = O.sphere(64, 48, 1.0, 0.02, 3)
     dpos = torch.from_numpy(pos).cuda()
     didx = torch.from_numpy(idx.view(np.int32)).cuda()
-    bvh = rt.GpuBVH4.build(dpos, didx)
+    bvh = rt.GpuBVH4.build(dpos, didx, algorithm="radix")
     r = O.random_rays(20_000, 17, (-1.5, -1.5, -1.5), (1.5, 1.5, 1.5), tmax=1e30)
     prim, t = bvh.intersect(rt.Rays.from_numpy(r.packed()))
     torch.cuda.synchronize()
-    nodes_c, tris_c, _, _ = O.lbvh_build(pos, idx)
+    nodes_c, tris_c, _, _ = O.rlbvh_build(pos, idx)
     pb, tb = O.trace_brute(tris_c, r)
     prim, t = u32(prim), to_np(t)
     same_t = t.view(np.uint32) == tb.view(np.uint32)
@@ -77,7 +102,7 @@ def test_refit_matches_rebuild_layout(rt, O):
     xyz0, _ = O.dynamic_grid(n, 0.0, 2.0)
     xyz1, _ = O.dynamic_grid(n, 0.7, 2.0)
     d0 = torch.from_numpy(xyz0).cuda()
-    bvh = rt.GpuBVH4.build(d0, None, keep_order=True)
+    bvh = rt.GpuBVH4.build(d0, None, algorithm="implicit", keep_order=True)
     nodes_c, tris_c, _, _ = O.lbvh_build(xyz0, None)
     nodes_r, tris_r = O.lbvh_refit(xyz1, None, nodes_c, tris_c)
     d1 = torch.from_numpy(xyz1).cuda()
@@ -92,7 +117,7 @@ def test_small_refit(rt, O):
     n = 4096
     xyz0, _ = O.dynamic_grid(n, 0.0, 1.25)
     xyz1, _ = O.dynamic_grid(n, 1.1, 1.25)
-    bvh = rt.GpuBVH4.build(torch.from_numpy(xyz0).cuda(), None)
+    bvh = rt.GpuBVH4.build(torch.from_numpy(xyz0).cuda(), None, algorithm="implicit")
     nodes_c, tris_c, _, _ = O.lbvh_build(xyz0, None)
     nodes_r, tris_r = O.lbvh_refit(xyz1, None, nodes_c, tris_c)
     bvh.refit(torch.from_numpy(xyz1).cuda(), None)

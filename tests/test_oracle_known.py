@@ -100,15 +100,17 @@ def test_fast_builder_verification_rays(O, n):
     assert p[n] == O.MISS
 
 
-@pytest.mark.parametrize("builder", ["software", "lbvh"])
+@pytest.mark.parametrize("builder", ["software", "lbvh", "rlbvh"])
 def test_traversal_equals_brute_force(O, builder):
     """Traversal result is independent of the tree: the closest t equals the
     brute-force minimum over every triangle, any-hit equals t < tmax."""
     pos, idx = O.sphere(40, 30, 1.0, 0.02, 5)
     if builder == "software":
         n4, tris = O.software_bvh4(pos, idx)
-    else:
+    elif builder == "lbvh":
         n4, tris, _, _ = O.lbvh_build(pos, idx)
+    else:
+        n4, tris, _, _ = O.rlbvh_build(pos, idx)
     r = O.random_rays(20000, 3, (-1.5, -1.5, -1.5), (1.5, 1.5, 1.5), tmax=1e30)
     r.tmax[::2] = 0.3
     p, t = O.trace_bvh4(n4, tris, r)
@@ -225,6 +227,57 @@ def test_lbvh_invariants(O, n):
     m = root["count"] != 0
     v = pos[idx.reshape(-1)]
     assert root["minX"][m].min() == v[:, 0].min() and root["maxY"][m].max() == v[:, 1].max()
+
+
+def _radix_split_brute(codes, f, l):
+    """Split of sorted range [f, l] by definition: the highest bit (of the
+    64-bit key code << 32 | position) that differs between f and l; the left
+    part is every position whose bit is 0."""
+    key = lambda i: (int(codes[i]) << 32) | i
+    b = (key(f) ^ key(l)).bit_length() - 1
+    s = f
+    while s + 1 <= l and not (key(s + 1) >> b) & 1:
+        s += 1
+    return s
+
+
+@pytest.mark.parametrize("maxp", [1, 2, 4])
+def test_rlbvh_structure(O, maxp):
+    """Radix-tree LBVH: every BVH2 split is the highest-differing-bit split of
+    its range (brute force on the 64-bit key, position as tie-break), leaves
+    hold <= maxp triangles, inner boxes are unions, and the Box4 nodes are
+    SoftwareBVH4::collapse of that BVH2 (node count = inner BVH2 nodes
+    reached)."""
+    pos, idx = O.terrain(3000)
+    idx = np.concatenate([idx, idx[:500]])  # duplicate triangles -> equal codes
+    n = len(idx)
+    n4, tris, ids, codes, n2 = O.rlbvh_build(pos, idx, max_prims=maxp, with_bvh2=True)
+    assert sorted(ids.tolist()) == list(range(n))
+    np.testing.assert_array_equal(O.morton30(pos, idx)[ids], codes)
+    np.testing.assert_array_equal(n4.tobytes(), O.bvh4_collapse(n2).tobytes())
+
+    def rng(u):
+        nd = n2[u]
+        if nd["leftChild"] < 0:
+            f = ~int(nd["leftChild"])
+            return f, f + int(nd["rightChild"]) - 1
+        return rng(int(nd["leftChild"]))[0], rng(int(nd["rightChild"]))[1]
+
+    stack = [0]
+    while stack:
+        u = stack.pop()
+        nd = n2[u]
+        f, l = rng(u)
+        if nd["leftChild"] < 0:
+            assert int(nd["rightChild"]) <= maxp
+            continue
+        assert l - f + 1 > maxp
+        a, b = int(nd["leftChild"]), int(nd["rightChild"])
+        assert rng(a) == (f, _radix_split_brute(codes, f, l)) and rng(b)[1] == l
+        np.testing.assert_array_equal(nd["aabbMin"], np.minimum(n2[a]["aabbMin"], n2[b]["aabbMin"]))
+        np.testing.assert_array_equal(nd["aabbMax"], np.maximum(n2[a]["aabbMax"], n2[b]["aabbMax"]))
+        stack += [a, b]
+    assert len(n4) <= n - 1
 
 
 def test_lbvh_boxes_have_no_negative_zero(O):
