@@ -118,23 +118,33 @@ def algorithmic_bytes_shadow(frame):
 
 def cpu_baseline(frame, threads):
     """The CPU oracle's BVH4 traversal on the host cores over the same tree and
-    a bounded sample of the same rays: all primary rays (closest-hit) + the
-    shadow rays of the first 2 lights (any-hit)."""
+    the same rays: one whole frame (all primary rays closest-hit, all shadow
+    rays any-hit), parallel over rays in 4,096-ray chunks."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     nodes = frame.bvh.nodes.cpu().numpy()[: frame.bvh.node_count * 128].view(O.NODE4)
     tris = frame.bvh.tris.cpu().numpy()[: frame.bvh.tri_count * 64].view(O.TRI64)
     prim_rays = O.Rays.stack(frame.primary.data.cpu().numpy())
-    nsh = frame.npix * 2
-    sh_rays = O.Rays.stack(frame.shadow.data[:, :nsh].cpu().numpy())
+    sh_rays = O.Rays.stack(frame.shadow.data.cpu().numpy())
     t0 = time.perf_counter()
     O.trace_bvh4(nodes, tris, prim_rays, mode=O.CLOSEST, threads=threads)
     O.trace_bvh4(nodes, tris, sh_rays, mode=O.ANY, threads=threads)
     dt = time.perf_counter() - t0
     n = len(prim_rays) + len(sh_rays)
     return {"value": round(n / dt / 1e6, 3), "unit": "Mrays/s", "cores": threads, "kind": "port",
-            "sample": f"{len(prim_rays)} primary closest-hit + {len(sh_rays)} shadow any-hit rays "
-                      f"(2 of 16 lights) of the same frame and tree; {dt:.1f} s"}
+            "sample": f"one whole frame of the same workload and tree: {len(prim_rays)} primary "
+                      f"closest-hit + {len(sh_rays)} shadow any-hit rays; {dt:.1f} s wall on "
+                      f"{threads} threads"}
+
+
+def copy_probe(nbytes=1 << 30, reps=3):
+    """A plain device copy of nbytes (read nbytes, write nbytes), the
+    calibration kernel for the FETCH_SIZE / WRITE_SIZE counters."""
+    x = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda").fill_(1.0)
+    y = torch.empty_like(x)
+    for _ in range(reps):
+        torch.mul(x, 1.0, out=y)
+    torch.cuda.synchronize()
 
 
 def main():
@@ -143,9 +153,11 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--schedule", choices=["persistent", "simple"], default="persistent",
+    ap.add_argument("--schedule", choices=["persistent", "simple"], default="simple",
                     help="traversal schedule (persistent lane-refill waves or one ray/thread)")
     ap.add_argument("--builder", choices=["radix", "implicit"], default="radix")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="first run a plain 1 GiB device copy (counter calibration)")
     a = ap.parse_args()
     rt.DEFAULT_PERSISTENT = a.schedule == "persistent"
     Frame.builder = a.builder
@@ -156,6 +168,8 @@ def main():
         sys.exit(2)
     torch.cuda.set_device(0)
     rt.lib()
+    if a.copy_probe:
+        copy_probe()
 
     b4k_p50, b4k_p99 = build_time_4k()
     f = Frame()

@@ -39,21 +39,35 @@ __device__ __forceinline__ void cx(Child& a, Child& b) {
     }
 }
 
+// The stack pointer is a separate local (a register); only the overflow
+// array lives in private memory (keeping both in one aggregate would put the
+// pointer in scratch too).
+// The LDS column is addressed through the __shared__ array itself (address
+// space 3), never through a generic pointer: a select between an LDS and a
+// private address would compile to a slower flat load.
 template <int PRIV>
 struct Stack {
-    int32_t* lds;  // this lane's column: lds[i * kBlock]
     int32_t priv[PRIV];
-    int ptr = 0;
-    __device__ __forceinline__ void push(int32_t v) {
-        if (ptr < kLdsStack) lds[ptr * kBlock] = v;
-        else priv[ptr - kLdsStack] = v;
-        ++ptr;
-    }
-    __device__ __forceinline__ int32_t pop() {
-        --ptr;
-        return ptr < kLdsStack ? lds[ptr * kBlock] : priv[ptr - kLdsStack];
-    }
 };
+
+#define RT_SPUSH(v)                                                      \
+    do {                                                                 \
+        if (sp < kLdsStack) lds_stack[sp * kBlock + tid] = (v);          \
+        else st.priv[sp - kLdsStack] = (v);                              \
+        ++sp;                                                            \
+    } while (0)
+
+// Pop: the LDS read uses a clamped, always-valid slot (a ds_read); private
+// memory is read only when the entry lives there.  The private read is
+// volatile so it cannot be merged with the LDS read into one flat load of a
+// selected address (which would put a flat-load latency on every pop).
+#define RT_SPOP(dst)                                                     \
+    do {                                                                 \
+        --sp;                                                            \
+        dst = lds_stack[min(sp, kLdsStack - 1) * kBlock + tid];          \
+        if (sp >= kLdsStack)                                             \
+            dst = reinterpret_cast<volatile int32_t*>(st.priv)[sp - kLdsStack]; \
+    } while (0)
 
 // Leaf range [first, first + cnt): Moller-Trumbore on each 64-B record.
 template <bool ANY>
@@ -107,11 +121,12 @@ __global__ __launch_bounds__(kBlock) void k_trace(const rt_node4* __restrict__ n
     bool done = false;
 
     Stack<PRIV> st;
-    st.lds = lds_stack + tid;
-    if (valid && node_count > 0) st.push(0);
+    int sp = 0;
+    if (valid && node_count > 0) RT_SPUSH(0);
 
-    while (st.ptr > 0) {
-        const int32_t ni = st.pop();
+    while (sp > 0) {
+        int32_t ni;
+        RT_SPOP(ni);
         const float4* np = reinterpret_cast<const float4*>(nodes + ni);
         const float4 mnx = np[0], mxx = np[1], mny = np[2], mxy = np[3], mnz = np[4], mxz = np[5];
         const int4 ch = reinterpret_cast<const int4*>(np)[6];
@@ -154,7 +169,7 @@ __global__ __launch_bounds__(kBlock) void k_trace(const rt_node4* __restrict__ n
                         break;
                     }
                 } else {
-                    st.push(chv[k]);
+                    RT_SPUSH(chv[k]);
                 }
             }
             if (done) break;
@@ -184,7 +199,7 @@ __global__ __launch_bounds__(kBlock) void k_trace(const rt_node4* __restrict__ n
                 test_leaf<false>(tris, (uint32_t)(~e[p].child), (uint32_t)e[p].count, ox, oy, oz,
                                  dx, dy, dz, tmin, hitT, hitPrim);
             } else if (e[p].count < 0) {
-                st.push(e[p].child);
+                RT_SPUSH(e[p].child);
             }
         }
     }
@@ -234,7 +249,7 @@ __global__ __launch_bounds__(kBlock) void k_trace_persist(
     float ix = 0.f, iy = 0.f, iz = 0.f, hitT = 0.f;
     uint32_t hitPrim = RT_MISS, boxTests = 0, triTests = 0;
     Stack<PRIV> st;
-    st.lds = lds_stack + tid;
+    int sp = 0;
 
     for (;;) {
         const uint64_t idle = __ballot(!active);
@@ -251,8 +266,8 @@ __global__ __launch_bounds__(kBlock) void k_trace_persist(
                     iz = fabsf(dz) > 1e-6f ? 1.0f / dz : 1e6f;
                     hitPrim = RT_MISS;
                     boxTests = triTests = 0;
-                    st.ptr = 0;
-                    if (node_count > 0) st.push(0);
+                    sp = 0;
+                    if (node_count > 0) RT_SPUSH(0);
                     active = true;
                 }
             }
@@ -260,9 +275,10 @@ __global__ __launch_bounds__(kBlock) void k_trace_persist(
         }
         if (!active) continue;
 
-        bool finished = st.ptr == 0;
+        bool finished = sp == 0;
         if (!finished) {
-            const int32_t ni = st.pop();
+            int32_t ni;
+        RT_SPOP(ni);
             const float4* np = reinterpret_cast<const float4*>(nodes + ni);
             const float4 mnx = np[0], mxx = np[1], mny = np[2], mxy = np[3], mnz = np[4],
                          mxz = np[5];
@@ -300,7 +316,7 @@ __global__ __launch_bounds__(kBlock) void k_trace_persist(
                         finished = test_leaf<true>(tris, (uint32_t)(~chv[k]), (uint32_t)cnv[k],
                                                    ox, oy, oz, dx, dy, dz, tmin, hitT, hitPrim);
                     } else {
-                        st.push(chv[k]);
+                        RT_SPUSH(chv[k]);
                     }
                 }
             } else {
@@ -326,12 +342,12 @@ __global__ __launch_bounds__(kBlock) void k_trace_persist(
                             test_leaf<false>(tris, (uint32_t)(~e[p].child), (uint32_t)e[p].count,
                                              ox, oy, oz, dx, dy, dz, tmin, hitT, hitPrim);
                         } else if (e[p].count < 0) {
-                            st.push(e[p].child);
+                            RT_SPUSH(e[p].child);
                         }
                     }
                 }
             }
-            if (st.ptr == 0) finished = true;
+            if (sp == 0) finished = true;
         }
         if (finished) {
             if (ANY) {

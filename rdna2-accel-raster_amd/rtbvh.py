@@ -23,8 +23,9 @@ MISS = 0xFFFFFFFF
 RT_CLOSEST, RT_ANY, RT_COUNTERS, RT_TRACE_PERSISTENT = 0x0, 0x1, 0x100, 0x200
 
 # Default traversal schedule (persistent lane-refill waves or one ray per
-# thread); per call override with persistent=True/False.
-DEFAULT_PERSISTENT = True
+# thread); per call override with persistent=True/False.  One ray per thread
+# measured faster on the 2.5 M-triangle shadow frame (25.4 vs 42.4 ms).
+DEFAULT_PERSISTENT = False
 
 
 def _sched(persistent):
