@@ -153,13 +153,16 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--schedule", choices=["persistent", "simple"], default="simple",
-                    help="traversal schedule (persistent lane-refill waves or one ray/thread)")
+    ap.add_argument("--schedule", choices=["refill", "simple"], default="simple",
+                    help="traversal schedule (waves refilling finished lanes, or one ray/thread)")
+    ap.add_argument("--rays-per-wave", type=int, default=1024)
+    ap.add_argument("--refill-lanes", type=int, default=16)
     ap.add_argument("--builder", choices=["radix", "implicit"], default="radix")
     ap.add_argument("--copy-probe", action="store_true",
                     help="first run a plain 1 GiB device copy (counter calibration)")
     a = ap.parse_args()
-    rt.DEFAULT_PERSISTENT = a.schedule == "persistent"
+    rt.DEFAULT_SCHEDULE = a.schedule
+    rt.TRACE_TUNING.update(rays_per_wave=a.rays_per_wave, refill_lanes=a.refill_lanes)
     Frame.builder = a.builder
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); the per-GPU shard "
@@ -227,7 +230,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic, "kernel": "k_trace_persist any-hit (shadow)"
-                     if a.schedule == "persistent" else "k_trace any-hit (shadow)",
+                     if a.schedule == "refill" else "k_trace any-hit (shadow)",
                      "kernel_ms": round(kern_avg, 4), "algorithmic_bytes": int(ab)},
         "build_us_4k_p50": round(b4k_p50, 2),
         "build_us_4k_p99": round(b4k_p99, 2),

@@ -178,6 +178,24 @@ rt_status rt_trace(rt_stream stream, const rt_bvh_dev* bvh, const rt_rays_soa* r
                    uint32_t flags, rt_hits_soa* hits, uint32_t* occl_bits, uint32_t* box_tests,
                    uint32_t* tri_tests);
 
+/* rt_trace with a schedule choice (same results for every choice):
+ *   RT_SCHED_SIMPLE  one ray per thread, the hardware dispatcher balances.
+ *   RT_SCHED_REFILL  each wave walks a chunk of rays_per_wave consecutive rays
+ *                    and hands new rays to its finished lanes whenever at
+ *                    least refill_lanes of 64 are idle.
+ * opts = NULL: RT_SCHED_SIMPLE (RT_TRACE_PERSISTENT in flags: RT_SCHED_REFILL). */
+#define RT_SCHED_SIMPLE 0u
+#define RT_SCHED_REFILL 1u
+typedef struct rt_trace_opts {
+    uint32_t schedule;
+    uint32_t rays_per_wave; /* multiple of 64, at most 4096; 0 = 1024 */
+    uint32_t refill_lanes;  /* 1..64; 0 = 16 */
+    uint32_t reserved;
+} rt_trace_opts;
+rt_status rt_trace_ex(rt_stream stream, const rt_bvh_dev* bvh, const rt_rays_soa* rays,
+                      uint32_t n, uint32_t flags, const rt_trace_opts* opts, rt_hits_soa* hits,
+                      uint32_t* occl_bits, uint32_t* box_tests, uint32_t* tri_tests);
+
 /* ---- generators (synthetic inputs made on the device) -------------------- */
 rt_status rt_gen_sphere(rt_stream stream, uint32_t nu, uint32_t nv, float radius, float disp,
                         uint32_t seed, float* d_pos, uint32_t* d_idx);

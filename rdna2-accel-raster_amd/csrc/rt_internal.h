@@ -6,6 +6,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cfloat>
 #include <cstdint>
 #include <string>
 

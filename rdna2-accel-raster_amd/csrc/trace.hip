@@ -7,9 +7,8 @@
 // pushed first), leaf-first processing and Moller-Trumbore predicate, so on
 // the same tree and rays the hit ID, t and box/triangle counters equal the
 // CPU path bit for bit.  Any hit (the role of the TerminateOnFirstHit ray
-// queries in rq_ao.comp / tiled_lighting_ra.comp) visits children in slot
-// order and stops at the first accepted triangle; its occlusion bit does not
-// depend on the order.
+// queries in rq_ao.comp / tiled_lighting_ra.comp) walks the same near-first
+// order and stops at the first accepted triangle.
 //
 // Per-lane stack: the top kLdsStack entries live in LDS (entry-major, so
 // lanes at the same depth hit distinct banks; 5 KB per wave keeps 8 waves per
@@ -157,38 +156,16 @@ __global__ __launch_bounds__(kBlock) void k_trace(const rt_node4* __restrict__ n
             hit[k] = cnv[k] != 0 && tx >= te[k];
         }
 
-        if (ANY) {
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                if (!hit[k]) continue;
-                if (cnv[k] > 0) {
-                    if (COUNT) triTests += (uint32_t)cnv[k];
-                    if (test_leaf<true>(tris, (uint32_t)(~chv[k]), (uint32_t)cnv[k], ox, oy, oz,
-                                        dx, dy, dz, tmin, hitT, hitPrim)) {
-                        done = true;
-                        break;
-                    }
-                } else {
-                    RT_SPUSH(chv[k]);
-                }
-            }
-            if (done) break;
-            continue;
-        }
-
-        // closest hit: compact hit children in slot order, then :149-156
+        // :149-156 exchange sort of the hit children, run in place over the 4
+        // slots with missed children keyed -inf (keys of hit children are at
+        // least -FLT_MAX): for every pattern of hits and ties this yields the
+        // reference's order of its compacted list, without compacting.
         Child e[4];
 #pragma unroll
-        for (int p = 0; p < 4; ++p) e[p] = Child{-INFINITY, 0, 0};
-        int c = 0;
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-#pragma unroll
-            for (int p = 0; p < 4; ++p)
-                if (hit[k] && p == c) e[p] = Child{te[k], chv[k], cnv[k]};
-            c += hit[k] ? 1 : 0;
-        }
-        if (c == 0) continue;
+        for (int k = 0; k < 4; ++k)
+            e[k] = hit[k] ? Child{fmaxf(te[k], -FLT_MAX), chv[k], cnv[k]}
+                          : Child{-INFINITY, 0, 0};
+        if (!(hit[0] || hit[1] || hit[2] || hit[3])) continue;
         cx(e[0], e[1]); cx(e[0], e[2]); cx(e[0], e[3]);
         cx(e[1], e[2]); cx(e[1], e[3]);
         cx(e[2], e[3]);
@@ -196,12 +173,16 @@ __global__ __launch_bounds__(kBlock) void k_trace(const rt_node4* __restrict__ n
         for (int p = 0; p < 4; ++p) {  // :158-189; padding (count 0) is skipped
             if (e[p].count > 0) {
                 if (COUNT) triTests += (uint32_t)e[p].count;
-                test_leaf<false>(tris, (uint32_t)(~e[p].child), (uint32_t)e[p].count, ox, oy, oz,
-                                 dx, dy, dz, tmin, hitT, hitPrim);
+                if (test_leaf<ANY>(tris, (uint32_t)(~e[p].child), (uint32_t)e[p].count, ox, oy,
+                                   oz, dx, dy, dz, tmin, hitT, hitPrim)) {
+                    done = true;  // any hit: first accepted triangle ends the ray
+                    break;
+                }
             } else if (e[p].count < 0) {
                 RT_SPUSH(e[p].child);
             }
         }
+        if (ANY && done) break;
     }
 
     if (ANY) {
@@ -221,27 +202,34 @@ __global__ __launch_bounds__(kBlock) void k_trace(const rt_node4* __restrict__ n
     }
 }
 
-// Persistent form ("while-while" with lane refill): each wave owns the
+// Refill form ("while-while" with lane refill): each wave owns the
 // contiguous rays [w * per_wave, (w + 1) * per_wave) and, whenever at least
-// kRefill lanes have finished, hands the next rays of its range to the idle
+// `refill` lanes have finished, hands the next rays of its range to the idle
 // lanes (ballot + mbcnt compaction), so a wave stays full while its slowest
-// rays are still traversing.  Any-hit bits are set with atomicOr (zeroed by
-// the launcher first).  Same per-ray arithmetic and order as k_trace.
-constexpr int kRefill = 16;
+// rays are still traversing; the hardware dispatcher balances the chunks
+// across CUs.  Any-hit bits collect in the wave's LDS words (LDS atomicOr) and
+// are written once at the end.  Same per-ray arithmetic and order as k_trace.
+constexpr uint32_t kMaxPerWave = 4096;  // rays per wave: 128 occlusion words in LDS
 
 template <int PRIV, bool ANY, bool COUNT>
 __global__ __launch_bounds__(kBlock) void k_trace_persist(
     const rt_node4* __restrict__ nodes, const rt_tri64* __restrict__ tris, uint32_t node_count,
-    rt_rays_soa rays, uint32_t n, uint32_t per_wave, uint32_t* __restrict__ prim_out,
+    rt_rays_soa rays, uint32_t n, uint32_t per_wave, uint32_t refill, uint32_t* __restrict__ prim_out,
     float* __restrict__ t_out, uint32_t* __restrict__ bits_out, uint32_t* __restrict__ box_out,
     uint32_t* __restrict__ tri_out) {
     __shared__ int32_t lds_stack[kLdsStack * kBlock];
+    // any hit: each wave's occlusion words (its range is whole 32-ray words)
+    __shared__ uint32_t lds_bits[ANY ? (kBlock / 64) * (kMaxPerWave / 32) : 1];
     const uint32_t tid = threadIdx.x, lane = tid & 63u;
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64u - lane));
     const uint32_t wave = (blockIdx.x * kBlock + tid) >> 6;
     const uint64_t wbeg = (uint64_t)wave * per_wave;
     uint32_t cur = (uint32_t)min<uint64_t>(wbeg, n);
     const uint32_t end = (uint32_t)min<uint64_t>(wbeg + per_wave, n);
+    const uint32_t wbeg32 = cur;  // a multiple of 64 (per_wave is)
+    uint32_t* wbits = lds_bits + (ANY ? (tid >> 6) * (kMaxPerWave / 32) : 0);
+    if (ANY)
+        for (uint32_t i = lane; i < kMaxPerWave / 32; i += 64) wbits[i] = 0u;
 
     bool active = false;
     uint32_t r = 0;
@@ -254,7 +242,7 @@ __global__ __launch_bounds__(kBlock) void k_trace_persist(
     for (;;) {
         const uint64_t idle = __ballot(!active);
         if (idle == ~0ull && cur >= end) break;
-        if (cur < end && (__popcll(idle) >= kRefill || idle == ~0ull)) {
+        if (cur < end && ((uint32_t)__popcll(idle) >= refill || idle == ~0ull)) {
             if (!active) {
                 const uint32_t rr = cur + __popcll(idle & lt);
                 if (rr < end) {
@@ -307,43 +295,26 @@ __global__ __launch_bounds__(kBlock) void k_trace_persist(
                 float tx = fminf(fminf(tmaxX, tmaxY), fminf(tmaxZ, hitT));
                 hit[k] = cnv[k] != 0 && tx >= te[k];
             }
-            if (ANY) {
+            Child e[4];  // in-place exchange sort, as in k_trace
 #pragma unroll
-                for (int k = 0; k < 4; ++k) {
-                    if (!hit[k] || finished) continue;
-                    if (cnv[k] > 0) {
-                        if (COUNT) triTests += (uint32_t)cnv[k];
-                        finished = test_leaf<true>(tris, (uint32_t)(~chv[k]), (uint32_t)cnv[k],
-                                                   ox, oy, oz, dx, dy, dz, tmin, hitT, hitPrim);
-                    } else {
-                        RT_SPUSH(chv[k]);
-                    }
-                }
-            } else {
-                Child e[4];
+            for (int k = 0; k < 4; ++k)
+                e[k] = hit[k] ? Child{fmaxf(te[k], -FLT_MAX), chv[k], cnv[k]}
+                              : Child{-INFINITY, 0, 0};
+            if (hit[0] || hit[1] || hit[2] || hit[3]) {
+                cx(e[0], e[1]); cx(e[0], e[2]); cx(e[0], e[3]);
+                cx(e[1], e[2]); cx(e[1], e[3]);
+                cx(e[2], e[3]);
 #pragma unroll
-                for (int p = 0; p < 4; ++p) e[p] = Child{-INFINITY, 0, 0};
-                int c = 0;
-#pragma unroll
-                for (int k = 0; k < 4; ++k) {
-#pragma unroll
-                    for (int p = 0; p < 4; ++p)
-                        if (hit[k] && p == c) e[p] = Child{te[k], chv[k], cnv[k]};
-                    c += hit[k] ? 1 : 0;
-                }
-                if (c > 0) {
-                    cx(e[0], e[1]); cx(e[0], e[2]); cx(e[0], e[3]);
-                    cx(e[1], e[2]); cx(e[1], e[3]);
-                    cx(e[2], e[3]);
-#pragma unroll
-                    for (int p = 0; p < 4; ++p) {
-                        if (e[p].count > 0) {
-                            if (COUNT) triTests += (uint32_t)e[p].count;
-                            test_leaf<false>(tris, (uint32_t)(~e[p].child), (uint32_t)e[p].count,
-                                             ox, oy, oz, dx, dy, dz, tmin, hitT, hitPrim);
-                        } else if (e[p].count < 0) {
-                            RT_SPUSH(e[p].child);
+                for (int p = 0; p < 4; ++p) {
+                    if (e[p].count > 0) {
+                        if (COUNT) triTests += (uint32_t)e[p].count;
+                        if (test_leaf<ANY>(tris, (uint32_t)(~e[p].child), (uint32_t)e[p].count,
+                                           ox, oy, oz, dx, dy, dz, tmin, hitT, hitPrim)) {
+                            finished = true;
+                            break;
                         }
+                    } else if (e[p].count < 0) {
+                        RT_SPUSH(e[p].child);
                     }
                 }
             }
@@ -351,7 +322,7 @@ __global__ __launch_bounds__(kBlock) void k_trace_persist(
         }
         if (finished) {
             if (ANY) {
-                if (hitPrim != RT_MISS) atomicOr(bits_out + (r >> 5), 1u << (r & 31u));
+                if (hitPrim != RT_MISS) atomicOr(wbits + ((r - wbeg32) >> 5), 1u << (r & 31u));
                 if (prim_out) prim_out[r] = hitPrim;
             } else {
                 prim_out[r] = hitPrim;
@@ -364,50 +335,34 @@ __global__ __launch_bounds__(kBlock) void k_trace_persist(
             active = false;
         }
     }
+    if (ANY) {  // flush this wave's occlusion words (whole words: no overlap)
+        const uint32_t words = (end - wbeg32 + 31) >> 5;
+        for (uint32_t i = lane; i < words; i += 64) bits_out[(wbeg32 >> 5) + i] = wbits[i];
+    }
 }
 
-__global__ void k_zero_words(uint32_t* __restrict__ p, uint32_t nwords) {
-    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
-    if (i < nwords) p[i] = 0u;
-}
-
-// Resident blocks of kernel f across the device (occupancy x CUs).
-template <class F>
-uint32_t resident_blocks(F f) {
-    int dev = 0, cus = 0, per_cu = 0;
-    if (hipGetDevice(&dev) != hipSuccess ||
-        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-        cus <= 0)
-        cus = 256;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, kBlock, 0) != hipSuccess ||
-        per_cu <= 0)
-        per_cu = 4;
-    return (uint32_t)(cus * per_cu);
-}
+struct TraceCfg {
+    bool refill;
+    uint32_t per_wave;      // multiple of 64, <= kMaxPerWave
+    uint32_t refill_lanes;  // 1..64
+};
 
 template <int PRIV>
 rt_status launch_v(hipStream_t s, const rt_bvh_dev* b, const rt_rays_soa* rays, uint32_t n,
-                   uint32_t flags, rt_hits_soa* hits, uint32_t* bits, uint32_t* box,
-                   uint32_t* tri) {
+                   uint32_t flags, const TraceCfg& cfg, rt_hits_soa* hits, uint32_t* bits,
+                   uint32_t* box, uint32_t* tri) {
     const bool any = flags & RT_ANY;
     const bool cnt = flags & RT_COUNTERS;
     uint32_t* prim = hits ? hits->prim : nullptr;
     float* t = hits ? hits->t : nullptr;
-    if (flags & RT_TRACE_PERSISTENT) {
-        if (any) {
-            const uint32_t words = ceil_div(n, 32);
-            hipLaunchKernelGGL(k_zero_words, dim3(ceil_div(words, 256)), dim3(256), 0, s, bits,
-                               words);
-        }
+    if (cfg.refill) {
+        // waves x per_wave covers [0, n); per_wave is a multiple of 64 and at
+        // most kMaxPerWave, so each wave owns whole occlusion words
+        const uint32_t blocks = ceil_div(n, (uint64_t)cfg.per_wave * (kBlock / 64));
 #define RT_P(A, C)                                                                            \
-    do {                                                                                      \
-        static const uint32_t res = resident_blocks(k_trace_persist<PRIV, A, C>);             \
-        const uint32_t blocks = std::min<uint32_t>(ceil_div(n, kBlock), res);                 \
-        const uint32_t per_wave = ceil_div(ceil_div(n, blocks * (kBlock / 64)), 64) * 64;     \
-        hipLaunchKernelGGL((k_trace_persist<PRIV, A, C>), dim3(blocks), dim3(kBlock), 0, s,   \
-                           b->nodes, b->tris, b->node_count, *rays, n, per_wave, prim, t,     \
-                           bits, box, tri);                                                   \
-    } while (0)
+    hipLaunchKernelGGL((k_trace_persist<PRIV, A, C>), dim3(blocks), dim3(kBlock), 0, s,       \
+                       b->nodes, b->tris, b->node_count, *rays, n, cfg.per_wave,              \
+                       cfg.refill_lanes, prim, t, bits, box, tri)
         if (any && cnt) RT_P(true, true);
         else if (any) RT_P(true, false);
         else if (cnt) RT_P(false, true);
@@ -432,11 +387,21 @@ rt_status launch_v(hipStream_t s, const rt_bvh_dev* b, const rt_rays_soa* rays, 
 
 using namespace rt;
 
-extern "C" rt_status rt_trace(rt_stream stream, const rt_bvh_dev* bvh, const rt_rays_soa* rays,
-                              uint32_t n, uint32_t flags, rt_hits_soa* hits, uint32_t* occl_bits,
-                              uint32_t* box_tests, uint32_t* tri_tests) {
+extern "C" rt_status rt_trace_ex(rt_stream stream, const rt_bvh_dev* bvh, const rt_rays_soa* rays,
+                                 uint32_t n, uint32_t flags, const rt_trace_opts* opts,
+                                 rt_hits_soa* hits, uint32_t* occl_bits, uint32_t* box_tests,
+                                 uint32_t* tri_tests) {
     if (!bvh || !rays) return RT_ERR_ARG;
     if (flags & ~(RT_ANY | RT_COUNTERS | RT_TRACE_PERSISTENT)) return RT_ERR_ARG;
+    TraceCfg cfg{(flags & RT_TRACE_PERSISTENT) != 0, 1024u, 16u};
+    if (opts) {
+        if (opts->schedule > RT_SCHED_REFILL) return RT_ERR_ARG;
+        cfg.refill = cfg.refill || opts->schedule == RT_SCHED_REFILL;
+        if (opts->rays_per_wave) cfg.per_wave = opts->rays_per_wave;
+        if (opts->refill_lanes) cfg.refill_lanes = opts->refill_lanes;
+    }
+    if (cfg.per_wave % 64 || cfg.per_wave > kMaxPerWave || cfg.refill_lanes > 64)
+        return RT_ERR_ARG;
     if (n == 0) return RT_OK;
     if (!rays->ox || !rays->oy || !rays->oz || !rays->tmin || !rays->dx || !rays->dy ||
         !rays->dz || !rays->tmax)
@@ -452,7 +417,14 @@ extern "C" rt_status rt_trace(rt_stream stream, const rt_bvh_dev* bvh, const rt_
     const uint32_t need = 3u * depth + 1u;
     hipStream_t s = as_stream(stream);
     if (need <= kLdsStack + 44)
-        return launch_v<44>(s, bvh, rays, n, flags, hits, occl_bits, box_tests, tri_tests);
-    return launch_v<3 * kMaxDepth + 1 - kLdsStack>(s, bvh, rays, n, flags, hits, occl_bits,
+        return launch_v<44>(s, bvh, rays, n, flags, cfg, hits, occl_bits, box_tests, tri_tests);
+    return launch_v<3 * kMaxDepth + 1 - kLdsStack>(s, bvh, rays, n, flags, cfg, hits, occl_bits,
                                                    box_tests, tri_tests);
+}
+
+extern "C" rt_status rt_trace(rt_stream stream, const rt_bvh_dev* bvh, const rt_rays_soa* rays,
+                              uint32_t n, uint32_t flags, rt_hits_soa* hits, uint32_t* occl_bits,
+                              uint32_t* box_tests, uint32_t* tri_tests) {
+    return rt_trace_ex(stream, bvh, rays, n, flags, nullptr, hits, occl_bits, box_tests,
+                       tri_tests);
 }

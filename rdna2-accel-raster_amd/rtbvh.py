@@ -21,16 +21,6 @@ LIB_PATH = os.path.join(_HERE, "librtbvh.so")
 MISS = 0xFFFFFFFF
 
 RT_CLOSEST, RT_ANY, RT_COUNTERS, RT_TRACE_PERSISTENT = 0x0, 0x1, 0x100, 0x200
-
-# Default traversal schedule (persistent lane-refill waves or one ray per
-# thread); per call override with persistent=True/False.  One ray per thread
-# measured faster on the 2.5 M-triangle shadow frame (25.4 vs 42.4 ms).
-DEFAULT_PERSISTENT = False
-
-
-def _sched(persistent):
-    p = DEFAULT_PERSISTENT if persistent is None else persistent
-    return RT_TRACE_PERSISTENT if p else 0
 _STATUS = {0: "RT_OK", 1: "RT_ERR_ARG", 2: "RT_ERR_CAPACITY", 3: "RT_ERR_HIP",
            4: "RT_ERR_NOMEM", 5: "RT_ERR_UNSUPPORTED"}
 
@@ -73,6 +63,25 @@ class rt_build_cfg(C.Structure):
 BUILD_ALGOS = {"auto": 0, "radix": 1, "implicit": 2}
 
 
+class rt_trace_opts(C.Structure):
+    _fields_ = [("schedule", u32), ("rays_per_wave", u32), ("refill_lanes", u32),
+                ("reserved", u32)]
+
+
+# Traversal schedule used when a call does not choose one: "simple" (one ray
+# per thread) or "refill" (waves walk ray chunks, refilling finished lanes),
+# with the chunk size and refill threshold of TRACE_TUNING.
+DEFAULT_SCHEDULE = "simple"
+TRACE_TUNING = {"rays_per_wave": 1024, "refill_lanes": 16}
+
+
+def trace_opts(schedule=None, rays_per_wave=None, refill_lanes=None):
+    s = schedule or DEFAULT_SCHEDULE
+    return rt_trace_opts(1 if s == "refill" else 0,
+                         rays_per_wave or TRACE_TUNING["rays_per_wave"],
+                         refill_lanes or TRACE_TUNING["refill_lanes"], 0)
+
+
 _lib = None
 
 # name -> argtypes (restype is rt_status = int for all but the string getters)
@@ -83,6 +92,7 @@ _SIGS = {
     "rt_gpu_refit": [vp, vp, vp, u32, vp, u32, vp, vp, C.c_size_t],
     "rt_convert_tris48": [vp, vp, vp, u32, vp],
     "rt_trace": [vp, vp, vp, u32, u32, vp, vp, vp, vp],
+    "rt_trace_ex": [vp, vp, vp, u32, u32, vp, vp, vp, vp, vp],
     "rt_gen_sphere": [vp, u32, u32, f32, f32, u32, vp, vp],
     "rt_gen_wave_grid": [vp, u32, u32, f32, f32, f32, f32, f32, vp, vp],
     "rt_gen_heightfield": [vp, u32, u32, f32, f32, u32, vp, vp],
@@ -303,14 +313,16 @@ class GpuBVH4:
         return rt_bvh_dev(ptr(self.nodes), ptr(self.tris), u32(self.node_count),
                           u32(self.tri_count), u32(self.depth), u32(0))
 
-    def intersect(self, rays, counters=False, stream=None, out=None, persistent=None):
-        """Closest hit per ray: (prim uint32-as-int32, t float32[, box, tri])."""
+    def intersect(self, rays, counters=False, stream=None, out=None, schedule=None, **tune):
+        """Closest hit per ray: (prim uint32-as-int32, t float32[, box, tri]).
+        schedule: "simple" | "refill" (default DEFAULT_SCHEDULE); tune:
+        rays_per_wave, refill_lanes for "refill"."""
         n = len(rays)
         dev = rays.data.device
         prim = out[0] if out else torch.empty(n, dtype=torch.int32, device=dev)
         t = out[1] if out else torch.empty(n, dtype=torch.float32, device=dev)
         box = tri = None
-        flags = RT_CLOSEST | _sched(persistent)
+        flags = RT_CLOSEST
         if counters:
             box = torch.empty(n, dtype=torch.int32, device=dev)
             tri = torch.empty(n, dtype=torch.int32, device=dev)
@@ -318,26 +330,28 @@ class GpuBVH4:
         hits = rt_hits_soa(ptr(prim), ptr(t))
         b = self.dev()
         r = rays.soa()
-        call("rt_trace", stream_handle(stream), C.byref(b), C.byref(r), u32(n), u32(flags),
-             C.byref(hits), None, ptr(box), ptr(tri))
+        o = trace_opts(schedule, **tune)
+        call("rt_trace_ex", stream_handle(stream), C.byref(b), C.byref(r), u32(n), u32(flags),
+             C.byref(o), C.byref(hits), None, ptr(box), ptr(tri))
         return (prim, t, box, tri) if counters else (prim, t)
 
-    def occluded(self, rays, stream=None, out=None, counters=False, persistent=None):
+    def occluded(self, rays, stream=None, out=None, counters=False, schedule=None, **tune):
         """Any-hit occlusion bits: int32 tensor of ceil(n/32) words."""
         n = len(rays)
         dev = rays.data.device
         bits = out if out is not None else torch.empty(max(1, (n + 31) // 32), dtype=torch.int32,
                                                        device=dev)
         box = tri = None
-        flags = RT_ANY | _sched(persistent)
+        flags = RT_ANY
         if counters:
             box = torch.empty(n, dtype=torch.int32, device=dev)
             tri = torch.empty(n, dtype=torch.int32, device=dev)
             flags |= RT_COUNTERS
         b = self.dev()
         r = rays.soa()
-        call("rt_trace", stream_handle(stream), C.byref(b), C.byref(r), u32(n), u32(flags), None,
-             ptr(bits), ptr(box), ptr(tri))
+        o = trace_opts(schedule, **tune)
+        call("rt_trace_ex", stream_handle(stream), C.byref(b), C.byref(r), u32(n), u32(flags),
+             C.byref(o), None, ptr(bits), ptr(box), ptr(tri))
         return (bits, box, tri) if counters else bits
 
 

@@ -1,6 +1,7 @@
 """GPU traversal parity: rt_trace (HIP) against the CPU oracle on the same
-tree and the same rays, for both schedules (one ray per thread, persistent
-lane-refill waves).  Closest-hit triangle IDs, t and box/triangle counters
+tree and the same rays, for every schedule (one ray per thread; waves that
+walk ray chunks and refill finished lanes, at the extreme chunk sizes and
+refill thresholds).  Closest-hit triangle IDs, t and box/triangle counters
 must be bit-identical (same traversal order and predicates as
 software_bvh_bvh4.comp:96-190); any-hit occlusion bits and counters
 identical (slot-order any-hit on both sides)."""
@@ -13,7 +14,11 @@ from conftest import u32, to_np
 pytestmark = pytest.mark.gpu
 
 
-@pytest.fixture(params=[False, True], ids=["simple", "persistent"])
+@pytest.fixture(params=[dict(schedule="simple"),
+                        dict(schedule="refill"),
+                        dict(schedule="refill", rays_per_wave=64, refill_lanes=1),
+                        dict(schedule="refill", rays_per_wave=4096, refill_lanes=64)],
+                ids=["simple", "refill", "refill-64-1", "refill-4096-64"])
 def sched(request):
     return request.param
 
@@ -35,7 +40,7 @@ def _bits(b, n):
 def _cmp_closest(rt, O, n4, tris, bvh, r, sched):
     prim_c, t_c, box_c, tri_c = O.trace_bvh4(n4, tris, r, counters=True)
     prim_g, t_g, box_g, tri_g = bvh.intersect(_rays_to_dev(rt, r), counters=True,
-                                              persistent=sched)
+                                              **sched)
     torch.cuda.synchronize()
     np.testing.assert_array_equal(u32(prim_g), prim_c)
     np.testing.assert_array_equal(to_np(t_g).view(np.uint32), t_c.view(np.uint32))
@@ -46,7 +51,7 @@ def _cmp_closest(rt, O, n4, tris, bvh, r, sched):
 
 def _cmp_any(rt, O, n4, tris, bvh, r, sched):
     prim_c, _, box_c, tri_c = O.trace_bvh4(n4, tris, r, mode=O.ANY, counters=True)
-    bits, box_g, tri_g = bvh.occluded(_rays_to_dev(rt, r), counters=True, persistent=sched)
+    bits, box_g, tri_g = bvh.occluded(_rays_to_dev(rt, r), counters=True, **sched)
     torch.cuda.synchronize()
     np.testing.assert_array_equal(_bits(u32(bits), len(r)), prim_c != O.MISS)
     np.testing.assert_array_equal(u32(box_g), box_c)
@@ -102,7 +107,7 @@ def test_known_answer_probe_triangle(rt, O, sched):
     r.dz[:] = 1.0
     r.tmin[:] = 0.001
     r.tmax[:] = 100.0
-    prim, t = bvh.intersect(_rays_to_dev(rt, r), persistent=sched)
+    prim, t = bvh.intersect(_rays_to_dev(rt, r), **sched)
     torch.cuda.synchronize()
     prim, t = u32(prim), to_np(t)
     assert (prim[:512] == 0).all() and (np.abs(t[:512] - 2.0) < 0.01).all()
@@ -120,7 +125,7 @@ def test_known_answer_forged_triangle(rt, O, sched):
     r.dz[:] = 1.0
     r.tmin[:] = 0.001
     r.tmax[:] = 100.0
-    prim, t = bvh.intersect(_rays_to_dev(rt, r), persistent=sched)
+    prim, t = bvh.intersect(_rays_to_dev(rt, r), **sched)
     torch.cuda.synchronize()
     prim, t = u32(prim), to_np(t)
     assert prim[0] == 0 and abs(t[0] - 5.0) < 1e-3 and prim[1] == O.MISS
@@ -138,7 +143,7 @@ def test_exp0_stats_against_published(rt, O, target, sched):
     n4, tris, bvh = _upload_tree(rt, O, pos, idx)
     for coherent in (True, False):
         r = O.exp0_rays(1 << 20, coherent)
-        prim, t, box, tri = bvh.intersect(_rays_to_dev(rt, r), counters=True, persistent=sched)
+        prim, t, box, tri = bvh.intersect(_rays_to_dev(rt, r), counters=True, **sched)
         torch.cuda.synchronize()
         prim, box, tri = u32(prim), u32(box), u32(tri)
         hit = prim != O.MISS
@@ -165,13 +170,13 @@ def test_zero_rays_and_empty_tree(rt, O, sched):
     p2, i2 = O.wave_grid(2, 2)
     _, _, bvh = _upload_tree(rt, O, p2, i2)
     empty = rt.Rays.empty(0)
-    prim, t = bvh.intersect(empty, persistent=sched)
+    prim, t = bvh.intersect(empty, **sched)
     assert prim.numel() == 0
     # an empty tree: every ray misses, t = tmax
     tree0 = rt.GpuBVH4(bvh.nodes, bvh.tris, 0, 0, 0)
     r = O.random_rays(100, 1, (-1, -1, -1), (1, 1, 1), tmax=3.0)
-    prim, t = tree0.intersect(_rays_to_dev(rt, r), persistent=sched)
-    bits = tree0.occluded(_rays_to_dev(rt, r), persistent=sched)
+    prim, t = tree0.intersect(_rays_to_dev(rt, r), **sched)
+    bits = tree0.occluded(_rays_to_dev(rt, r), **sched)
     torch.cuda.synchronize()
     assert (u32(prim) == O.MISS).all() and (to_np(t) == 3.0).all()
     assert not _bits(u32(bits), 100).any()
