@@ -612,6 +612,9 @@ __global__ __launch_bounds__(kSmallBlock) void k_build_small(
 // Node ids: internal node i in [0, m), leaf (sorted position) p -> m + p,
 // m = n - 1.  Internal node i covers sorted range [rf[i], rl[i]].
 // ---------------------------------------------------------------------------
+// Sorted positions per bottom-up window (one workgroup, LDS counters).
+constexpr uint32_t kWindow = 4096;
+
 struct RadixArrays {
     const uint32_t* keys;  // sorted codes
     float* box;            // (2n - 1) x 6: min xyz, max xyz
@@ -620,7 +623,9 @@ struct RadixArrays {
     int32_t* left;         // m
     int32_t* right;        // m
     int32_t* parent;       // 2n - 1
-    uint32_t* counter;     // m
+    uint32_t* done;        // m: arrivals at a straddling node (k_rt_top)
+    uint32_t* top;         // straddling nodes (range crosses a kWindow boundary)
+    uint32_t* top_count;   // 1 word
     int4* cand;            // m, -1 = empty
     uint32_t* size4;       // m: Box4 nodes in the subtree rooted here
     uint32_t n, m, maxp;
@@ -651,8 +656,10 @@ __global__ __launch_bounds__(256) void k_rt_leaves(const float* __restrict__ pos
                                                    const uint32_t* __restrict__ idx, uint32_t n,
                                                    const uint32_t* __restrict__ sorted,
                                                    rt_tri64* __restrict__ tris,
-                                                   float* __restrict__ box, uint32_t m) {
+                                                   float* __restrict__ box, uint32_t m,
+                                                   uint32_t* __restrict__ top_count) {
     const uint32_t p = blockIdx.x * 256 + threadIdx.x;
+    if (p == 0) *top_count = 0;  // appended to by k_rt_radix
     if (p >= n) return;
     const uint32_t id = sorted[p];
     float v[9];
@@ -701,7 +708,9 @@ __global__ __launch_bounds__(256) void k_rt_radix(RadixArrays A, rt_build_info* 
     A.right[i] = R;
     A.parent[L] = (int32_t)i;
     A.parent[R] = (int32_t)i;
-    A.counter[i] = 0;
+    A.done[i] = 0;
+    if (lo / kWindow != hi / kWindow)  // finished by k_rt_top, not by a window
+        A.top[atomicAdd(A.top_count, 1u)] = i;
     if (i == 0) {
         A.parent[0] = -1;
         info->node_count = 0;
@@ -715,52 +724,100 @@ __device__ __forceinline__ float box_sa(const float* b) {
     return 2.0f * (dx * dy + dy * dz + dz * dx);
 }
 
-// Leaf p climbs towards the root; at each internal node the second thread
-// to arrive (acquire-release counter) finishes the node: box = union of the
+// Finish internal node x once both subtrees are: box = union of the
 // children, the greedy Box4 candidates and the Box4 subtree size.
-__global__ __launch_bounds__(256) void k_rt_bottom_up(RadixArrays A) {
-    const uint32_t p = blockIdx.x * 256 + threadIdx.x;
-    if (p >= A.n) return;
-    int32_t x = A.parent[A.m + p];
-    while (x >= 0) {
-        const uint32_t old = __hip_atomic_fetch_add(A.counter + x, 1u, __ATOMIC_ACQ_REL,
-                                                    __HIP_MEMORY_SCOPE_AGENT);
-        if (old == 0) return;  // the sibling subtree is still in flight
-        const int32_t L = A.left[x], R = A.right[x];
-        float* bx = A.box + 6 * (size_t)x;
-        const float* bl = A.box + 6 * (size_t)L;
-        const float* br = A.box + 6 * (size_t)R;
+__device__ void finish_node(const RadixArrays& A, int32_t x) {
+    const int32_t L = A.left[x], R = A.right[x];
+    float* bx = A.box + 6 * (size_t)x;
+    const float* bl = A.box + 6 * (size_t)L;
+    const float* br = A.box + 6 * (size_t)R;
 #pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            bx[a] = fminf(bl[a], br[a]);
-            bx[3 + a] = fmaxf(bl[3 + a], br[3 + a]);
-        }
-        if (A.rl[x] - A.rf[x] + 1 > A.maxp) {
-            int32_t c[4] = {L, R, -1, -1};
-            int nc = 2;
-            while (nc < 4) {  // software_bvh.hpp:225-247
-                int best = -1;
-                float bestSa = -1.0f;
-                for (int q = 0; q < nc; ++q) {
-                    if (!ninner(A, c[q])) continue;
-                    const float sa = box_sa(A.box + 6 * (size_t)c[q]);
-                    if (sa > bestSa) {
-                        bestSa = sa;
-                        best = q;
-                    }
+    for (int a = 0; a < 3; ++a) {
+        bx[a] = fminf(bl[a], br[a]);
+        bx[3 + a] = fmaxf(bl[3 + a], br[3 + a]);
+    }
+    if (A.rl[x] - A.rf[x] + 1 > A.maxp) {
+        int32_t c[4] = {L, R, -1, -1};
+        int nc = 2;
+        while (nc < 4) {  // software_bvh.hpp:225-247
+            int best = -1;
+            float bestSa = -1.0f;
+            for (int q = 0; q < nc; ++q) {
+                if (!ninner(A, c[q])) continue;
+                const float sa = box_sa(A.box + 6 * (size_t)c[q]);
+                if (sa > bestSa) {
+                    bestSa = sa;
+                    best = q;
                 }
-                if (best < 0) break;
-                const int32_t e = c[best];
-                c[best] = A.left[e];
-                c[nc++] = A.right[e];
             }
-            uint32_t sz = 1;
-            for (int q = 0; q < nc; ++q)
-                if (ninner(A, c[q])) sz += A.size4[c[q]];
-            A.cand[x] = make_int4(c[0], c[1], c[2], c[3]);
-            A.size4[x] = sz;
+            if (best < 0) break;
+            const int32_t e = c[best];
+            c[best] = A.left[e];
+            c[nc++] = A.right[e];
         }
-        x = A.parent[x];
+        uint32_t sz = 1;
+        for (int q = 0; q < nc; ++q)
+            if (ninner(A, c[q])) sz += A.size4[c[q]];
+        A.cand[x] = make_int4(c[0], c[1], c[2], c[3]);
+        A.size4[x] = sz;
+    }
+}
+
+__device__ __forceinline__ bool straddles(const RadixArrays& A, int32_t x) {
+    return (uint32_t)x < A.m && A.rf[x] / kWindow != A.rl[x] / kWindow;
+}
+
+// Pass 1: block b owns the kWindow sorted positions [b kWindow, ...); each
+// leaf climbs and, at every internal node whose range lies inside the
+// window, the second thread to arrive (LDS counter, workgroup-scope
+// acquire-release) finishes it.  A climb stops at the first node that
+// straddles a window boundary: those are finished by k_rt_top.  (Agent-scope
+// release writes back this XCD's L2; one such atomic per node made this pass
+// ~10 ms at 2.5 M triangles, so no agent-scope atomic is used at all.)
+__global__ __launch_bounds__(1024) void k_rt_bottom_up(RadixArrays A) {
+    __shared__ uint32_t wcnt[kWindow];
+    const uint32_t lo = blockIdx.x * kWindow;
+    const uint32_t hi = min(lo + kWindow, A.n) - 1;
+    for (uint32_t i = threadIdx.x; i < kWindow; i += 1024) wcnt[i] = 0;
+    __syncthreads();
+    for (uint32_t p = lo + threadIdx.x; p <= hi; p += 1024) {
+        int32_t x = A.parent[A.m + p];
+        while (x >= 0) {
+            // x covers [rf[x], rl[x]] with x at one end, so an inside node's
+            // counter slot x - lo is in [0, kWindow)
+            if (!(A.rf[x] >= lo && A.rl[x] <= hi)) break;
+            const uint32_t old = __hip_atomic_fetch_add(wcnt + (x - lo), 1u, __ATOMIC_ACQ_REL,
+                                                        __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (old == 0) break;  // the sibling subtree is still in flight
+            finish_node(A, x);
+            x = A.parent[x];
+        }
+    }
+}
+
+// Pass 2, one workgroup: the straddling nodes (at most depth per window
+// boundary; their parents straddle too).  A child that does not straddle was
+// finished by pass 1 and counts as an arrival up front; a straddling child
+// arrives when its own climb finishes it.  The second arrival finishes the
+// node and climbs on.  The arrival counters are in global memory but every
+// access is from this one workgroup, so workgroup scope suffices.
+__global__ __launch_bounds__(1024) void k_rt_top(RadixArrays A) {
+    const uint32_t cnt = *A.top_count;
+    for (uint32_t i = threadIdx.x; i < cnt; i += 1024) {
+        int32_t x = (int32_t)A.top[i];
+        const int ready = (straddles(A, A.left[x]) ? 0 : 1) + (straddles(A, A.right[x]) ? 0 : 1);
+        if (ready == 0) continue;
+        if (ready == 1 && __hip_atomic_fetch_add(A.done + x, 1u, __ATOMIC_ACQ_REL,
+                                                 __HIP_MEMORY_SCOPE_WORKGROUP) == 0)
+            continue;
+        for (;;) {
+            finish_node(A, x);
+            x = A.parent[x];
+            if (x < 0) break;
+            if (__hip_atomic_fetch_add(A.done + x, 1u, __ATOMIC_ACQ_REL,
+                                       __HIP_MEMORY_SCOPE_WORKGROUP) == 0)
+                break;
+        }
     }
 }
 
@@ -850,8 +907,8 @@ LevelArgs level_args(const TreeLayout& t) {
 
 struct ScratchPlan {
     size_t bounds = 0, keys0 = 0, vals0 = 0, keys1 = 0, vals1 = 0, hist = 0, totals = 0,
-           boxes = 0, rbox = 0, rf = 0, rl = 0, left = 0, right = 0, parent = 0, counter = 0,
-           cand = 0, size4 = 0, total = 0;
+           boxes = 0, rbox = 0, rf = 0, rl = 0, left = 0, right = 0, parent = 0, done = 0,
+           top = 0, top_count = 0, cand = 0, size4 = 0, total = 0;
     uint32_t tiles = 0;
 };
 
@@ -879,7 +936,9 @@ ScratchPlan plan_scratch(uint32_t n, uint32_t algo) {
         p.left = off; off += al(m * 4);
         p.right = off; off += al(m * 4);
         p.parent = off; off += al(u * 4);
-        p.counter = off; off += al(m * 4);
+        p.done = off; off += al(m * 4);
+        p.top = off; off += al(m * 4);
+        p.top_count = off; off += al(4);
         p.cand = off; off += al(m * 16);
         p.size4 = off; off += al(m * 4);
     }
@@ -954,20 +1013,23 @@ rt_status build_radix(hipStream_t s, const float* pos, const uint32_t* idx, uint
     A.left = reinterpret_cast<int32_t*>(base + p.left);
     A.right = reinterpret_cast<int32_t*>(base + p.right);
     A.parent = reinterpret_cast<int32_t*>(base + p.parent);
-    A.counter = reinterpret_cast<uint32_t*>(base + p.counter);
+    A.done = reinterpret_cast<uint32_t*>(base + p.done);
+    A.top = reinterpret_cast<uint32_t*>(base + p.top);
+    A.top_count = reinterpret_cast<uint32_t*>(base + p.top_count);
     A.cand = reinterpret_cast<int4*>(base + p.cand);
     A.size4 = reinterpret_cast<uint32_t*>(base + p.size4);
     A.n = n;
     A.m = n - 1;
     A.maxp = maxp;
     hipLaunchKernelGGL(k_rt_leaves, dim3(ceil_div(n, 256)), dim3(256), 0, s, pos, idx, n, vals,
-                       tris, A.box, A.m);
+                       tris, A.box, A.m, A.top_count);
     if (n <= maxp) {
         hipLaunchKernelGGL(k_rt_single, dim3(1), dim3(64), 0, s, A.box, n, A.m, nodes, info);
         return launch_status("k_rt_single");
     }
     hipLaunchKernelGGL(k_rt_radix, dim3(ceil_div(A.m, 256)), dim3(256), 0, s, A, info);
-    hipLaunchKernelGGL(k_rt_bottom_up, dim3(ceil_div(n, 256)), dim3(256), 0, s, A);
+    hipLaunchKernelGGL(k_rt_bottom_up, dim3(ceil_div(n, kWindow)), dim3(1024), 0, s, A);
+    hipLaunchKernelGGL(k_rt_top, dim3(1), dim3(1024), 0, s, A);
     hipLaunchKernelGGL(k_rt_emit, dim3(ceil_div(A.m, 256)), dim3(256), 0, s, A, nodes, info);
     return launch_status("k_rt_emit");
 }
