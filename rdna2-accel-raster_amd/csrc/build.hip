@@ -53,37 +53,49 @@ struct LevelArgs {
 __device__ __forceinline__ void tri_verts(const float* __restrict__ pos,
                                           const uint32_t* __restrict__ idx, uint32_t t,
                                           float v[9]) {
+    // one pointer per vertex (both layouts), each read at offsets 0, 4, 8
+    const float *p0, *p1, *p2;
     if (idx) {
         const uint32_t i0 = idx[3 * (size_t)t], i1 = idx[3 * (size_t)t + 1],
                        i2 = idx[3 * (size_t)t + 2];
-        v[0] = pos[3 * (size_t)i0]; v[1] = pos[3 * (size_t)i0 + 1]; v[2] = pos[3 * (size_t)i0 + 2];
-        v[3] = pos[3 * (size_t)i1]; v[4] = pos[3 * (size_t)i1 + 1]; v[5] = pos[3 * (size_t)i1 + 2];
-        v[6] = pos[3 * (size_t)i2]; v[7] = pos[3 * (size_t)i2 + 1]; v[8] = pos[3 * (size_t)i2 + 2];
+        p0 = pos + 3 * (size_t)i0;
+        p1 = pos + 3 * (size_t)i1;
+        p2 = pos + 3 * (size_t)i2;
     } else {
-        const float* p = pos + 9 * (size_t)t;
+        p0 = pos + 9 * (size_t)t;
+        p1 = p0 + 3;
+        p2 = p0 + 6;
+    }
 #pragma unroll
-        for (int k = 0; k < 9; ++k) v[k] = p[k];
+    for (int k = 0; k < 3; ++k) {
+        v[k] = p0[k];
+        v[3 + k] = p1[k];
+        v[6 + k] = p2[k];
     }
 }
 
-// software_bvh.hpp:53 centroid: (p0 + p1 + p2) / 3
+// software_bvh.hpp:53 centroid: (p0 + p1 + p2) / 3 (div3 == the division)
 __device__ __forceinline__ void centroid(const float v[9], float c[3]) {
-    c[0] = (v[0] + v[3] + v[6]) / 3.0f;
-    c[1] = (v[1] + v[4] + v[7]) / 3.0f;
-    c[2] = (v[2] + v[5] + v[8]) / 3.0f;
+    c[0] = div3(v[0] + v[3] + v[6]);
+    c[1] = div3(v[1] + v[4] + v[7]);
+    c[2] = div3(v[2] + v[5] + v[8]);
 }
 
 // q_a = clamp(floor((c_a - lo_a) * s_a), 0, 1023), s_a = 1024 / (hi_a - lo_a)
 // (0 on a flat axis); x bits at 3k+2, y at 3k+1, z at 3k.
-__device__ __forceinline__ uint32_t morton(const float c[3], const float lo[3],
-                                           const float s[3]) {
-    uint32_t q[3];
+__device__ __forceinline__ void quantize(const float c[3], const float lo[3], const float s[3],
+                                         uint32_t q[3]) {
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
         float f = (c[a] - lo[a]) * s[a];
         f = fminf(fmaxf(f, 0.0f), 1023.0f);
         q[a] = (uint32_t)f;
     }
+}
+__device__ __forceinline__ uint32_t morton(const float c[3], const float lo[3],
+                                           const float s[3]) {
+    uint32_t q[3];
+    quantize(c, lo, s, q);
     return (expand_bits10(q[0]) << 2) | (expand_bits10(q[1]) << 1) | expand_bits10(q[2]);
 }
 
@@ -99,26 +111,38 @@ __device__ __forceinline__ void axis_scale(const float lo[3], const float hi[3],
 // Block-wide primitives
 // ---------------------------------------------------------------------------
 
-__device__ __forceinline__ float wave_min(float v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o));
-    return v;
-}
-__device__ __forceinline__ float wave_max(float v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
+// Inclusive wave scan with DPP (the gfx9 sequence): row_shr 1, 2, 4, 8 scan
+// each row of 16 lanes, row_bcast:15 and row_bcast:31 carry across rows.
+// All 64 lanes must be active; lane i ends with op over lanes 0..i.  (A
+// shuffle-based scan goes through ds_bpermute, an LDS round trip per step.)
+template <typename Op>
+__device__ __forceinline__ uint32_t wave_scan_dpp(uint32_t v, uint32_t id, Op op) {
+    v = op(v, (uint32_t)__builtin_amdgcn_update_dpp((int)id, (int)v, 0x111, 0xf, 0xf, false));
+    v = op(v, (uint32_t)__builtin_amdgcn_update_dpp((int)id, (int)v, 0x112, 0xf, 0xf, false));
+    v = op(v, (uint32_t)__builtin_amdgcn_update_dpp((int)id, (int)v, 0x114, 0xf, 0xf, false));
+    v = op(v, (uint32_t)__builtin_amdgcn_update_dpp((int)id, (int)v, 0x118, 0xf, 0xf, false));
+    v = op(v, (uint32_t)__builtin_amdgcn_update_dpp((int)id, (int)v, 0x142, 0xa, 0xf, false));
+    v = op(v, (uint32_t)__builtin_amdgcn_update_dpp((int)id, (int)v, 0x143, 0xc, 0xf, false));
     return v;
 }
 
-// Inclusive scan across the 64 lanes of a wave.
 __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
-    const uint32_t lane = threadIdx.x & 63u;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        uint32_t u = __shfl_up(v, o);
-        if (lane >= (uint32_t)o) v += u;
-    }
-    return v;
+    return wave_scan_dpp(v, 0u, [](uint32_t a, uint32_t b) { return a + b; });
+}
+__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
+    v = wave_scan_dpp(v, 0u, [](uint32_t a, uint32_t b) { return a > b ? a : b; });
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, 63);
+}
+// Float min / max through the order-preserving integer map (so -0 < +0; the
+// callers only quantize against the result, where the sign of zero is moot).
+__device__ __forceinline__ float wave_min(float f) {
+    uint32_t v = wave_scan_dpp(f2o(f), 0xFFFFFFFFu,
+                               [](uint32_t a, uint32_t b) { return a < b ? a : b; });
+    return o2f((uint32_t)__builtin_amdgcn_readlane((int)v, 63));
+}
+__device__ __forceinline__ float wave_max(float f) {
+    uint32_t v = wave_scan_dpp(f2o(f), 0u, [](uint32_t a, uint32_t b) { return a > b ? a : b; });
+    return o2f((uint32_t)__builtin_amdgcn_readlane((int)v, 63));
 }
 
 // Stable rank of BLOCK*ITEMS elements by an 8-bit digit.  Element order is
@@ -134,7 +158,7 @@ __device__ __forceinline__ void block_rank(const uint32_t (&dig)[ITEMS], uint32_
     constexpr int W = BLOCK / 64;
     const uint32_t tid = threadIdx.x, lane = tid & 63u, w = tid >> 6;
     for (uint32_t i = tid; i < (uint32_t)W * 256u; i += BLOCK) cnt[i] = 0;
-    __syncthreads();
+    lds_barrier();
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64u - lane));
     uint32_t* row = cnt + w * 256u;
 #pragma unroll
@@ -154,7 +178,7 @@ __device__ __forceinline__ void block_rank(const uint32_t (&dig)[ITEMS], uint32_
         __builtin_amdgcn_wave_barrier();
         pos[i] = base + below;
     }
-    __syncthreads();
+    lds_barrier();
     uint32_t total = 0;
     if (tid < 256) {
         uint32_t run = 0;
@@ -172,13 +196,13 @@ __device__ __forceinline__ void block_rank(const uint32_t (&dig)[ITEMS], uint32_
         inc = wave_incl_scan(total);
         if (lane == 63) wsum[w] = inc;
     }
-    __syncthreads();
+    lds_barrier();
     if (tid < 256) {
         uint32_t add = 0;
         for (uint32_t ww = 0; ww < w; ++ww) add += wsum[ww];
         dstart[tid] = add + inc - total;
     }
-    __syncthreads();
+    lds_barrier();
 #pragma unroll
     for (int i = 0; i < ITEMS; ++i) pos[i] += dstart[dig[i]] + row[dig[i]];
 }
@@ -369,6 +393,26 @@ __device__ __forceinline__ void node_store(rt_node4* dst, const NodeReg& r) {
     reinterpret_cast<int4*>(dst)[7] = r.cn;
 }
 
+// 64-B leaf record (rt_tri64 / RadvBvhTriangleNode): vertices, triangle id,
+// geometry flags 0x80000000, node id 9.
+__device__ __forceinline__ void store_tri(rt_tri64* dst, const float v[9], uint32_t id) {
+    float4* tp = reinterpret_cast<float4*>(dst);
+    tp[0] = make_float4(v[0], v[1], v[2], v[3]);
+    tp[1] = make_float4(v[4], v[5], v[6], v[7]);
+    tp[2] = make_float4(v[8], 0.f, 0.f, 0.f);
+    reinterpret_cast<uint4*>(tp)[3] = make_uint4(id, 0x80000000u, 0u, 9u);
+}
+
+// Triangle box.  + 0.0f maps -0 to +0: which zero min/max return for
+// (+0, -0) differs between devices; canonical zeros keep the bytes exact.
+__device__ __forceinline__ void tri_box(const float v[9], float bb[6]) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        bb[a] = fminf(v[a], fminf(v[3 + a], v[6 + a])) + 0.0f;
+        bb[3 + a] = fmaxf(v[a], fmaxf(v[3 + a], v[6 + a])) + 0.0f;
+    }
+}
+
 // Leaf box b: sorted triangles 4b..4b+3 -> four rt_tri64 records and the
 // leaf node (one triangle per slot, child = ~sorted position, count 1).
 // REFIT takes each triangle's index from the existing leaf record.
@@ -390,18 +434,11 @@ __device__ __forceinline__ void emit_leaf_box(const float* __restrict__ pos,
         const uint32_t id = REFIT ? tris[s].triangle_id : sorted[s];
         float v[9];
         tri_verts(pos, idx, id, v);
-        float4* tp = reinterpret_cast<float4*>(tris + s);
-        tp[0] = make_float4(v[0], v[1], v[2], v[3]);
-        tp[1] = make_float4(v[4], v[5], v[6], v[7]);
-        tp[2] = make_float4(v[8], 0.f, 0.f, 0.f);
-        reinterpret_cast<uint4*>(tp)[3] = make_uint4(id, 0x80000000u, 0u, 9u);
+        store_tri(tris + s, v, id);
         float bb[6];
+        tri_box(v, bb);
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
-            // + 0.0f maps -0 to +0: which zero min/max return for (+0, -0)
-            // differs between devices; canonical zeros keep the bytes exact.
-            bb[a] = fminf(v[a], fminf(v[3 + a], v[6 + a])) + 0.0f;
-            bb[3 + a] = fmaxf(v[a], fmaxf(v[3 + a], v[6 + a])) + 0.0f;
             box[a] = fminf(box[a], bb[a]);
             box[3 + a] = fmaxf(box[3 + a], bb[3 + a]);
         }
@@ -486,120 +523,476 @@ __global__ __launch_bounds__(kEmitBlock) void k_emit_levels(
 }
 
 // ---------------------------------------------------------------------------
-// Small-n fused path: one 1024-thread workgroup, everything in LDS.
+// Small-n fused path: n <= 4096 in one launch of ceil(n / 256) workgroups of
+// 1024 threads.  Every workgroup loads all triangles, sorts them and reduces
+// every box in its own LDS (identical work, identical results); workgroup k
+// then writes only its share of the output: the records, sorted indices and
+// codes of sorted positions [256k, 256k + 256) and 1/ceil(n/256) of the node
+// array.  One CU stores about 57 GB/s even fully coalesced, so the 463 KB of
+// output of a 4096-triangle build is spread over 16 CUs rather than 1.
+//
+// Thread t owns triangles p = w*256 + i*64 + lane (w = t/64, i < 4) and keeps
+// their 36 vertex floats in registers from the load to the output.  The
+// centroid divide is div3 (the IEEE division sequence's temporaries pushed
+// the load phase past 128 VGPRs and into spills).  The sort is one MSD
+// bucket pass over the top 13 code bits (LDS histogram, scan, scatter); a
+// triangle's rank inside its bucket is the number of members with a smaller
+// (low 17 code bits, triangle index) pair, one u32 since n <= 4096.  That is
+// the order (code, index), i.e. the order of a stable sort by code.  When a
+// bucket holds more than kBucketCap triangles (clustered centroids) the
+// workgroup runs the 4-pass LSD ballot sort instead.
+//
+// LDS boxes are SoA (one array per bound) with level offsets padded to 4, so
+// a node's 4 child bounds are one 16-B read; stores put consecutive lanes on
+// consecutive 16-B chunks.
 // ---------------------------------------------------------------------------
 constexpr int kSmallItems = kSmallMax / kSmallBlock;  // 4
-constexpr uint32_t kSmallBoxes = 1365;                // total boxes for n = 4096
+constexpr uint32_t kBoxPitch = 1368;  // padded box slots for n = 4096: 4+4+16+64+256+1024
+constexpr uint32_t kBucketBits = 13;
+constexpr uint32_t kBuckets = 1u << kBucketBits;
+constexpr uint32_t kBucketShift = 30 - kBucketBits;
+constexpr uint32_t kBucketCap = 256;
+constexpr uint32_t kPartRecs = 256;   // sorted positions written per workgroup
+constexpr uint32_t kStageWords = 11;  // 9 vertex floats, triangle index, code
 
 struct SmallLds {
-    uint32_t key[2][kSmallMax];
-    uint32_t val[2][kSmallMax];
-    uint32_t cnt[(kSmallBlock / 64) * 256];
+    union {
+        struct {
+            uint32_t start[kBuckets];   // bucket counts, then first ranks
+            uint32_t cursor[kBuckets];  // scatter cursors, then bucket ends
+            uint32_t comp[kSmallMax];   // (low 17 code bits << 12) | index
+        } msd;
+        struct {
+            uint32_t key[2][kSmallMax];
+            uint32_t val[2][kSmallMax];
+            uint32_t cnt[(kSmallBlock / 64) * 256];
+        } lsd;
+        float tbox[6][kSmallMax];                  // triangle bounds at their ranks
+        uint32_t stage[kPartRecs * kStageWords];  // this workgroup's records
+    } u;
+    float box[6][kBoxPitch];  // node bounds: level lv from slot pad[lv]
+    uint32_t spread[1024];    // expand_bits10 of every 10-bit value
     uint32_t dstart[256];
     uint32_t wsum[16];
+    uint32_t wmax[16];
     float red[16][6];
-    float box[kSmallBoxes * 6];
 };
 
-template <bool REFIT>
+// Level table of a fused-path tree (n <= 4096: at most 6 levels).  Read only
+// with compile-time indices (pick), so it stays in scalar registers; a
+// runtime index into a kernel argument array is a scalar memory load.
+constexpr int kSmallLevels = 7;
+struct SmallTree {
+    uint32_t levels;              // node levels, root first
+    uint32_t count[kSmallLevels];  // nodes per level
+    uint32_t off[kSmallLevels];    // node index of the level's first node
+    uint32_t pad[kSmallLevels];    // first LDS box slot of the level (multiple of 4)
+};
+
+// Copy into scalar registers (readfirstlane keeps the compiler from folding
+// pick's selects back into indexed loads of the argument).
+__device__ __forceinline__ SmallTree in_regs(const SmallTree& a) {
+    SmallTree r;
+    r.levels = __builtin_amdgcn_readfirstlane(a.levels);
+#pragma unroll
+    for (int q = 0; q < kSmallLevels; ++q) {
+        r.count[q] = __builtin_amdgcn_readfirstlane(a.count[q]);
+        r.off[q] = __builtin_amdgcn_readfirstlane(a.off[q]);
+        r.pad[q] = __builtin_amdgcn_readfirstlane(a.pad[q]);
+    }
+    return r;
+}
+
+template <int N>
+__device__ __forceinline__ uint32_t pick(const uint32_t (&a)[N], uint32_t i) {
+    uint32_t r = a[0];
+#pragma unroll
+    for (int q = 1; q < N; ++q) r = i == (uint32_t)q ? a[q] : r;
+    return r;
+}
+
+// Union of slots first..first+3 of src (SoA, pitch floats per bound, first a
+// multiple of 4).  Empty slots hold the identity box (+inf mins, -inf maxes),
+// so no slot is masked.
+__device__ __forceinline__ void union4(const float* src, uint32_t pitch, uint32_t first,
+                                       float out[6]) {
+#pragma unroll
+    for (int a = 0; a < 6; ++a) {
+        const float4 q = *reinterpret_cast<const float4*>(src + a * pitch + first);
+        out[a] = a < 3 ? fminf(fminf(q.x, q.y), fminf(q.z, q.w))
+                       : fmaxf(fmaxf(q.x, q.y), fmaxf(q.z, q.w));
+    }
+}
+
+__device__ __forceinline__ void put_box(float (*box)[kBoxPitch], uint32_t slot, const float b[6]) {
+#pragma unroll
+    for (int a = 0; a < 6; ++a) box[a][slot] = b[a];
+}
+
+// Node j of a level whose children are slots 4j..4j+3 of src, or the identity
+// box for a padding slot j >= count.
+__device__ __forceinline__ void level_box(const float* src, uint32_t pitch, uint32_t j,
+                                          uint32_t count, float b[6]) {
+    if (j < count) {
+        union4(src, pitch, 4 * j, b);
+    } else {
+        b[0] = b[1] = b[2] = INFINITY;
+        b[3] = b[4] = b[5] = -INFINITY;
+    }
+}
+
+// Every node box, from the triangle boxes in tbox (identity-padded to a
+// multiple of 4): leaves by all threads, the level above them (at most 256
+// nodes) by the first 4 waves, the rest (at most 64 nodes per level) by wave
+// 0 alone.  Each level is padded with identity boxes to a multiple of 4.
+// Ends with a barrier.
+__device__ void small_boxes(const SmallTree& T, SmallLds& S) {
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, w = tid >> 6;
+    const uint32_t lb = T.levels - 1;
+    {
+        const uint32_t c = pick(T.count, lb);
+        if (tid < ((c + 3u) & ~3u)) {
+            float b[6];
+            level_box(S.u.tbox[0], kSmallMax, tid, c, b);
+            put_box(S.box, pick(T.pad, lb) + tid, b);
+        }
+    }
+    lds_barrier();
+    if (lb == 0) return;
+    {
+        const uint32_t c = pick(T.count, lb - 1);
+        if (tid < ((c + 3u) & ~3u)) {
+            float b[6];
+            level_box(S.box[0] + pick(T.pad, lb), kBoxPitch, tid, c, b);
+            put_box(S.box, pick(T.pad, lb - 1) + tid, b);
+        }
+    }
+    lds_barrier();
+    if (lb == 1) return;
+    if (w == 0) {
+        for (int lv = (int)lb - 2; lv >= 0; --lv) {
+            wave_lds_sync();
+            const uint32_t c = pick(T.count, lv);
+            if (lane < ((c + 3u) & ~3u)) {
+                float b[6];
+                level_box(S.box[0] + pick(T.pad, lv + 1), kBoxPitch, lane, c, b);
+                put_box(S.box, pick(T.pad, lv) + lane, b);
+            }
+        }
+    }
+    lds_barrier();
+}
+
+// Chunk c (16 B) of node `node`: c < 6 one bound of the 4 slots (minX, maxX,
+// minY, maxY, minZ, maxZ: software_bvh.hpp:178-187 order), 6 child indices,
+// 7 child counts.  Leaf slot s = 4j + k: triangle bounds, child ~s, count 1;
+// inner slot: child bounds, child node index, count -1; empty slots zero.
+__device__ __forceinline__ uint4 node_chunk(const SmallTree& T, uint32_t n, uint32_t node,
+                                            uint32_t c, const SmallLds& S) {
+    const uint32_t lb = T.levels - 1;
+    uint32_t lv = 0;
+#pragma unroll
+    for (int q = 1; q < kSmallLevels; ++q)
+        if ((uint32_t)q < T.levels && node >= T.off[q]) lv = q;
+    const uint32_t j = node - pick(T.off, lv);
+    const bool leaf = lv == lb;
+    const uint32_t cnt = leaf ? n : pick(T.count, lv + 1);
+    const uint32_t next = pick(T.off, lv + 1);
+    uint32_t o[4] = {0u, 0u, 0u, 0u};
+    if (c < 6) {
+        const uint32_t comp = (c & 1u) * 3u + (c >> 1);  // 0,3,1,4,2,5
+        const float* src = leaf ? S.u.tbox[comp] + 4 * j : S.box[comp] + pick(T.pad, lv + 1) + 4 * j;
+        const float4 q = *reinterpret_cast<const float4*>(src);
+        o[0] = __float_as_uint(q.x); o[1] = __float_as_uint(q.y);
+        o[2] = __float_as_uint(q.z); o[3] = __float_as_uint(q.w);
+    }
+#pragma unroll
+    for (uint32_t k = 0; k < 4; ++k) {
+        const uint32_t ch = 4 * j + k;
+        if (ch >= cnt) o[k] = 0u;
+        else if (c == 6) o[k] = leaf ? ~ch : next + ch;
+        else if (c == 7) o[k] = leaf ? 1u : 0xFFFFFFFFu;
+    }
+    return make_uint4(o[0], o[1], o[2], o[3]);
+}
+
+// Output of the fused build and refit.  Owner item i (ok[i]): vertices v[i],
+// sorted position r[i], triangle index id[i], code key[i].  Every workgroup
+// computes all boxes; workgroup `part` of `parts` stores its share.
+__device__ void small_emit(SmallLds& S, const SmallTree& T, uint32_t n,
+                           const float (&v)[kSmallItems][9], const uint32_t (&r)[kSmallItems],
+                           const uint32_t (&id)[kSmallItems], const uint32_t (&key)[kSmallItems],
+                           const bool (&ok)[kSmallItems], rt_node4* __restrict__ nodes,
+                           rt_tri64* __restrict__ tris, uint32_t* __restrict__ ids_out,
+                           uint32_t* __restrict__ codes_out) {
+    const uint32_t tid = threadIdx.x, part = blockIdx.x, parts = gridDim.x;
+#pragma unroll
+    for (int i = 0; i < kSmallItems; ++i) {
+        if (!ok[i]) continue;
+        float bb[6];
+        tri_box(v[i], bb);
+#pragma unroll
+        for (int a = 0; a < 6; ++a) S.u.tbox[a][r[i]] = bb[a];
+    }
+    if (tid < ((n + 3u) & ~3u) - n) {  // identity boxes up to a multiple of 4
+#pragma unroll
+        for (int a = 0; a < 6; ++a) S.u.tbox[a][n + tid] = a < 3 ? INFINITY : -INFINITY;
+    }
+    lds_barrier();
+    small_boxes(T, S);
+    const uint32_t lb = T.levels - 1;
+    // this workgroup's slice of the node array, 16-B chunks
+    const uint32_t nchunks = (pick(T.off, lb) + pick(T.count, lb)) * 8;
+    const uint32_t per = (nchunks + parts - 1) / parts;
+    const uint32_t g0 = part * per, g1 = min(nchunks, g0 + per);
+    uint4* nd = reinterpret_cast<uint4*>(nodes);
+    for (uint32_t g = g0 + tid; g < g1; g += kSmallBlock) nd[g] = node_chunk(T, n, g >> 3, g & 7u, S);
+    lds_barrier();  // triangle bounds dead: the union stages records
+    const uint32_t base = part * kPartRecs;
+    if (base >= n) return;
+#pragma unroll
+    for (int i = 0; i < kSmallItems; ++i) {
+        if (!ok[i] || r[i] < base || r[i] - base >= kPartRecs) continue;
+        uint32_t* st = S.u.stage + (r[i] - base) * kStageWords;
+#pragma unroll
+        for (int a = 0; a < 9; ++a) st[a] = __float_as_uint(v[i][a]);
+        st[9] = id[i];
+        st[10] = key[i];
+    }
+    lds_barrier();
+    const uint32_t cnt = min(kPartRecs, n - base);
+    uint4* td = reinterpret_cast<uint4*>(tris + base);
+    for (uint32_t g = tid; g < 4 * cnt; g += kSmallBlock) {
+        const uint32_t* st = S.u.stage + (g >> 2) * kStageWords;
+        const uint32_t c = g & 3u;
+        uint4 o;
+        if (c == 0) o = make_uint4(st[0], st[1], st[2], st[3]);
+        else if (c == 1) o = make_uint4(st[4], st[5], st[6], st[7]);
+        else if (c == 2) o = make_uint4(st[8], 0u, 0u, 0u);
+        else o = make_uint4(st[9], 0x80000000u, 0u, 9u);  // store_tri's last 16 B
+        td[g] = o;
+    }
+    if (tid < cnt) {
+        if (ids_out) ids_out[base + tid] = S.u.stage[tid * kStageWords + 9];
+        if (codes_out) codes_out[base + tid] = S.u.stage[tid * kStageWords + 10];
+    }
+}
+
+__device__ __forceinline__ void small_info(rt_build_info* info, const SmallTree& T) {
+    if (info && blockIdx.x == 0 && threadIdx.x == 0) {
+        info->node_count = pick(T.off, T.levels - 1) + pick(T.count, T.levels - 1);
+        info->depth = T.levels;
+    }
+}
+
 __global__ __launch_bounds__(kSmallBlock) void k_build_small(
     const float* __restrict__ pos, const uint32_t* __restrict__ idx, uint32_t n,
     rt_node4* __restrict__ nodes, rt_tri64* __restrict__ tris, uint32_t* __restrict__ ids_out,
-    uint32_t* __restrict__ codes_out, LevelArgs L, rt_build_info* info) {
+    uint32_t* __restrict__ codes_out, SmallTree targ, rt_build_info* info) {
     __shared__ SmallLds S;
+    const SmallTree T = in_regs(targ);
     const uint32_t tid = threadIdx.x, lane = tid & 63u, w = tid >> 6;
-    int cur = 0;
-    if (info && tid == 0) {
-        info->node_count = L.offsets[L.levels - 1] + L.counts[L.levels - 1];
-        info->depth = L.levels;
+    small_info(info, T);
+    // bucket counters cleared while the triangle loads are in flight
+    static_assert(kBuckets == 8 * kSmallBlock, "8 bucket counters per thread");
+    uint4* start4 = reinterpret_cast<uint4*>(S.u.msd.start);
+    start4[2 * tid] = make_uint4(0u, 0u, 0u, 0u);
+    start4[2 * tid + 1] = make_uint4(0u, 0u, 0u, 0u);
+    S.spread[tid] = expand_bits10(tid);
+
+    float v[kSmallItems][9], cen[kSmallItems][3];
+    float mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+#pragma unroll
+    for (int i = 0; i < kSmallItems; ++i) {
+        const uint32_t p = w * 64 * kSmallItems + i * 64 + lane;
+        if (p < n) {
+            tri_verts(pos, idx, p, v[i]);
+            centroid(v[i], cen[i]);
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                mn[a] = fminf(mn[a], cen[i][a]);
+                mx[a] = fmaxf(mx[a], cen[i][a]);
+            }
+        }
     }
-    if (!REFIT) {
-        // centroids of triangles p = w*256 + i*64 + lane (warp-striped order)
-        float cen[kSmallItems][3];
-        float mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+    // bounds: the 1024 per-thread partials through LDS (the box arrays are
+    // free until the output), bound a reduced by wave a alone
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        S.box[a][tid] = mn[a];
+        S.box[3 + a][tid] = mx[a];
+    }
+    lds_barrier();
+    if (w < 6) {
+        const float4* src = reinterpret_cast<const float4*>(S.box[w] + 16 * lane);
+        float m = w < 3 ? INFINITY : -INFINITY;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const float4 x = src[q];
+            m = w < 3 ? fminf(m, fminf(fminf(x.x, x.y), fminf(x.z, x.w)))
+                      : fmaxf(m, fmaxf(fmaxf(x.x, x.y), fmaxf(x.z, x.w)));
+        }
+        m = w < 3 ? wave_min(m) : wave_max(m);
+        if (lane == 0) S.red[0][w] = m;
+    }
+    lds_barrier();
+    float lo[3], hi[3], sc[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        lo[a] = S.red[0][a];
+        hi[a] = S.red[0][3 + a];
+    }
+    axis_scale(lo, hi, sc);
+    uint32_t key[kSmallItems], rank[kSmallItems];
+#pragma unroll
+    for (int i = 0; i < kSmallItems; ++i) {
+        const uint32_t p = w * 64 * kSmallItems + i * 64 + lane;
+        uint32_t q[3];
+        quantize(cen[i], lo, sc, q);
+        key[i] = p < n ? (S.spread[q[0]] << 2) | (S.spread[q[1]] << 1) | S.spread[q[2]]
+                       : 0xFFFFFFFFu;
+        if (p < n) atomicAdd(&S.u.msd.start[key[i] >> kBucketShift], 1u);
+    }
+    lds_barrier();
+    // exclusive scan of the bucket counts (8 per thread) and the largest bucket
+    {
+        const uint4 c0 = start4[2 * tid], c1 = start4[2 * tid + 1];
+        const uint32_t c[8] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w};
+        uint32_t total = 0, big = 0;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            total += c[k];
+            big = max(big, c[k]);
+        }
+        const uint32_t inc = wave_incl_scan(total);
+        big = wave_max_u32(big);
+        if (lane == 63) S.wsum[w] = inc;
+        if (lane == 0) S.wmax[w] = big;
+        lds_barrier();
+        // waves before this one: lane l < 16 holds wave l's total
+        const uint32_t waves = wave_incl_scan(lane < 16 ? S.wsum[lane] : 0u);
+        uint32_t run = inc - total + (w ? (uint32_t)__builtin_amdgcn_readlane((int)waves, (int)w - 1) : 0u);
+        uint32_t e[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            e[k] = run;
+            run += c[k];
+        }
+        uint4* cursor4 = reinterpret_cast<uint4*>(S.u.msd.cursor);
+        start4[2 * tid] = cursor4[2 * tid] = make_uint4(e[0], e[1], e[2], e[3]);
+        start4[2 * tid + 1] = cursor4[2 * tid + 1] = make_uint4(e[4], e[5], e[6], e[7]);
+    }
+    const uint32_t largest = wave_max_u32(lane < 16 ? S.wmax[lane] : 0u);
+    lds_barrier();
+    if (largest <= kBucketCap) {
 #pragma unroll
         for (int i = 0; i < kSmallItems; ++i) {
             const uint32_t p = w * 64 * kSmallItems + i * 64 + lane;
             if (p < n) {
-                float v[9];
-                tri_verts(pos, idx, p, v);
-                centroid(v, cen[i]);
-#pragma unroll
-                for (int a = 0; a < 3; ++a) {
-                    mn[a] = fminf(mn[a], cen[i][a]);
-                    mx[a] = fmaxf(mx[a], cen[i][a]);
-                }
+                const uint32_t slot = atomicAdd(&S.u.msd.cursor[key[i] >> kBucketShift], 1u);
+                S.u.msd.comp[slot] = ((key[i] & ((1u << kBucketShift) - 1)) << 12) | p;
             }
         }
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            float a0 = wave_min(mn[a]), a1 = wave_max(mx[a]);
-            if (lane == 0) { S.red[w][a] = a0; S.red[w][3 + a] = a1; }
-        }
-        __syncthreads();
-        float lo[3], hi[3], s[3];
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            lo[a] = S.red[0][a];
-            hi[a] = S.red[0][3 + a];
-            for (int ww = 1; ww < 16; ++ww) {
-                lo[a] = fminf(lo[a], S.red[ww][a]);
-                hi[a] = fmaxf(hi[a], S.red[ww][3 + a]);
-            }
-        }
-        axis_scale(lo, hi, s);
-        uint32_t key[kSmallItems], val[kSmallItems], dig[kSmallItems], rk[kSmallItems];
+        lds_barrier();
+        // rank = bucket start + members below; the 4 items' scans interleaved
+        uint32_t me[kSmallItems], first[kSmallItems], len[kSmallItems], most = 0;
 #pragma unroll
         for (int i = 0; i < kSmallItems; ++i) {
             const uint32_t p = w * 64 * kSmallItems + i * 64 + lane;
-            key[i] = p < n ? morton(cen[i], lo, s) : 0xFFFFFFFFu;
-            val[i] = p;
+            len[i] = 0;
+            first[i] = 0;
+            me[i] = 0;
+            if (p < n) {
+                const uint32_t bkt = key[i] >> kBucketShift;
+                first[i] = S.u.msd.start[bkt];
+                len[i] = S.u.msd.cursor[bkt] - first[i];
+                me[i] = ((key[i] & ((1u << kBucketShift) - 1)) << 12) | p;
+                most = max(most, len[i]);
+            }
+            rank[i] = first[i];
+        }
+        for (uint32_t j = 0; j < most; ++j) {
+#pragma unroll
+            for (int i = 0; i < kSmallItems; ++i)
+                if (j < len[i]) rank[i] += S.u.msd.comp[first[i] + j] < me[i] ? 1u : 0u;
+        }
+    } else {
+        // LSD: 4 stable passes of 8 bits; padding keys (p >= n) sort last
+        uint32_t k2[kSmallItems], val[kSmallItems], dig[kSmallItems], rk[kSmallItems];
+        int cur = 0;
+#pragma unroll
+        for (int i = 0; i < kSmallItems; ++i) {
+            k2[i] = key[i];
+            val[i] = w * 64 * kSmallItems + i * 64 + lane;
         }
         for (uint32_t shift = 0; shift < 32; shift += 8) {
             if (shift > 0) {
 #pragma unroll
                 for (int i = 0; i < kSmallItems; ++i) {
                     const uint32_t p = w * 64 * kSmallItems + i * 64 + lane;
-                    key[i] = S.key[cur][p];
-                    val[i] = S.val[cur][p];
+                    k2[i] = S.u.lsd.key[cur][p];
+                    val[i] = S.u.lsd.val[cur][p];
                 }
                 cur ^= 1;
             }
 #pragma unroll
-            for (int i = 0; i < kSmallItems; ++i) dig[i] = (key[i] >> shift) & 0xFFu;
-            block_rank<kSmallBlock, kSmallItems>(dig, rk, S.cnt, S.dstart, S.wsum);
+            for (int i = 0; i < kSmallItems; ++i) dig[i] = (k2[i] >> shift) & 0xFFu;
+            block_rank<kSmallBlock, kSmallItems>(dig, rk, S.u.lsd.cnt, S.dstart, S.wsum);
             const int dst = shift == 0 ? 0 : cur;
 #pragma unroll
             for (int i = 0; i < kSmallItems; ++i) {
-                S.key[dst][rk[i]] = key[i];
-                S.val[dst][rk[i]] = val[i];
+                S.u.lsd.key[dst][rk[i]] = k2[i];
+                S.u.lsd.val[dst][rk[i]] = val[i];
             }
             cur = dst;
-            __syncthreads();
+            lds_barrier();
         }
-        if (ids_out || codes_out) {
-            for (uint32_t p = tid; p < n; p += kSmallBlock) {
-                if (ids_out) ids_out[p] = S.val[cur][p];
-                if (codes_out) codes_out[p] = S.key[cur][p];
-            }
-        }
-    }
-    // leaves + bottom-up levels, all boxes in LDS
-    const uint32_t lb = L.levels - 1;
-    if (tid < L.counts[lb]) {
-        float box[6];
-        emit_leaf_box<REFIT>(pos, idx, n, REFIT ? nullptr : S.val[cur], nodes, tris,
-                             L.offsets[lb], tid, box);
-        for (int a = 0; a < 6; ++a) S.box[6 * (L.offsets[lb] + tid) + a] = box[a];
-    }
-    for (int lv = (int)lb - 1; lv >= 0; --lv) {
-        __syncthreads();
-        if (tid < L.counts[lv]) {
-            float box[6];
-            emit_inner(nodes, L, lv, tid, S.box + 6 * L.offsets[lv + 1], 0, box);
-            for (int a = 0; a < 6; ++a) S.box[6 * (L.offsets[lv] + tid) + a] = box[a];
+        // rank of triangle p: invert the sorted index array into key[cur]
+        for (uint32_t q = tid; q < n; q += kSmallBlock) S.u.lsd.key[cur][S.u.lsd.val[cur][q]] = q;
+        lds_barrier();
+#pragma unroll
+        for (int i = 0; i < kSmallItems; ++i) {
+            const uint32_t p = w * 64 * kSmallItems + i * 64 + lane;
+            rank[i] = p < n ? S.u.lsd.key[cur][p] : 0u;
         }
     }
+    lds_barrier();  // sort arrays dead from here
+    uint32_t id[kSmallItems];
+    bool ok[kSmallItems];
+#pragma unroll
+    for (int i = 0; i < kSmallItems; ++i) {
+        id[i] = w * 64 * kSmallItems + i * 64 + lane;
+        ok[i] = id[i] < n;
+    }
+    small_emit(S, T, n, v, rank, id, key, ok, nodes, tris, ids_out, codes_out);
+}
+
+// Refit of a fused-path tree: sorted position s = i*1024 + t keeps its
+// triangle (the index in its record); vertices are re-read and every box
+// and record rewritten through the same output path.  Every workgroup reads
+// every record's triangle index while others rewrite their slices; the
+// index is rewritten with its own value, so a read sees it either way.
+__global__ __launch_bounds__(kSmallBlock) void k_refit_small(
+    const float* __restrict__ pos, const uint32_t* __restrict__ idx, uint32_t n,
+    rt_node4* __restrict__ nodes, rt_tri64* __restrict__ tris, SmallTree targ,
+    rt_build_info* info) {
+    __shared__ SmallLds S;
+    const SmallTree T = in_regs(targ);
+    small_info(info, T);
+    float v[kSmallItems][9];
+    uint32_t r[kSmallItems], id[kSmallItems], key[kSmallItems];
+    bool ok[kSmallItems];
+#pragma unroll
+    for (int i = 0; i < kSmallItems; ++i) {
+        r[i] = i * kSmallBlock + threadIdx.x;
+        ok[i] = r[i] < n;
+        id[i] = ok[i] ? tris[r[i]].triangle_id : 0u;
+        key[i] = 0;
+    }
+#pragma unroll
+    for (int i = 0; i < kSmallItems; ++i)
+        if (ok[i]) tri_verts(pos, idx, id[i], v[i]);
+    small_emit(S, T, n, v, r, id, key, ok, nodes, tris, nullptr, nullptr);
 }
 
 // ---------------------------------------------------------------------------
@@ -626,6 +1019,7 @@ struct RadixArrays {
     uint32_t* done;        // m: arrivals at a straddling node (k_rt_top)
     uint32_t* top;         // straddling nodes (range crosses a kWindow boundary)
     uint32_t* top_count;   // 1 word
+    uint32_t* starts;      // n / kWindow + 1: k_rt_top climb starts
     int4* cand;            // m, -1 = empty
     uint32_t* size4;       // m: Box4 nodes in the subtree rooted here
     uint32_t n, m, maxp;
@@ -664,17 +1058,12 @@ __global__ __launch_bounds__(256) void k_rt_leaves(const float* __restrict__ pos
     const uint32_t id = sorted[p];
     float v[9];
     tri_verts(pos, idx, id, v);
-    float4* tp = reinterpret_cast<float4*>(tris + p);
-    tp[0] = make_float4(v[0], v[1], v[2], v[3]);
-    tp[1] = make_float4(v[4], v[5], v[6], v[7]);
-    tp[2] = make_float4(v[8], 0.f, 0.f, 0.f);
-    reinterpret_cast<uint4*>(tp)[3] = make_uint4(id, 0x80000000u, 0u, 9u);
+    store_tri(tris + p, v, id);
+    float bb[6];
+    tri_box(v, bb);
     float* b = box + 6 * (size_t)(m + p);
 #pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        b[a] = fminf(v[a], fminf(v[3 + a], v[6 + a])) + 0.0f;
-        b[3 + a] = fmaxf(v[a], fmaxf(v[3 + a], v[6 + a])) + 0.0f;
-    }
+    for (int a = 0; a < 6; ++a) b[a] = bb[a];
 }
 
 // Internal node i (Karras 2012, Alg. 1 with the position tie-break).
@@ -798,18 +1187,28 @@ __global__ __launch_bounds__(1024) void k_rt_bottom_up(RadixArrays A) {
 // Pass 2, one workgroup: the straddling nodes (at most depth per window
 // boundary; their parents straddle too).  A child that does not straddle was
 // finished by pass 1 and counts as an arrival up front; a straddling child
-// arrives when its own climb finishes it.  The second arrival finishes the
-// node and climbs on.  The arrival counters are in global memory but every
+// arrives when its own climb finishes it; the second arrival finishes the
+// node and climbs on.  Climbs start at the nodes whose children both do not
+// straddle (one per boundary at most), gathered first so that no thread
+// holds two of them.  The arrival counters are in global memory, but every
 // access is from this one workgroup, so workgroup scope suffices.
 __global__ __launch_bounds__(1024) void k_rt_top(RadixArrays A) {
+    __shared__ uint32_t nstart;
     const uint32_t cnt = *A.top_count;
+    if (threadIdx.x == 0) nstart = 0;
+    __syncthreads();
     for (uint32_t i = threadIdx.x; i < cnt; i += 1024) {
-        int32_t x = (int32_t)A.top[i];
+        const int32_t x = (int32_t)A.top[i];
         const int ready = (straddles(A, A.left[x]) ? 0 : 1) + (straddles(A, A.right[x]) ? 0 : 1);
-        if (ready == 0) continue;
-        if (ready == 1 && __hip_atomic_fetch_add(A.done + x, 1u, __ATOMIC_ACQ_REL,
-                                                 __HIP_MEMORY_SCOPE_WORKGROUP) == 0)
-            continue;
+        if (ready == 1)
+            A.done[x] = 1;  // no climb has started yet: plain store
+        else if (ready == 2)
+            A.starts[atomicAdd(&nstart, 1u)] = (uint32_t)x;
+    }
+    __syncthreads();
+    const uint32_t ns = nstart;
+    for (uint32_t j = threadIdx.x; j < ns; j += 1024) {
+        int32_t x = (int32_t)A.starts[j];
         for (;;) {
             finish_node(A, x);
             x = A.parent[x];
@@ -905,10 +1304,27 @@ LevelArgs level_args(const TreeLayout& t) {
     return a;
 }
 
+// Level table of the fused path (t.levels <= 6 for n <= kSmallMax); unused
+// entries stay zero.
+SmallTree small_tree(const TreeLayout& t) {
+    SmallTree a = {};
+    a.levels = t.levels;
+    uint32_t pad = 0;
+    for (uint32_t i = 0; i < t.levels && i < (uint32_t)kSmallLevels; ++i) {
+        a.count[i] = t.counts[i];
+        a.off[i] = t.offsets[i];
+        a.pad[i] = pad;
+        pad += (t.counts[i] + 3u) & ~3u;
+    }
+    return a;
+}
+static_assert(kSmallMax == 4096 && kBoxPitch == 4 + 4 + 16 + 64 + 256 + 1024,
+              "LDS box pitch covers the padded levels of a 4096-triangle tree");
+
 struct ScratchPlan {
     size_t bounds = 0, keys0 = 0, vals0 = 0, keys1 = 0, vals1 = 0, hist = 0, totals = 0,
            boxes = 0, rbox = 0, rf = 0, rl = 0, left = 0, right = 0, parent = 0, done = 0,
-           top = 0, top_count = 0, cand = 0, size4 = 0, total = 0;
+           top = 0, top_count = 0, starts = 0, cand = 0, size4 = 0, total = 0;
     uint32_t tiles = 0;
 };
 
@@ -939,6 +1355,7 @@ ScratchPlan plan_scratch(uint32_t n, uint32_t algo) {
         p.done = off; off += al(m * 4);
         p.top = off; off += al(m * 4);
         p.top_count = off; off += al(4);
+        p.starts = off; off += al(((size_t)n / kWindow + 1) * 4);
         p.cand = off; off += al(m * 16);
         p.size4 = off; off += al(m * 4);
     }
@@ -1016,6 +1433,7 @@ rt_status build_radix(hipStream_t s, const float* pos, const uint32_t* idx, uint
     A.done = reinterpret_cast<uint32_t*>(base + p.done);
     A.top = reinterpret_cast<uint32_t*>(base + p.top);
     A.top_count = reinterpret_cast<uint32_t*>(base + p.top_count);
+    A.starts = reinterpret_cast<uint32_t*>(base + p.starts);
     A.cand = reinterpret_cast<int4*>(base + p.cand);
     A.size4 = reinterpret_cast<uint32_t*>(base + p.size4);
     A.n = n;
@@ -1114,8 +1532,9 @@ extern "C" rt_status rt_gpu_build(rt_stream stream, const float* d_pos, const ui
     if (node_capacity < max_nodes(n, algo)) return RT_ERR_CAPACITY;
     const TreeLayout t = tree_layout(n);
     if (algo == RT_BUILD_IMPLICIT && n <= kSmallMax) {
-        hipLaunchKernelGGL((k_build_small<false>), dim3(1), dim3(kSmallBlock), 0, s, d_pos, d_idx,
-                           n, d_nodes, d_tris, d_sorted_ids, d_codes, level_args(t), d_info);
+        hipLaunchKernelGGL(k_build_small, dim3(ceil_div(n, kPartRecs)), dim3(kSmallBlock), 0, s,
+                           d_pos, d_idx, n, d_nodes, d_tris, d_sorted_ids, d_codes,
+                           small_tree(t), d_info);
         return launch_status("k_build_small");
     }
     const ScratchPlan p = plan_scratch(n, algo);
@@ -1146,9 +1565,9 @@ extern "C" rt_status rt_gpu_refit(rt_stream stream, const float* d_pos, const ui
     if (node_count < t.total_boxes) return RT_ERR_CAPACITY;
     hipStream_t s = as_stream(stream);
     if (n <= kSmallMax) {
-        hipLaunchKernelGGL((k_build_small<true>), dim3(1), dim3(kSmallBlock), 0, s, d_pos, d_idx,
-                           n, d_nodes, d_tris, nullptr, nullptr, level_args(t), nullptr);
-        return launch_status("k_build_small");
+        hipLaunchKernelGGL(k_refit_small, dim3(ceil_div(n, kPartRecs)), dim3(kSmallBlock), 0, s,
+                           d_pos, d_idx, n, d_nodes, d_tris, small_tree(t), nullptr);
+        return launch_status("k_refit_small");
     }
     const ScratchPlan p = plan_scratch(n, RT_BUILD_IMPLICIT);
     if (!d_scratch || scratch_bytes < p.total) return RT_ERR_CAPACITY;

@@ -89,6 +89,36 @@ __device__ __forceinline__ bool mt_hit(float v0x, float v0y, float v0z, float v1
     return true;
 }
 
+// Workgroup barrier that orders LDS only.  __syncthreads() also waits for
+// every outstanding global store (gfx9 counts loads and stores on one
+// counter), which a kernel that never reads its own global writes back does
+// not need.
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// Orders this wave's LDS accesses before and after (a wave runs its LDS
+// instructions in order; this stops the compiler from moving them).
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
+}
+
+// x / 3 rounded to nearest without the division sequence: q = x * RN(1/3),
+// one residual step with the exact FMA residual x - 3q, sign taken from x
+// (the step turns -0 into +0).  Bit-identical to IEEE x / 3.0f for every
+// finite float (all 2^32 patterns checked on the host; tests repeat a
+// sample of that check).
+__host__ __device__ inline float div3(float x) {
+    const float y = 0x1.555556p-2f;  // RN(1/3)
+    const float q = x * y;
+    const float r = fmaf(-3.0f, q, x);
+    return copysignf(fmaf(r, y, q), x);
+}
+
 // Float <-> order-preserving uint (for atomicMin / atomicMax on bounds).
 __device__ __forceinline__ uint32_t f2o(float f) {
     uint32_t b = __float_as_uint(f);

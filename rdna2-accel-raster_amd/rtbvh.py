@@ -247,7 +247,7 @@ class GpuBVH4:
         cap, need = build_requirements(n, cfg)
         nodes = torch.empty(max(cap, 1) * NODE_BYTES, dtype=torch.uint8, device=dev)
         tris = torch.empty(max(n, 1) * TRI_BYTES, dtype=torch.uint8, device=dev)
-        info = torch.zeros(4, dtype=torch.int32, device=dev)
+        info = torch.empty(4, dtype=torch.int32, device=dev)  # every build path writes it
         bvh = GpuBVH4(nodes, tris, cap, n, 0)
         bvh.algorithm = algorithm
         ids = codes = None
