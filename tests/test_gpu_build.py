@@ -113,8 +113,10 @@ def test_refit_matches_rebuild_layout(rt, O):
     np.testing.assert_array_equal(to_np(bvh.tris)[: n * 64], tris_r.view(np.uint8).reshape(-1))
 
 
-def test_small_refit(rt, O):
-    n = 4096
+@pytest.mark.parametrize("n", [4096, 3002, 130])
+def test_small_refit(rt, O, n):
+    """Fused-path refit (one workgroup per 256 sorted positions, partial last
+    one included): nodes and records equal the oracle's refit."""
     xyz0, _ = O.dynamic_grid(n, 0.0, 1.25)
     xyz1, _ = O.dynamic_grid(n, 1.1, 1.25)
     bvh = rt.GpuBVH4.build(torch.from_numpy(xyz0).cuda(), None, algorithm="implicit")
@@ -124,6 +126,28 @@ def test_small_refit(rt, O):
     torch.cuda.synchronize()
     np.testing.assert_array_equal(to_np(bvh.nodes)[: bvh.node_count * 128],
                                   nodes_r.view(np.uint8).reshape(-1))
+    np.testing.assert_array_equal(to_np(bvh.tris)[: n * 64], tris_r.view(np.uint8).reshape(-1))
+
+
+@pytest.mark.parametrize("nu,nv", [(64, 32), (40, 20), (7, 3)])
+def test_small_path_indexed_sphere(rt, O, nu, nv):
+    """Indexed UV sphere: thin pole triangles crowd a few Morton buckets (up
+    to 35 of 4096 triangles share the top 13 code bits), so ranks inside a
+    bucket decide the order."""
+    pos, idx = O.sphere(nu, nv, 1.0, 0.02, 5)
+    _cmp_build(rt, O, pos, idx)
+
+
+@pytest.mark.parametrize("dups", [300, 4096])
+def test_small_path_clustered_fallback(rt, O, dups):
+    """More than 256 triangles in one bucket sends the fused build to its LSD
+    sort: `dups` copies of one triangle (equal codes: index order decides)
+    among a grid, and a mesh of 4096 identical triangles."""
+    xyz, _ = O.dynamic_grid(4096, 0.2, 1.25)
+    xyz = xyz.copy()
+    pick = np.random.default_rng(3).choice(4096, size=dups, replace=False)
+    xyz[pick] = xyz[17]
+    _cmp_build(rt, O, xyz, None)
 
 
 def test_fast_builder_verification_rays(rt, O):
