@@ -196,6 +196,30 @@ rt_status rt_trace_ex(rt_stream stream, const rt_bvh_dev* bvh, const rt_rays_soa
                       uint32_t n, uint32_t flags, const rt_trace_opts* opts, rt_hits_soa* hits,
                       uint32_t* occl_bits, uint32_t* box_tests, uint32_t* tri_tests);
 
+/* Batched trace: many (tree, ray set, mode) items in ONE launch, each with
+ * the results rt_trace gives it alone.  No reference counterpart (the
+ * reference dispatches one ray set per submit, microbench_main.cpp:344);
+ * this is the batching path over the same rt_trace contract.
+ *   1. rt_trace_batch_plan (host): checks the items and fills
+ *      chunk_start[0..count] (item i owns the 64-ray chunks
+ *      [chunk_start[i], chunk_start[i+1])) and the deepest tree.
+ *   2. the caller copies items[] and chunk_start[] to device memory;
+ *   3. rt_trace_batch launches over those device copies (one kernel, no
+ *      host work, no allocation: capturable in a graph).                    */
+typedef struct rt_trace_item {
+    rt_bvh_dev bvh;
+    rt_rays_soa rays;
+    uint32_t count;           /* rays */
+    uint32_t flags;           /* RT_CLOSEST or RT_ANY */
+    rt_hits_soa hits;         /* closest: prim and t; any: prim optional (NULL) */
+    uint32_t* occlusion_bits; /* any: ceil(count / 32) words */
+} rt_trace_item;
+rt_status rt_trace_batch_plan(const rt_trace_item* items, uint32_t count, uint32_t* chunk_start,
+                              uint32_t* depth);
+rt_status rt_trace_batch(rt_stream stream, const rt_trace_item* d_items,
+                         const uint32_t* d_chunk_start, uint32_t count, uint32_t total_chunks,
+                         uint32_t depth);
+
 /* ---- generators (synthetic inputs made on the device) -------------------- */
 rt_status rt_gen_sphere(rt_stream stream, uint32_t nu, uint32_t nv, float radius, float disp,
                         uint32_t seed, float* d_pos, uint32_t* d_idx);

@@ -38,6 +38,34 @@ __device__ __forceinline__ void cx(Child& a, Child& b) {
     }
 }
 
+// Global-address-space views.  Kernel pointer arguments are global to the
+// compiler, but pointers read from memory (the batch items) are generic, and
+// generic accesses compile to flat loads (which also wait on the LDS
+// counter); the traversal is written over either pointer kind.
+#define RT_GLOBAL __attribute__((address_space(1)))
+__device__ __forceinline__ const float4* f4(const rt_node4* p) {
+    return reinterpret_cast<const float4*>(p);
+}
+__device__ __forceinline__ const RT_GLOBAL float4* f4(const RT_GLOBAL rt_node4* p) {
+    return reinterpret_cast<const RT_GLOBAL float4*>(p);
+}
+__device__ __forceinline__ const float4* f4(const rt_tri64* p) {
+    return reinterpret_cast<const float4*>(p);
+}
+__device__ __forceinline__ const RT_GLOBAL float4* f4(const RT_GLOBAL rt_tri64* p) {
+    return reinterpret_cast<const RT_GLOBAL float4*>(p);
+}
+__device__ __forceinline__ const int4* i4(const rt_node4* p) {
+    return reinterpret_cast<const int4*>(p);
+}
+__device__ __forceinline__ const RT_GLOBAL int4* i4(const RT_GLOBAL rt_node4* p) {
+    return reinterpret_cast<const RT_GLOBAL int4*>(p);
+}
+template <typename T>
+__device__ __forceinline__ RT_GLOBAL T* glb(T* p) {
+    return (RT_GLOBAL T*)p;
+}
+
 // The stack pointer is a separate local (a register); only the overflow
 // array lives in private memory (keeping both in one aggregate would put the
 // pointer in scratch too).
@@ -69,13 +97,13 @@ struct Stack {
     } while (0)
 
 // Leaf range [first, first + cnt): Moller-Trumbore on each 64-B record.
-template <bool ANY>
-__device__ __forceinline__ bool test_leaf(const rt_tri64* __restrict__ tris, uint32_t first,
+template <bool ANY, typename TP>
+__device__ __forceinline__ bool test_leaf(TP __restrict__ tris, uint32_t first,
                                           uint32_t cnt, float ox, float oy, float oz, float dx,
                                           float dy, float dz, float tmin, float& hitT,
                                           uint32_t& hitPrim) {
     for (uint32_t q = 0; q < cnt; ++q) {
-        const float4* tp = reinterpret_cast<const float4*>(tris + first + q);
+        const auto tp = f4(tris + first + q);
         const float4 a = tp[0], b = tp[1], c = tp[2];
         float t;
         if (!mt_hit(a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w, c.x, ox, oy, oz, dx, dy, dz, &t))
@@ -89,47 +117,32 @@ __device__ __forceinline__ bool test_leaf(const rt_tri64* __restrict__ tris, uin
     return false;
 }
 
-template <int PRIV, bool ANY, bool COUNT>
-__global__ __launch_bounds__(kBlock) void k_trace(const rt_node4* __restrict__ nodes,
-                                                  const rt_tri64* __restrict__ tris,
-                                                  uint32_t node_count, rt_rays_soa rays,
-                                                  uint32_t n, uint32_t* __restrict__ prim_out,
-                                                  float* __restrict__ t_out,
-                                                  uint32_t* __restrict__ bits_out,
-                                                  uint32_t* __restrict__ box_out,
-                                                  uint32_t* __restrict__ tri_out) {
-    __shared__ int32_t lds_stack[kLdsStack * kBlock];
-    const uint32_t tid = threadIdx.x;
-    const uint32_t r = blockIdx.x * kBlock + tid;
-    const bool valid = r < n;
-
-    float ox = 0.0f, oy = 0.0f, oz = 0.0f, tmin = 1.0f;
-    float dx = 0.0f, dy = 0.0f, dz = 1.0f, tmax = 0.0f;
-    if (valid) {
-        ox = rays.ox[r]; oy = rays.oy[r]; oz = rays.oz[r]; tmin = rays.tmin[r];
-        dx = rays.dx[r]; dy = rays.dy[r]; dz = rays.dz[r]; tmax = rays.tmax[r];
-    }
+// One ray's traversal (the body shared by k_trace and k_trace_batch).  lds
+// is the block's LDS stack array; the thread uses column tid.  Returns with
+// hitT / hitPrim (closest: nearest accepted hit; any: first accepted hit).
+template <int PRIV, bool ANY, bool COUNT, typename NP, typename TP>
+__device__ __forceinline__ void trace_ray(NP __restrict__ nodes, TP __restrict__ tris, bool start,
+                                          float ox, float oy, float oz, float tmin, float dx,
+                                          float dy, float dz, int32_t* lds_stack, uint32_t tid,
+                                          float& hitT, uint32_t& hitPrim, uint32_t& boxTests,
+                                          uint32_t& triTests) {
     // software_bvh_bvh4.comp:96-100
     const float ix = fabsf(dx) > 1e-6f ? 1.0f / dx : 1e6f;
     const float iy = fabsf(dy) > 1e-6f ? 1.0f / dy : 1e6f;
     const float iz = fabsf(dz) > 1e-6f ? 1.0f / dz : 1e6f;
-
-    float hitT = tmax;
-    uint32_t hitPrim = RT_MISS;
-    uint32_t boxTests = 0, triTests = 0;
     bool done = false;
 
     Stack<PRIV> st;
     int sp = 0;
-    if (valid && node_count > 0) RT_SPUSH(0);
+    if (start) RT_SPUSH(0);
 
     while (sp > 0) {
         int32_t ni;
         RT_SPOP(ni);
-        const float4* np = reinterpret_cast<const float4*>(nodes + ni);
+        const auto np = f4(nodes + ni);
         const float4 mnx = np[0], mxx = np[1], mny = np[2], mxy = np[3], mnz = np[4], mxz = np[5];
-        const int4 ch = reinterpret_cast<const int4*>(np)[6];
-        const int4 cn = reinterpret_cast<const int4*>(np)[7];
+        const int4 ch = i4(nodes + ni)[6];
+        const int4 cn = i4(nodes + ni)[7];
         if (COUNT) boxTests += 4;
 
         const float bminx[4] = {mnx.x, mnx.y, mnx.z, mnx.w};
@@ -184,13 +197,45 @@ __global__ __launch_bounds__(kBlock) void k_trace(const rt_node4* __restrict__ n
         }
         if (ANY && done) break;
     }
+}
 
+// Any-hit bits of a wave's 64 rays r0..r0+63 (r0 a multiple of 64): lane 0
+// writes the first word, lane 32 the second, each if it holds a ray < n.
+__device__ __forceinline__ void write_wave_bits(uint32_t* bits_out, uint32_t r0, uint32_t n,
+                                                bool occluded) {
+    const uint64_t m = __ballot(occluded);
+    const uint32_t lane = threadIdx.x & 63u;
+    if (lane == 0 && r0 < n) bits_out[r0 >> 5] = (uint32_t)m;
+    if (lane == 32 && r0 + 32 < n) bits_out[(r0 >> 5) + 1] = (uint32_t)(m >> 32);
+}
+
+template <int PRIV, bool ANY, bool COUNT>
+__global__ __launch_bounds__(kBlock) void k_trace(const rt_node4* __restrict__ nodes,
+                                                  const rt_tri64* __restrict__ tris,
+                                                  uint32_t node_count, rt_rays_soa rays,
+                                                  uint32_t n, uint32_t* __restrict__ prim_out,
+                                                  float* __restrict__ t_out,
+                                                  uint32_t* __restrict__ bits_out,
+                                                  uint32_t* __restrict__ box_out,
+                                                  uint32_t* __restrict__ tri_out) {
+    __shared__ int32_t lds_stack[kLdsStack * kBlock];
+    const uint32_t tid = threadIdx.x;
+    const uint32_t r = blockIdx.x * kBlock + tid;
+    const bool valid = r < n;
+
+    float ox = 0.0f, oy = 0.0f, oz = 0.0f, tmin = 1.0f;
+    float dx = 0.0f, dy = 0.0f, dz = 1.0f, tmax = 0.0f;
+    if (valid) {
+        ox = rays.ox[r]; oy = rays.oy[r]; oz = rays.oz[r]; tmin = rays.tmin[r];
+        dx = rays.dx[r]; dy = rays.dy[r]; dz = rays.dz[r]; tmax = rays.tmax[r];
+    }
+    float hitT = tmax;
+    uint32_t hitPrim = RT_MISS;
+    uint32_t boxTests = 0, triTests = 0;
+    trace_ray<PRIV, ANY, COUNT>(nodes, tris, valid && node_count > 0, ox, oy, oz, tmin, dx, dy,
+                                dz, lds_stack, tid, hitT, hitPrim, boxTests, triTests);
     if (ANY) {
-        const uint64_t m = __ballot(valid && hitPrim != RT_MISS);
-        const uint32_t lane = tid & 63u;
-        const uint32_t r0 = r - lane;
-        if (lane == 0 && r0 < n) bits_out[r0 >> 5] = (uint32_t)m;
-        if (lane == 32 && r0 + 32 < n) bits_out[(r0 >> 5) + 1] = (uint32_t)(m >> 32);
+        write_wave_bits(bits_out, r - (tid & 63u), n, valid && hitPrim != RT_MISS);
         if (valid && prim_out) prim_out[r] = hitPrim;
     } else if (valid) {
         prim_out[r] = hitPrim;
@@ -199,6 +244,58 @@ __global__ __launch_bounds__(kBlock) void k_trace(const rt_node4* __restrict__ n
     if (COUNT && valid) {
         box_out[r] = boxTests;
         tri_out[r] = triTests;
+    }
+}
+
+// Many (tree, rays, mode) items in one launch.  Item i owns the 64-ray chunks
+// [chunk[i], chunk[i+1]) of the launch, so each wave has one item: its tree,
+// rays and mode are wave-uniform (found by a binary search over chunk[]).
+template <int PRIV>
+__global__ __launch_bounds__(kBlock) void k_trace_batch(const rt_trace_item* __restrict__ items,
+                                                        const uint32_t* __restrict__ chunk,
+                                                        uint32_t count, uint32_t chunks) {
+    __shared__ int32_t lds_stack[kLdsStack * kBlock];
+    const uint32_t tid = threadIdx.x, lane = tid & 63u;
+    const uint32_t c = __builtin_amdgcn_readfirstlane(blockIdx.x * (kBlock / 64) + (tid >> 6));
+    if (c >= chunks) return;
+    uint32_t lo = 0, hi = count;  // last item with chunk[item] <= c
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (chunk[mid] <= c) lo = mid;
+        else hi = mid;
+    }
+    const rt_trace_item& it = items[lo];
+    const uint32_t n = it.count;
+    const uint32_t r0 = (c - chunk[lo]) * 64u;
+    const uint32_t r = r0 + lane;
+    const bool valid = r < n;
+    const rt_rays_soa& rays = it.rays;
+    float ox = 0.0f, oy = 0.0f, oz = 0.0f, tmin = 1.0f;
+    float dx = 0.0f, dy = 0.0f, dz = 1.0f, tmax = 0.0f;
+    if (valid) {
+        ox = glb(rays.ox)[r]; oy = glb(rays.oy)[r]; oz = glb(rays.oz)[r];
+        tmin = glb(rays.tmin)[r];
+        dx = glb(rays.dx)[r]; dy = glb(rays.dy)[r]; dz = glb(rays.dz)[r];
+        tmax = glb(rays.tmax)[r];
+    }
+    float hitT = tmax;
+    uint32_t hitPrim = RT_MISS;
+    uint32_t boxTests = 0, triTests = 0;
+    const bool start = valid && it.bvh.node_count > 0;
+    const auto nodes = glb(it.bvh.nodes);
+    const auto tris = glb(it.bvh.tris);
+    if (it.flags & RT_ANY) {
+        trace_ray<PRIV, true, false>(nodes, tris, start, ox, oy, oz, tmin, dx, dy, dz, lds_stack,
+                                     tid, hitT, hitPrim, boxTests, triTests);
+        write_wave_bits(glb(it.occlusion_bits), r0, n, valid && hitPrim != RT_MISS);
+        if (valid && it.hits.prim) glb(it.hits.prim)[r] = hitPrim;
+    } else {
+        trace_ray<PRIV, false, false>(nodes, tris, start, ox, oy, oz, tmin, dx, dy, dz, lds_stack,
+                                      tid, hitT, hitPrim, boxTests, triTests);
+        if (valid) {
+            glb(it.hits.prim)[r] = hitPrim;
+            glb(it.hits.t)[r] = hitT;
+        }
     }
 }
 
@@ -427,4 +524,49 @@ extern "C" rt_status rt_trace(rt_stream stream, const rt_bvh_dev* bvh, const rt_
                               uint32_t* box_tests, uint32_t* tri_tests) {
     return rt_trace_ex(stream, bvh, rays, n, flags, nullptr, hits, occl_bits, box_tests,
                        tri_tests);
+}
+
+extern "C" rt_status rt_trace_batch_plan(const rt_trace_item* items, uint32_t count,
+                                         uint32_t* chunk_start, uint32_t* depth) {
+    if ((count && !items) || !chunk_start) return RT_ERR_ARG;
+    uint64_t c = 0;
+    uint32_t deep = 1;
+    for (uint32_t i = 0; i < count; ++i) {
+        const rt_trace_item& it = items[i];
+        if (it.flags & ~RT_ANY) return RT_ERR_ARG;
+        if (it.count) {
+            const rt_rays_soa& r = it.rays;
+            if (!r.ox || !r.oy || !r.oz || !r.tmin || !r.dx || !r.dy || !r.dz || !r.tmax)
+                return RT_ERR_ARG;
+            if ((it.flags & RT_ANY) ? !it.occlusion_bits : (!it.hits.prim || !it.hits.t))
+                return RT_ERR_ARG;
+            if (it.bvh.node_count > 0 && (!it.bvh.nodes || !it.bvh.tris)) return RT_ERR_ARG;
+        }
+        const uint32_t d = it.bvh.depth ? it.bvh.depth : kMaxDepth;
+        if (d > kMaxDepth) return RT_ERR_CAPACITY;
+        deep = d > deep ? d : deep;
+        chunk_start[i] = (uint32_t)c;
+        c += ceil_div(it.count, 64);
+        if (c > 0xFFFFFFFFull / kBlock) return RT_ERR_CAPACITY;
+    }
+    chunk_start[count] = (uint32_t)c;
+    if (depth) *depth = deep;
+    return RT_OK;
+}
+
+extern "C" rt_status rt_trace_batch(rt_stream stream, const rt_trace_item* d_items,
+                                    const uint32_t* d_chunk_start, uint32_t count,
+                                    uint32_t total_chunks, uint32_t depth) {
+    if (total_chunks == 0) return RT_OK;
+    if (!d_items || !d_chunk_start || count == 0) return RT_ERR_ARG;
+    if (depth == 0 || depth > kMaxDepth) return RT_ERR_ARG;
+    hipStream_t s = as_stream(stream);
+    const dim3 grid(ceil_div(total_chunks, kBlock / 64)), block(kBlock);
+    if (3u * depth + 1u <= kLdsStack + 44)
+        hipLaunchKernelGGL(k_trace_batch<44>, grid, block, 0, s, d_items, d_chunk_start, count,
+                           total_chunks);
+    else
+        hipLaunchKernelGGL(k_trace_batch<3 * kMaxDepth + 1 - kLdsStack>, grid, block, 0, s,
+                           d_items, d_chunk_start, count, total_chunks);
+    return launch_status("k_trace_batch");
 }

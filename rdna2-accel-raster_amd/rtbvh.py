@@ -56,6 +56,11 @@ class rt_hits_soa(C.Structure):
     _fields_ = [("prim", vp), ("t", vp)]
 
 
+class rt_trace_item(C.Structure):
+    _fields_ = [("bvh", rt_bvh_dev), ("rays", rt_rays_soa), ("count", u32), ("flags", u32),
+                ("hits", rt_hits_soa), ("occlusion_bits", vp)]
+
+
 class rt_build_cfg(C.Structure):
     _fields_ = [("algorithm", u32), ("max_leaf_prims", u32), ("reserved", u32 * 2)]
 
@@ -102,6 +107,8 @@ _SIGS = {
     "rt_gen_ao_rays": [vp, vp, vp, vp, vp, vp, u32, u32, u32, f32, f32, f32, vp],
     "rt_gen_random_rays": [vp, u32, u32, vp, vp, f32, vp],
     "rt_pack_light_masks": [vp, vp, u32, u32, vp],
+    "rt_trace_batch_plan": [vp, u32, vp, vp],
+    "rt_trace_batch": [vp, vp, vp, u32, u32, u32],
     "rt_device_sync": [],
 }
 
@@ -353,6 +360,55 @@ class GpuBVH4:
         call("rt_trace_ex", stream_handle(stream), C.byref(b), C.byref(r), u32(n), u32(flags),
              C.byref(o), None, ptr(bits), ptr(box), ptr(tri))
         return (bits, box, tri) if counters else bits
+
+
+class TraceBatch:
+    """Many (tree, rays, mode) items traced in ONE launch (rt_trace_batch).
+
+    items: list of (bvh, rays, mode, out) with mode "closest" (out = (prim,
+    t) int32 / float32 tensors, or None to allocate) or "any" (out = int32
+    bits tensor of ceil(n/32) words, or None).  The item table and chunk
+    offsets are uploaded once here; launch() is then a single kernel with
+    device-resident arguments (no host work: it can be captured in a graph).
+    The tensors of every item must stay alive while the batch is used."""
+
+    def __init__(self, items, device=None):
+        dev = device or _dev()
+        self.count = len(items)
+        arr = (rt_trace_item * max(1, self.count))()
+        self.outputs, self._keep = [], []
+        for i, (bvh, rays, mode, out) in enumerate(items):
+            n = len(rays)
+            it = arr[i]
+            it.bvh, it.rays, it.count = bvh.dev(), rays.soa(), n
+            if mode == "any":
+                bits = out if out is not None else torch.empty(max(1, (n + 31) // 32),
+                                                               dtype=torch.int32, device=dev)
+                it.flags, it.occlusion_bits = RT_ANY, ptr(bits)
+                it.hits = rt_hits_soa(None, None)
+                self.outputs.append(bits)
+            elif mode == "closest":
+                prim, t = out if out is not None else (
+                    torch.empty(n, dtype=torch.int32, device=dev),
+                    torch.empty(n, dtype=torch.float32, device=dev))
+                it.flags, it.occlusion_bits = RT_CLOSEST, None
+                it.hits = rt_hits_soa(ptr(prim), ptr(t))
+                self.outputs.append((prim, t))
+            else:
+                raise ValueError(f"mode {mode!r}")
+            self._keep.append((bvh, rays))
+        chunk = (u32 * (self.count + 1))()
+        depth = u32(0)
+        call("rt_trace_batch_plan", arr, u32(self.count), chunk, C.byref(depth))
+        self.depth = depth.value
+        self.total_chunks = chunk[self.count]
+        self.d_items = torch.frombuffer(bytearray(bytes(arr)), dtype=torch.uint8).to(dev)
+        self.d_chunk = torch.tensor(list(chunk), dtype=torch.int64).to(torch.int32).to(dev)
+
+    def launch(self, stream=None):
+        call("rt_trace_batch", stream_handle(stream), ptr(self.d_items), ptr(self.d_chunk),
+             u32(self.count), u32(self.total_chunks), u32(self.depth))
+        return self.outputs
 
 
 def _depth_host(nodes4):

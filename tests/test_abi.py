@@ -85,3 +85,37 @@ def test_struct_layouts():
     # rt_tri64 = RadvBvhTriangleNode: triangle_id at byte 48 (radv_bvh.hpp:146)
     assert oracle.TRI64.fields["triangle_id"][1] == 48
     assert oracle.NODE4.fields["child"][1] == 96 and oracle.NODE4.fields["count"][1] == 112
+
+
+def test_trace_batch_plan_on_host():
+    """rt_trace_batch_plan is host-only: chunk offsets (64-ray chunks per
+    item), the deepest tree, and argument checks; the item record is 128 B."""
+    import rtbvh
+    assert C.sizeof(rtbvh.rt_trace_item) == 128
+    lib = rtbvh.lib()
+    fake = C.c_void_p(16)
+    items = (rtbvh.rt_trace_item * 4)()
+    counts = [1, 64, 65, 0]
+    for i, n in enumerate(counts):
+        it = items[i]
+        it.bvh = rtbvh.rt_bvh_dev(fake, fake, 5, 9, 3 + i, 0)
+        it.rays = rtbvh.rt_rays_soa(*([fake] * 8))
+        it.count = n
+        it.flags = rtbvh.RT_ANY if i % 2 else rtbvh.RT_CLOSEST
+        it.hits = rtbvh.rt_hits_soa(fake, fake)
+        it.occlusion_bits = fake
+    chunk = (C.c_uint32 * 5)()
+    depth = C.c_uint32(0)
+    assert lib.rt_trace_batch_plan(items, 4, chunk, C.byref(depth)) == 0
+    assert list(chunk) == [0, 1, 2, 4, 4] and depth.value == 6
+    items[2].flags = 0x100  # counters are not part of the batch path
+    assert lib.rt_trace_batch_plan(items, 4, chunk, C.byref(depth)) == 1
+    items[2].flags = rtbvh.RT_CLOSEST
+    items[2].hits = rtbvh.rt_hits_soa(None, None)  # closest without outputs
+    assert lib.rt_trace_batch_plan(items, 4, chunk, C.byref(depth)) == 1
+    items[2].hits = rtbvh.rt_hits_soa(fake, fake)
+    items[0].bvh.depth = 65
+    assert lib.rt_trace_batch_plan(items, 4, chunk, C.byref(depth)) == 2
+    assert lib.rt_trace_batch(None, None, None, 0, 0, 1) == 0  # nothing to trace
+    assert lib.rt_trace_batch(None, None, None, 3, 7, 1) == 1
+    assert lib.rt_trace_batch(None, fake, fake, 3, 7, 0) == 1
