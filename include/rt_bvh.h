@@ -161,6 +161,35 @@ rt_status rt_gpu_refit(rt_stream stream, const float* d_pos, const uint32_t* d_i
                        rt_node4* d_nodes, uint32_t node_count, rt_tri64* d_tris,
                        void* d_scratch, size_t scratch_bytes);
 
+/* Batched build: many meshes of 1..RT_BATCH_BUILD_MAX triangles built in ONE
+ * persistent launch, each with exactly the output rt_gpu_build gives it alone
+ * with RT_BUILD_IMPLICIT (the fast_bvh_builder layout; fused path).  No
+ * reference counterpart (fast_bvh_builder.comp builds one mesh per dispatch,
+ * src/main.cpp:871-885).  Larger meshes go through rt_gpu_build.
+ *   1. rt_gpu_build_batch_plan (host) checks the items and writes the device
+ *      table image (rt_gpu_build_batch_table_bytes(count) bytes) into host
+ *      memory; order the items largest first for balance;
+ *   2. the caller copies the table to device memory;
+ *   3. rt_gpu_build_batch launches over it (no host work, no allocation:
+ *      capturable in a graph).  workgroups = 0: min(count, 256).          */
+#define RT_BATCH_BUILD_MAX 4096u
+typedef struct rt_build_item {
+    const float* pos;       /* as rt_gpu_build */
+    const uint32_t* idx;    /* NULL: 9 floats per triangle */
+    rt_node4* nodes;        /* node_capacity >= rt_build_requirements' */
+    rt_tri64* tris;
+    rt_build_info* info;
+    uint32_t* sorted_ids;   /* optional */
+    uint32_t* codes;        /* optional */
+    uint32_t n;
+    uint32_t node_capacity;
+} rt_build_item;
+rt_status rt_gpu_build_batch_table_bytes(uint32_t count, size_t* bytes);
+rt_status rt_gpu_build_batch_plan(const rt_build_item* items, uint32_t count, void* table,
+                                  size_t table_bytes);
+rt_status rt_gpu_build_batch(rt_stream stream, const void* d_table, uint32_t count,
+                             uint32_t workgroups);
+
 /* GpuTriangle records (48 B, software_bvh.hpp:18-22, leaf order) plus the
  * leaf-order original indices -> rt_tri64 leaves (software_bvh.hpp:66-84
  * drops the index; d_order restores it, NULL = identity).                  */

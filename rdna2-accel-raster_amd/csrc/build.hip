@@ -555,6 +555,39 @@ constexpr uint32_t kBucketCap = 256;
 constexpr uint32_t kPartRecs = 256;   // sorted positions written per workgroup
 constexpr uint32_t kStageWords = 11;  // 9 vertex floats, triangle index, code
 
+// Level table of a fused-path tree (n <= 4096: at most 6 levels).  Read only
+// with compile-time indices (pick), so it stays in scalar registers; a
+// runtime index into a kernel argument array is a scalar memory load.
+constexpr int kSmallLevels = 7;
+struct SmallTree {
+    uint32_t levels;              // node levels, root first
+    uint32_t count[kSmallLevels];  // nodes per level
+    uint32_t off[kSmallLevels];    // node index of the level's first node
+    uint32_t pad[kSmallLevels];    // first LDS box slot of the level (multiple of 4)
+};
+
+// The level table lives in LDS for the kernel's lifetime: lookups are
+// wave-uniform broadcast reads.  (Kept in a private array it went to
+// scratch: indexing a kernel-argument or register array by a runtime level
+// compiles to memory loads either way.)
+struct LevelLds {
+    uint32_t levels;
+    uint32_t count[kSmallLevels];
+    uint32_t off[kSmallLevels];
+    uint32_t pad[kSmallLevels];
+};
+
+// Threads 0..kSmallLevels-1 copy the table; visible after the next barrier.
+__device__ __forceinline__ void load_levels(LevelLds& lv, const SmallTree& T) {
+    const uint32_t q = threadIdx.x;
+    if (q < (uint32_t)kSmallLevels) {
+        lv.count[q] = T.count[q];
+        lv.off[q] = T.off[q];
+        lv.pad[q] = T.pad[q];
+        if (q == 0) lv.levels = T.levels;
+    }
+}
+
 struct SmallLds {
     union {
         struct {
@@ -572,44 +605,13 @@ struct SmallLds {
     } u;
     float box[6][kBoxPitch];  // node bounds: level lv from slot pad[lv]
     uint32_t spread[1024];    // expand_bits10 of every 10-bit value
+    LevelLds lv;              // level table of the mesh being built
     uint32_t dstart[256];
     uint32_t wsum[16];
     uint32_t wmax[16];
     float red[16][6];
 };
 
-// Level table of a fused-path tree (n <= 4096: at most 6 levels).  Read only
-// with compile-time indices (pick), so it stays in scalar registers; a
-// runtime index into a kernel argument array is a scalar memory load.
-constexpr int kSmallLevels = 7;
-struct SmallTree {
-    uint32_t levels;              // node levels, root first
-    uint32_t count[kSmallLevels];  // nodes per level
-    uint32_t off[kSmallLevels];    // node index of the level's first node
-    uint32_t pad[kSmallLevels];    // first LDS box slot of the level (multiple of 4)
-};
-
-// Copy into scalar registers (readfirstlane keeps the compiler from folding
-// pick's selects back into indexed loads of the argument).
-__device__ __forceinline__ SmallTree in_regs(const SmallTree& a) {
-    SmallTree r;
-    r.levels = __builtin_amdgcn_readfirstlane(a.levels);
-#pragma unroll
-    for (int q = 0; q < kSmallLevels; ++q) {
-        r.count[q] = __builtin_amdgcn_readfirstlane(a.count[q]);
-        r.off[q] = __builtin_amdgcn_readfirstlane(a.off[q]);
-        r.pad[q] = __builtin_amdgcn_readfirstlane(a.pad[q]);
-    }
-    return r;
-}
-
-template <int N>
-__device__ __forceinline__ uint32_t pick(const uint32_t (&a)[N], uint32_t i) {
-    uint32_t r = a[0];
-#pragma unroll
-    for (int q = 1; q < N; ++q) r = i == (uint32_t)q ? a[q] : r;
-    return r;
-}
 
 // Union of slots first..first+3 of src (SoA, pitch floats per bound, first a
 // multiple of 4).  Empty slots hold the identity box (+inf mins, -inf maxes),
@@ -646,25 +648,26 @@ __device__ __forceinline__ void level_box(const float* src, uint32_t pitch, uint
 // nodes) by the first 4 waves, the rest (at most 64 nodes per level) by wave
 // 0 alone.  Each level is padded with identity boxes to a multiple of 4.
 // Ends with a barrier.
-__device__ void small_boxes(const SmallTree& T, SmallLds& S) {
+__device__ void small_boxes(SmallLds& S) {
+    const LevelLds& L = S.lv;
     const uint32_t tid = threadIdx.x, lane = tid & 63u, w = tid >> 6;
-    const uint32_t lb = T.levels - 1;
+    const uint32_t lb = L.levels - 1;
     {
-        const uint32_t c = pick(T.count, lb);
+        const uint32_t c = L.count[lb];
         if (tid < ((c + 3u) & ~3u)) {
             float b[6];
             level_box(S.u.tbox[0], kSmallMax, tid, c, b);
-            put_box(S.box, pick(T.pad, lb) + tid, b);
+            put_box(S.box, L.pad[lb] + tid, b);
         }
     }
     lds_barrier();
     if (lb == 0) return;
     {
-        const uint32_t c = pick(T.count, lb - 1);
+        const uint32_t c = L.count[lb - 1];
         if (tid < ((c + 3u) & ~3u)) {
             float b[6];
-            level_box(S.box[0] + pick(T.pad, lb), kBoxPitch, tid, c, b);
-            put_box(S.box, pick(T.pad, lb - 1) + tid, b);
+            level_box(S.box[0] + L.pad[lb], kBoxPitch, tid, c, b);
+            put_box(S.box, L.pad[lb - 1] + tid, b);
         }
     }
     lds_barrier();
@@ -672,11 +675,11 @@ __device__ void small_boxes(const SmallTree& T, SmallLds& S) {
     if (w == 0) {
         for (int lv = (int)lb - 2; lv >= 0; --lv) {
             wave_lds_sync();
-            const uint32_t c = pick(T.count, lv);
+            const uint32_t c = L.count[lv];
             if (lane < ((c + 3u) & ~3u)) {
                 float b[6];
-                level_box(S.box[0] + pick(T.pad, lv + 1), kBoxPitch, lane, c, b);
-                put_box(S.box, pick(T.pad, lv) + lane, b);
+                level_box(S.box[0] + L.pad[lv + 1], kBoxPitch, lane, c, b);
+                put_box(S.box, L.pad[lv] + lane, b);
             }
         }
     }
@@ -687,21 +690,22 @@ __device__ void small_boxes(const SmallTree& T, SmallLds& S) {
 // minY, maxY, minZ, maxZ: software_bvh.hpp:178-187 order), 6 child indices,
 // 7 child counts.  Leaf slot s = 4j + k: triangle bounds, child ~s, count 1;
 // inner slot: child bounds, child node index, count -1; empty slots zero.
-__device__ __forceinline__ uint4 node_chunk(const SmallTree& T, uint32_t n, uint32_t node,
-                                            uint32_t c, const SmallLds& S) {
-    const uint32_t lb = T.levels - 1;
+__device__ __forceinline__ uint4 node_chunk(uint32_t n, uint32_t node, uint32_t c,
+                                            const SmallLds& S) {
+    const LevelLds& L = S.lv;
+    const uint32_t lb = L.levels - 1;
     uint32_t lv = 0;
 #pragma unroll
     for (int q = 1; q < kSmallLevels; ++q)
-        if ((uint32_t)q < T.levels && node >= T.off[q]) lv = q;
-    const uint32_t j = node - pick(T.off, lv);
+        if ((uint32_t)q < L.levels && node >= L.off[q]) lv = q;
+    const uint32_t j = node - L.off[lv];
     const bool leaf = lv == lb;
-    const uint32_t cnt = leaf ? n : pick(T.count, lv + 1);
-    const uint32_t next = pick(T.off, lv + 1);
+    const uint32_t cnt = leaf ? n : L.count[lv + 1];
+    const uint32_t next = L.off[lv + 1];
     uint32_t o[4] = {0u, 0u, 0u, 0u};
     if (c < 6) {
         const uint32_t comp = (c & 1u) * 3u + (c >> 1);  // 0,3,1,4,2,5
-        const float* src = leaf ? S.u.tbox[comp] + 4 * j : S.box[comp] + pick(T.pad, lv + 1) + 4 * j;
+        const float* src = leaf ? S.u.tbox[comp] + 4 * j : S.box[comp] + L.pad[lv + 1] + 4 * j;
         const float4 q = *reinterpret_cast<const float4*>(src);
         o[0] = __float_as_uint(q.x); o[1] = __float_as_uint(q.y);
         o[2] = __float_as_uint(q.z); o[3] = __float_as_uint(q.w);
@@ -719,13 +723,14 @@ __device__ __forceinline__ uint4 node_chunk(const SmallTree& T, uint32_t n, uint
 // Output of the fused build and refit.  Owner item i (ok[i]): vertices v[i],
 // sorted position r[i], triangle index id[i], code key[i].  Every workgroup
 // computes all boxes; workgroup `part` of `parts` stores its share.
-__device__ void small_emit(SmallLds& S, const SmallTree& T, uint32_t n,
+__device__ void small_emit(SmallLds& S, uint32_t n,
                            const float (&v)[kSmallItems][9], const uint32_t (&r)[kSmallItems],
                            const uint32_t (&id)[kSmallItems], const uint32_t (&key)[kSmallItems],
                            const bool (&ok)[kSmallItems], rt_node4* __restrict__ nodes,
                            rt_tri64* __restrict__ tris, uint32_t* __restrict__ ids_out,
-                           uint32_t* __restrict__ codes_out) {
-    const uint32_t tid = threadIdx.x, part = blockIdx.x, parts = gridDim.x;
+                           uint32_t* __restrict__ codes_out, uint32_t part, uint32_t parts) {
+    const uint32_t tid = threadIdx.x;
+    const LevelLds& L = S.lv;
 #pragma unroll
     for (int i = 0; i < kSmallItems; ++i) {
         if (!ok[i]) continue;
@@ -739,60 +744,67 @@ __device__ void small_emit(SmallLds& S, const SmallTree& T, uint32_t n,
         for (int a = 0; a < 6; ++a) S.u.tbox[a][n + tid] = a < 3 ? INFINITY : -INFINITY;
     }
     lds_barrier();
-    small_boxes(T, S);
-    const uint32_t lb = T.levels - 1;
+    small_boxes(S);
+    const uint32_t lb = L.levels - 1;
     // this workgroup's slice of the node array, 16-B chunks
-    const uint32_t nchunks = (pick(T.off, lb) + pick(T.count, lb)) * 8;
+    const uint32_t nchunks = (L.off[lb] + L.count[lb]) * 8;
     const uint32_t per = (nchunks + parts - 1) / parts;
     const uint32_t g0 = part * per, g1 = min(nchunks, g0 + per);
     uint4* nd = reinterpret_cast<uint4*>(nodes);
-    for (uint32_t g = g0 + tid; g < g1; g += kSmallBlock) nd[g] = node_chunk(T, n, g >> 3, g & 7u, S);
+    for (uint32_t g = g0 + tid; g < g1; g += kSmallBlock) nd[g] = node_chunk(n, g >> 3, g & 7u, S);
     lds_barrier();  // triangle bounds dead: the union stages records
-    const uint32_t base = part * kPartRecs;
-    if (base >= n) return;
+    // sorted positions [base, base + kPartRecs) for base = part * kPartRecs,
+    // stepping by parts * kPartRecs (one slice per workgroup when parts
+    // workgroups share the mesh, all of them when one builds it alone)
+    for (uint32_t base = part * kPartRecs; base < n; base += parts * kPartRecs) {
 #pragma unroll
-    for (int i = 0; i < kSmallItems; ++i) {
-        if (!ok[i] || r[i] < base || r[i] - base >= kPartRecs) continue;
-        uint32_t* st = S.u.stage + (r[i] - base) * kStageWords;
+        for (int i = 0; i < kSmallItems; ++i) {
+            if (!ok[i] || r[i] < base || r[i] - base >= kPartRecs) continue;
+            uint32_t* st = S.u.stage + (r[i] - base) * kStageWords;
 #pragma unroll
-        for (int a = 0; a < 9; ++a) st[a] = __float_as_uint(v[i][a]);
-        st[9] = id[i];
-        st[10] = key[i];
-    }
-    lds_barrier();
-    const uint32_t cnt = min(kPartRecs, n - base);
-    uint4* td = reinterpret_cast<uint4*>(tris + base);
-    for (uint32_t g = tid; g < 4 * cnt; g += kSmallBlock) {
-        const uint32_t* st = S.u.stage + (g >> 2) * kStageWords;
-        const uint32_t c = g & 3u;
-        uint4 o;
-        if (c == 0) o = make_uint4(st[0], st[1], st[2], st[3]);
-        else if (c == 1) o = make_uint4(st[4], st[5], st[6], st[7]);
-        else if (c == 2) o = make_uint4(st[8], 0u, 0u, 0u);
-        else o = make_uint4(st[9], 0x80000000u, 0u, 9u);  // store_tri's last 16 B
-        td[g] = o;
-    }
-    if (tid < cnt) {
-        if (ids_out) ids_out[base + tid] = S.u.stage[tid * kStageWords + 9];
-        if (codes_out) codes_out[base + tid] = S.u.stage[tid * kStageWords + 10];
+            for (int a = 0; a < 9; ++a) st[a] = __float_as_uint(v[i][a]);
+            st[9] = id[i];
+            st[10] = key[i];
+        }
+        lds_barrier();
+        const uint32_t cnt = min(kPartRecs, n - base);
+        uint4* td = reinterpret_cast<uint4*>(tris + base);
+        for (uint32_t g = tid; g < 4 * cnt; g += kSmallBlock) {
+            const uint32_t* st = S.u.stage + (g >> 2) * kStageWords;
+            const uint32_t c = g & 3u;
+            uint4 o;
+            if (c == 0) o = make_uint4(st[0], st[1], st[2], st[3]);
+            else if (c == 1) o = make_uint4(st[4], st[5], st[6], st[7]);
+            else if (c == 2) o = make_uint4(st[8], 0u, 0u, 0u);
+            else o = make_uint4(st[9], 0x80000000u, 0u, 9u);  // store_tri's last 16 B
+            td[g] = o;
+        }
+        if (tid < cnt) {
+            if (ids_out) ids_out[base + tid] = S.u.stage[tid * kStageWords + 9];
+            if (codes_out) codes_out[base + tid] = S.u.stage[tid * kStageWords + 10];
+        }
+        lds_barrier();  // staging reused by the next slice
     }
 }
 
-__device__ __forceinline__ void small_info(rt_build_info* info, const SmallTree& T) {
-    if (info && blockIdx.x == 0 && threadIdx.x == 0) {
-        info->node_count = pick(T.off, T.levels - 1) + pick(T.count, T.levels - 1);
+__device__ __forceinline__ void small_info(rt_build_info* info, const SmallTree& T,
+                                           uint32_t part) {
+    if (info && part == 0 && threadIdx.x == 0) {
+        info->node_count = T.off[T.levels - 1] + T.count[T.levels - 1];
         info->depth = T.levels;
     }
 }
 
-__global__ __launch_bounds__(kSmallBlock) void k_build_small(
-    const float* __restrict__ pos, const uint32_t* __restrict__ idx, uint32_t n,
-    rt_node4* __restrict__ nodes, rt_tri64* __restrict__ tris, uint32_t* __restrict__ ids_out,
-    uint32_t* __restrict__ codes_out, SmallTree targ, rt_build_info* info) {
-    __shared__ SmallLds S;
-    const SmallTree T = in_regs(targ);
+// The whole fused build of one mesh by this workgroup, which writes slice
+// `part` of `parts` of the output (see small_emit).
+__device__ void build_small(SmallLds& S, const SmallTree& T, const float* __restrict__ pos,
+                            const uint32_t* __restrict__ idx, uint32_t n,
+                            rt_node4* __restrict__ nodes, rt_tri64* __restrict__ tris,
+                            uint32_t* __restrict__ ids_out, uint32_t* __restrict__ codes_out,
+                            rt_build_info* info, uint32_t part, uint32_t parts) {
     const uint32_t tid = threadIdx.x, lane = tid & 63u, w = tid >> 6;
-    small_info(info, T);
+    small_info(info, T, part);
+    load_levels(S.lv, T);  // first read after the barriers below
     // bucket counters cleared while the triangle loads are in flight
     static_assert(kBuckets == 8 * kSmallBlock, "8 bucket counters per thread");
     uint4* start4 = reinterpret_cast<uint4*>(S.u.msd.start);
@@ -964,7 +976,37 @@ __global__ __launch_bounds__(kSmallBlock) void k_build_small(
         id[i] = w * 64 * kSmallItems + i * 64 + lane;
         ok[i] = id[i] < n;
     }
-    small_emit(S, T, n, v, rank, id, key, ok, nodes, tris, ids_out, codes_out);
+    small_emit(S, n, v, rank, id, key, ok, nodes, tris, ids_out, codes_out, part, parts);
+}
+
+__global__ __launch_bounds__(kSmallBlock) void k_build_small(
+    const float* __restrict__ pos, const uint32_t* __restrict__ idx, uint32_t n,
+    rt_node4* __restrict__ nodes, rt_tri64* __restrict__ tris, uint32_t* __restrict__ ids_out,
+    uint32_t* __restrict__ codes_out, SmallTree targ, rt_build_info* info) {
+    __shared__ SmallLds S;
+    build_small(S, targ, pos, idx, n, nodes, tris, ids_out, codes_out, info, blockIdx.x,
+                gridDim.x);
+}
+
+// Device table entry of the batched build: the caller's item and the level
+// table of its fused-path tree.
+struct BatchBuild {
+    rt_build_item item;
+    SmallTree tree;
+};
+
+// Persistent batched build: workgroup b builds items b, b + grid, b + 2 grid,
+// ... (the host orders them largest first), each alone (parts = 1).
+__global__ __launch_bounds__(kSmallBlock) void k_build_batch(const BatchBuild* __restrict__ table,
+                                                              uint32_t count) {
+    __shared__ SmallLds S;
+    for (uint32_t k = blockIdx.x; k < count; k += gridDim.x) {
+        const BatchBuild& b = table[k];
+        const rt_build_item& it = b.item;
+        build_small(S, b.tree, it.pos, it.idx, it.n, it.nodes, it.tris, it.sorted_ids,
+                    it.codes, it.info, 0u, 1u);
+        lds_barrier();  // LDS reused by the next mesh
+    }
 }
 
 // Refit of a fused-path tree: sorted position s = i*1024 + t keeps its
@@ -977,8 +1019,8 @@ __global__ __launch_bounds__(kSmallBlock) void k_refit_small(
     rt_node4* __restrict__ nodes, rt_tri64* __restrict__ tris, SmallTree targ,
     rt_build_info* info) {
     __shared__ SmallLds S;
-    const SmallTree T = in_regs(targ);
-    small_info(info, T);
+    small_info(info, targ, blockIdx.x);
+    load_levels(S.lv, targ);  // first read after small_emit's first barrier
     float v[kSmallItems][9];
     uint32_t r[kSmallItems], id[kSmallItems], key[kSmallItems];
     bool ok[kSmallItems];
@@ -992,7 +1034,7 @@ __global__ __launch_bounds__(kSmallBlock) void k_refit_small(
 #pragma unroll
     for (int i = 0; i < kSmallItems; ++i)
         if (ok[i]) tri_verts(pos, idx, id[i], v[i]);
-    small_emit(S, T, n, v, r, id, key, ok, nodes, tris, nullptr, nullptr);
+    small_emit(S, n, v, r, id, key, ok, nodes, tris, nullptr, nullptr, blockIdx.x, gridDim.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -1573,4 +1615,37 @@ extern "C" rt_status rt_gpu_refit(rt_stream stream, const float* d_pos, const ui
     if (!d_scratch || scratch_bytes < p.total) return RT_ERR_CAPACITY;
     float* gboxes = reinterpret_cast<float*>(static_cast<char*>(d_scratch) + p.boxes);
     return emit_levels(s, d_pos, d_idx, n, nullptr, d_nodes, d_tris, gboxes, t, true);
+}
+
+extern "C" rt_status rt_gpu_build_batch_table_bytes(uint32_t count, size_t* bytes) {
+    if (!bytes) return RT_ERR_ARG;
+    *bytes = (size_t)count * sizeof(BatchBuild);
+    return RT_OK;
+}
+
+extern "C" rt_status rt_gpu_build_batch_plan(const rt_build_item* items, uint32_t count,
+                                             void* table, size_t table_bytes) {
+    if ((count && (!items || !table)) || table_bytes < (size_t)count * sizeof(BatchBuild))
+        return RT_ERR_ARG;
+    BatchBuild* out = static_cast<BatchBuild*>(table);
+    for (uint32_t k = 0; k < count; ++k) {
+        const rt_build_item& it = items[k];
+        if (it.n == 0 || it.n > RT_BATCH_BUILD_MAX) return RT_ERR_CAPACITY;
+        if (!it.pos || !it.nodes || !it.tris || !it.info) return RT_ERR_ARG;
+        const TreeLayout t = tree_layout(it.n);
+        if (it.node_capacity < t.total_boxes) return RT_ERR_CAPACITY;
+        out[k].item = it;
+        out[k].tree = small_tree(t);
+    }
+    return RT_OK;
+}
+
+extern "C" rt_status rt_gpu_build_batch(rt_stream stream, const void* d_table, uint32_t count,
+                                        uint32_t workgroups) {
+    if (count == 0) return RT_OK;
+    if (!d_table) return RT_ERR_ARG;
+    const uint32_t g = workgroups ? min(workgroups, count) : min(count, 256u);
+    hipLaunchKernelGGL(k_build_batch, dim3(g), dim3(kSmallBlock), 0, as_stream(stream),
+                       static_cast<const BatchBuild*>(d_table), count);
+    return launch_status("k_build_batch");
 }

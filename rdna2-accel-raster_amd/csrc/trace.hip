@@ -42,24 +42,28 @@ __device__ __forceinline__ void cx(Child& a, Child& b) {
 // compiler, but pointers read from memory (the batch items) are generic, and
 // generic accesses compile to flat loads (which also wait on the LDS
 // counter); the traversal is written over either pointer kind.
+// (clang vector types, not HIP's float4 class, whose copy constructor takes
+// a generic reference.)
 #define RT_GLOBAL __attribute__((address_space(1)))
-__device__ __forceinline__ const float4* f4(const rt_node4* p) {
-    return reinterpret_cast<const float4*>(p);
+typedef float f4v __attribute__((ext_vector_type(4)));
+typedef int32_t i4v __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ const f4v* f4(const rt_node4* p) {
+    return reinterpret_cast<const f4v*>(p);
 }
-__device__ __forceinline__ const RT_GLOBAL float4* f4(const RT_GLOBAL rt_node4* p) {
-    return reinterpret_cast<const RT_GLOBAL float4*>(p);
+__device__ __forceinline__ const RT_GLOBAL f4v* f4(const RT_GLOBAL rt_node4* p) {
+    return reinterpret_cast<const RT_GLOBAL f4v*>(p);
 }
-__device__ __forceinline__ const float4* f4(const rt_tri64* p) {
-    return reinterpret_cast<const float4*>(p);
+__device__ __forceinline__ const f4v* f4(const rt_tri64* p) {
+    return reinterpret_cast<const f4v*>(p);
 }
-__device__ __forceinline__ const RT_GLOBAL float4* f4(const RT_GLOBAL rt_tri64* p) {
-    return reinterpret_cast<const RT_GLOBAL float4*>(p);
+__device__ __forceinline__ const RT_GLOBAL f4v* f4(const RT_GLOBAL rt_tri64* p) {
+    return reinterpret_cast<const RT_GLOBAL f4v*>(p);
 }
-__device__ __forceinline__ const int4* i4(const rt_node4* p) {
-    return reinterpret_cast<const int4*>(p);
+__device__ __forceinline__ const i4v* i4(const rt_node4* p) {
+    return reinterpret_cast<const i4v*>(p);
 }
-__device__ __forceinline__ const RT_GLOBAL int4* i4(const RT_GLOBAL rt_node4* p) {
-    return reinterpret_cast<const RT_GLOBAL int4*>(p);
+__device__ __forceinline__ const RT_GLOBAL i4v* i4(const RT_GLOBAL rt_node4* p) {
+    return reinterpret_cast<const RT_GLOBAL i4v*>(p);
 }
 template <typename T>
 __device__ __forceinline__ RT_GLOBAL T* glb(T* p) {
@@ -104,7 +108,7 @@ __device__ __forceinline__ bool test_leaf(TP __restrict__ tris, uint32_t first,
                                           uint32_t& hitPrim) {
     for (uint32_t q = 0; q < cnt; ++q) {
         const auto tp = f4(tris + first + q);
-        const float4 a = tp[0], b = tp[1], c = tp[2];
+        const f4v a = tp[0], b = tp[1], c = tp[2];
         float t;
         if (!mt_hit(a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w, c.x, ox, oy, oz, dx, dy, dz, &t))
             continue;
@@ -140,9 +144,9 @@ __device__ __forceinline__ void trace_ray(NP __restrict__ nodes, TP __restrict__
         int32_t ni;
         RT_SPOP(ni);
         const auto np = f4(nodes + ni);
-        const float4 mnx = np[0], mxx = np[1], mny = np[2], mxy = np[3], mnz = np[4], mxz = np[5];
-        const int4 ch = i4(nodes + ni)[6];
-        const int4 cn = i4(nodes + ni)[7];
+        const f4v mnx = np[0], mxx = np[1], mny = np[2], mxy = np[3], mnz = np[4], mxz = np[5];
+        const i4v ch = i4(nodes + ni)[6];
+        const i4v cn = i4(nodes + ni)[7];
         if (COUNT) boxTests += 4;
 
         const float bminx[4] = {mnx.x, mnx.y, mnx.z, mnx.w};
@@ -201,7 +205,8 @@ __device__ __forceinline__ void trace_ray(NP __restrict__ nodes, TP __restrict__
 
 // Any-hit bits of a wave's 64 rays r0..r0+63 (r0 a multiple of 64): lane 0
 // writes the first word, lane 32 the second, each if it holds a ray < n.
-__device__ __forceinline__ void write_wave_bits(uint32_t* bits_out, uint32_t r0, uint32_t n,
+template <typename BP>
+__device__ __forceinline__ void write_wave_bits(BP bits_out, uint32_t r0, uint32_t n,
                                                 bool occluded) {
     const uint64_t m = __ballot(occluded);
     const uint32_t lane = threadIdx.x & 63u;

@@ -61,6 +61,11 @@ class rt_trace_item(C.Structure):
                 ("hits", rt_hits_soa), ("occlusion_bits", vp)]
 
 
+class rt_build_item(C.Structure):
+    _fields_ = [("pos", vp), ("idx", vp), ("nodes", vp), ("tris", vp), ("info", vp),
+                ("sorted_ids", vp), ("codes", vp), ("n", u32), ("node_capacity", u32)]
+
+
 class rt_build_cfg(C.Structure):
     _fields_ = [("algorithm", u32), ("max_leaf_prims", u32), ("reserved", u32 * 2)]
 
@@ -107,6 +112,9 @@ _SIGS = {
     "rt_gen_ao_rays": [vp, vp, vp, vp, vp, vp, u32, u32, u32, f32, f32, f32, vp],
     "rt_gen_random_rays": [vp, u32, u32, vp, vp, f32, vp],
     "rt_pack_light_masks": [vp, vp, u32, u32, vp],
+    "rt_gpu_build_batch_table_bytes": [u32, vp],
+    "rt_gpu_build_batch_plan": [vp, u32, vp, C.c_size_t],
+    "rt_gpu_build_batch": [vp, vp, u32, u32],
     "rt_trace_batch_plan": [vp, u32, vp, vp],
     "rt_trace_batch": [vp, vp, vp, u32, u32, u32],
     "rt_device_sync": [],
@@ -360,6 +368,58 @@ class GpuBVH4:
         call("rt_trace_ex", stream_handle(stream), C.byref(b), C.byref(r), u32(n), u32(flags),
              C.byref(o), None, ptr(bits), ptr(box), ptr(tri))
         return (bits, box, tri) if counters else bits
+
+
+BATCH_BUILD_MAX = 4096
+
+
+class BuildBatch:
+    """Many meshes of at most BATCH_BUILD_MAX triangles built in ONE persistent
+    launch (rt_gpu_build_batch), each tree identical to GpuBVH4.build(...,
+    algorithm="implicit") of that mesh alone.
+
+    meshes: list of (pos, idx or None).  The table is planned and uploaded
+    here, largest meshes first; launch() is a single kernel with
+    device-resident arguments (capturable in a graph).  .trees[i] is mesh
+    i's GpuBVH4 (filled by launch)."""
+
+    def __init__(self, meshes, keep_order=False, workgroups=0, device=None):
+        dev = device or _dev()
+        self.count = len(meshes)
+        self.workgroups = workgroups
+        self.trees = []
+        arr = (rt_build_item * max(1, self.count))()
+        order = sorted(range(self.count), key=lambda i: -_tri_count(*meshes[i]))
+        for slot, i in enumerate(order):
+            pos, idx = meshes[i]
+            n = _tri_count(pos, idx)
+            cap = build_requirements(n, build_cfg("implicit"))[0]
+            nodes = torch.empty(max(cap, 1) * NODE_BYTES, dtype=torch.uint8, device=dev)
+            tris = torch.empty(max(n, 1) * TRI_BYTES, dtype=torch.uint8, device=dev)
+            info = torch.empty(4, dtype=torch.int32, device=dev)
+            bvh = GpuBVH4(nodes, tris, cap, n, len(tree_layout(n)[0]))
+            bvh.info, bvh.pos, bvh.idx, bvh.algorithm = info, pos, idx, "implicit"
+            if keep_order:
+                bvh.sorted_ids = torch.empty(n, dtype=torch.int32, device=dev)
+                bvh.codes = torch.empty(n, dtype=torch.int32, device=dev)
+            arr[slot] = rt_build_item(ptr(pos), ptr(idx), ptr(nodes), ptr(tris), ptr(info),
+                                      ptr(bvh.sorted_ids), ptr(bvh.codes), n, cap)
+            self.trees.append((i, bvh))
+        self.trees = [b for _, b in sorted(self.trees, key=lambda x: x[0])]
+        nbytes = C.c_size_t(0)
+        call("rt_gpu_build_batch_table_bytes", u32(self.count), C.byref(nbytes))
+        host = (C.c_uint8 * max(1, nbytes.value))()
+        call("rt_gpu_build_batch_plan", arr, u32(self.count), host, nbytes)
+        self.d_table = torch.frombuffer(bytearray(bytes(host)), dtype=torch.uint8).to(dev)
+
+    def launch(self, stream=None):
+        call("rt_gpu_build_batch", stream_handle(stream), ptr(self.d_table), u32(self.count),
+             u32(self.workgroups))
+        return self.trees
+
+
+def _tri_count(pos, idx):
+    return idx.shape[0] if idx is not None else pos.numel() // 9
 
 
 class TraceBatch:

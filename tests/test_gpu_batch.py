@@ -105,3 +105,65 @@ def test_batch_graph_replays(rt, O):
                 np.testing.assert_array_equal(u32(got[1])[:n], u32(want[1])[:n])
             else:
                 np.testing.assert_array_equal(_bits(u32(got), n), _bits(u32(want), n))
+
+
+def _meshes(rt, O):
+    rng = np.random.default_rng(4)
+    out = []
+    for k, n in enumerate([4096, 1, 2, 5, 64, 100, 255, 256, 257, 1000, 3001, 4095, 777]):
+        if k % 3 == 0:
+            nu = max(2, int(np.sqrt(n / 2)))
+            pos, idx = O.sphere(nu, max(2, (n + 2 * nu - 1) // (2 * nu)), 1.0, 0.02, k)
+            idx = idx[:n]
+            out.append((torch.from_numpy(pos).cuda(),
+                        torch.from_numpy(np.ascontiguousarray(idx).view(np.int32)).cuda()))
+        else:
+            xyz = rng.random((n, 9), dtype=np.float32) * 2.0 - 1.0
+            out.append((torch.from_numpy(xyz).cuda(), None))
+    return out
+
+
+def test_build_batch_equals_alone(rt, O):
+    """One persistent launch over meshes of 1..4096 triangles (indexed and
+    not): every tree, leaf record, sorted index and code equals the fused
+    build of that mesh alone; one also against the oracle."""
+    meshes = _meshes(rt, O)
+    for wg in (0, 3):
+        bb = rt.BuildBatch(meshes, keep_order=True, workgroups=wg)
+        trees = bb.launch()
+        torch.cuda.synchronize()
+        for (pos, idx), got in zip(meshes, trees):
+            want = rt.GpuBVH4.build(pos, idx, algorithm="implicit", keep_order=True)
+            torch.cuda.synchronize()
+            n = got.tri_count
+            assert [int(v) for v in got.info[:2].cpu()] == [want.node_count, want.depth]
+            np.testing.assert_array_equal(to_np(got.nodes)[: want.node_count * 128],
+                                          to_np(want.nodes)[: want.node_count * 128])
+            np.testing.assert_array_equal(to_np(got.tris)[: n * 64], to_np(want.tris)[: n * 64])
+            np.testing.assert_array_equal(u32(got.sorted_ids), u32(want.sorted_ids)[:n])
+            np.testing.assert_array_equal(u32(got.codes), u32(want.codes)[:n])
+    pos, idx = meshes[4]
+    nodes_c, tris_c, ids_c, _ = O.lbvh_build(to_np(pos).reshape(-1, 9), None)
+    np.testing.assert_array_equal(to_np(trees[4].nodes)[: len(nodes_c) * 128],
+                                  nodes_c.view(np.uint8).reshape(-1))
+
+
+def test_build_batch_graph_replays(rt, O):
+    meshes = _meshes(rt, O)
+    bb = rt.BuildBatch(meshes)
+    trees = bb.launch()
+    torch.cuda.synchronize()
+    ref = [(t.nodes.clone(), t.tris.clone()) for t in trees]
+    s = torch.cuda.Stream()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g, stream=s):
+        bb.launch(stream=s)
+    for rep in range(3):
+        for t in trees:
+            t.nodes.fill_(0)
+            t.tris.fill_(0)
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        for t, (rn, rt_) in zip(trees, ref):
+            assert torch.equal(t.nodes, rn) and torch.equal(t.tris, rt_)
