@@ -256,6 +256,9 @@ rt_status rt_gen_wave_grid(rt_stream stream, uint32_t nx, uint32_t ny, float ext
                            float amp, float fx, float fy, float* d_pos, uint32_t* d_idx);
 rt_status rt_gen_heightfield(rt_stream stream, uint32_t nx, uint32_t ny, float extent, float amp,
                              uint32_t seed, float* d_pos, uint32_t* d_idx);
+/* include/radv_bvh.hpp:224-283 generate_dynamic_grid_mesh: 2 triangles per
+ * quad, so d_xyz9 receives 2 * floor(n / 2) triangles (9 floats each); an
+ * odd n leaves the last of n slots untouched.                              */
 rt_status rt_gen_dynamic_grid(rt_stream stream, uint32_t n, float t, float extent, float* d_xyz9);
 rt_status rt_gen_camera_rays(rt_stream stream, uint32_t w, uint32_t h, const float eye[3],
                              const float look[3], const float up[3], float vfov_deg, float tmin,

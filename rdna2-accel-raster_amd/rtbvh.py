@@ -432,7 +432,7 @@ class TraceBatch:
     device-resident arguments (no host work: it can be captured in a graph).
     The tensors of every item must stay alive while the batch is used."""
 
-    def __init__(self, items, device=None):
+    def __init__(self, items, device=None, stream=None):
         dev = device or _dev()
         self.count = len(items)
         arr = (rt_trace_item * max(1, self.count))()
@@ -462,8 +462,17 @@ class TraceBatch:
         call("rt_trace_batch_plan", arr, u32(self.count), chunk, C.byref(depth))
         self.depth = depth.value
         self.total_chunks = chunk[self.count]
-        self.d_items = torch.frombuffer(bytearray(bytes(arr)), dtype=torch.uint8).to(dev)
-        self.d_chunk = torch.tensor(list(chunk), dtype=torch.int64).to(torch.int32).to(dev)
+        h_items = torch.frombuffer(bytearray(bytes(arr)), dtype=torch.uint8)
+        h_chunk = torch.tensor(list(chunk), dtype=torch.int64).to(torch.int32)
+        if stream is None:
+            self.d_items, self.d_chunk = h_items.to(dev), h_chunk.to(dev)
+        else:
+            # stream-ordered upload from pinned copies (no host wait); the
+            # pinned buffers live as long as this batch
+            self._pinned = (h_items.pin_memory(), h_chunk.pin_memory())
+            with torch.cuda.stream(stream):
+                self.d_items = self._pinned[0].to(dev, non_blocking=True)
+                self.d_chunk = self._pinned[1].to(dev, non_blocking=True)
 
     def launch(self, stream=None):
         call("rt_trace_batch", stream_handle(stream), ptr(self.d_items), ptr(self.d_chunk),
@@ -519,7 +528,11 @@ def gen_heightfield(nx, ny, extent=1.0, amp=0.05, seed=11, stream=None):
 
 
 def gen_dynamic_grid(n, t=0.0, extent=2.0, stream=None, out=None):
-    xyz = out if out is not None else torch.empty((n, 9), dtype=torch.float32, device=_dev())
+    """generate_dynamic_grid_mesh layout, 9 floats per triangle; two
+    triangles per quad, so the result has 2 * (n // 2) rows (an odd n loses
+    its last triangle, as in the reference generator)."""
+    xyz = out if out is not None else torch.empty((2 * (n // 2), 9), dtype=torch.float32,
+                                                   device=_dev())
     call("rt_gen_dynamic_grid", stream_handle(stream), u32(n), f32(t), f32(extent), ptr(xyz))
     return xyz
 
