@@ -148,6 +148,124 @@ def cpu_baseline(frame, threads):
                       f"{threads} threads"}
 
 
+def batched_leg(meshes=256, seed=4):
+    """Batched build and trace: `meshes` seeded random-displaced grids with
+    triangle counts log-uniform in [64, 65536], each with its own ray set of
+    4k-512k rays (origins uniform in the mesh bounds, directions uniform on
+    the sphere), closest-hit or any-hit 50/50.  Batched: meshes of at most
+    4096 triangles built in one persistent launch (larger ones one build
+    each), all ray sets traced in one launch replayed from a graph.  Plain:
+    one build and one trace launch per mesh, waiting for each.  Both trace
+    legs give identical results (checked)."""
+    rng = np.random.default_rng(seed)
+    geo, rays, modes = [], [], []
+    for k in range(meshes):
+        n = int(np.exp(rng.uniform(np.log(64), np.log(65536))))
+        nx = max(1, int(np.sqrt(n / 2)))
+        ny = max(1, n // (2 * nx))
+        geo.append(rt.gen_heightfield(nx, ny, 1.0, 0.05, 1000 + k))
+        nr = int(np.exp(rng.uniform(np.log(4096), np.log(524288))))
+        rays.append(rt.gen_random_rays(nr, 5000 + k, (-1.1, -1.1, -0.2), (1.1, 1.1, 0.2),
+                                       0.3 if k % 2 else 1e30))
+        modes.append("any" if rng.random() < 0.5 else "closest")
+    tris = sum(int(i.shape[0]) for _, i in geo)
+    nrays = sum(len(r) for r in rays)
+    torch.cuda.synchronize()
+    small = [k for k, (_, i) in enumerate(geo) if i.shape[0] <= rt.BATCH_BUILD_MAX]
+    big = [k for k in range(meshes) if k not in small]
+    # batched build: one persistent launch for the small meshes + the rest
+    bb = rt.BuildBatch([geo[k] for k in small])
+    s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    trees = [None] * meshes
+    for k, t in zip(small, bb.launch()):
+        trees[k] = t
+    for k in big:
+        trees[k] = rt.GpuBVH4.build(*geo[k], algorithm="radix")
+    torch.cuda.synchronize()
+    s.record()
+    bb.launch()
+    for k in big:
+        rt.GpuBVH4.build(*geo[k], algorithm="radix", sync=False)
+    e.record()
+    e.synchronize()
+    build_ms = s.elapsed_time(e)
+    # batched trace, captured in a graph
+    batch = rt.TraceBatch([(trees[k], rays[k], modes[k], None) for k in range(meshes)])
+    batch.launch()
+    torch.cuda.synchronize()
+    gs = torch.cuda.Stream()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g, stream=gs):
+        batch.launch(stream=gs)
+    g.replay()
+    torch.cuda.synchronize()
+    reps = 5
+    s.record()
+    for _ in range(reps):
+        g.replay()
+    e.record()
+    e.synchronize()
+    trace_ms = s.elapsed_time(e) / reps
+    # plain: one build + one trace launch per mesh, waiting for each
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    plain_out = []
+    for k in range(meshes):
+        tk = rt.GpuBVH4.build(*geo[k], algorithm="auto")
+        o = tk.intersect(rays[k]) if modes[k] == "closest" else tk.occluded(rays[k])
+        torch.cuda.synchronize()
+        plain_out.append(o)
+    plain_ms = (time.perf_counter() - t0) * 1e3
+    t0 = time.perf_counter()
+    for k in range(meshes):
+        _ = trees[k].intersect(rays[k]) if modes[k] == "closest" else trees[k].occluded(rays[k])
+        torch.cuda.synchronize()
+    plain_trace_ms = (time.perf_counter() - t0) * 1e3
+    same = True
+    for k, o in enumerate(batch.outputs):
+        n = len(rays[k])
+        if modes[k] == "closest":
+            same &= bool(torch.equal(o[0], plain_out[k][0]) and torch.equal(o[1], plain_out[k][1]))
+        else:
+            same &= bool(torch.equal(o[: n // 32], plain_out[k][: n // 32]))
+    return {"workload": f"{meshes} random-displaced grids, 64-65536 tris log-uniform "
+                        f"({tris} tris), 4k-512k random rays each ({nrays} rays), "
+                        "closest/any 50/50",
+            "build_ms": round(build_ms, 3), "small_meshes_one_launch": len(small),
+            "trace_ms": round(trace_ms, 3), "trace_mrays_per_s": round(nrays / trace_ms / 1e3, 1),
+            "plain_trace_ms": round(plain_trace_ms, 3),
+            "plain_trace_mrays_per_s": round(nrays / plain_trace_ms / 1e3, 1),
+            "plain_build_and_trace_ms": round(plain_ms, 3),
+            "batched_build_and_trace_ms": round(build_ms + trace_ms, 3),
+            "speedup_build_and_trace": round(plain_ms / (build_ms + trace_ms), 2),
+            "speedup_trace": round(plain_trace_ms / trace_ms, 2),
+            "identical_results": same}
+
+
+def serving_leg(seed=7):
+    """Serving: 32 resident scenes (10k-2.5M tris, log-uniform), requests of
+    4k-2M rays (shadow, AO, closest) or 4k-64k-triangle deform-and-rebuilds.
+    Capacity: 300 requests all waiting at t = 0, served (batched) and plain
+    (one launch per request, waiting for each).  Latency: 800 Poisson
+    arrivals at half the served capacity, p50 / p99 over all requests."""
+    import serve as S
+    scenes = S.make_scenes(32)
+    budget = 4 << 30
+    reqs = S.make_requests(300, 0, scenes, seed=seed)
+    sat = S.Server(scenes, budget).run(reqs)
+    plain = S.run_plain(scenes, S.clone_requests(reqs))
+    qps = round(0.5 * len(reqs) / sat["span_s"])
+    lat = S.Server(scenes, budget).run(S.make_requests(800, qps, scenes, seed=seed + 1))
+    return {"workload": f"32 resident scenes ({sum(x.tris for x in scenes)} tris), 4k-2M-ray "
+                        "shadow/AO/closest requests and 4k-64k-tri rebuilds",
+            "capacity_mrays_per_s": sat["mrays_per_s"],
+            "plain_mrays_per_s": plain["mrays_per_s"],
+            "speedup": round(sat["mrays_per_s"] / plain["mrays_per_s"], 2),
+            "qps": qps, "p50_ms": lat["p50_ms"], "p99_ms": lat["p99_ms"],
+            "latency_mrays_per_s": lat["mrays_per_s"],
+            "p99_ms_by_kind": {k: lat.get(f"p99_ms_{k}") for k in S.KINDS}}
+
+
 def copy_probe(nbytes=1 << 30, reps=3):
     """A plain device copy of nbytes (read nbytes, write nbytes), the
     calibration kernel for the FETCH_SIZE / WRITE_SIZE counters."""
@@ -171,6 +289,8 @@ def main():
     ap.add_argument("--builder", choices=["radix", "implicit"], default="radix")
     ap.add_argument("--copy-probe", action="store_true",
                     help="first run a plain 1 GiB device copy (counter calibration)")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the batched and serving legs")
     a = ap.parse_args()
     rt.DEFAULT_SCHEDULE = a.schedule
     rt.TRACE_TUNING.update(rays_per_wave=a.rays_per_wave, refill_lanes=a.refill_lanes)
@@ -251,6 +371,10 @@ def main():
         "frame_latency_ms_p50": round(percentile(step_ms, 0.5), 4),
         "frame_latency_ms_p99": round(percentile(step_ms, 0.99), 4),
     }
+    if not a.no_extra:
+        out["batched"] = batched_leg()
+        out["serving"] = serving_leg()
+        out["p99_request_latency_ms"] = out["serving"]["p99_ms"]
     if not a.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         out["cpu_baseline"] = cpu_baseline(f, threads)

@@ -13,9 +13,10 @@ Scheduling (one host thread):
     in the budget and fewer than `max_inflight` requests are in flight
     (FIFO: a large head is never overtaken);
   * all ray requests admitted in a tick get their rays generated and are
-    traced together in ONE rt_trace_batch launch; the tick's rebuilds go to
-    their own stream; ticks rotate over `streams` streams, so work admitted
-    while earlier work is still on the device overlaps with it;
+    traced together in ONE rt_trace_batch launch; rebuilds rotate over
+    `rebuild_streams` streams of their own; ticks rotate over `streams`
+    streams, so work admitted while earlier work is still on the device
+    overlaps with it;
   * completion is polled through events; a finished request releases its
     bytes and its latency (arrival to observed completion) is recorded.
 
@@ -159,15 +160,16 @@ def _trace_alone(r, scene, stream=None):
 
 # --------------------------------------------------------------- server ----
 class Server:
-    def __init__(self, scenes, budget_bytes=2 << 30, streams=3, max_inflight=256,
-                 max_batch=128, keep_results=False):
+    def __init__(self, scenes, budget_bytes=2 << 30, streams=3, rebuild_streams=4,
+                 max_inflight=256, max_batch=128, keep_results=False):
         self.scenes = scenes
         self.keep_results = keep_results
         self.budget = budget_bytes
         self.max_inflight = max_inflight
         self.max_batch = max_batch
         self.streams = [torch.cuda.Stream() for _ in range(streams)]
-        self.rebuild_stream = torch.cuda.Stream()
+        self.rebuild_streams = [torch.cuda.Stream() for _ in range(rebuild_streams)]
+        self.rebuilds = 0
         self.used = 0
         self.peak = 0
         self.launches = 0
@@ -210,16 +212,17 @@ class Server:
                 ev = torch.cuda.Event()
                 ev.record(s)
             groups.append((ev, ray_reqs))
-        rb = [r for r in took if r.kind == "rebuild"]
-        if rb:
-            s = self.rebuild_stream
+        # rebuilds: a small build is a chain of short kernels, so each goes to
+        # the next of several streams and independent rebuilds overlap
+        for r in (r for r in took if r.kind == "rebuild"):
+            s = self.rebuild_streams[self.rebuilds % len(self.rebuild_streams)]
+            self.rebuilds += 1
             with torch.cuda.stream(s):
-                for r in rb:
-                    xyz, r.bvh = _rebuild(r, stream=s)
-                    r.keep.append(xyz)
+                xyz, r.bvh = _rebuild(r, stream=s)
+                r.keep.append(xyz)
                 ev = torch.cuda.Event()
                 ev.record(s)
-            groups.append((ev, rb))
+            groups.append((ev, [r]))
         return groups
 
     def run(self, requests):
