@@ -249,6 +249,50 @@ rt_status rt_trace_batch(rt_stream stream, const rt_trace_item* d_items,
                          const uint32_t* d_chunk_start, uint32_t count, uint32_t total_chunks,
                          uint32_t depth);
 
+/* ---- two-level scene (the reference traces through a TLAS of instances:
+ * src/main.cpp:920-1000, fullframe_benchmark.cpp:207-219) ------------------
+ * A flat top level: every ray visits the instances in order, each instance a
+ * BLAS placed by a world-to-object affine map.  Closest hit carries the best
+ * t from instance to instance (a later instance must be strictly nearer, so
+ * ties keep the lower instance); any hit stops at the first occluding
+ * instance.  An instance's result equals rt_trace of its BLAS on the mapped
+ * ray with tmax = the best t so far.  d_inst (closest, optional) receives the
+ * instance index of the hit (RT_MISS on a miss).                            */
+#define RT_INST_GENERAL 0u   /* o' = M o + T, d' = M d (M = 3x3 part)   */
+#define RT_INST_TRANSLATE 1u /* M is the identity: o' = o + T, d' = d   */
+#define RT_INST_IDENTITY 2u  /* world space is object space             */
+typedef struct rt_instance {
+    rt_bvh_dev blas;
+    const float* pos;          /* the instance's mesh, for hit normals: vertices */
+    const uint32_t* idx;       /* triangles; NULL = 9 floats per triangle in pos */
+    float world_to_object[12]; /* row-major 3x4 [M | T] */
+    uint32_t kind;             /* RT_INST_* */
+    uint32_t reserved[7];
+} rt_instance;
+rt_status rt_trace_scene(rt_stream stream, const rt_instance* d_instances, uint32_t count,
+                         uint32_t max_depth, const rt_rays_soa* rays, uint32_t n,
+                         uint32_t flags, rt_hits_soa* hits, uint32_t* d_inst,
+                         uint32_t* occl_bits);
+
+/* ---- frame post-processing (shaders/rq_ao.comp, shaders/bilateral_blur.comp) */
+/* AO rays of rt_gen_ao_rays for a two-level scene: the hit normal comes from
+ * instance d_inst[p]'s mesh (mapped to world space for RT_INST_GENERAL).    */
+rt_status rt_gen_ao_rays_scene(rt_stream stream, const rt_instance* d_instances, uint32_t count,
+                               const rt_rays_soa* primary, const uint32_t* d_prim,
+                               const uint32_t* d_inst, const float* d_t, uint32_t w,
+                               uint32_t h, uint32_t frame, float offset, float tmin, float tmax,
+                               const rt_rays_out* out);
+/* 1 SPP AO: ao[p] = 1 for a primary miss, else 0 if the AO ray is occluded. */
+rt_status rt_ao_resolve(rt_stream stream, const uint32_t* d_bits, const uint32_t* d_prim,
+                        uint32_t npix, float* d_ao);
+/* 7x7 depth-gated bilateral filter: spatial weights w[|i|] w[|j|] from the
+ * first four taps of bilateral_blur.comp:28-31, a neighbour counts iff it is
+ * inside the image and |depth - centre depth| < depth_thr
+ * (bilateral_blur.comp:41,49); out = sum / max(total, 0.001); pixels with
+ * depth >= 1e30 (background) pass through (bilateral_blur.comp:23-26).     */
+rt_status rt_blur7(rt_stream stream, const float* d_in, const float* d_depth, uint32_t w,
+                   uint32_t h, float depth_thr, float* d_out);
+
 /* ---- generators (synthetic inputs made on the device) -------------------- */
 rt_status rt_gen_sphere(rt_stream stream, uint32_t nu, uint32_t nv, float radius, float disp,
                         uint32_t seed, float* d_pos, uint32_t* d_idx);

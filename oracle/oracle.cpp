@@ -697,11 +697,13 @@ void or_camera_rays(uint32_t w, uint32_t h, const float eye[3], const float look
 }
 
 // Geometric normal of triangle prim facing against the incoming direction.
+// Unit normal of triangle prim (indexed, or 9 floats per triangle when idx
+// is null), turned to face against d.
 static inline void face_normal(const float* pos, const uint32_t* idx, uint32_t prim,
                                float ddx, float ddy, float ddz, float n[3]) {
-    const float* p0 = pos + 3 * idx[prim * 3 + 0];
-    const float* p1 = pos + 3 * idx[prim * 3 + 1];
-    const float* p2 = pos + 3 * idx[prim * 3 + 2];
+    const float* p0 = idx ? pos + 3 * (size_t)idx[prim * 3 + 0] : pos + 9 * (size_t)prim;
+    const float* p1 = idx ? pos + 3 * (size_t)idx[prim * 3 + 1] : p0 + 3;
+    const float* p2 = idx ? pos + 3 * (size_t)idx[prim * 3 + 2] : p0 + 6;
     float e1x = p1[0] - p0[0], e1y = p1[1] - p0[1], e1z = p1[2] - p0[2];
     float e2x = p2[0] - p0[0], e2y = p2[1] - p0[1], e2z = p2[2] - p0[2];
     float nx = e1y * e2z - e1z * e2y;
@@ -757,9 +759,46 @@ static inline uint32_t ao_hash(uint32_t x) {
     return x;
 }
 
-// One cosine-weighted AO ray per pixel (rq_ao.comp:70-87): seed
-// y*W + x + 5544 + frame*W*H, origin P + offset N, [tmin, tmax).  Missed
-// pixels get an empty interval.
+// One cosine-weighted AO ray for pixel p about the unit normal n
+// (rq_ao.comp:32-97, 70-87): seed y*W + x + 5544 + frame*W*H, origin
+// P + offset n, [tmin, tmax).  Missed pixels get an empty interval.
+static void ao_ray(uint32_t p, const float n[3], float Px, float Py, float Pz, uint32_t w,
+                   uint32_t h, uint32_t frame, float offset, float tmin, float tmax,
+                   float* ox, float* oy, float* oz, float* tn,
+                   float* dx, float* dy, float* dz, float* tx) {
+    uint32_t seed = p + 5544u + frame * w * h;
+    seed = ao_hash(seed);
+    float u1 = (float)seed / 4294967295.0f;
+    seed = ao_hash(seed);
+    float u2 = (float)seed / 4294967295.0f;
+    float r = std::sqrt(u1);
+    float th = 6.2831853f * u2;
+    float lx = r * std::cos(th), ly = r * std::sin(th);
+    float lz = std::sqrt(std::max(0.0f, 1.0f - u1));
+    float upx = 0.0f, upy = 0.0f, upz = 1.0f;
+    if (!(std::fabs(n[2]) < 0.999f)) { upx = 1.0f; upz = 0.0f; }
+    float tx0 = upy * n[2] - upz * n[1];
+    float ty0 = upz * n[0] - upx * n[2];
+    float tz0 = upx * n[1] - upy * n[0];
+    normalize3(tx0, ty0, tz0);
+    float bx = n[1] * tz0 - n[2] * ty0;
+    float by = n[2] * tx0 - n[0] * tz0;
+    float bz = n[0] * ty0 - n[1] * tx0;
+    float d0 = tx0 * lx + bx * ly + n[0] * lz;
+    float d1 = ty0 * lx + by * ly + n[1] * lz;
+    float d2 = tz0 * lx + bz * ly + n[2] * lz;
+    normalize3(d0, d1, d2);
+    ox[p] = Px + offset * n[0]; oy[p] = Py + offset * n[1]; oz[p] = Pz + offset * n[2];
+    tn[p] = tmin; tx[p] = tmax;
+    dx[p] = d0; dy[p] = d1; dz[p] = d2;
+}
+
+static void empty_ray(uint32_t p, float* ox, float* oy, float* oz, float* tn, float* dx,
+                      float* dy, float* dz, float* tx) {
+    ox[p] = 0.0f; oy[p] = 0.0f; oz[p] = 0.0f; tn[p] = 1.0f;
+    dx[p] = 0.0f; dy[p] = 0.0f; dz[p] = 1.0f; tx[p] = 0.0f;
+}
+
 void or_ao_rays(const float* pos, const uint32_t* idx,
                 const float* pox, const float* poy, const float* poz,
                 const float* pdx, const float* pdy, const float* pdz,
@@ -769,40 +808,49 @@ void or_ao_rays(const float* pos, const uint32_t* idx,
                 float* dx, float* dy, float* dz, float* tx) {
     for (uint32_t p = 0; p < w * h; ++p) {
         if (prim[p] == 0xFFFFFFFFu) {
-            ox[p] = 0.0f; oy[p] = 0.0f; oz[p] = 0.0f; tn[p] = 1.0f;
-            dx[p] = 0.0f; dy[p] = 0.0f; dz[p] = 1.0f; tx[p] = 0.0f;
+            empty_ray(p, ox, oy, oz, tn, dx, dy, dz, tx);
             continue;
         }
         float n[3];
         face_normal(pos, idx, prim[p], pdx[p], pdy[p], pdz[p], n);
-        uint32_t seed = p + 5544u + frame * w * h;
-        seed = ao_hash(seed);
-        float u1 = (float)seed / 4294967295.0f;
-        seed = ao_hash(seed);
-        float u2 = (float)seed / 4294967295.0f;
-        float r = std::sqrt(u1);
-        float th = 6.2831853f * u2;
-        float lx = r * std::cos(th), ly = r * std::sin(th);
-        float lz = std::sqrt(std::max(0.0f, 1.0f - u1));
-        float upx = 0.0f, upy = 0.0f, upz = 1.0f;
-        if (!(std::fabs(n[2]) < 0.999f)) { upx = 1.0f; upz = 0.0f; }
-        float tx0 = upy * n[2] - upz * n[1];
-        float ty0 = upz * n[0] - upx * n[2];
-        float tz0 = upx * n[1] - upy * n[0];
-        normalize3(tx0, ty0, tz0);
-        float bx = n[1] * tz0 - n[2] * ty0;
-        float by = n[2] * tx0 - n[0] * tz0;
-        float bz = n[0] * ty0 - n[1] * tx0;
-        float d0 = tx0 * lx + bx * ly + n[0] * lz;
-        float d1 = ty0 * lx + by * ly + n[1] * lz;
-        float d2 = tz0 * lx + bz * ly + n[2] * lz;
-        normalize3(d0, d1, d2);
-        float Px = pox[p] + t[p] * pdx[p];
-        float Py = poy[p] + t[p] * pdy[p];
-        float Pz = poz[p] + t[p] * pdz[p];
-        ox[p] = Px + offset * n[0]; oy[p] = Py + offset * n[1]; oz[p] = Pz + offset * n[2];
-        tn[p] = tmin; tx[p] = tmax;
-        dx[p] = d0; dy[p] = d1; dz[p] = d2;
+        ao_ray(p, n, pox[p] + t[p] * pdx[p], poy[p] + t[p] * pdy[p], poz[p] + t[p] * pdz[p], w, h,
+               frame, offset, tmin, tmax, ox, oy, oz, tn, dx, dy, dz, tx);
+    }
+}
+
+// AO rays over a two-level scene (the device's k_ao_scene): the normal of
+// instance inst[p]'s hit triangle, oriented against the mapped direction,
+// mapped back by M^T for a general instance (kind 0).
+void or_ao_rays_scene(const or_instance* insts, uint32_t count,
+                      const float* pox, const float* poy, const float* poz,
+                      const float* pdx, const float* pdy, const float* pdz,
+                      const uint32_t* prim, const uint32_t* inst, const float* t, uint32_t w,
+                      uint32_t h, uint32_t frame, float offset, float tmin, float tmax,
+                      float* ox, float* oy, float* oz, float* tn,
+                      float* dx, float* dy, float* dz, float* tx) {
+    for (uint32_t p = 0; p < w * h; ++p) {
+        if (prim[p] == 0xFFFFFFFFu || inst[p] >= count) {
+            empty_ray(p, ox, oy, oz, tn, dx, dy, dz, tx);
+            continue;
+        }
+        const or_instance& I = insts[inst[p]];
+        float n[3];
+        if (I.kind == 0) {
+            const float* m = I.world_to_object;
+            const float ex = m[0] * pdx[p] + m[1] * pdy[p] + m[2] * pdz[p];
+            const float ey = m[4] * pdx[p] + m[5] * pdy[p] + m[6] * pdz[p];
+            const float ez = m[8] * pdx[p] + m[9] * pdy[p] + m[10] * pdz[p];
+            float q[3];
+            face_normal(I.pos, I.idx, prim[p], ex, ey, ez, q);
+            n[0] = m[0] * q[0] + m[4] * q[1] + m[8] * q[2];
+            n[1] = m[1] * q[0] + m[5] * q[1] + m[9] * q[2];
+            n[2] = m[2] * q[0] + m[6] * q[1] + m[10] * q[2];
+            normalize3(n[0], n[1], n[2]);
+        } else {
+            face_normal(I.pos, I.idx, prim[p], pdx[p], pdy[p], pdz[p], n);
+        }
+        ao_ray(p, n, pox[p] + t[p] * pdx[p], poy[p] + t[p] * pdy[p], poz[p] + t[p] * pdz[p], w, h,
+               frame, offset, tmin, tmax, ox, oy, oz, tn, dx, dy, dz, tx);
     }
 }
 

@@ -120,6 +120,22 @@ void or_ao_rays(const float* pos, const uint32_t* idx,
                 uint32_t frame, float offset, float tmin, float tmax,
                 float* ox, float* oy, float* oz, float* tn,
                 float* dx, float* dy, float* dz, float* tx);
+/* A two-level scene instance as the device's rt_instance: mesh for normals,
+ * row-major 3x4 world-to-object map, kind 0 general / 1 translate /
+ * 2 identity. */
+typedef struct or_instance {
+    const float* pos;
+    const uint32_t* idx;
+    float world_to_object[12];
+    uint32_t kind;
+} or_instance;
+void or_ao_rays_scene(const or_instance* insts, uint32_t count,
+                      const float* pox, const float* poy, const float* poz,
+                      const float* pdx, const float* pdy, const float* pdz,
+                      const uint32_t* prim, const uint32_t* inst, const float* t, uint32_t w,
+                      uint32_t h, uint32_t frame, float offset, float tmin, float tmax,
+                      float* ox, float* oy, float* oz, float* tn,
+                      float* dx, float* dy, float* dz, float* tx);
 void or_random_rays(uint32_t count, uint32_t seed, const float lo[3], const float hi[3],
                     float tmax, float* ox, float* oy, float* oz, float* tn,
                     float* dx, float* dy, float* dz, float* tx);

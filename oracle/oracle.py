@@ -261,10 +261,85 @@ def ao_rays(pos, idx, prim_rays, prim, t, w, h, frame=0, offset=0.003, tmin=0.00
             tmax=0.5):
     r = Rays(w * h)
     pr = prim_rays
-    lib().or_ao_rays(_p(_f(pos)), _p(_u(idx)), _p(pr.ox), _p(pr.oy), _p(pr.oz),
+    lib().or_ao_rays(_p(_f(pos)), _p(None if idx is None else _u(idx)), _p(pr.ox), _p(pr.oy),
+                     _p(pr.oz),
                      _p(pr.dx), _p(pr.dy), _p(pr.dz), _p(_u(prim)), _p(_f(t)), u32(w), u32(h),
                      u32(frame), f32(offset), f32(tmin), f32(tmax),
                      *[_p(a) for a in r.arrays()])
+    return r
+
+
+class _Inst(C.Structure):
+    _fields_ = [("pos", C.c_void_p), ("idx", C.c_void_p), ("world_to_object", C.c_float * 12),
+                ("kind", C.c_uint32)]
+
+
+INST_GENERAL, INST_TRANSLATE, INST_IDENTITY = 0, 1, 2
+
+
+def map_rays(rays, m, kind):
+    """World-to-object map of a ray batch as the device applies it
+    (float32, ((m0 x + m1 y) + m2 z) + t; direction by the 3x3 part)."""
+    if kind == INST_IDENTITY:
+        return rays
+    m = np.asarray(m, np.float32).reshape(3, 4)
+    o = [rays.ox, rays.oy, rays.oz]
+    d = [rays.dx, rays.dy, rays.dz]
+    if kind == INST_TRANSLATE:
+        mo = [o[a] + m[a, 3] for a in range(3)]
+        md = d
+    else:
+        mo = [m[a, 0] * o[0] + m[a, 1] * o[1] + m[a, 2] * o[2] + m[a, 3] for a in range(3)]
+        md = [m[a, 0] * d[0] + m[a, 1] * d[1] + m[a, 2] * d[2] for a in range(3)]
+    return Rays(ox=mo[0], oy=mo[1], oz=mo[2], tmin=rays.tmin, dx=md[0], dy=md[1], dz=md[2],
+                tmax=rays.tmax)
+
+
+def trace_scene(instances, rays, mode=CLOSEST):
+    """Two-level scene on the CPU path: instances = [(nodes4, tris64, m, kind)]
+    visited in order; each BLAS traced (or_trace_bvh4) on the mapped ray with
+    tmax = the best t so far (closest), any hit stops at the first occluding
+    instance.  Returns (prim, t, inst) for closest, the occluded mask for any."""
+    n = len(rays)
+    best_t = rays.tmax.copy()
+    prim = np.full(n, MISS, np.uint32)
+    inst = np.full(n, MISS, np.uint32)
+    occ = np.zeros(n, bool)
+    for i, (nodes, tris, m, kind) in enumerate(instances):
+        r = map_rays(rays, m, kind)
+        if mode == ANY:
+            live = ~occ
+            if not live.any():
+                break
+            sub = r.subset(live)
+            p, _ = trace_bvh4(nodes, tris, sub, mode=ANY)
+            occ[np.nonzero(live)[0][p != MISS]] = True
+        else:
+            r = Rays(ox=r.ox, oy=r.oy, oz=r.oz, tmin=r.tmin, dx=r.dx, dy=r.dy, dz=r.dz,
+                     tmax=best_t)
+            p, t = trace_bvh4(nodes, tris, r)
+            hit = p != MISS
+            prim[hit], best_t[hit], inst[hit] = p[hit], t[hit], i
+    return occ if mode == ANY else (prim, best_t, inst)
+
+
+def ao_rays_scene(instances, prim_rays, prim, inst, t, w, h, frame=0, offset=0.003,
+                  tmin=0.001, tmax=0.5):
+    """instances = [(pos, idx or None, m (3x4), kind)] (the device's
+    rt_instance meshes); or_ao_rays_scene."""
+    keep = [(_f(pos), None if idx is None else _u(idx)) for pos, idx, _, _ in instances]
+    arr = (_Inst * len(instances))()
+    for k, (pos, idx, m, kind) in enumerate(instances):
+        arr[k].pos = _p(keep[k][0])
+        arr[k].idx = _p(keep[k][1])
+        arr[k].world_to_object = (C.c_float * 12)(*np.asarray(m, np.float32).reshape(-1))
+        arr[k].kind = kind
+    r = Rays(w * h)
+    pr = prim_rays
+    lib().or_ao_rays_scene(arr, u32(len(instances)), _p(pr.ox), _p(pr.oy), _p(pr.oz),
+                           _p(pr.dx), _p(pr.dy), _p(pr.dz), _p(_u(prim)), _p(_u(inst)),
+                           _p(_f(t)), u32(w), u32(h), u32(frame), f32(offset), f32(tmin),
+                           f32(tmax), *[_p(a) for a in r.arrays()])
     return r
 
 

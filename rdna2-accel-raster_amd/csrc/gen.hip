@@ -128,12 +128,21 @@ __global__ void k_camera(uint32_t w, uint32_t h, CamBasis cb, rt_rays_out o) {
     put_ray(o, i, cb.eye[0], cb.eye[1], cb.eye[2], cb.tmin, d0, d1, d2, cb.tmax);
 }
 
+// Unit normal of triangle prim (indexed, or 9 floats per triangle when idx
+// is null), turned to face against the direction d.
 __device__ __forceinline__ void face_normal(const float* __restrict__ pos,
                                             const uint32_t* __restrict__ idx, uint32_t prim,
                                             float ddx, float ddy, float ddz, float n[3]) {
-    const float* p0 = pos + 3 * (size_t)idx[3 * (size_t)prim + 0];
-    const float* p1 = pos + 3 * (size_t)idx[3 * (size_t)prim + 1];
-    const float* p2 = pos + 3 * (size_t)idx[3 * (size_t)prim + 2];
+    const float *p0, *p1, *p2;
+    if (idx) {
+        p0 = pos + 3 * (size_t)idx[3 * (size_t)prim + 0];
+        p1 = pos + 3 * (size_t)idx[3 * (size_t)prim + 1];
+        p2 = pos + 3 * (size_t)idx[3 * (size_t)prim + 2];
+    } else {
+        p0 = pos + 9 * (size_t)prim;
+        p1 = p0 + 3;
+        p2 = p0 + 6;
+    }
     const float e1x = p1[0] - p0[0], e1y = p1[1] - p0[1], e1z = p1[2] - p0[2];
     const float e2x = p2[0] - p0[0], e2y = p2[1] - p0[1], e2z = p2[2] - p0[2];
     float nx = e1y * e2z - e1z * e2y;
@@ -185,20 +194,11 @@ __device__ __forceinline__ uint32_t ao_hash(uint32_t x) {
 }
 
 // One cosine-weighted hemisphere ray per pixel (rq_ao.comp:32-45,70-87).
-__global__ void k_ao(const float* __restrict__ pos, const uint32_t* __restrict__ idx,
-                     rt_rays_soa pr, const uint32_t* __restrict__ prim,
-                     const float* __restrict__ tt, uint32_t w, uint32_t h, uint32_t frame,
-                     float offset, float tmin, float tmax, rt_rays_out o) {
-    const uint32_t p = blockIdx.x * kB + threadIdx.x;
-    if (p >= w * h) return;
-    const uint32_t hp = prim[p];
-    if (hp == RT_MISS) {
-        put_ray(o, p, 0.f, 0.f, 0.f, 1.f, 0.f, 0.f, 1.f, 0.f);
-        return;
-    }
-    const float dx = pr.dx[p], dy = pr.dy[p], dz = pr.dz[p];
-    float n[3];
-    face_normal(pos, idx, hp, dx, dy, dz, n);
+// One cosine-weighted hemisphere ray (rq_ao.comp:32-97) about the unit
+// normal n for pixel p of a w x h frame, from P + offset n.
+__device__ __forceinline__ void ao_ray(uint32_t p, const float n[3], float Px, float Py,
+                                       float Pz, uint32_t w, uint32_t h, uint32_t frame,
+                                       float offset, float tmin, float tmax, rt_rays_out o) {
     uint32_t seed = p + 5544u + frame * w * h;
     seed = ao_hash(seed);
     const float u1 = (float)seed / 4294967295.0f;
@@ -224,10 +224,114 @@ __global__ void k_ao(const float* __restrict__ pos, const uint32_t* __restrict__
     float d1 = ty0 * lx + by * ly + n[1] * lz;
     float d2 = tz0 * lx + bz * ly + n[2] * lz;
     normalize3(d0, d1, d2);
-    const float t = tt[p];
-    const float Px = pr.ox[p] + t * dx, Py = pr.oy[p] + t * dy, Pz = pr.oz[p] + t * dz;
     put_ray(o, p, Px + offset * n[0], Py + offset * n[1], Pz + offset * n[2], tmin, d0, d1, d2,
             tmax);
+}
+
+__global__ void k_ao(const float* __restrict__ pos, const uint32_t* __restrict__ idx,
+                     rt_rays_soa pr, const uint32_t* __restrict__ prim,
+                     const float* __restrict__ tt, uint32_t w, uint32_t h, uint32_t frame,
+                     float offset, float tmin, float tmax, rt_rays_out o) {
+    const uint32_t p = blockIdx.x * kB + threadIdx.x;
+    if (p >= w * h) return;
+    const uint32_t hp = prim[p];
+    if (hp == RT_MISS) {
+        put_ray(o, p, 0.f, 0.f, 0.f, 1.f, 0.f, 0.f, 1.f, 0.f);
+        return;
+    }
+    const float dx = pr.dx[p], dy = pr.dy[p], dz = pr.dz[p];
+    float n[3];
+    face_normal(pos, idx, hp, dx, dy, dz, n);
+    const float t = tt[p];
+    ao_ray(p, n, pr.ox[p] + t * dx, pr.oy[p] + t * dy, pr.oz[p] + t * dz, w, h, frame, offset,
+           tmin, tmax, o);
+}
+
+// AO rays over a two-level scene: the normal of the hit triangle of
+// instance inst[p], oriented against the mapped direction in object space,
+// then mapped to world space by M^T (the inverse transpose of the
+// object-to-world map) for a general instance.
+__global__ void k_ao_scene(const rt_instance* __restrict__ inst, rt_rays_soa pr,
+                           const uint32_t* __restrict__ prim, const uint32_t* __restrict__ hit_inst,
+                           const float* __restrict__ tt, uint32_t w, uint32_t h, uint32_t frame,
+                           float offset, float tmin, float tmax, rt_rays_out o) {
+    const uint32_t p = blockIdx.x * kB + threadIdx.x;
+    if (p >= w * h) return;
+    const uint32_t hp = prim[p];
+    if (hp == RT_MISS) {
+        put_ray(o, p, 0.f, 0.f, 0.f, 1.f, 0.f, 0.f, 1.f, 0.f);
+        return;
+    }
+    const rt_instance& I = inst[hit_inst[p]];
+    const float dx = pr.dx[p], dy = pr.dy[p], dz = pr.dz[p];
+    float n[3];
+    if (I.kind == RT_INST_GENERAL) {
+        const float* m = I.world_to_object;
+        const float ex = m[0] * dx + m[1] * dy + m[2] * dz;
+        const float ey = m[4] * dx + m[5] * dy + m[6] * dz;
+        const float ez = m[8] * dx + m[9] * dy + m[10] * dz;
+        float q[3];
+        face_normal(I.pos, I.idx, hp, ex, ey, ez, q);
+        n[0] = m[0] * q[0] + m[4] * q[1] + m[8] * q[2];
+        n[1] = m[1] * q[0] + m[5] * q[1] + m[9] * q[2];
+        n[2] = m[2] * q[0] + m[6] * q[1] + m[10] * q[2];
+        normalize3(n[0], n[1], n[2]);
+    } else {
+        face_normal(I.pos, I.idx, hp, dx, dy, dz, n);
+    }
+    const float t = tt[p];
+    ao_ray(p, n, pr.ox[p] + t * dx, pr.oy[p] + t * dy, pr.oz[p] + t * dz, w, h, frame, offset,
+           tmin, tmax, o);
+}
+
+// rq_ao.comp's 1 SPP resolve: 1 for a primary miss, else 0 / 1 by occlusion.
+__global__ void k_ao_resolve(const uint32_t* __restrict__ bits, const uint32_t* __restrict__ prim,
+                             uint32_t npix, float* __restrict__ ao) {
+    const uint32_t p = blockIdx.x * kB + threadIdx.x;
+    if (p >= npix) return;
+    const bool occ = (bits[p >> 5] >> (p & 31u)) & 1u;
+    ao[p] = prim[p] == RT_MISS ? 1.0f : (occ ? 0.0f : 1.0f);
+}
+
+// 7x7 depth-gated bilateral filter over a 16 x 16 tile per workgroup, the
+// tile and its 3-pixel apron staged in LDS.  Taps accumulate row by row,
+// left to right, as the CPU filter does, so sums round identically.
+constexpr int kBT = 16;
+__global__ __launch_bounds__(kBT * kBT) void k_blur7(const float* __restrict__ in,
+                                                      const float* __restrict__ depth,
+                                                      uint32_t w, uint32_t h, float thr,
+                                                      float* __restrict__ out) {
+    constexpr int A = kBT + 6;
+    __shared__ float si[A][A], sd[A][A];
+    const int x0 = (int)(blockIdx.x * kBT) - 3, y0 = (int)(blockIdx.y * kBT) - 3;
+    for (int k = threadIdx.y * kBT + threadIdx.x; k < A * A; k += kBT * kBT) {
+        const int ly = k / A, lx = k % A, gx = x0 + lx, gy = y0 + ly;
+        const bool inside = gx >= 0 && gy >= 0 && gx < (int)w && gy < (int)h;
+        si[ly][lx] = inside ? in[(size_t)gy * w + gx] : 0.0f;
+        sd[ly][lx] = inside ? depth[(size_t)gy * w + gx] : 0.0f;
+    }
+    __syncthreads();
+    const int x = (int)(blockIdx.x * kBT + threadIdx.x), y = (int)(blockIdx.y * kBT + threadIdx.y);
+    if (x >= (int)w || y >= (int)h) return;
+    const float cd = sd[threadIdx.y + 3][threadIdx.x + 3];
+    if (cd >= 1e30f) {
+        out[(size_t)y * w + x] = si[threadIdx.y + 3][threadIdx.x + 3];
+        return;
+    }
+    const float wt[4] = {0.227027f, 0.1945946f, 0.1216216f, 0.054054f};
+    float acc = 0.0f, tot = 0.0f;
+    for (int j = -3; j <= 3; ++j) {
+        if (y + j < 0 || y + j >= (int)h) continue;
+        for (int i = -3; i <= 3; ++i) {
+            if (x + i < 0 || x + i >= (int)w) continue;
+            const float dq = sd[threadIdx.y + 3 + j][threadIdx.x + 3 + i];
+            if (!(fabsf(dq - cd) < thr)) continue;
+            const float ww = wt[i < 0 ? -i : i] * wt[j < 0 ? -j : j];
+            acc = acc + si[threadIdx.y + 3 + j][threadIdx.x + 3 + i] * ww;
+            tot = tot + ww;
+        }
+    }
+    out[(size_t)y * w + x] = acc / fmaxf(tot, 0.001f);
 }
 
 __global__ void k_random_rays(uint32_t count, uint32_t seed, float lx, float ly, float lz,
@@ -375,11 +479,44 @@ extern "C" rt_status rt_gen_ao_rays(rt_stream stream, const float* d_pos, const 
                                     const float* d_t, uint32_t w, uint32_t h, uint32_t frame,
                                     float offset, float tmin, float tmax,
                                     const rt_rays_out* out) {
-    if (!d_pos || !d_idx || !primary || !d_prim || !d_t || !out) return RT_ERR_ARG;
+    if (!d_pos || !primary || !d_prim || !d_t || !out) return RT_ERR_ARG;  // d_idx may be null
     if ((uint64_t)w * h == 0) return RT_OK;
     hipLaunchKernelGGL(k_ao, dim3(ceil_div((uint64_t)w * h, kB)), dim3(kB), 0, as_stream(stream),
                        d_pos, d_idx, *primary, d_prim, d_t, w, h, frame, offset, tmin, tmax, *out);
     return launch_status("rt_gen_ao_rays");
+}
+
+extern "C" rt_status rt_gen_ao_rays_scene(rt_stream stream, const rt_instance* d_instances,
+                                          uint32_t count, const rt_rays_soa* primary,
+                                          const uint32_t* d_prim, const uint32_t* d_inst,
+                                          const float* d_t, uint32_t w, uint32_t h,
+                                          uint32_t frame, float offset, float tmin, float tmax,
+                                          const rt_rays_out* out) {
+    if (!d_instances || count == 0 || !primary || !d_prim || !d_inst || !d_t || !out)
+        return RT_ERR_ARG;
+    if ((uint64_t)w * h == 0) return RT_OK;
+    hipLaunchKernelGGL(k_ao_scene, dim3(ceil_div((uint64_t)w * h, kB)), dim3(kB), 0,
+                       as_stream(stream), d_instances, *primary, d_prim, d_inst, d_t, w, h, frame,
+                       offset, tmin, tmax, *out);
+    return launch_status("rt_gen_ao_rays_scene");
+}
+
+extern "C" rt_status rt_ao_resolve(rt_stream stream, const uint32_t* d_bits,
+                                   const uint32_t* d_prim, uint32_t npix, float* d_ao) {
+    if (npix == 0) return RT_OK;
+    if (!d_bits || !d_prim || !d_ao) return RT_ERR_ARG;
+    hipLaunchKernelGGL(k_ao_resolve, dim3(ceil_div(npix, kB)), dim3(kB), 0, as_stream(stream),
+                       d_bits, d_prim, npix, d_ao);
+    return launch_status("rt_ao_resolve");
+}
+
+extern "C" rt_status rt_blur7(rt_stream stream, const float* d_in, const float* d_depth,
+                              uint32_t w, uint32_t h, float depth_thr, float* d_out) {
+    if ((uint64_t)w * h == 0) return RT_OK;
+    if (!d_in || !d_depth || !d_out || d_in == d_out) return RT_ERR_ARG;
+    hipLaunchKernelGGL(k_blur7, dim3(ceil_div(w, kBT), ceil_div(h, kBT)), dim3(kBT, kBT), 0,
+                       as_stream(stream), d_in, d_depth, w, h, depth_thr, d_out);
+    return launch_status("rt_blur7");
 }
 
 extern "C" rt_status rt_gen_random_rays(rt_stream stream, uint32_t count, uint32_t seed,

@@ -304,6 +304,69 @@ __global__ __launch_bounds__(kBlock) void k_trace_batch(const rt_trace_item* __r
     }
 }
 
+// Two-level scene: every ray visits the instances in order; instance i's
+// BLAS is traversed on the mapped ray with hitT = the best t so far, so a
+// later instance must be strictly nearer (ties keep the lower instance), and
+// an any-hit ray stops at the first occluding instance.  Instance records
+// are wave-uniform (scalar loads); the map is applied as written, in the
+// order ((m0 x + m1 y) + m2 z) + t.
+template <int PRIV, bool ANY>
+__global__ __launch_bounds__(kBlock) void k_trace_scene(const rt_instance* __restrict__ inst,
+                                                        uint32_t count, rt_rays_soa rays,
+                                                        uint32_t n, uint32_t* __restrict__ prim_out,
+                                                        float* __restrict__ t_out,
+                                                        uint32_t* __restrict__ inst_out,
+                                                        uint32_t* __restrict__ bits_out) {
+    __shared__ int32_t lds_stack[kLdsStack * kBlock];
+    const uint32_t tid = threadIdx.x;
+    const uint32_t r = blockIdx.x * kBlock + tid;
+    const bool valid = r < n;
+    float ox = 0.0f, oy = 0.0f, oz = 0.0f, tmin = 1.0f;
+    float dx = 0.0f, dy = 0.0f, dz = 1.0f, tmax = 0.0f;
+    if (valid) {
+        ox = rays.ox[r]; oy = rays.oy[r]; oz = rays.oz[r]; tmin = rays.tmin[r];
+        dx = rays.dx[r]; dy = rays.dy[r]; dz = rays.dz[r]; tmax = rays.tmax[r];
+    }
+    float best = tmax;
+    uint32_t bprim = RT_MISS, binst = RT_MISS;
+    for (uint32_t i = 0; i < count; ++i) {
+        const rt_instance& I = inst[i];
+        float mox = ox, moy = oy, moz = oz, mdx = dx, mdy = dy, mdz = dz;
+        if (I.kind == RT_INST_TRANSLATE) {
+            const float* m = I.world_to_object;
+            mox = ox + m[3];
+            moy = oy + m[7];
+            moz = oz + m[11];
+        } else if (I.kind == RT_INST_GENERAL) {
+            const float* m = I.world_to_object;
+            mox = m[0] * ox + m[1] * oy + m[2] * oz + m[3];
+            moy = m[4] * ox + m[5] * oy + m[6] * oz + m[7];
+            moz = m[8] * ox + m[9] * oy + m[10] * oz + m[11];
+            mdx = m[0] * dx + m[1] * dy + m[2] * dz;
+            mdy = m[4] * dx + m[5] * dy + m[6] * dz;
+            mdz = m[8] * dx + m[9] * dy + m[10] * dz;
+        }
+        float hitT = best;
+        uint32_t hitPrim = RT_MISS, bt = 0, tt = 0;
+        trace_ray<PRIV, ANY, false>(glb(I.blas.nodes), glb(I.blas.tris),
+                                    valid && I.blas.node_count > 0, mox, moy, moz, tmin, mdx, mdy,
+                                    mdz, lds_stack, tid, hitT, hitPrim, bt, tt);
+        if (hitPrim != RT_MISS) {
+            best = hitT;
+            bprim = hitPrim;
+            binst = i;
+            if (ANY) break;
+        }
+    }
+    if (ANY) {
+        write_wave_bits(bits_out, r - (tid & 63u), n, valid && bprim != RT_MISS);
+    } else if (valid) {
+        prim_out[r] = bprim;
+        t_out[r] = best;
+        if (inst_out) inst_out[r] = binst;
+    }
+}
+
 // Refill form ("while-while" with lane refill): each wave owns the
 // contiguous rays [w * per_wave, (w + 1) * per_wave) and, whenever at least
 // `refill` lanes have finished, hands the next rays of its range to the idle
@@ -574,4 +637,36 @@ extern "C" rt_status rt_trace_batch(rt_stream stream, const rt_trace_item* d_ite
         hipLaunchKernelGGL(k_trace_batch<3 * kMaxDepth + 1 - kLdsStack>, grid, block, 0, s,
                            d_items, d_chunk_start, count, total_chunks);
     return launch_status("k_trace_batch");
+}
+
+extern "C" rt_status rt_trace_scene(rt_stream stream, const rt_instance* d_instances,
+                                    uint32_t count, uint32_t max_depth, const rt_rays_soa* rays,
+                                    uint32_t n, uint32_t flags, rt_hits_soa* hits,
+                                    uint32_t* d_inst, uint32_t* occl_bits) {
+    if (!rays || (flags & ~RT_ANY)) return RT_ERR_ARG;
+    if (n == 0) return RT_OK;
+    if ((count && !d_instances) || !rays->ox || !rays->oy || !rays->oz || !rays->tmin ||
+        !rays->dx || !rays->dy || !rays->dz || !rays->tmax)
+        return RT_ERR_ARG;
+    const bool any = flags & RT_ANY;
+    if (any && !occl_bits) return RT_ERR_ARG;
+    if (!any && (!hits || !hits->prim || !hits->t)) return RT_ERR_ARG;
+    const uint32_t depth = max_depth ? max_depth : kMaxDepth;
+    if (depth > kMaxDepth) return RT_ERR_CAPACITY;
+    uint32_t* prim = hits ? hits->prim : nullptr;
+    float* t = hits ? hits->t : nullptr;
+    const dim3 grid(ceil_div(n, kBlock)), block(kBlock);
+    hipStream_t s = as_stream(stream);
+#define RT_S(P, A)                                                                             \
+    hipLaunchKernelGGL((k_trace_scene<P, A>), grid, block, 0, s, d_instances, count, *rays, n, \
+                       prim, t, d_inst, occl_bits)
+    if (3u * depth + 1u <= kLdsStack + 44) {
+        if (any) RT_S(44, true);
+        else RT_S(44, false);
+    } else {
+        if (any) RT_S(3 * kMaxDepth + 1 - kLdsStack, true);
+        else RT_S(3 * kMaxDepth + 1 - kLdsStack, false);
+    }
+#undef RT_S
+    return launch_status("k_trace_scene");
 }

@@ -14,6 +14,7 @@ import ctypes as C
 import math
 import os
 
+import numpy as np
 import torch
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -59,6 +60,14 @@ class rt_hits_soa(C.Structure):
 class rt_trace_item(C.Structure):
     _fields_ = [("bvh", rt_bvh_dev), ("rays", rt_rays_soa), ("count", u32), ("flags", u32),
                 ("hits", rt_hits_soa), ("occlusion_bits", vp)]
+
+
+class rt_instance(C.Structure):
+    _fields_ = [("blas", rt_bvh_dev), ("pos", vp), ("idx", vp),
+                ("world_to_object", f32 * 12), ("kind", u32), ("reserved", u32 * 7)]
+
+
+INST_GENERAL, INST_TRANSLATE, INST_IDENTITY = 0, 1, 2
 
 
 class rt_build_item(C.Structure):
@@ -117,6 +126,10 @@ _SIGS = {
     "rt_gpu_build_batch": [vp, vp, u32, u32],
     "rt_trace_batch_plan": [vp, u32, vp, vp],
     "rt_trace_batch": [vp, vp, vp, u32, u32, u32],
+    "rt_trace_scene": [vp, vp, u32, u32, vp, u32, u32, vp, vp, vp],
+    "rt_gen_ao_rays_scene": [vp, vp, u32, vp, vp, vp, vp, u32, u32, u32, f32, f32, f32, vp],
+    "rt_ao_resolve": [vp, vp, vp, u32, vp],
+    "rt_blur7": [vp, vp, vp, u32, u32, f32, vp],
     "rt_device_sync": [],
 }
 
@@ -300,7 +313,6 @@ class GpuBVH4:
         nodes4: GpuBvh4Node records (128 B each).  Either tris64 (64-B leaf
         records) or tris48 (GpuTriangle, 48 B) plus order (leaf-order original
         triangle indices), which rt_convert_tris48 turns into 64-B leaves."""
-        import numpy as np
         dev = device or _dev()
         nb = np.ascontiguousarray(nodes4).view(np.uint8).reshape(-1)
         ncount = nb.size // NODE_BYTES
@@ -480,9 +492,79 @@ class TraceBatch:
         return self.outputs
 
 
+class Scene:
+    """Two-level scene (rt_trace_scene): instances [(bvh, pos, idx, m, kind)]
+    with m a 3x4 row-major world-to-object map (None for the identity) and
+    kind INST_GENERAL / INST_TRANSLATE / INST_IDENTITY.  The instance table
+    is uploaded once; rebuild a Scene when an instance's tree changes
+    buffers (a refit or a rebuild into the same buffers needs nothing)."""
+
+    def __init__(self, instances, device=None):
+        dev = device or _dev()
+        self.instances = list(instances)
+        arr = (rt_instance * max(1, len(self.instances)))()
+        depth = 1
+        for k, (bvh, pos, idx, m, kind) in enumerate(self.instances):
+            arr[k].blas = bvh.dev()
+            arr[k].pos, arr[k].idx = ptr(pos), ptr(idx)
+            mm = [1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0] if m is None else \
+                [float(v) for v in np.asarray(m, np.float32).reshape(-1)]
+            arr[k].world_to_object = (f32 * 12)(*mm)
+            arr[k].kind = kind
+            depth = max(depth, bvh.depth or 64)
+        self.count = len(self.instances)
+        self.depth = depth
+        self.d_table = torch.frombuffer(bytearray(bytes(arr)), dtype=torch.uint8).to(dev)
+
+    def intersect(self, rays, stream=None, out=None):
+        """Closest hit: (prim, t, instance index) tensors (int32, f32, int32)."""
+        n = len(rays)
+        dev = rays.data.device
+        prim, t, inst = out if out else (torch.empty(n, dtype=torch.int32, device=dev),
+                                         torch.empty(n, dtype=torch.float32, device=dev),
+                                         torch.empty(n, dtype=torch.int32, device=dev))
+        hits = rt_hits_soa(ptr(prim), ptr(t))
+        r = rays.soa()
+        call("rt_trace_scene", stream_handle(stream), ptr(self.d_table), u32(self.count),
+             u32(self.depth), C.byref(r), u32(n), u32(RT_CLOSEST), C.byref(hits), ptr(inst),
+             None)
+        return prim, t, inst
+
+    def occluded(self, rays, stream=None, out=None):
+        n = len(rays)
+        bits = out if out is not None else torch.empty(max(1, (n + 31) // 32),
+                                                       dtype=torch.int32, device=rays.data.device)
+        r = rays.soa()
+        call("rt_trace_scene", stream_handle(stream), ptr(self.d_table), u32(self.count),
+             u32(self.depth), C.byref(r), u32(n), u32(RT_ANY), None, None, ptr(bits))
+        return bits
+
+    def gen_ao_rays(self, primary, prim, inst, t, w, h, frame=0, offset=0.003, tmin=0.001,
+                    tmax=0.5, stream=None, out=None):
+        rays = out if out is not None else Rays.empty(w * h)
+        p, o = primary.soa(), rays.out()
+        call("rt_gen_ao_rays_scene", stream_handle(stream), ptr(self.d_table), u32(self.count),
+             C.byref(p), ptr(prim), ptr(inst), ptr(t), u32(w), u32(h), u32(frame), f32(offset),
+             f32(tmin), f32(tmax), C.byref(o))
+        return rays
+
+
+def ao_resolve(bits, prim, stream=None, out=None):
+    n = prim.numel()
+    ao = out if out is not None else torch.empty(n, dtype=torch.float32, device=prim.device)
+    call("rt_ao_resolve", stream_handle(stream), ptr(bits), ptr(prim), u32(n), ptr(ao))
+    return ao
+
+
+def blur7(img, depth, w, h, thr=0.05, stream=None, out=None):
+    o = out if out is not None else torch.empty_like(img)
+    call("rt_blur7", stream_handle(stream), ptr(img), ptr(depth), u32(w), u32(h), f32(thr),
+         ptr(o))
+    return o
+
+
 def _depth_host(nodes4):
     """Node levels on the longest path of a host tree (numpy structured)."""
-    import numpy as np
     if len(nodes4) == 0:
         return 0
     a = nodes4.view(np.uint8).reshape(-1, NODE_BYTES)

@@ -119,3 +119,21 @@ def test_trace_batch_plan_on_host():
     assert lib.rt_trace_batch(None, None, None, 0, 0, 1) == 0  # nothing to trace
     assert lib.rt_trace_batch(None, None, None, 3, 7, 1) == 1
     assert lib.rt_trace_batch(None, fake, fake, 3, 7, 0) == 1
+
+
+def test_scene_and_post_entry_points_on_host():
+    import rtbvh
+    assert C.sizeof(rtbvh.rt_instance) == 128
+    lib = rtbvh.lib()
+    fake = C.c_void_p(16)
+    r = rtbvh.rt_rays_soa(*([fake] * 8))
+    h = rtbvh.rt_hits_soa(fake, fake)
+    assert lib.rt_trace_scene(None, fake, 1, 0, C.byref(r), 0, 0, C.byref(h), None, None) == 0
+    assert lib.rt_trace_scene(None, fake, 1, 0, C.byref(r), 8, 0x100, C.byref(h), None,
+                              None) == 1  # counters: not a scene flag
+    assert lib.rt_trace_scene(None, fake, 1, 0, C.byref(r), 8, 1, None, None, None) == 1
+    assert lib.rt_trace_scene(None, fake, 1, 65, C.byref(r), 8, 0, C.byref(h), None, None) == 2
+    assert lib.rt_blur7(None, fake, fake, 4, 4, C.c_float(0.05), fake) == 1  # in place
+    assert lib.rt_blur7(None, None, fake, 4, 4, C.c_float(0.05), fake) == 1
+    assert lib.rt_ao_resolve(None, None, None, 0, None) == 0
+    assert lib.rt_ao_resolve(None, None, fake, 5, fake) == 1
