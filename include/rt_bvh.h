@@ -161,6 +161,23 @@ rt_status rt_gpu_refit(rt_stream stream, const float* d_pos, const uint32_t* d_i
                        rt_node4* d_nodes, uint32_t node_count, rt_tri64* d_tris,
                        void* d_scratch, size_t scratch_bytes);
 
+/* The RADV acceleration-structure image fast_bvh_builder.comp writes
+ * (shaders/fast_bvh_builder.comp:91-257, formats radv_bvh.hpp:22-152):
+ * 192-B header with the pre-root links at 0xB4 / 0xBC, box32 nodes (child
+ * IDs (offset >> 3) | type, 24-B child boxes, empty slot ID 0xFFFFFFFF with
+ * a NaN box) in the compute_tree_layout order, then the 64-B triangle nodes;
+ * total_as_size bytes (rt_tree_layout).  n <= 4096, the reference builder's
+ * limit (one workgroup there; here the fused path).  order:
+ *   RT_RADV_INPUT_ORDER  leaves of 4 consecutive input triangles, as the
+ *                        reference builder (d_sorted_ids, if given, = 0..n-1)
+ *   RT_RADV_MORTON       the Morton order of rt_gpu_build.
+ * d_image must be 64-byte aligned.                                          */
+#define RT_RADV_INPUT_ORDER 0u
+#define RT_RADV_MORTON 1u
+rt_status rt_gpu_build_radv(rt_stream stream, const float* d_pos, const uint32_t* d_idx,
+                            uint32_t n, uint32_t order, void* d_image, uint64_t image_bytes,
+                            uint32_t* d_sorted_ids);
+
 /* Batched build: many meshes of 1..RT_BATCH_BUILD_MAX triangles built in ONE
  * persistent launch, each with exactly the output rt_gpu_build gives it alone
  * with RT_BUILD_IMPLICIT (the fast_bvh_builder layout; fused path).  No

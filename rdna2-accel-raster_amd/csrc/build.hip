@@ -513,7 +513,8 @@ __global__ __launch_bounds__(kEmitBlock) void k_emit_levels(
             // children 4p..4p+3 of level lv live at LDS slots 4*tid + c
             emit_inner(nodes, L, lv - 1, p, sbox[cur], 4 * p - 4 * tid, box);
             if (lv - 1 == top) {
-                for (int a = 0; a < 6; ++a) gboxes[6 * (size_t)(L.offsets[lv - 1] + p) + a] = box[a];
+                for (int a = 0; a < 6; ++a)
+                    gboxes[6 * (size_t)(L.offsets[lv - 1] + p) + a] = box[a];
             } else {
                 for (int a = 0; a < 6; ++a) sbox[cur ^ 1][tid * 6 + a] = box[a];
             }
@@ -720,9 +721,78 @@ __device__ __forceinline__ uint4 node_chunk(uint32_t n, uint32_t node, uint32_t 
     return make_uint4(o[0], o[1], o[2], o[3]);
 }
 
+// Chunk c (16 B) of box node `node` in the RADV image
+// (radv_bvh.hpp:79-135, written by fast_bvh_builder.comp:141-212): chunk 0
+// the 4 child node IDs (byte offset from the first box, >> 3, | type: 0
+// triangle, 5 box32), chunks 1-6 the children's AABBs as 24-B records (min
+// xyz, max xyz), chunk 7 zero.  Empty slot: ID 0xFFFFFFFF, NaN box.
+__device__ __forceinline__ uint4 radv_chunk(uint32_t n, uint32_t node, uint32_t c,
+                                            const SmallLds& S) {
+    const LevelLds& L = S.lv;
+    const uint32_t lb = L.levels - 1;
+    uint32_t lv = 0;
+#pragma unroll
+    for (int q = 1; q < kSmallLevels; ++q)
+        if ((uint32_t)q < L.levels && node >= L.off[q]) lv = q;
+    const uint32_t j = node - L.off[lv];
+    const bool leaf = lv == lb;
+    const uint32_t cnt = leaf ? n : L.count[lv + 1];
+    const uint32_t tb = L.off[lb] + L.count[lb];
+    uint32_t o[4] = {0u, 0u, 0u, 0u};
+    if (c == 0) {
+#pragma unroll
+        for (uint32_t k = 0; k < 4; ++k) {
+            const uint32_t ch = 4 * j + k;
+            o[k] = ch >= cnt ? 0xFFFFFFFFu
+                 : leaf      ? (128u * tb + 64u * ch) >> 3
+                             : (((L.off[lv + 1] + ch) * 128u) >> 3) | 5u;
+        }
+    } else if (c < 7) {
+#pragma unroll
+        for (uint32_t q = 0; q < 4; ++q) {
+            const uint32_t e = 4 * (c - 1) + q, k = e / 6, a = e % 6, ch = 4 * j + k;
+            const float* src = leaf ? S.u.tbox[a] : S.box[a] + L.pad[lv + 1];
+            o[q] = ch < cnt ? __float_as_uint(src[ch]) : 0x7FC00000u;
+        }
+    }
+    return make_uint4(o[0], o[1], o[2], o[3]);
+}
+
+// The 192-B RADV header and pre-root links (fast_bvh_builder.comp:220-256):
+// root bounds, sizes, counts and the pre-root box's links at 0xB4 / 0xBC.
+__device__ void radv_header(uint32_t* __restrict__ h, uint32_t n, const SmallLds& S) {
+    const LevelLds& L = S.lv;
+    const uint32_t lb = L.levels - 1, tb = L.off[lb] + L.count[lb];
+    const uint64_t sz = 192ull + 128ull * tb + 64ull * n;
+    uint32_t w[48] = {0u};
+    w[0] = 192u;
+    for (int a = 0; a < 3; ++a) {
+        w[2 + a] = __float_as_uint(S.box[a][L.pad[0]]);
+        w[5 + a] = __float_as_uint(S.box[3 + a][L.pad[0]]);
+    }
+    w[8] = 8u; w[9] = 1u; w[10] = 1u;
+    w[12] = (uint32_t)sz; w[13] = (uint32_t)(sz >> 32);
+    w[14] = (uint32_t)(sz + 128); w[15] = (uint32_t)((sz + 128) >> 32);
+    w[16] = 1u; w[17] = 1u; w[18] = 1u;
+    w[20] = (uint32_t)sz; w[21] = (uint32_t)(sz >> 32);
+    w[23] = 1u;
+    const uint64_t tri0 = 192ull + 128ull * tb;
+    w[24] = (uint32_t)tri0; w[25] = (uint32_t)(tri0 >> 32);
+    w[26] = n;
+    w[31] = 4u;
+    w[32] = 1u;
+    w[45] = 5u;
+    w[47] = 0xFFFFFFFFu;
+    uint4* d = reinterpret_cast<uint4*>(h);
+#pragma unroll
+    for (int k = 0; k < 12; ++k)
+        d[k] = make_uint4(w[4 * k], w[4 * k + 1], w[4 * k + 2], w[4 * k + 3]);
+}
+
 // Output of the fused build and refit.  Owner item i (ok[i]): vertices v[i],
 // sorted position r[i], triangle index id[i], code key[i].  Every workgroup
 // computes all boxes; workgroup `part` of `parts` stores its share.
+template <bool RADV>
 __device__ void small_emit(SmallLds& S, uint32_t n,
                            const float (&v)[kSmallItems][9], const uint32_t (&r)[kSmallItems],
                            const uint32_t (&id)[kSmallItems], const uint32_t (&key)[kSmallItems],
@@ -751,7 +821,10 @@ __device__ void small_emit(SmallLds& S, uint32_t n,
     const uint32_t per = (nchunks + parts - 1) / parts;
     const uint32_t g0 = part * per, g1 = min(nchunks, g0 + per);
     uint4* nd = reinterpret_cast<uint4*>(nodes);
-    for (uint32_t g = g0 + tid; g < g1; g += kSmallBlock) nd[g] = node_chunk(n, g >> 3, g & 7u, S);
+    for (uint32_t g = g0 + tid; g < g1; g += kSmallBlock)
+        nd[g] = RADV ? radv_chunk(n, g >> 3, g & 7u, S) : node_chunk(n, g >> 3, g & 7u, S);
+    if (RADV && part == 0 && tid == 0)  // the image starts 192 B before the first box
+        radv_header(reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(nodes) - 192), n, S);
     lds_barrier();  // triangle bounds dead: the union stages records
     // sorted positions [base, base + kPartRecs) for base = part * kPartRecs,
     // stepping by parts * kPartRecs (one slice per workgroup when parts
@@ -797,11 +870,13 @@ __device__ __forceinline__ void small_info(rt_build_info* info, const SmallTree&
 
 // The whole fused build of one mesh by this workgroup, which writes slice
 // `part` of `parts` of the output (see small_emit).
+template <bool RADV>
 __device__ void build_small(SmallLds& S, const SmallTree& T, const float* __restrict__ pos,
                             const uint32_t* __restrict__ idx, uint32_t n,
                             rt_node4* __restrict__ nodes, rt_tri64* __restrict__ tris,
                             uint32_t* __restrict__ ids_out, uint32_t* __restrict__ codes_out,
-                            rt_build_info* info, uint32_t part, uint32_t parts) {
+                            rt_build_info* info, uint32_t part, uint32_t parts,
+                            bool sort = true) {
     const uint32_t tid = threadIdx.x, lane = tid & 63u, w = tid >> 6;
     small_info(info, T, part);
     load_levels(S.lv, T);  // first read after the barriers below
@@ -827,145 +902,155 @@ __device__ void build_small(SmallLds& S, const SmallTree& T, const float* __rest
             }
         }
     }
-    // bounds: the 1024 per-thread partials through LDS (the box arrays are
-    // free until the output), bound a reduced by wave a alone
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        S.box[a][tid] = mn[a];
-        S.box[3 + a][tid] = mx[a];
-    }
-    lds_barrier();
-    if (w < 6) {
-        const float4* src = reinterpret_cast<const float4*>(S.box[w] + 16 * lane);
-        float m = w < 3 ? INFINITY : -INFINITY;
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const float4 x = src[q];
-            m = w < 3 ? fminf(m, fminf(fminf(x.x, x.y), fminf(x.z, x.w)))
-                      : fmaxf(m, fmaxf(fmaxf(x.x, x.y), fmaxf(x.z, x.w)));
-        }
-        m = w < 3 ? wave_min(m) : wave_max(m);
-        if (lane == 0) S.red[0][w] = m;
-    }
-    lds_barrier();
-    float lo[3], hi[3], sc[3];
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        lo[a] = S.red[0][a];
-        hi[a] = S.red[0][3 + a];
-    }
-    axis_scale(lo, hi, sc);
     uint32_t key[kSmallItems], rank[kSmallItems];
-#pragma unroll
-    for (int i = 0; i < kSmallItems; ++i) {
-        const uint32_t p = w * 64 * kSmallItems + i * 64 + lane;
-        uint32_t q[3];
-        quantize(cen[i], lo, sc, q);
-        key[i] = p < n ? (S.spread[q[0]] << 2) | (S.spread[q[1]] << 1) | S.spread[q[2]]
-                       : 0xFFFFFFFFu;
-        if (p < n) atomicAdd(&S.u.msd.start[key[i] >> kBucketShift], 1u);
-    }
-    lds_barrier();
-    // exclusive scan of the bucket counts (8 per thread) and the largest bucket
-    {
-        const uint4 c0 = start4[2 * tid], c1 = start4[2 * tid + 1];
-        const uint32_t c[8] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w};
-        uint32_t total = 0, big = 0;
-#pragma unroll
-        for (int k = 0; k < 8; ++k) {
-            total += c[k];
-            big = max(big, c[k]);
-        }
-        const uint32_t inc = wave_incl_scan(total);
-        big = wave_max_u32(big);
-        if (lane == 63) S.wsum[w] = inc;
-        if (lane == 0) S.wmax[w] = big;
-        lds_barrier();
-        // waves before this one: lane l < 16 holds wave l's total
-        const uint32_t waves = wave_incl_scan(lane < 16 ? S.wsum[lane] : 0u);
-        uint32_t run = inc - total + (w ? (uint32_t)__builtin_amdgcn_readlane((int)waves, (int)w - 1) : 0u);
-        uint32_t e[8];
-#pragma unroll
-        for (int k = 0; k < 8; ++k) {
-            e[k] = run;
-            run += c[k];
-        }
-        uint4* cursor4 = reinterpret_cast<uint4*>(S.u.msd.cursor);
-        start4[2 * tid] = cursor4[2 * tid] = make_uint4(e[0], e[1], e[2], e[3]);
-        start4[2 * tid + 1] = cursor4[2 * tid + 1] = make_uint4(e[4], e[5], e[6], e[7]);
-    }
-    const uint32_t largest = wave_max_u32(lane < 16 ? S.wmax[lane] : 0u);
-    lds_barrier();
-    if (largest <= kBucketCap) {
+    if (!sort) {  // input order (fast_bvh_builder's leaves of consecutive triangles)
 #pragma unroll
         for (int i = 0; i < kSmallItems; ++i) {
-            const uint32_t p = w * 64 * kSmallItems + i * 64 + lane;
-            if (p < n) {
-                const uint32_t slot = atomicAdd(&S.u.msd.cursor[key[i] >> kBucketShift], 1u);
-                S.u.msd.comp[slot] = ((key[i] & ((1u << kBucketShift) - 1)) << 12) | p;
-            }
-        }
-        lds_barrier();
-        // rank = bucket start + members below; the 4 items' scans interleaved
-        uint32_t me[kSmallItems], first[kSmallItems], len[kSmallItems], most = 0;
-#pragma unroll
-        for (int i = 0; i < kSmallItems; ++i) {
-            const uint32_t p = w * 64 * kSmallItems + i * 64 + lane;
-            len[i] = 0;
-            first[i] = 0;
-            me[i] = 0;
-            if (p < n) {
-                const uint32_t bkt = key[i] >> kBucketShift;
-                first[i] = S.u.msd.start[bkt];
-                len[i] = S.u.msd.cursor[bkt] - first[i];
-                me[i] = ((key[i] & ((1u << kBucketShift) - 1)) << 12) | p;
-                most = max(most, len[i]);
-            }
-            rank[i] = first[i];
-        }
-        for (uint32_t j = 0; j < most; ++j) {
-#pragma unroll
-            for (int i = 0; i < kSmallItems; ++i)
-                if (j < len[i]) rank[i] += S.u.msd.comp[first[i] + j] < me[i] ? 1u : 0u;
+            key[i] = 0u;
+            rank[i] = w * 64 * kSmallItems + i * 64 + lane;
         }
     } else {
-        // LSD: 4 stable passes of 8 bits; padding keys (p >= n) sort last
-        uint32_t k2[kSmallItems], val[kSmallItems], dig[kSmallItems], rk[kSmallItems];
-        int cur = 0;
+        // bounds: the 1024 per-thread partials through LDS (the box arrays are
+        // free until the output), bound a reduced by wave a alone
 #pragma unroll
-        for (int i = 0; i < kSmallItems; ++i) {
-            k2[i] = key[i];
-            val[i] = w * 64 * kSmallItems + i * 64 + lane;
+        for (int a = 0; a < 3; ++a) {
+            S.box[a][tid] = mn[a];
+            S.box[3 + a][tid] = mx[a];
         }
-        for (uint32_t shift = 0; shift < 32; shift += 8) {
-            if (shift > 0) {
-#pragma unroll
-                for (int i = 0; i < kSmallItems; ++i) {
-                    const uint32_t p = w * 64 * kSmallItems + i * 64 + lane;
-                    k2[i] = S.u.lsd.key[cur][p];
-                    val[i] = S.u.lsd.val[cur][p];
-                }
-                cur ^= 1;
-            }
-#pragma unroll
-            for (int i = 0; i < kSmallItems; ++i) dig[i] = (k2[i] >> shift) & 0xFFu;
-            block_rank<kSmallBlock, kSmallItems>(dig, rk, S.u.lsd.cnt, S.dstart, S.wsum);
-            const int dst = shift == 0 ? 0 : cur;
-#pragma unroll
-            for (int i = 0; i < kSmallItems; ++i) {
-                S.u.lsd.key[dst][rk[i]] = k2[i];
-                S.u.lsd.val[dst][rk[i]] = val[i];
-            }
-            cur = dst;
-            lds_barrier();
-        }
-        // rank of triangle p: invert the sorted index array into key[cur]
-        for (uint32_t q = tid; q < n; q += kSmallBlock) S.u.lsd.key[cur][S.u.lsd.val[cur][q]] = q;
         lds_barrier();
+        if (w < 6) {
+            const float4* src = reinterpret_cast<const float4*>(S.box[w] + 16 * lane);
+            float m = w < 3 ? INFINITY : -INFINITY;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const float4 x = src[q];
+                m = w < 3 ? fminf(m, fminf(fminf(x.x, x.y), fminf(x.z, x.w)))
+                          : fmaxf(m, fmaxf(fmaxf(x.x, x.y), fmaxf(x.z, x.w)));
+            }
+            m = w < 3 ? wave_min(m) : wave_max(m);
+            if (lane == 0) S.red[0][w] = m;
+        }
+        lds_barrier();
+        float lo[3], hi[3], sc[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = S.red[0][a];
+            hi[a] = S.red[0][3 + a];
+        }
+        axis_scale(lo, hi, sc);
 #pragma unroll
         for (int i = 0; i < kSmallItems; ++i) {
             const uint32_t p = w * 64 * kSmallItems + i * 64 + lane;
-            rank[i] = p < n ? S.u.lsd.key[cur][p] : 0u;
+            uint32_t q[3];
+            quantize(cen[i], lo, sc, q);
+            key[i] = p < n ? (S.spread[q[0]] << 2) | (S.spread[q[1]] << 1) | S.spread[q[2]]
+                           : 0xFFFFFFFFu;
+            if (p < n) atomicAdd(&S.u.msd.start[key[i] >> kBucketShift], 1u);
+        }
+        lds_barrier();
+        // exclusive scan of the bucket counts (8 per thread) and the largest bucket
+        {
+            const uint4 c0 = start4[2 * tid], c1 = start4[2 * tid + 1];
+            const uint32_t c[8] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w};
+            uint32_t total = 0, big = 0;
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                total += c[k];
+                big = max(big, c[k]);
+            }
+            const uint32_t inc = wave_incl_scan(total);
+            big = wave_max_u32(big);
+            if (lane == 63) S.wsum[w] = inc;
+            if (lane == 0) S.wmax[w] = big;
+            lds_barrier();
+            // waves before this one: lane l < 16 holds wave l's total
+            const uint32_t waves = wave_incl_scan(lane < 16 ? S.wsum[lane] : 0u);
+            uint32_t run = inc - total;
+            if (w) run += (uint32_t)__builtin_amdgcn_readlane((int)waves, (int)w - 1);
+            uint32_t e[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                e[k] = run;
+                run += c[k];
+            }
+            uint4* cursor4 = reinterpret_cast<uint4*>(S.u.msd.cursor);
+            start4[2 * tid] = cursor4[2 * tid] = make_uint4(e[0], e[1], e[2], e[3]);
+            start4[2 * tid + 1] = cursor4[2 * tid + 1] = make_uint4(e[4], e[5], e[6], e[7]);
+        }
+        const uint32_t largest = wave_max_u32(lane < 16 ? S.wmax[lane] : 0u);
+        lds_barrier();
+        if (largest <= kBucketCap) {
+#pragma unroll
+            for (int i = 0; i < kSmallItems; ++i) {
+                const uint32_t p = w * 64 * kSmallItems + i * 64 + lane;
+                if (p < n) {
+                    const uint32_t slot = atomicAdd(&S.u.msd.cursor[key[i] >> kBucketShift], 1u);
+                    S.u.msd.comp[slot] = ((key[i] & ((1u << kBucketShift) - 1)) << 12) | p;
+                }
+            }
+            lds_barrier();
+            // rank = bucket start + members below; the 4 items' scans interleaved
+            uint32_t me[kSmallItems], first[kSmallItems], len[kSmallItems], most = 0;
+#pragma unroll
+            for (int i = 0; i < kSmallItems; ++i) {
+                const uint32_t p = w * 64 * kSmallItems + i * 64 + lane;
+                len[i] = 0;
+                first[i] = 0;
+                me[i] = 0;
+                if (p < n) {
+                    const uint32_t bkt = key[i] >> kBucketShift;
+                    first[i] = S.u.msd.start[bkt];
+                    len[i] = S.u.msd.cursor[bkt] - first[i];
+                    me[i] = ((key[i] & ((1u << kBucketShift) - 1)) << 12) | p;
+                    most = max(most, len[i]);
+                }
+                rank[i] = first[i];
+            }
+            for (uint32_t j = 0; j < most; ++j) {
+#pragma unroll
+                for (int i = 0; i < kSmallItems; ++i)
+                    if (j < len[i]) rank[i] += S.u.msd.comp[first[i] + j] < me[i] ? 1u : 0u;
+            }
+        } else {
+            // LSD: 4 stable passes of 8 bits; padding keys (p >= n) sort last
+            uint32_t k2[kSmallItems], val[kSmallItems], dig[kSmallItems], rk[kSmallItems];
+            int cur = 0;
+#pragma unroll
+            for (int i = 0; i < kSmallItems; ++i) {
+                k2[i] = key[i];
+                val[i] = w * 64 * kSmallItems + i * 64 + lane;
+            }
+            for (uint32_t shift = 0; shift < 32; shift += 8) {
+                if (shift > 0) {
+#pragma unroll
+                    for (int i = 0; i < kSmallItems; ++i) {
+                        const uint32_t p = w * 64 * kSmallItems + i * 64 + lane;
+                        k2[i] = S.u.lsd.key[cur][p];
+                        val[i] = S.u.lsd.val[cur][p];
+                    }
+                    cur ^= 1;
+                }
+#pragma unroll
+                for (int i = 0; i < kSmallItems; ++i) dig[i] = (k2[i] >> shift) & 0xFFu;
+                block_rank<kSmallBlock, kSmallItems>(dig, rk, S.u.lsd.cnt, S.dstart, S.wsum);
+                const int dst = shift == 0 ? 0 : cur;
+#pragma unroll
+                for (int i = 0; i < kSmallItems; ++i) {
+                    S.u.lsd.key[dst][rk[i]] = k2[i];
+                    S.u.lsd.val[dst][rk[i]] = val[i];
+                }
+                cur = dst;
+                lds_barrier();
+            }
+            // rank of triangle p: invert the sorted index array into key[cur]
+            for (uint32_t q = tid; q < n; q += kSmallBlock)
+                S.u.lsd.key[cur][S.u.lsd.val[cur][q]] = q;
+            lds_barrier();
+#pragma unroll
+            for (int i = 0; i < kSmallItems; ++i) {
+                const uint32_t p = w * 64 * kSmallItems + i * 64 + lane;
+                rank[i] = p < n ? S.u.lsd.key[cur][p] : 0u;
+            }
         }
     }
     lds_barrier();  // sort arrays dead from here
@@ -976,7 +1061,7 @@ __device__ void build_small(SmallLds& S, const SmallTree& T, const float* __rest
         id[i] = w * 64 * kSmallItems + i * 64 + lane;
         ok[i] = id[i] < n;
     }
-    small_emit(S, n, v, rank, id, key, ok, nodes, tris, ids_out, codes_out, part, parts);
+    small_emit<RADV>(S, n, v, rank, id, key, ok, nodes, tris, ids_out, codes_out, part, parts);
 }
 
 __global__ __launch_bounds__(kSmallBlock) void k_build_small(
@@ -984,8 +1069,22 @@ __global__ __launch_bounds__(kSmallBlock) void k_build_small(
     rt_node4* __restrict__ nodes, rt_tri64* __restrict__ tris, uint32_t* __restrict__ ids_out,
     uint32_t* __restrict__ codes_out, SmallTree targ, rt_build_info* info) {
     __shared__ SmallLds S;
-    build_small(S, targ, pos, idx, n, nodes, tris, ids_out, codes_out, info, blockIdx.x,
-                gridDim.x);
+    build_small<false>(S, targ, pos, idx, n, nodes, tris, ids_out, codes_out, info, blockIdx.x,
+                       gridDim.x);
+}
+
+// The RADV image of fast_bvh_builder.comp (header, box32 nodes, 64-B
+// triangle nodes) through the fused path; `sort` = 0 keeps the input order
+// (the reference builder's leaves of consecutive triangles), 1 the Morton
+// order of rt_gpu_build.  tb = total boxes.
+__global__ __launch_bounds__(kSmallBlock) void k_build_radv(
+    const float* __restrict__ pos, const uint32_t* __restrict__ idx, uint32_t n,
+    char* __restrict__ image, uint32_t tb, uint32_t* __restrict__ ids_out, SmallTree targ,
+    uint32_t sort) {
+    __shared__ SmallLds S;
+    build_small<true>(S, targ, pos, idx, n, reinterpret_cast<rt_node4*>(image + 192),
+                      reinterpret_cast<rt_tri64*>(image + 192 + 128 * (size_t)tb), ids_out,
+                      nullptr, nullptr, blockIdx.x, gridDim.x, sort != 0);
 }
 
 // Device table entry of the batched build: the caller's item and the level
@@ -1003,8 +1102,8 @@ __global__ __launch_bounds__(kSmallBlock) void k_build_batch(const BatchBuild* _
     for (uint32_t k = blockIdx.x; k < count; k += gridDim.x) {
         const BatchBuild& b = table[k];
         const rt_build_item& it = b.item;
-        build_small(S, b.tree, it.pos, it.idx, it.n, it.nodes, it.tris, it.sorted_ids,
-                    it.codes, it.info, 0u, 1u);
+        build_small<false>(S, b.tree, it.pos, it.idx, it.n, it.nodes, it.tris, it.sorted_ids,
+                           it.codes, it.info, 0u, 1u);
         lds_barrier();  // LDS reused by the next mesh
     }
 }
@@ -1034,7 +1133,8 @@ __global__ __launch_bounds__(kSmallBlock) void k_refit_small(
 #pragma unroll
     for (int i = 0; i < kSmallItems; ++i)
         if (ok[i]) tri_verts(pos, idx, id[i], v[i]);
-    small_emit(S, n, v, r, id, key, ok, nodes, tris, nullptr, nullptr, blockIdx.x, gridDim.x);
+    small_emit<false>(S, n, v, r, id, key, ok, nodes, tris, nullptr, nullptr, blockIdx.x,
+                      gridDim.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -1648,4 +1748,19 @@ extern "C" rt_status rt_gpu_build_batch(rt_stream stream, const void* d_table, u
     hipLaunchKernelGGL(k_build_batch, dim3(g), dim3(kSmallBlock), 0, as_stream(stream),
                        static_cast<const BatchBuild*>(d_table), count);
     return launch_status("k_build_batch");
+}
+
+extern "C" rt_status rt_gpu_build_radv(rt_stream stream, const float* d_pos,
+                                       const uint32_t* d_idx, uint32_t n, uint32_t order,
+                                       void* d_image, uint64_t image_bytes,
+                                       uint32_t* d_sorted_ids) {
+    if (n == 0 || n > RT_BATCH_BUILD_MAX) return RT_ERR_CAPACITY;
+    if (!d_pos || !d_image || order > RT_RADV_MORTON) return RT_ERR_ARG;
+    if (reinterpret_cast<uintptr_t>(d_image) % 64) return RT_ERR_ARG;
+    const TreeLayout t = tree_layout(n);
+    if (image_bytes < t.total_as_size) return RT_ERR_CAPACITY;
+    hipLaunchKernelGGL(k_build_radv, dim3(ceil_div(n, kPartRecs)), dim3(kSmallBlock), 0,
+                       as_stream(stream), d_pos, d_idx, n, static_cast<char*>(d_image),
+                       t.total_boxes, d_sorted_ids, small_tree(t), order);
+    return launch_status("k_build_radv");
 }

@@ -121,6 +121,7 @@ _SIGS = {
     "rt_gen_ao_rays": [vp, vp, vp, vp, vp, vp, u32, u32, u32, f32, f32, f32, vp],
     "rt_gen_random_rays": [vp, u32, u32, vp, vp, f32, vp],
     "rt_pack_light_masks": [vp, vp, u32, u32, vp],
+    "rt_gpu_build_radv": [vp, vp, vp, u32, u32, vp, C.c_uint64, vp],
     "rt_gpu_build_batch_table_bytes": [u32, vp],
     "rt_gpu_build_batch_plan": [vp, u32, vp, C.c_size_t],
     "rt_gpu_build_batch": [vp, vp, u32, u32],
@@ -383,6 +384,20 @@ class GpuBVH4:
 
 
 BATCH_BUILD_MAX = 4096
+
+
+def build_radv(pos, idx=None, order="input", stream=None, out=None):
+    """The RADV acceleration-structure image of fast_bvh_builder.comp
+    (rt_gpu_build_radv): returns (uint8 image of total_as_size bytes, int32
+    leaf-order triangle indices).  order: "input" (the reference builder's
+    leaves of consecutive triangles) or "morton"."""
+    n = _tri_count(pos, idx)
+    size = tree_layout(n)[3]
+    img = out if out is not None else torch.empty(size, dtype=torch.uint8, device=pos.device)
+    ids = torch.empty(n, dtype=torch.int32, device=pos.device)
+    call("rt_gpu_build_radv", stream_handle(stream), ptr(pos), ptr(idx), u32(n),
+         u32(0 if order == "input" else 1), ptr(img), C.c_uint64(img.numel()), ptr(ids))
+    return img, ids
 
 
 class BuildBatch:

@@ -137,3 +137,14 @@ def test_scene_and_post_entry_points_on_host():
     assert lib.rt_blur7(None, None, fake, 4, 4, C.c_float(0.05), fake) == 1
     assert lib.rt_ao_resolve(None, None, None, 0, None) == 0
     assert lib.rt_ao_resolve(None, None, fake, 5, fake) == 1
+
+
+def test_radv_argument_errors():
+    import rtbvh
+    lib = rtbvh.lib()
+    fake = C.c_void_p(64)
+    assert lib.rt_gpu_build_radv(None, fake, None, 0, 0, fake, 1 << 20, None) == 2
+    assert lib.rt_gpu_build_radv(None, fake, None, 4097, 0, fake, 1 << 20, None) == 2
+    assert lib.rt_gpu_build_radv(None, fake, None, 16, 2, fake, 1 << 20, None) == 1
+    assert lib.rt_gpu_build_radv(None, fake, None, 16, 0, C.c_void_p(72), 1 << 20, None) == 1
+    assert lib.rt_gpu_build_radv(None, fake, None, 16, 0, fake, 100, None) == 2

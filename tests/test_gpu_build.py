@@ -169,3 +169,31 @@ def test_fast_builder_verification_rays(rt, O):
         assert (prim[:n] != O.MISS).all()
         assert (np.abs(t[:n] - cen[:, 2]) < 1e-2).all()
         assert prim[n] == O.MISS
+
+
+@pytest.mark.parametrize("n", [1, 3, 4, 5, 64, 255, 1000, 4094, 4096])
+@pytest.mark.parametrize("order", ["input", "morton"])
+def test_radv_image_bytes(rt, O, n, order):
+    """rt_gpu_build_radv: the whole RADV image (header, box32 nodes with
+    NaN empty slots, triangle nodes) byte-identical to the oracle's
+    restatement of fast_bvh_builder.comp, in input order and in the Morton
+    order of the fused build."""
+    xyz, _ = O.dynamic_grid(max(2, n + (n % 2)), 0.6, 1.25)
+    xyz = np.ascontiguousarray(xyz[:n])
+    img, ids = rt.build_radv(torch.from_numpy(xyz).cuda(), None, order=order)
+    torch.cuda.synchronize()
+    want_ids = np.arange(n, dtype=np.uint32) if order == "input" else O.lbvh_build(xyz, None)[2]
+    np.testing.assert_array_equal(u32(ids), want_ids)
+    ref = O.radv_encode(xyz, order=None if order == "input" else want_ids)
+    np.testing.assert_array_equal(to_np(img), ref)
+
+
+def test_radv_image_indexed(rt, O):
+    pos, idx = O.sphere(40, 30, 1.0, 0.02, 8)
+    img, ids = rt.build_radv(torch.from_numpy(pos).cuda(),
+                             torch.from_numpy(idx.view(np.int32)).cuda(), order="morton")
+    torch.cuda.synchronize()
+    xyz = pos[idx.astype(np.int64)].reshape(-1, 9)
+    want = O.lbvh_build(pos, idx)[2]
+    np.testing.assert_array_equal(u32(ids), want)
+    np.testing.assert_array_equal(to_np(img), O.radv_encode(xyz, order=want))
