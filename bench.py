@@ -26,6 +26,7 @@ import torch
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "rdna2-accel-raster_amd"))
 import rtbvh as rt  # noqa: E402
+import autotune  # noqa: E402
 
 METRIC = "Mrays/s any-hit+closest-hit @2.5M tris; BVH build µs @4K tris; p99 req latency"
 HBM_PEAK_GBS = 8000.0
@@ -99,16 +100,25 @@ class Frame:
         self.bits = torch.empty((self.npix * NLIGHTS + 31) // 32, dtype=torch.int32, device="cuda")
         self.masks = torch.empty(self.npix, dtype=torch.int16, device="cuda")
         self.ev = None
+        self.primary_opts, self.shadow_opts = {}, {}
+
+    def tune(self, tuner):
+        """Schedule / launch shape of both traces, measured on this frame's
+        rays (every choice gives the same hits)."""
+        self.step()
+        torch.cuda.synchronize()
+        self.primary_opts = tuner.trace(self.bvh, self.primary, "closest", tag="primary")
+        self.shadow_opts = tuner.trace(self.bvh, self.shadow, "any", tag="shadow")
 
     def step(self, events=None):
         rt.gen_camera_rays(W, H, (0.0, 0.0, 1.25), (0.0, 0.0, 0.0), (0.0, 1.0, 0.0), 60.0,
                            0.0, 1e30, out=self.primary)
-        self.bvh.intersect(self.primary, out=(self.prim, self.t))
+        self.bvh.intersect(self.primary, out=(self.prim, self.t), **self.primary_opts)
         rt.gen_shadow_rays(self.pos, self.idx, self.primary, self.prim, self.t, self.lights, EPS,
                            out=self.shadow)
         if events:
             events[0].record()
-        self.bvh.occluded(self.shadow, out=self.bits)
+        self.bvh.occluded(self.shadow, out=self.bits, **self.shadow_opts)
         if events:
             events[1].record()
         rt.pack_light_masks(self.bits, self.npix, NLIGHTS, out=self.masks)
@@ -148,7 +158,7 @@ def cpu_baseline(frame, threads):
                       f"{threads} threads"}
 
 
-def batched_leg(meshes=256, seed=4):
+def batched_leg(meshes=256, seed=4, tuner=None):
     """Batched build and trace: `meshes` seeded random-displaced grids with
     triangle counts log-uniform in [64, 65536], each with its own ray set of
     4k-512k rays (origins uniform in the mesh bounds, directions uniform on
@@ -156,7 +166,10 @@ def batched_leg(meshes=256, seed=4):
     4096 triangles built in one persistent launch (larger ones one build
     each), all ray sets traced in one launch replayed from a graph.  Plain:
     one build and one trace launch per mesh, waiting for each.  Both trace
-    legs give identical results (checked)."""
+    legs give identical results (checked).  With a tuner, the persistent grid,
+    the radix digit width of the larger builds and the batched trace's launch
+    shape are picked by measurement (the default-shape times are reported
+    beside the tuned ones)."""
     rng = np.random.default_rng(seed)
     geo, rays, modes = [], [], []
     for k in range(meshes):
@@ -182,30 +195,49 @@ def batched_leg(meshes=256, seed=4):
     for k in big:
         trees[k] = rt.GpuBVH4.build(*geo[k], algorithm="radix")
     torch.cuda.synchronize()
-    s.record()
-    bb.launch()
-    for k in big:
-        rt.GpuBVH4.build(*geo[k], algorithm="radix", sync=False)
-    e.record()
-    e.synchronize()
-    build_ms = s.elapsed_time(e)
+
+    def build_all(bits):
+        s.record()
+        bb.launch()
+        for k in big:
+            rt.GpuBVH4.build(*geo[k], algorithm="radix", sync=False,
+                             sort_bits=bits.get(int(geo[k][1].shape[0]).bit_length()))
+        e.record()
+        e.synchronize()
+        return s.elapsed_time(e)
+
+    build_ms_default = build_all({})
+    bits = {}
+    if tuner is not None:
+        tuner.build_batch(bb, tag="grids")
+        for k in big:  # one measurement per size class (log2 of the triangle count)
+            c = int(geo[k][1].shape[0]).bit_length()
+            if c not in bits:
+                bits[c] = tuner.sort_bits(*geo[k], algorithm="radix", tag="grid")
+        torch.cuda.synchronize()
+    build_ms = build_all(bits)
     # batched trace, captured in a graph
     batch = rt.TraceBatch([(trees[k], rays[k], modes[k], None) for k in range(meshes)])
     batch.launch()
     torch.cuda.synchronize()
-    gs = torch.cuda.Stream()
-    g = torch.cuda.CUDAGraph()
-    with torch.cuda.graph(g, stream=gs):
-        batch.launch(stream=gs)
-    g.replay()
-    torch.cuda.synchronize()
-    reps = 5
-    s.record()
-    for _ in range(reps):
+
+    def graph_ms(shape, reps=5):
+        gs = torch.cuda.Stream()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g, stream=gs):
+            batch.launch(stream=gs, **shape)
         g.replay()
-    e.record()
-    e.synchronize()
-    trace_ms = s.elapsed_time(e) / reps
+        torch.cuda.synchronize()
+        s.record()
+        for _ in range(reps):
+            g.replay()
+        e.record()
+        e.synchronize()
+        return s.elapsed_time(e) / reps
+
+    trace_ms_default = graph_ms({})
+    shape = tuner.trace_batch(batch, tag="grids") if tuner is not None else {}
+    trace_ms = graph_ms(shape)
     # plain: one build + one trace launch per mesh, waiting for each
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -227,12 +259,17 @@ def batched_leg(meshes=256, seed=4):
         if modes[k] == "closest":
             same &= bool(torch.equal(o[0], plain_out[k][0]) and torch.equal(o[1], plain_out[k][1]))
         else:
-            same &= bool(torch.equal(o[: n // 32], plain_out[k][: n // 32]))
+            w = (n + 31) // 32
+            same &= bool(torch.equal(o[:w], plain_out[k][:w]))
     return {"workload": f"{meshes} random-displaced grids, 64-65536 tris log-uniform "
                         f"({tris} tris), 4k-512k random rays each ({nrays} rays), "
                         "closest/any 50/50",
-            "build_ms": round(build_ms, 3), "small_meshes_one_launch": len(small),
-            "trace_ms": round(trace_ms, 3), "trace_mrays_per_s": round(nrays / trace_ms / 1e3, 1),
+            "build_ms": round(build_ms, 3), "build_ms_default_shape": round(build_ms_default, 3),
+            "small_meshes_one_launch": len(small),
+            "build_workgroups": bb.workgroups, "sort_bits_by_log2_tris": bits,
+            "trace_ms": round(trace_ms, 3), "trace_ms_default_shape": round(trace_ms_default, 3),
+            "trace_shape": shape,
+            "trace_mrays_per_s": round(nrays / trace_ms / 1e3, 1),
             "plain_trace_ms": round(plain_trace_ms, 3),
             "plain_trace_mrays_per_s": round(nrays / plain_trace_ms / 1e3, 1),
             "plain_build_and_trace_ms": round(plain_ms, 3),
@@ -339,7 +376,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--schedule", choices=["refill", "simple"], default="simple",
-                    help="traversal schedule (waves refilling finished lanes, or one ray/thread)")
+                    help="traversal schedule when not tuning (waves refilling finished lanes, "
+                         "or one ray/thread)")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="fixed launch shapes instead of the autotuner's measured choices")
     ap.add_argument("--rays-per-wave", type=int, default=1024)
     ap.add_argument("--refill-lanes", type=int, default=16)
     ap.add_argument("--builder", choices=["radix", "implicit"], default="radix")
@@ -361,8 +401,12 @@ def main():
     if a.copy_probe:
         copy_probe()
 
+    tuner = None if a.no_tune else autotune.Tuner(
+        log=lambda m: print(m, file=sys.stderr, flush=True))
     b4k_p50, b4k_p99, b4k_b2b = build_time_4k()
     f = Frame()
+    if tuner is not None:
+        f.tune(tuner)
     for _ in range(a.warmup):
         f.step()
     torch.cuda.synchronize()
@@ -412,12 +456,15 @@ def main():
                                "closest-hit + 16 shadow rays/pixel",
                    "tris": f.bvh.tri_count, "width": W, "height": H, "lights": NLIGHTS,
                    "rays_per_step": f.rays_per_step, "builder": f.builder,
-                   "schedule": a.schedule, "bvh_nodes": f.bvh.node_count,
-                   "bvh_depth": f.bvh.depth},
+                   "schedule": f.shadow_opts.get("schedule", a.schedule),
+                   "primary_trace": f.primary_opts or "default",
+                   "shadow_trace": f.shadow_opts or "default",
+                   "bvh_nodes": f.bvh.node_count, "bvh_depth": f.bvh.depth},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic, "kernel": "k_trace_persist any-hit (shadow)"
-                     if a.schedule == "refill" else "k_trace any-hit (shadow)",
+                     if f.shadow_opts.get("schedule", a.schedule) == "refill"
+                     else "k_trace any-hit (shadow)",
                      "kernel_ms": round(kern_avg, 4), "algorithmic_bytes": int(ab)},
         "build_us_4k_p50": round(b4k_p50, 2),
         "build_us_4k_p99": round(b4k_p99, 2),
@@ -429,9 +476,11 @@ def main():
     }
     if not a.no_extra:
         out["ao_frame"] = ao_frame_leg()
-        out["batched"] = batched_leg()
+        out["batched"] = batched_leg(tuner=tuner)
         out["serving"] = serving_leg()
         out["p99_request_latency_ms"] = out["serving"]["p99_ms"]
+    if tuner is not None:
+        out["tuning"] = tuner.report()
     if not a.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         out["cpu_baseline"] = cpu_baseline(f, threads)

@@ -139,7 +139,10 @@ rt_status rt_tree_layout(uint32_t n, uint32_t* counts, uint32_t* offsets, uint32
 typedef struct rt_build_cfg {
     uint32_t algorithm;      /* RT_BUILD_AUTO / RT_BUILD_RADIX / RT_BUILD_IMPLICIT */
     uint32_t max_leaf_prims; /* RADIX leaf size, 1..16; 0 = 4 (maxPrimsPerLeaf) */
-    uint32_t reserved[2];
+    uint32_t sort_bits;      /* digit width of the multi-launch Morton radix sort
+                                (n > 4096): 4, 6, 8 or 10 bits = 8, 5, 4 or 3
+                                passes; 0 = 8.  The output does not depend on it. */
+    uint32_t reserved;
 } rt_build_cfg;
 
 typedef struct rt_build_info {
@@ -224,11 +227,16 @@ rt_status rt_trace(rt_stream stream, const rt_bvh_dev* bvh, const rt_rays_soa* r
                    uint32_t flags, rt_hits_soa* hits, uint32_t* occl_bits, uint32_t* box_tests,
                    uint32_t* tri_tests);
 
-/* rt_trace with a schedule choice (same results for every choice):
+/* rt_trace with a schedule and launch-shape choice (same results for every
+ * choice; the tuner in the Python host layer picks them by measurement):
  *   RT_SCHED_SIMPLE  one ray per thread, the hardware dispatcher balances.
  *   RT_SCHED_REFILL  each wave walks a chunk of rays_per_wave consecutive rays
  *                    and hands new rays to its finished lanes whenever at
  *                    least refill_lanes of 64 are idle.
+ *   block            threads per workgroup: 64, 128 or 256 (0 = 256).
+ *   lds_stack        traversal-stack entries per lane kept in LDS, 4..48
+ *                    (0 = 20); deeper entries spill to private memory.  More
+ *                    entries mean fewer spills and fewer resident waves.
  * opts = NULL: RT_SCHED_SIMPLE (RT_TRACE_PERSISTENT in flags: RT_SCHED_REFILL). */
 #define RT_SCHED_SIMPLE 0u
 #define RT_SCHED_REFILL 1u
@@ -236,7 +244,9 @@ typedef struct rt_trace_opts {
     uint32_t schedule;
     uint32_t rays_per_wave; /* multiple of 64, at most 4096; 0 = 1024 */
     uint32_t refill_lanes;  /* 1..64; 0 = 16 */
-    uint32_t reserved;
+    uint32_t block;         /* 64 / 128 / 256; 0 = 256 */
+    uint32_t lds_stack;     /* 4..48; 0 = 20 */
+    uint32_t reserved[3];
 } rt_trace_opts;
 rt_status rt_trace_ex(rt_stream stream, const rt_bvh_dev* bvh, const rt_rays_soa* rays,
                       uint32_t n, uint32_t flags, const rt_trace_opts* opts, rt_hits_soa* hits,
@@ -265,6 +275,11 @@ rt_status rt_trace_batch_plan(const rt_trace_item* items, uint32_t count, uint32
 rt_status rt_trace_batch(rt_stream stream, const rt_trace_item* d_items,
                          const uint32_t* d_chunk_start, uint32_t count, uint32_t total_chunks,
                          uint32_t depth);
+/* rt_trace_batch with the launch shape of opts (block, lds_stack; schedule
+ * must be RT_SCHED_SIMPLE, the other fields are unused).                    */
+rt_status rt_trace_batch_ex(rt_stream stream, const rt_trace_item* d_items,
+                            const uint32_t* d_chunk_start, uint32_t count, uint32_t total_chunks,
+                            uint32_t depth, const rt_trace_opts* opts);
 
 /* ---- two-level scene (the reference traces through a TLAS of instances:
  * src/main.cpp:920-1000, fullframe_benchmark.cpp:207-219) ------------------

@@ -1,0 +1,224 @@
+"""Autotuner: picks launch shapes and the radix digit width by measuring the
+candidates on the caller's own workload (or a slice of it) on the device.
+
+Every knob below changes only how the work is scheduled, never the result,
+so the tuner also checks that each candidate's output equals the default
+configuration's bit for bit (a mismatch raises: it would be a kernel bug).
+
+  trace (rt_trace_ex)          schedule simple | refill, block (threads per
+                               workgroup, the ray tile), lds_stack (traversal
+                               stack entries per lane held in LDS: more
+                               entries = fewer private-memory spills, fewer
+                               resident waves), and for refill rays_per_wave
+                               (the wave's ray tile) and refill_lanes
+  batched trace (rt_trace_batch_ex)   block, lds_stack
+  device build (rt_gpu_build)  sort_bits: digit width of the Morton radix
+                               sort, 4 / 6 / 8 / 10 bits = 8 / 5 / 4 / 3 passes
+  batched build (rt_gpu_build_batch)  workgroups of the persistent launch
+
+Choices are cached per workload signature (kind, mode, tree depth, size
+classes); `Tuner(cache_path=...)` also keeps them in a JSON file across
+processes.  Timing uses HIP events on the stream the work runs on.
+"""
+import json
+import math
+import os
+import time
+
+import torch
+
+import rtbvh as rt
+
+TRACE_BLOCKS = (64, 128, 256)
+TRACE_LDS = (8, 12, 16, 20, 24, 32)
+REFILL_RAYS_PER_WAVE = (256, 1024, 4096)
+REFILL_LANES = (8, 16, 32)
+SORT_BITS = rt.SORT_BITS
+BUILD_WORKGROUPS = (64, 128, 256, 512, 1024)
+
+
+def _size_class(n):
+    return int(math.log2(max(1, n)))
+
+
+def time_ms(fn, stream=None, reps=3, target_ms=2.0):
+    """Median over `reps` trials of fn()'s device time (ms per call), each
+    trial a loop of enough calls to last about target_ms, bracketed by HIP
+    events recorded on `stream` (the stream fn launches on)."""
+    s = stream or torch.cuda.current_stream()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    fn()  # warm-up (module load, caches)
+    ev0.record(s)
+    fn()
+    ev1.record(s)
+    ev1.synchronize()
+    one = max(ev0.elapsed_time(ev1), 1e-3)
+    inner = max(1, min(200, int(target_ms / one)))
+    trials = []
+    for _ in range(reps):
+        ev0.record(s)
+        for _ in range(inner):
+            fn()
+        ev1.record(s)
+        ev1.synchronize()
+        trials.append(ev0.elapsed_time(ev1) / inner)
+    trials.sort()
+    return trials[len(trials) // 2]
+
+
+def _same(a, b):
+    if isinstance(a, (tuple, list)):
+        return all(_same(x, y) for x, y in zip(a, b))
+    return bool(torch.equal(a, b))
+
+
+def _clone(o):
+    if isinstance(o, (tuple, list)):
+        return type(o)(_clone(x) for x in o)
+    return o.clone()
+
+
+class Tuner:
+    def __init__(self, cache_path=None, reps=3, sample_rays=1 << 21, log=None):
+        self.cache_path = cache_path
+        self.reps = reps
+        self.sample_rays = sample_rays
+        self.log = log
+        self.cache = {}
+        self.history = []  # (key, candidate, ms) of every measurement
+        self.seconds = 0.0
+        if cache_path and os.path.exists(cache_path):
+            with open(cache_path) as f:
+                self.cache = json.load(f)
+
+    def _save(self):
+        if self.cache_path:
+            tmp = self.cache_path + ".tmp"
+            with open(tmp, "w") as f:
+                json.dump(self.cache, f, indent=1, sort_keys=True)
+            os.replace(tmp, self.cache_path)
+
+    def _pick(self, key, candidates, run, check):
+        """Times run(c) for every candidate c, checks check(c) == the first
+        candidate's output, caches and returns the fastest."""
+        if key in self.cache:
+            return self.cache[key]["choice"]
+        t0 = time.perf_counter()
+        ref = _clone(check(candidates[0]))
+        torch.cuda.synchronize()
+        best, best_ms, table = None, float("inf"), []
+        for c in candidates:
+            out = check(c)
+            torch.cuda.synchronize()
+            if not _same(out, ref):
+                raise AssertionError(f"tuner: candidate {c} changed the result of {key}")
+            ms = time_ms(lambda: run(c), reps=self.reps)
+            table.append((c, ms))
+            self.history.append((key, c, ms))
+            if ms < best_ms:
+                best, best_ms = c, ms
+        self.seconds += time.perf_counter() - t0
+        default_ms = table[0][1]
+        self.cache[key] = {"choice": best, "ms": round(best_ms, 5),
+                           "default_ms": round(default_ms, 5), "candidates": len(candidates)}
+        if self.log:
+            self.log(f"tuned {key}: {best} {best_ms:.4f} ms (default {default_ms:.4f} ms, "
+                     f"{len(candidates)} candidates)")
+        self._save()
+        return best
+
+    # -- trace -----------------------------------------------------------------
+    def trace(self, bvh, rays, mode, tag=""):
+        """Schedule and launch shape for bvh.intersect / bvh.occluded on rays
+        like these (a leading slice of at most sample_rays is measured).
+        Returns the keyword arguments for intersect / occluded."""
+        n = len(rays)
+        m = min(n, self.sample_rays)
+        sample = rays if m == n else rt.Rays(rays.data[:, :m])
+        key = (f"trace|{mode}|{tag}|d{bvh.depth}|t{_size_class(bvh.tri_count)}"
+               f"|r{_size_class(m)}")
+        fn = bvh.intersect if mode == "closest" else bvh.occluded
+        out = None
+
+        def run(c):
+            nonlocal out
+            if mode == "closest":
+                out = fn(sample, out=out, **c)
+            else:
+                out = fn(sample, out=out, **c)
+            return out
+
+        def check(c):
+            return _clone(run(c))
+
+        simple = [{"schedule": "simple", "block": b, "lds_stack": l}
+                  for b in TRACE_BLOCKS for l in TRACE_LDS]
+        # the default first: its output is the reference
+        simple.sort(key=lambda c: (c["block"], c["lds_stack"]) != (256, 20))
+        best = self._pick(key + "|simple", simple, run, check)
+        refill = [{"schedule": "refill", "block": best["block"],
+                   "lds_stack": best["lds_stack"], "rays_per_wave": w, "refill_lanes": r}
+                  for w in REFILL_RAYS_PER_WAVE for r in REFILL_LANES]
+        return self._pick(key, [best] + refill, run, check)
+
+    def trace_batch(self, batch, tag=""):
+        """Launch shape (block, lds_stack) for TraceBatch.launch."""
+        key = (f"batch|{tag}|n{batch.count}|d{batch.depth}"
+               f"|c{_size_class(batch.total_chunks)}")
+        cands = [{"block": b, "lds_stack": l} for b in TRACE_BLOCKS for l in TRACE_LDS]
+        cands.sort(key=lambda c: (c["block"], c["lds_stack"]) != (256, 20))
+
+        def run(c):
+            return batch.launch(**c)
+
+        def check(c):
+            return _clone(run(c))
+
+        return self._pick(key, cands, run, check)
+
+    # -- build -----------------------------------------------------------------
+    def sort_bits(self, pos, idx=None, algorithm="radix", tag=""):
+        """Radix digit width for GpuBVH4.build of meshes like this one
+        (meshes of more than 4096 triangles; the fused path has one sort)."""
+        n = idx.shape[0] if idx is not None else pos.numel() // 9
+        key = f"sort|{algorithm}|{tag}|t{_size_class(n)}"
+        cands = [8] + [b for b in SORT_BITS if b != 8]
+        scratch = torch.empty(rt.build_requirements(n, rt.build_cfg(algorithm))[1] + 256,
+                              dtype=torch.uint8, device=pos.device)
+        holder = {}
+
+        def run(b):
+            holder["bvh"] = rt.GpuBVH4.build(pos, idx, algorithm=algorithm, sort_bits=b,
+                                             scratch=scratch, sync=False, keep_order=True)
+            return holder["bvh"]
+
+        def check(b):
+            # the written part only: a radix tree fills fewer nodes than its capacity
+            t = run(b)
+            count, depth = [int(v) for v in t.info[:2].cpu().tolist()]
+            return (t.nodes[: count * rt.NODE_BYTES].clone(), t.tris.clone(),
+                    t.sorted_ids.clone(), t.codes.clone(), torch.tensor([count, depth]))
+
+        return self._pick(key, cands, run, check)
+
+    def build_batch(self, batch, tag=""):
+        """Persistent-grid size for BuildBatch.launch."""
+        key = f"buildbatch|{tag}|n{batch.count}"
+        cands = [0] + [w for w in BUILD_WORKGROUPS if w < batch.count] + [batch.count]
+        cands = list(dict.fromkeys(cands))
+
+        def run(w):
+            batch.workgroups = w
+            return batch.launch()
+
+        def check(w):
+            return [(t.nodes[: t.node_count * rt.NODE_BYTES].clone(), t.tris.clone(),
+                     t.info[:2].clone()) for t in run(w)]
+
+        w = self._pick(key, cands, run, check)
+        batch.workgroups = w
+        return w
+
+    def report(self):
+        return {"seconds": round(self.seconds, 2),
+                "choices": {k: v for k, v in self.cache.items()}}

@@ -43,6 +43,7 @@ constexpr int kSortBlock = 256;
 constexpr int kSortItems = 16;
 constexpr uint32_t kTile = kSortBlock * kSortItems;  // 4096 keys per tile
 constexpr int kEmitBlock = 256;
+constexpr uint32_t kMaxDigits = 1024;  // widest radix digit: 10 bits
 
 struct LevelArgs {
     uint32_t counts[32];
@@ -145,28 +146,62 @@ __device__ __forceinline__ float wave_max(float f) {
     return o2f((uint32_t)__builtin_amdgcn_readlane((int)v, 63));
 }
 
-// Stable rank of BLOCK*ITEMS elements by an 8-bit digit.  Element order is
+// Exclusive scan of D per-digit values held in LDS or registers by the
+// block's first threads: thread t < ST owns the R = D / ST consecutive digits
+// [t*R, t*R + R) (ST = min(D, 256)), passes their values in v[] and gets
+// their exclusive prefixes back.  wsum: 4 words of LDS.  Every thread of the
+// block calls it (it contains a barrier).
+template <uint32_t D>
+__device__ __forceinline__ void digit_excl_scan(uint32_t (&v)[(D > 256 ? D / 256 : 1)],
+                                                uint32_t* wsum) {
+    constexpr uint32_t R = D > 256 ? D / 256 : 1, ST = D / R, SW = (ST + 63) / 64;
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, w = tid >> 6;
+    uint32_t local = 0;
+#pragma unroll
+    for (uint32_t r = 0; r < R; ++r) local += tid < ST ? v[r] : 0u;
+    uint32_t inc = 0;
+    if (tid < SW * 64) {
+        inc = wave_incl_scan(local);
+        if (lane == 63) wsum[w] = inc;
+    }
+    lds_barrier();
+    if (tid < ST) {
+        uint32_t run = inc - local;
+        for (uint32_t ww = 0; ww < w; ++ww) run += wsum[ww];
+#pragma unroll
+        for (uint32_t r = 0; r < R; ++r) {
+            const uint32_t x = v[r];
+            v[r] = run;
+            run += x;
+        }
+    }
+}
+
+// Stable rank of BLOCK*ITEMS elements by a BITS-bit digit.  Element order is
 // warp-striped: wave w, item i, lane l <-> position w*64*ITEMS + i*64 + l.
 // On return pos[i] is the element's position in the digit-sorted tile and
-// dstart[d] (LDS, 256 entries) the first position of digit d.  cnt is LDS of
-// (BLOCK/64)*256 words, wsum LDS of 4 words.  Peer lanes sharing a digit are
-// found with 8 ballots; each wave keeps its own running digit counters, so
-// only the owning wave touches a counter row between the barriers.
-template <int BLOCK, int ITEMS>
+// dstart[d] (LDS, 2^BITS entries) the first position of digit d.  cnt is LDS
+// of (BLOCK/64)*2^BITS words, wsum LDS of 4 words.  Peer lanes sharing a
+// digit are found with BITS ballots; each wave keeps its own running digit
+// counters, so only the owning wave touches a counter row between the
+// barriers.
+template <int BLOCK, int ITEMS, int BITS = 8>
 __device__ __forceinline__ void block_rank(const uint32_t (&dig)[ITEMS], uint32_t (&pos)[ITEMS],
                                            uint32_t* cnt, uint32_t* dstart, uint32_t* wsum) {
-    constexpr int W = BLOCK / 64;
+    constexpr uint32_t D = 1u << BITS, W = BLOCK / 64;
+    constexpr uint32_t R = D > 256 ? D / 256 : 1, ST = D / R;
+    static_assert(ST <= (uint32_t)BLOCK, "one scanning thread per R digits");
     const uint32_t tid = threadIdx.x, lane = tid & 63u, w = tid >> 6;
-    for (uint32_t i = tid; i < (uint32_t)W * 256u; i += BLOCK) cnt[i] = 0;
+    for (uint32_t i = tid; i < W * D; i += BLOCK) cnt[i] = 0;
     lds_barrier();
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64u - lane));
-    uint32_t* row = cnt + w * 256u;
+    uint32_t* row = cnt + w * D;
 #pragma unroll
     for (int i = 0; i < ITEMS; ++i) {
         const uint32_t d = dig[i];
         uint64_t peers = ~0ull;
 #pragma unroll
-        for (int b = 0; b < 8; ++b) {
+        for (int b = 0; b < BITS; ++b) {
             const bool bit = (d >> b) & 1u;
             const uint64_t bal = __ballot(bit);
             peers &= bit ? bal : ~bal;
@@ -179,28 +214,27 @@ __device__ __forceinline__ void block_rank(const uint32_t (&dig)[ITEMS], uint32_
         pos[i] = base + below;
     }
     lds_barrier();
-    uint32_t total = 0;
-    if (tid < 256) {
-        uint32_t run = 0;
+    // per digit: exclusive prefix over the waves' rows, then over digits
+    uint32_t tot[R];
 #pragma unroll
-        for (int ww = 0; ww < W; ++ww) {
-            const uint32_t c = cnt[ww * 256 + tid];
-            cnt[ww * 256 + tid] = run;
-            run += c;
+    for (uint32_t r = 0; r < R; ++r) {
+        tot[r] = 0;
+        if (tid < ST) {
+            const uint32_t d = tid * R + r;
+            uint32_t run = 0;
+#pragma unroll
+            for (uint32_t ww = 0; ww < W; ++ww) {
+                const uint32_t c = cnt[ww * D + d];
+                cnt[ww * D + d] = run;
+                run += c;
+            }
+            tot[r] = run;
         }
-        total = run;
     }
-    // exclusive scan of the 256 digit totals (threads 0..255 = 4 waves)
-    uint32_t inc = 0;
-    if (tid < 256) {
-        inc = wave_incl_scan(total);
-        if (lane == 63) wsum[w] = inc;
-    }
-    lds_barrier();
-    if (tid < 256) {
-        uint32_t add = 0;
-        for (uint32_t ww = 0; ww < w; ++ww) add += wsum[ww];
-        dstart[tid] = add + inc - total;
+    digit_excl_scan<D>(tot, wsum);
+    if (tid < ST) {
+#pragma unroll
+        for (uint32_t r = 0; r < R; ++r) dstart[tid * R + r] = tot[r];
     }
     lds_barrier();
 #pragma unroll
@@ -262,21 +296,24 @@ __global__ __launch_bounds__(256) void k_morton(const float* __restrict__ pos,
     vals[t] = t;
 }
 
-// Per-tile digit histogram, digit-major: hist[d * tiles + tile].
+// Per-tile digit histogram of a BITS-bit digit, digit-major:
+// hist[d * tiles + tile].
+template <int BITS>
 __global__ __launch_bounds__(kSortBlock) void k_hist(const uint32_t* __restrict__ keys, uint32_t n,
                                                      uint32_t shift, uint32_t tiles,
                                                      uint32_t* __restrict__ hist) {
-    __shared__ uint32_t h[256];
-    h[threadIdx.x] = 0;
+    constexpr uint32_t D = 1u << BITS;
+    __shared__ uint32_t h[D];
+    for (uint32_t d = threadIdx.x; d < D; d += kSortBlock) h[d] = 0;
     __syncthreads();
     const uint32_t base = blockIdx.x * kTile;
 #pragma unroll 4
     for (int i = 0; i < kSortItems; ++i) {
         const uint32_t p = base + i * kSortBlock + threadIdx.x;
-        if (p < n) atomicAdd(&h[(keys[p] >> shift) & 0xFFu], 1u);
+        if (p < n) atomicAdd(&h[(keys[p] >> shift) & (D - 1u)], 1u);
     }
     __syncthreads();
-    hist[threadIdx.x * tiles + blockIdx.x] = h[threadIdx.x];
+    for (uint32_t d = threadIdx.x; d < D; d += kSortBlock) hist[d * tiles + blockIdx.x] = h[d];
 }
 
 // One block per digit: exclusive scan of that digit's tile counts in place,
@@ -305,6 +342,7 @@ __global__ __launch_bounds__(1024) void k_digit_scan(uint32_t* __restrict__ hist
     if (tid == 0) totals[d] = carry;
 }
 
+template <int BITS>
 __global__ __launch_bounds__(kSortBlock) void k_scatter(const uint32_t* __restrict__ kin,
                                                         const uint32_t* __restrict__ vin,
                                                         uint32_t* __restrict__ kout,
@@ -312,9 +350,11 @@ __global__ __launch_bounds__(kSortBlock) void k_scatter(const uint32_t* __restri
                                                         uint32_t shift, uint32_t tiles,
                                                         const uint32_t* __restrict__ hist,
                                                         const uint32_t* __restrict__ totals) {
-    __shared__ uint32_t cnt[(kSortBlock / 64) * 256];
-    __shared__ uint32_t dstart[256];
-    __shared__ uint32_t gbase[256];
+    constexpr uint32_t D = 1u << BITS;
+    constexpr uint32_t R = D > 256 ? D / 256 : 1, ST = D / R;
+    __shared__ uint32_t cnt[(kSortBlock / 64) * D];
+    __shared__ uint32_t dstart[D];
+    __shared__ uint32_t gbase[D];
     __shared__ uint32_t wsum[4];
     __shared__ uint32_t sk[kTile];
     __shared__ uint32_t sv[kTile];
@@ -324,14 +364,18 @@ __global__ __launch_bounds__(kSortBlock) void k_scatter(const uint32_t* __restri
 
     // global base of each digit for this tile = exclusive(totals)[d] + hist[d][tile]
     {
-        const uint32_t v = totals[tid];
-        const uint32_t inc = wave_incl_scan(v);
-        if (lane == 63) wsum[w] = inc;
-        __syncthreads();
-        uint32_t add = 0;
-        for (uint32_t ww = 0; ww < w; ++ww) add += wsum[ww];
-        gbase[tid] = add + inc - v + hist[(size_t)tid * tiles + blockIdx.x];
-        __syncthreads();
+        uint32_t v[R];
+#pragma unroll
+        for (uint32_t r = 0; r < R; ++r) v[r] = tid < ST ? totals[tid * R + r] : 0u;
+        digit_excl_scan<D>(v, wsum);
+        if (tid < ST) {
+#pragma unroll
+            for (uint32_t r = 0; r < R; ++r) {
+                const uint32_t d = tid * R + r;
+                gbase[d] = v[r] + hist[(size_t)d * tiles + blockIdx.x];
+            }
+        }
+        lds_barrier();
     }
     uint32_t key[kSortItems], val[kSortItems], dig[kSortItems], pos[kSortItems];
 #pragma unroll
@@ -345,9 +389,9 @@ __global__ __launch_bounds__(kSortBlock) void k_scatter(const uint32_t* __restri
             key[i] = 0xFFFFFFFFu;
             val[i] = 0u;
         }
-        dig[i] = (key[i] >> shift) & 0xFFu;
+        dig[i] = (key[i] >> shift) & (D - 1u);
     }
-    block_rank<kSortBlock, kSortItems>(dig, pos, cnt, dstart, wsum);
+    block_rank<kSortBlock, kSortItems, BITS>(dig, pos, cnt, dstart, wsum);
 #pragma unroll
     for (int i = 0; i < kSortItems; ++i) {
         sk[pos[i]] = key[i];
@@ -359,7 +403,7 @@ __global__ __launch_bounds__(kSortBlock) void k_scatter(const uint32_t* __restri
         const uint32_t p = i * kSortBlock + tid;
         if (p < valid) {
             const uint32_t k = sk[p];
-            const uint32_t d = (k >> shift) & 0xFFu;
+            const uint32_t d = (k >> shift) & (D - 1u);
             const uint32_t g = gbase[d] + (p - dstart[d]);
             kout[g] = k;
             vout[g] = sv[p];
@@ -1483,8 +1527,8 @@ ScratchPlan plan_scratch(uint32_t n, uint32_t algo) {
     p.vals0 = off; off += al((size_t)n * 4);
     p.keys1 = off; off += al((size_t)n * 4);
     p.vals1 = off; off += al((size_t)n * 4);
-    p.hist = off; off += al((size_t)256 * p.tiles * 4);
-    p.totals = off; off += al(256 * 4);
+    p.hist = off; off += al((size_t)kMaxDigits * p.tiles * 4);
+    p.totals = off; off += al(kMaxDigits * 4);
     if (algo == RT_BUILD_IMPLICIT) {
         p.boxes = off; off += al((size_t)tree_layout(n).total_boxes * 24);
     } else {
@@ -1511,6 +1555,14 @@ uint32_t resolve_algo(const rt_build_cfg* cfg, uint32_t n) {
     return a;
 }
 
+// Digit width of the multi-launch radix sort (the sort's result does not
+// depend on it): 0 = 8 bits; 4, 6, 8 or 10 bits.
+uint32_t resolve_bits(const rt_build_cfg* cfg) {
+    const uint32_t b = cfg ? cfg->sort_bits : 0u;
+    if (b == 0) return 8u;
+    return (b == 4 || b == 6 || b == 8 || b == 10) ? b : 0u;
+}
+
 uint32_t resolve_maxp(const rt_build_cfg* cfg) {
     return (cfg && cfg->max_leaf_prims) ? cfg->max_leaf_prims : 4u;
 }
@@ -1527,9 +1579,31 @@ __global__ void k_set_info(rt_build_info* info, uint32_t nodes, uint32_t depth) 
     }
 }
 
-// Bounds -> Morton -> 4-pass radix sort; sorted codes in k0, ids in v0.
+template <int BITS>
+rt_status sort_passes(hipStream_t s, uint32_t n, const ScratchPlan& p, uint32_t* hist,
+                      uint32_t* totals, uint32_t** ki, uint32_t** vi, uint32_t** ko,
+                      uint32_t** vo) {
+    constexpr uint32_t D = 1u << BITS;
+    for (uint32_t shift = 0; shift < 30; shift += BITS) {  // codes have 30 bits
+        hipLaunchKernelGGL(k_hist<BITS>, dim3(p.tiles), dim3(kSortBlock), 0, s, *ki, n, shift,
+                           p.tiles, hist);
+        hipLaunchKernelGGL(k_digit_scan, dim3(D), dim3(1024), 0, s, hist, p.tiles, totals);
+        hipLaunchKernelGGL(k_scatter<BITS>, dim3(p.tiles), dim3(kSortBlock), 0, s, *ki, *vi, *ko,
+                           *vo, n, shift, p.tiles, hist, totals);
+        const rt_status st = launch_status("k_scatter");
+        if (st != RT_OK) return st;
+        std::swap(*ki, *ko);
+        std::swap(*vi, *vo);
+    }
+    return RT_OK;
+}
+
+// Bounds -> Morton -> LSD radix sort of BITS-bit digits (ceil(30 / BITS)
+// stable passes); *keys / *vals point at the sorted codes / ids (in either
+// pair of sort buffers).
 rt_status morton_sort(hipStream_t s, const float* pos, const uint32_t* idx, uint32_t n,
-                      const ScratchPlan& p, char* base, uint32_t** keys, uint32_t** vals) {
+                      uint32_t bits, const ScratchPlan& p, char* base, uint32_t** keys,
+                      uint32_t** vals) {
     uint32_t* ob = reinterpret_cast<uint32_t*>(base + p.bounds);
     uint32_t* k0 = reinterpret_cast<uint32_t*>(base + p.keys0);
     uint32_t* v0 = reinterpret_cast<uint32_t*>(base + p.vals0);
@@ -1544,20 +1618,15 @@ rt_status morton_sort(hipStream_t s, const float* pos, const uint32_t* idx, uint
     rt_status st = launch_status("k_morton");
     if (st != RT_OK) return st;
     uint32_t *ki = k0, *vi = v0, *ko = k1, *vo = v1;
-    for (uint32_t shift = 0; shift < 32; shift += 8) {
-        hipLaunchKernelGGL(k_hist, dim3(p.tiles), dim3(kSortBlock), 0, s, ki, n, shift, p.tiles,
-                           hist);
-        hipLaunchKernelGGL(k_digit_scan, dim3(256), dim3(1024), 0, s, hist, p.tiles, totals);
-        hipLaunchKernelGGL(k_scatter, dim3(p.tiles), dim3(kSortBlock), 0, s, ki, vi, ko, vo, n,
-                           shift, p.tiles, hist, totals);
-        st = launch_status("k_scatter");
-        if (st != RT_OK) return st;
-        std::swap(ki, ko);
-        std::swap(vi, vo);
+    switch (bits) {
+        case 4: st = sort_passes<4>(s, n, p, hist, totals, &ki, &vi, &ko, &vo); break;
+        case 6: st = sort_passes<6>(s, n, p, hist, totals, &ki, &vi, &ko, &vo); break;
+        case 10: st = sort_passes<10>(s, n, p, hist, totals, &ki, &vi, &ko, &vo); break;
+        default: st = sort_passes<8>(s, n, p, hist, totals, &ki, &vi, &ko, &vo); break;
     }
-    *keys = k0;  // four passes: back in the first buffers
-    *vals = v0;
-    return RT_OK;
+    *keys = ki;
+    *vals = vi;
+    return st;
 }
 
 rt_status build_radix(hipStream_t s, const float* pos, const uint32_t* idx, uint32_t n,
@@ -1646,6 +1715,7 @@ extern "C" rt_status rt_build_requirements(uint32_t n, const rt_build_cfg* cfg,
                                            uint32_t* node_capacity, size_t* scratch_bytes) {
     const uint32_t algo = resolve_algo(cfg, n);
     if (algo != RT_BUILD_IMPLICIT && algo != RT_BUILD_RADIX) return RT_ERR_ARG;
+    if (resolve_bits(cfg) == 0) return RT_ERR_ARG;
     if (node_capacity) *node_capacity = max_nodes(n, algo);
     if (scratch_bytes) {
         const bool fused = algo == RT_BUILD_IMPLICIT && n <= kSmallMax;
@@ -1669,7 +1739,9 @@ extern "C" rt_status rt_gpu_build(rt_stream stream, const float* d_pos, const ui
     if (n > (1u << 30)) return RT_ERR_CAPACITY;
     const uint32_t algo = resolve_algo(cfg, n);
     const uint32_t maxp = resolve_maxp(cfg);
+    const uint32_t bits = resolve_bits(cfg);
     if (algo != RT_BUILD_IMPLICIT && algo != RT_BUILD_RADIX) return RT_ERR_ARG;
+    if (bits == 0) return RT_ERR_ARG;
     if (maxp < 1 || maxp > 16) return RT_ERR_ARG;
     if (node_capacity < max_nodes(n, algo)) return RT_ERR_CAPACITY;
     const TreeLayout t = tree_layout(n);
@@ -1683,7 +1755,7 @@ extern "C" rt_status rt_gpu_build(rt_stream stream, const float* d_pos, const ui
     if (!d_scratch || scratch_bytes < p.total) return RT_ERR_CAPACITY;
     char* base = static_cast<char*>(d_scratch);
     uint32_t *keys = nullptr, *vals = nullptr;
-    rt_status st = morton_sort(s, d_pos, d_idx, n, p, base, &keys, &vals);
+    rt_status st = morton_sort(s, d_pos, d_idx, n, bits, p, base, &keys, &vals);
     if (st != RT_OK) return st;
     if (d_sorted_ids)
         RT_TRY_HIP(hipMemcpyAsync(d_sorted_ids, vals, (size_t)n * 4, hipMemcpyDeviceToDevice, s));

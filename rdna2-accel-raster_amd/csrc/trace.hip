@@ -10,18 +10,23 @@
 // queries in rq_ao.comp / tiled_lighting_ra.comp) walks the same near-first
 // order and stops at the first accepted triangle.
 //
-// Per-lane stack: the top kLdsStack entries live in LDS (entry-major, so
-// lanes at the same depth hit distinct banks; 5 KB per wave keeps 8 waves per
-// SIMD resident), deeper entries go to a private array.  Nothing is ever
+// Per-lane stack: the top `ls` entries live in LDS (entry-major, so lanes at
+// the same depth hit distinct banks; the default 20 entries = 5 KB per wave
+// keep 8 waves per SIMD resident), deeper entries go to a private array.
+// The LDS depth and the block size are launch parameters (dynamic LDS), so a
+// tuner can trade occupancy against private-memory spills; nothing is ever
 // dropped (the shader drops on overflow, :185-187): 3 * depth + 1 entries
-// bound the live stack and the variant is chosen from the tree depth.
+// bound the live stack and the private size is chosen from the tree depth.
 #include "rt_internal.h"
 
 namespace rt {
 namespace {
 
-constexpr int kBlock = 256;
-constexpr int kLdsStack = 20;
+constexpr int kBlock = 256;         // largest block; launches use 64, 128 or 256
+constexpr uint32_t kLdsStack = 20;  // default LDS entries per lane
+constexpr uint32_t kLdsMin = 4, kLdsMax = 48;
+constexpr int kPrivSmall = 48;      // private entries when 3 * depth + 1 - ls <= 48
+constexpr int kPrivLarge = 3 * (int)kMaxDepth + 1 - (int)kLdsMin;
 
 struct Child {
     float key;
@@ -73,9 +78,10 @@ __device__ __forceinline__ RT_GLOBAL T* glb(T* p) {
 // The stack pointer is a separate local (a register); only the overflow
 // array lives in private memory (keeping both in one aggregate would put the
 // pointer in scratch too).
-// The LDS column is addressed through the __shared__ array itself (address
-// space 3), never through a generic pointer: a select between an LDS and a
-// private address would compile to a slower flat load.
+// The LDS column (`col`, entry e at col[e * bs], bs = block size) is derived
+// from the __shared__ array itself (address space 3), never through a
+// generic pointer: a select between an LDS and a private address would
+// compile to a slower flat load.
 template <int PRIV>
 struct Stack {
     int32_t priv[PRIV];
@@ -83,8 +89,8 @@ struct Stack {
 
 #define RT_SPUSH(v)                                                      \
     do {                                                                 \
-        if (sp < kLdsStack) lds_stack[sp * kBlock + tid] = (v);          \
-        else st.priv[sp - kLdsStack] = (v);                              \
+        if (sp < ls) col[sp * bs] = (v);                                 \
+        else st.priv[sp - ls] = (v);                                     \
         ++sp;                                                            \
     } while (0)
 
@@ -95,9 +101,9 @@ struct Stack {
 #define RT_SPOP(dst)                                                     \
     do {                                                                 \
         --sp;                                                            \
-        dst = lds_stack[min(sp, kLdsStack - 1) * kBlock + tid];          \
-        if (sp >= kLdsStack)                                             \
-            dst = reinterpret_cast<volatile int32_t*>(st.priv)[sp - kLdsStack]; \
+        dst = col[min(sp, ls - 1) * bs];                                 \
+        if (sp >= ls)                                                    \
+            dst = reinterpret_cast<volatile int32_t*>(st.priv)[sp - ls]; \
     } while (0)
 
 // Leaf range [first, first + cnt): Moller-Trumbore on each 64-B record.
@@ -121,13 +127,14 @@ __device__ __forceinline__ bool test_leaf(TP __restrict__ tris, uint32_t first,
     return false;
 }
 
-// One ray's traversal (the body shared by k_trace and k_trace_batch).  lds
-// is the block's LDS stack array; the thread uses column tid.  Returns with
-// hitT / hitPrim (closest: nearest accepted hit; any: first accepted hit).
+// One ray's traversal (the body shared by k_trace, k_trace_batch and
+// k_trace_scene).  col is the thread's LDS stack column (ls entries, stride
+// bs).  Returns with hitT / hitPrim (closest: nearest accepted hit; any:
+// first accepted hit).
 template <int PRIV, bool ANY, bool COUNT, typename NP, typename TP>
 __device__ __forceinline__ void trace_ray(NP __restrict__ nodes, TP __restrict__ tris, bool start,
                                           float ox, float oy, float oz, float tmin, float dx,
-                                          float dy, float dz, int32_t* lds_stack, uint32_t tid,
+                                          float dy, float dz, int32_t* col, int ls, int bs,
                                           float& hitT, uint32_t& hitPrim, uint32_t& boxTests,
                                           uint32_t& triTests) {
     // software_bvh_bvh4.comp:96-100
@@ -218,14 +225,15 @@ template <int PRIV, bool ANY, bool COUNT>
 __global__ __launch_bounds__(kBlock) void k_trace(const rt_node4* __restrict__ nodes,
                                                   const rt_tri64* __restrict__ tris,
                                                   uint32_t node_count, rt_rays_soa rays,
-                                                  uint32_t n, uint32_t* __restrict__ prim_out,
+                                                  uint32_t n, uint32_t ls,
+                                                  uint32_t* __restrict__ prim_out,
                                                   float* __restrict__ t_out,
                                                   uint32_t* __restrict__ bits_out,
                                                   uint32_t* __restrict__ box_out,
                                                   uint32_t* __restrict__ tri_out) {
-    __shared__ int32_t lds_stack[kLdsStack * kBlock];
-    const uint32_t tid = threadIdx.x;
-    const uint32_t r = blockIdx.x * kBlock + tid;
+    extern __shared__ int32_t lds_stack[];  // dynamic: ls * blockDim.x entries
+    const uint32_t tid = threadIdx.x, bs = blockDim.x;
+    const uint32_t r = blockIdx.x * bs + tid;
     const bool valid = r < n;
 
     float ox = 0.0f, oy = 0.0f, oz = 0.0f, tmin = 1.0f;
@@ -238,7 +246,8 @@ __global__ __launch_bounds__(kBlock) void k_trace(const rt_node4* __restrict__ n
     uint32_t hitPrim = RT_MISS;
     uint32_t boxTests = 0, triTests = 0;
     trace_ray<PRIV, ANY, COUNT>(nodes, tris, valid && node_count > 0, ox, oy, oz, tmin, dx, dy,
-                                dz, lds_stack, tid, hitT, hitPrim, boxTests, triTests);
+                                dz, lds_stack + tid, (int)ls, (int)bs, hitT, hitPrim, boxTests,
+                                triTests);
     if (ANY) {
         write_wave_bits(bits_out, r - (tid & 63u), n, valid && hitPrim != RT_MISS);
         if (valid && prim_out) prim_out[r] = hitPrim;
@@ -258,10 +267,11 @@ __global__ __launch_bounds__(kBlock) void k_trace(const rt_node4* __restrict__ n
 template <int PRIV>
 __global__ __launch_bounds__(kBlock) void k_trace_batch(const rt_trace_item* __restrict__ items,
                                                         const uint32_t* __restrict__ chunk,
-                                                        uint32_t count, uint32_t chunks) {
-    __shared__ int32_t lds_stack[kLdsStack * kBlock];
-    const uint32_t tid = threadIdx.x, lane = tid & 63u;
-    const uint32_t c = __builtin_amdgcn_readfirstlane(blockIdx.x * (kBlock / 64) + (tid >> 6));
+                                                        uint32_t count, uint32_t chunks,
+                                                        uint32_t ls) {
+    extern __shared__ int32_t lds_stack[];  // dynamic: ls * blockDim.x entries
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, bs = blockDim.x;
+    const uint32_t c = __builtin_amdgcn_readfirstlane(blockIdx.x * (bs / 64) + (tid >> 6));
     if (c >= chunks) return;
     uint32_t lo = 0, hi = count;  // last item with chunk[item] <= c
     while (hi - lo > 1) {
@@ -290,13 +300,15 @@ __global__ __launch_bounds__(kBlock) void k_trace_batch(const rt_trace_item* __r
     const auto nodes = glb(it.bvh.nodes);
     const auto tris = glb(it.bvh.tris);
     if (it.flags & RT_ANY) {
-        trace_ray<PRIV, true, false>(nodes, tris, start, ox, oy, oz, tmin, dx, dy, dz, lds_stack,
-                                     tid, hitT, hitPrim, boxTests, triTests);
+        trace_ray<PRIV, true, false>(nodes, tris, start, ox, oy, oz, tmin, dx, dy, dz,
+                                     lds_stack + tid, (int)ls, (int)bs, hitT, hitPrim, boxTests,
+                                     triTests);
         write_wave_bits(glb(it.occlusion_bits), r0, n, valid && hitPrim != RT_MISS);
         if (valid && it.hits.prim) glb(it.hits.prim)[r] = hitPrim;
     } else {
-        trace_ray<PRIV, false, false>(nodes, tris, start, ox, oy, oz, tmin, dx, dy, dz, lds_stack,
-                                      tid, hitT, hitPrim, boxTests, triTests);
+        trace_ray<PRIV, false, false>(nodes, tris, start, ox, oy, oz, tmin, dx, dy, dz,
+                                      lds_stack + tid, (int)ls, (int)bs, hitT, hitPrim, boxTests,
+                                      triTests);
         if (valid) {
             glb(it.hits.prim)[r] = hitPrim;
             glb(it.hits.t)[r] = hitT;
@@ -316,10 +328,11 @@ __global__ __launch_bounds__(kBlock) void k_trace_scene(const rt_instance* __res
                                                         uint32_t n, uint32_t* __restrict__ prim_out,
                                                         float* __restrict__ t_out,
                                                         uint32_t* __restrict__ inst_out,
-                                                        uint32_t* __restrict__ bits_out) {
-    __shared__ int32_t lds_stack[kLdsStack * kBlock];
-    const uint32_t tid = threadIdx.x;
-    const uint32_t r = blockIdx.x * kBlock + tid;
+                                                        uint32_t* __restrict__ bits_out,
+                                                        uint32_t ls) {
+    extern __shared__ int32_t lds_stack[];  // dynamic: ls * blockDim.x entries
+    const uint32_t tid = threadIdx.x, bs = blockDim.x;
+    const uint32_t r = blockIdx.x * bs + tid;
     const bool valid = r < n;
     float ox = 0.0f, oy = 0.0f, oz = 0.0f, tmin = 1.0f;
     float dx = 0.0f, dy = 0.0f, dz = 1.0f, tmax = 0.0f;
@@ -350,7 +363,8 @@ __global__ __launch_bounds__(kBlock) void k_trace_scene(const rt_instance* __res
         uint32_t hitPrim = RT_MISS, bt = 0, tt = 0;
         trace_ray<PRIV, ANY, false>(glb(I.blas.nodes), glb(I.blas.tris),
                                     valid && I.blas.node_count > 0, mox, moy, moz, tmin, mdx, mdy,
-                                    mdz, lds_stack, tid, hitT, hitPrim, bt, tt);
+                                    mdz, lds_stack + tid, (int)ls, (int)bs, hitT, hitPrim, bt,
+                                    tt);
         if (hitPrim != RT_MISS) {
             best = hitT;
             bprim = hitPrim;
@@ -379,15 +393,17 @@ constexpr uint32_t kMaxPerWave = 4096;  // rays per wave: 128 occlusion words in
 template <int PRIV, bool ANY, bool COUNT>
 __global__ __launch_bounds__(kBlock) void k_trace_persist(
     const rt_node4* __restrict__ nodes, const rt_tri64* __restrict__ tris, uint32_t node_count,
-    rt_rays_soa rays, uint32_t n, uint32_t per_wave, uint32_t refill, uint32_t* __restrict__ prim_out,
-    float* __restrict__ t_out, uint32_t* __restrict__ bits_out, uint32_t* __restrict__ box_out,
-    uint32_t* __restrict__ tri_out) {
-    __shared__ int32_t lds_stack[kLdsStack * kBlock];
+    rt_rays_soa rays, uint32_t n, uint32_t per_wave, uint32_t refill, uint32_t lsu,
+    uint32_t* __restrict__ prim_out, float* __restrict__ t_out, uint32_t* __restrict__ bits_out,
+    uint32_t* __restrict__ box_out, uint32_t* __restrict__ tri_out) {
+    extern __shared__ int32_t lds_stack[];  // dynamic: ls * blockDim.x entries
     // any hit: each wave's occlusion words (its range is whole 32-ray words)
     __shared__ uint32_t lds_bits[ANY ? (kBlock / 64) * (kMaxPerWave / 32) : 1];
     const uint32_t tid = threadIdx.x, lane = tid & 63u;
+    const int bs = (int)blockDim.x, ls = (int)lsu;
+    int32_t* col = lds_stack + tid;
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64u - lane));
-    const uint32_t wave = (blockIdx.x * kBlock + tid) >> 6;
+    const uint32_t wave = (blockIdx.x * blockDim.x + tid) >> 6;
     const uint64_t wbeg = (uint64_t)wave * per_wave;
     uint32_t cur = (uint32_t)min<uint64_t>(wbeg, n);
     const uint32_t end = (uint32_t)min<uint64_t>(wbeg + per_wave, n);
@@ -510,7 +526,27 @@ struct TraceCfg {
     bool refill;
     uint32_t per_wave;      // multiple of 64, <= kMaxPerWave
     uint32_t refill_lanes;  // 1..64
+    uint32_t block;         // 64, 128 or 256
+    uint32_t lds;           // LDS stack entries per lane, kLdsMin..kLdsMax
 };
+
+// Checks the launch-shape fields of opts (0 = default) into cfg.
+bool shape_opts(const rt_trace_opts* o, TraceCfg& cfg) {
+    if (!o) return true;
+    if (o->block) cfg.block = o->block;
+    if (o->lds_stack) cfg.lds = o->lds_stack;
+    return (cfg.block == 64 || cfg.block == 128 || cfg.block == 256) && cfg.lds >= kLdsMin &&
+           cfg.lds <= kLdsMax;
+}
+
+size_t lds_bytes(const TraceCfg& cfg) { return (size_t)cfg.lds * cfg.block * 4; }
+
+// Private overflow entries needed beyond the LDS part: at most 3 * depth + 1
+// entries are ever live.
+bool small_priv(uint32_t depth, uint32_t lds) {
+    const uint32_t need = 3u * depth + 1u;
+    return need <= lds + (uint32_t)kPrivSmall;
+}
 
 template <int PRIV>
 rt_status launch_v(hipStream_t s, const rt_bvh_dev* b, const rt_rays_soa* rays, uint32_t n,
@@ -520,14 +556,15 @@ rt_status launch_v(hipStream_t s, const rt_bvh_dev* b, const rt_rays_soa* rays, 
     const bool cnt = flags & RT_COUNTERS;
     uint32_t* prim = hits ? hits->prim : nullptr;
     float* t = hits ? hits->t : nullptr;
+    const size_t shm = lds_bytes(cfg);
     if (cfg.refill) {
         // waves x per_wave covers [0, n); per_wave is a multiple of 64 and at
         // most kMaxPerWave, so each wave owns whole occlusion words
-        const uint32_t blocks = ceil_div(n, (uint64_t)cfg.per_wave * (kBlock / 64));
+        const uint32_t blocks = ceil_div(n, (uint64_t)cfg.per_wave * (cfg.block / 64));
 #define RT_P(A, C)                                                                            \
-    hipLaunchKernelGGL((k_trace_persist<PRIV, A, C>), dim3(blocks), dim3(kBlock), 0, s,       \
+    hipLaunchKernelGGL((k_trace_persist<PRIV, A, C>), dim3(blocks), dim3(cfg.block), shm, s,  \
                        b->nodes, b->tris, b->node_count, *rays, n, cfg.per_wave,              \
-                       cfg.refill_lanes, prim, t, bits, box, tri)
+                       cfg.refill_lanes, cfg.lds, prim, t, bits, box, tri)
         if (any && cnt) RT_P(true, true);
         else if (any) RT_P(true, false);
         else if (cnt) RT_P(false, true);
@@ -535,10 +572,10 @@ rt_status launch_v(hipStream_t s, const rt_bvh_dev* b, const rt_rays_soa* rays, 
 #undef RT_P
         return launch_status("k_trace_persist");
     }
-    dim3 grid(ceil_div(n, kBlock)), block(kBlock);
+    dim3 grid(ceil_div(n, cfg.block)), block(cfg.block);
 #define RT_L(A, C)                                                                            \
-    hipLaunchKernelGGL((k_trace<PRIV, A, C>), grid, block, 0, s, b->nodes, b->tris,           \
-                       b->node_count, *rays, n, prim, t, bits, box, tri)
+    hipLaunchKernelGGL((k_trace<PRIV, A, C>), grid, block, shm, s, b->nodes, b->tris,         \
+                       b->node_count, *rays, n, cfg.lds, prim, t, bits, box, tri)
     if (any && cnt) RT_L(true, true);
     else if (any) RT_L(true, false);
     else if (cnt) RT_L(false, true);
@@ -558,13 +595,14 @@ extern "C" rt_status rt_trace_ex(rt_stream stream, const rt_bvh_dev* bvh, const 
                                  uint32_t* tri_tests) {
     if (!bvh || !rays) return RT_ERR_ARG;
     if (flags & ~(RT_ANY | RT_COUNTERS | RT_TRACE_PERSISTENT)) return RT_ERR_ARG;
-    TraceCfg cfg{(flags & RT_TRACE_PERSISTENT) != 0, 1024u, 16u};
+    TraceCfg cfg{(flags & RT_TRACE_PERSISTENT) != 0, 1024u, 16u, (uint32_t)kBlock, kLdsStack};
     if (opts) {
         if (opts->schedule > RT_SCHED_REFILL) return RT_ERR_ARG;
         cfg.refill = cfg.refill || opts->schedule == RT_SCHED_REFILL;
         if (opts->rays_per_wave) cfg.per_wave = opts->rays_per_wave;
         if (opts->refill_lanes) cfg.refill_lanes = opts->refill_lanes;
     }
+    if (!shape_opts(opts, cfg)) return RT_ERR_ARG;
     if (cfg.per_wave % 64 || cfg.per_wave > kMaxPerWave || cfg.refill_lanes > 64)
         return RT_ERR_ARG;
     if (n == 0) return RT_OK;
@@ -576,15 +614,15 @@ extern "C" rt_status rt_trace_ex(rt_stream stream, const rt_bvh_dev* bvh, const 
     if (!any && (!hits || !hits->prim || !hits->t)) return RT_ERR_ARG;
     if ((flags & RT_COUNTERS) && (!box_tests || !tri_tests)) return RT_ERR_ARG;
     if (bvh->node_count > 0 && (!bvh->nodes || !bvh->tris)) return RT_ERR_ARG;
-    // live stack <= 3 * depth + 1 entries; depth 0 = unknown (<= kMaxDepth)
+    // depth 0 = unknown (<= kMaxDepth)
     const uint32_t depth = bvh->depth ? bvh->depth : kMaxDepth;
     if (depth > kMaxDepth) return RT_ERR_CAPACITY;
-    const uint32_t need = 3u * depth + 1u;
     hipStream_t s = as_stream(stream);
-    if (need <= kLdsStack + 44)
-        return launch_v<44>(s, bvh, rays, n, flags, cfg, hits, occl_bits, box_tests, tri_tests);
-    return launch_v<3 * kMaxDepth + 1 - kLdsStack>(s, bvh, rays, n, flags, cfg, hits, occl_bits,
-                                                   box_tests, tri_tests);
+    if (small_priv(depth, cfg.lds))
+        return launch_v<kPrivSmall>(s, bvh, rays, n, flags, cfg, hits, occl_bits, box_tests,
+                                    tri_tests);
+    return launch_v<kPrivLarge>(s, bvh, rays, n, flags, cfg, hits, occl_bits, box_tests,
+                                tri_tests);
 }
 
 extern "C" rt_status rt_trace(rt_stream stream, const rt_bvh_dev* bvh, const rt_rays_soa* rays,
@@ -622,21 +660,32 @@ extern "C" rt_status rt_trace_batch_plan(const rt_trace_item* items, uint32_t co
     return RT_OK;
 }
 
-extern "C" rt_status rt_trace_batch(rt_stream stream, const rt_trace_item* d_items,
-                                    const uint32_t* d_chunk_start, uint32_t count,
-                                    uint32_t total_chunks, uint32_t depth) {
+extern "C" rt_status rt_trace_batch_ex(rt_stream stream, const rt_trace_item* d_items,
+                                       const uint32_t* d_chunk_start, uint32_t count,
+                                       uint32_t total_chunks, uint32_t depth,
+                                       const rt_trace_opts* opts) {
+    TraceCfg cfg{false, 0u, 0u, (uint32_t)kBlock, kLdsStack};
+    if (opts && opts->schedule != RT_SCHED_SIMPLE) return RT_ERR_ARG;
+    if (!shape_opts(opts, cfg)) return RT_ERR_ARG;
     if (total_chunks == 0) return RT_OK;
     if (!d_items || !d_chunk_start || count == 0) return RT_ERR_ARG;
     if (depth == 0 || depth > kMaxDepth) return RT_ERR_ARG;
     hipStream_t s = as_stream(stream);
-    const dim3 grid(ceil_div(total_chunks, kBlock / 64)), block(kBlock);
-    if (3u * depth + 1u <= kLdsStack + 44)
-        hipLaunchKernelGGL(k_trace_batch<44>, grid, block, 0, s, d_items, d_chunk_start, count,
-                           total_chunks);
+    const dim3 grid(ceil_div(total_chunks, cfg.block / 64)), block(cfg.block);
+    const size_t shm = lds_bytes(cfg);
+    if (small_priv(depth, cfg.lds))
+        hipLaunchKernelGGL(k_trace_batch<kPrivSmall>, grid, block, shm, s, d_items, d_chunk_start,
+                           count, total_chunks, cfg.lds);
     else
-        hipLaunchKernelGGL(k_trace_batch<3 * kMaxDepth + 1 - kLdsStack>, grid, block, 0, s,
-                           d_items, d_chunk_start, count, total_chunks);
+        hipLaunchKernelGGL(k_trace_batch<kPrivLarge>, grid, block, shm, s, d_items, d_chunk_start,
+                           count, total_chunks, cfg.lds);
     return launch_status("k_trace_batch");
+}
+
+extern "C" rt_status rt_trace_batch(rt_stream stream, const rt_trace_item* d_items,
+                                    const uint32_t* d_chunk_start, uint32_t count,
+                                    uint32_t total_chunks, uint32_t depth) {
+    return rt_trace_batch_ex(stream, d_items, d_chunk_start, count, total_chunks, depth, nullptr);
 }
 
 extern "C" rt_status rt_trace_scene(rt_stream stream, const rt_instance* d_instances,
@@ -655,17 +704,19 @@ extern "C" rt_status rt_trace_scene(rt_stream stream, const rt_instance* d_insta
     if (depth > kMaxDepth) return RT_ERR_CAPACITY;
     uint32_t* prim = hits ? hits->prim : nullptr;
     float* t = hits ? hits->t : nullptr;
-    const dim3 grid(ceil_div(n, kBlock)), block(kBlock);
+    const TraceCfg cfg{false, 0u, 0u, (uint32_t)kBlock, kLdsStack};
+    const dim3 grid(ceil_div(n, cfg.block)), block(cfg.block);
+    const size_t shm = lds_bytes(cfg);
     hipStream_t s = as_stream(stream);
 #define RT_S(P, A)                                                                             \
-    hipLaunchKernelGGL((k_trace_scene<P, A>), grid, block, 0, s, d_instances, count, *rays, n, \
-                       prim, t, d_inst, occl_bits)
-    if (3u * depth + 1u <= kLdsStack + 44) {
-        if (any) RT_S(44, true);
-        else RT_S(44, false);
+    hipLaunchKernelGGL((k_trace_scene<P, A>), grid, block, shm, s, d_instances, count, *rays,  \
+                       n, prim, t, d_inst, occl_bits, cfg.lds)
+    if (small_priv(depth, cfg.lds)) {
+        if (any) RT_S(kPrivSmall, true);
+        else RT_S(kPrivSmall, false);
     } else {
-        if (any) RT_S(3 * kMaxDepth + 1 - kLdsStack, true);
-        else RT_S(3 * kMaxDepth + 1 - kLdsStack, false);
+        if (any) RT_S(kPrivLarge, true);
+        else RT_S(kPrivLarge, false);
     }
 #undef RT_S
     return launch_status("k_trace_scene");

@@ -76,7 +76,14 @@ class rt_build_item(C.Structure):
 
 
 class rt_build_cfg(C.Structure):
-    _fields_ = [("algorithm", u32), ("max_leaf_prims", u32), ("reserved", u32 * 2)]
+    _fields_ = [("algorithm", u32), ("max_leaf_prims", u32), ("sort_bits", u32),
+                ("reserved", u32)]
+
+
+# Digit widths of the multi-launch Morton radix sort (same output for each);
+# DEFAULT_SORT_BITS is used when a build does not choose one.
+SORT_BITS = (4, 6, 8, 10)
+DEFAULT_SORT_BITS = 8
 
 
 BUILD_ALGOS = {"auto": 0, "radix": 1, "implicit": 2}
@@ -84,21 +91,30 @@ BUILD_ALGOS = {"auto": 0, "radix": 1, "implicit": 2}
 
 class rt_trace_opts(C.Structure):
     _fields_ = [("schedule", u32), ("rays_per_wave", u32), ("refill_lanes", u32),
-                ("reserved", u32)]
+                ("block", u32), ("lds_stack", u32), ("reserved", u32 * 3)]
 
 
 # Traversal schedule used when a call does not choose one: "simple" (one ray
 # per thread) or "refill" (waves walk ray chunks, refilling finished lanes),
-# with the chunk size and refill threshold of TRACE_TUNING.
+# with the chunk size, refill threshold, block size and LDS stack depth of
+# TRACE_TUNING.  Every choice gives the same results (autotune.py measures
+# them).
 DEFAULT_SCHEDULE = "simple"
-TRACE_TUNING = {"rays_per_wave": 1024, "refill_lanes": 16}
+TRACE_TUNING = {"rays_per_wave": 1024, "refill_lanes": 16, "block": 256, "lds_stack": 20}
+TRACE_BLOCKS = (64, 128, 256)
+LDS_STACK_RANGE = (4, 48)
 
 
-def trace_opts(schedule=None, rays_per_wave=None, refill_lanes=None):
+def trace_opts(schedule=None, rays_per_wave=None, refill_lanes=None, block=None,
+               lds_stack=None):
     s = schedule or DEFAULT_SCHEDULE
+    if s not in ("simple", "refill"):
+        raise ValueError(f"schedule {s!r}")
     return rt_trace_opts(1 if s == "refill" else 0,
                          rays_per_wave or TRACE_TUNING["rays_per_wave"],
-                         refill_lanes or TRACE_TUNING["refill_lanes"], 0)
+                         refill_lanes or TRACE_TUNING["refill_lanes"],
+                         block or TRACE_TUNING["block"],
+                         lds_stack or TRACE_TUNING["lds_stack"])
 
 
 _lib = None
@@ -127,6 +143,7 @@ _SIGS = {
     "rt_gpu_build_batch": [vp, vp, u32, u32],
     "rt_trace_batch_plan": [vp, u32, vp, vp],
     "rt_trace_batch": [vp, vp, vp, u32, u32, u32],
+    "rt_trace_batch_ex": [vp, vp, vp, u32, u32, u32, vp],
     "rt_trace_scene": [vp, vp, u32, u32, vp, u32, u32, vp, vp, vp],
     "rt_gen_ao_rays_scene": [vp, vp, u32, vp, vp, vp, vp, u32, u32, u32, f32, f32, f32, vp],
     "rt_ao_resolve": [vp, vp, vp, u32, vp],
@@ -199,8 +216,9 @@ def tree_layout(n):
     return list(counts[:L.value]), list(offs[:L.value]), tb.value, sz.value
 
 
-def build_cfg(algorithm="auto", max_leaf_prims=4):
-    return rt_build_cfg(BUILD_ALGOS[algorithm], max_leaf_prims, (u32 * 2)(0, 0))
+def build_cfg(algorithm="auto", max_leaf_prims=4, sort_bits=None):
+    return rt_build_cfg(BUILD_ALGOS[algorithm], max_leaf_prims,
+                        sort_bits or DEFAULT_SORT_BITS, 0)
 
 
 def build_requirements(n, cfg=None):
@@ -262,17 +280,19 @@ class GpuBVH4:
     # -- build ---------------------------------------------------------------
     @staticmethod
     def build(pos, idx=None, algorithm="auto", max_leaf_prims=4, stream=None, keep_order=False,
-              scratch=None, sync=True):
+              scratch=None, sync=True, sort_bits=None):
         """Device build (rt_gpu_build) of a Morton-sorted Box4 tree.
 
         pos: float32 (V, 3) device tensor; idx: uint32/int32 (T, 3) or None
         for 9 floats per triangle (pos then has shape (T, 9) or (3T, 3)).
-        algorithm: "auto" | "radix" | "implicit" (see rt_bvh.h).  With
-        sync=True the node count and depth are read back from the device;
-        otherwise node_count is the capacity and depth 0 (unknown)."""
+        algorithm: "auto" | "radix" | "implicit" (see rt_bvh.h).  sort_bits:
+        radix digit width of the multi-launch sort (SORT_BITS; default
+        DEFAULT_SORT_BITS).  With sync=True the node count and depth are read
+        back from the device; otherwise node_count is the capacity and depth 0
+        (unknown)."""
         dev = pos.device
         n = idx.shape[0] if idx is not None else pos.numel() // 9
-        cfg = build_cfg(algorithm, max_leaf_prims)
+        cfg = build_cfg(algorithm, max_leaf_prims, sort_bits)
         cap, need = build_requirements(n, cfg)
         nodes = torch.empty(max(cap, 1) * NODE_BYTES, dtype=torch.uint8, device=dev)
         tris = torch.empty(max(n, 1) * TRI_BYTES, dtype=torch.uint8, device=dev)
@@ -344,7 +364,7 @@ class GpuBVH4:
     def intersect(self, rays, counters=False, stream=None, out=None, schedule=None, **tune):
         """Closest hit per ray: (prim uint32-as-int32, t float32[, box, tri]).
         schedule: "simple" | "refill" (default DEFAULT_SCHEDULE); tune:
-        rays_per_wave, refill_lanes for "refill"."""
+        block, lds_stack, and rays_per_wave, refill_lanes for "refill"."""
         n = len(rays)
         dev = rays.data.device
         prim = out[0] if out else torch.empty(n, dtype=torch.int32, device=dev)
@@ -501,9 +521,12 @@ class TraceBatch:
                 self.d_items = self._pinned[0].to(dev, non_blocking=True)
                 self.d_chunk = self._pinned[1].to(dev, non_blocking=True)
 
-    def launch(self, stream=None):
-        call("rt_trace_batch", stream_handle(stream), ptr(self.d_items), ptr(self.d_chunk),
-             u32(self.count), u32(self.total_chunks), u32(self.depth))
+    def launch(self, stream=None, block=None, lds_stack=None):
+        """One launch over every item; block / lds_stack: launch shape (same
+        results for every choice; default TRACE_TUNING's)."""
+        o = trace_opts("simple", block=block, lds_stack=lds_stack)
+        call("rt_trace_batch_ex", stream_handle(stream), ptr(self.d_items), ptr(self.d_chunk),
+             u32(self.count), u32(self.total_chunks), u32(self.depth), C.byref(o))
         return self.outputs
 
 

@@ -50,7 +50,14 @@ def test_build_requirements():
     cap, _ = rtbvh.build_requirements(10, rtbvh.build_cfg("auto"))
     assert cap == rtbvh.tree_layout(10)[2]
     with pytest.raises(rtbvh.RtError):
-        rtbvh.build_requirements(10, rtbvh.rt_build_cfg(7, 4, (C.c_uint32 * 2)(0, 0)))
+        rtbvh.build_requirements(10, rtbvh.rt_build_cfg(7, 4, 0, 0))
+    # radix digit widths: 4, 6, 8, 10 (0 = 8); the scratch size does not depend on it
+    sizes = {b: rtbvh.build_requirements(100_000, rtbvh.build_cfg("radix", 4, b))[1]
+             for b in rtbvh.SORT_BITS}
+    assert len(set(sizes.values())) == 1
+    for bad in (1, 5, 7, 12):
+        with pytest.raises(rtbvh.RtError):
+            rtbvh.build_requirements(10, rtbvh.rt_build_cfg(1, 4, bad, 0))
 
 
 def test_argument_errors_before_device_work():
@@ -73,6 +80,28 @@ def test_argument_errors_before_device_work():
                             None, 0) == 1
     assert lib.rt_gen_sphere(None, 2, 2, 1.0, 0.0, 0, None, None) == 1
     assert lib.rt_pack_light_masks(None, fake, 10, 17, fake) == 1
+
+
+def test_trace_launch_shape_checks():
+    """rt_trace_ex / rt_trace_batch_ex reject launch shapes outside block in
+    {64, 128, 256} and LDS stack depths outside 4..48 before any device work."""
+    import rtbvh
+    lib = rtbvh.lib()
+    assert C.sizeof(rtbvh.rt_trace_opts) == 32
+    b = rtbvh.rt_bvh_dev()
+    r = rtbvh.rt_rays_soa()
+    for block, lds, ok in [(64, 4, True), (128, 48, True), (256, 20, True), (0, 0, True),
+                           (32, 20, False), (512, 20, False), (96, 20, False),
+                           (256, 3, False), (256, 49, False)]:
+        o = rtbvh.rt_trace_opts(0, 0, 0, block, lds)
+        want = 0 if ok else 1
+        assert lib.rt_trace_ex(None, C.byref(b), C.byref(r), 0, 0, C.byref(o), None, None, None,
+                               None) == want, (block, lds)
+        assert lib.rt_trace_batch_ex(None, None, None, 0, 0, 1, C.byref(o)) == want
+    o = rtbvh.rt_trace_opts(1, 0, 0, 0, 0)  # the batch path has one schedule
+    assert lib.rt_trace_batch_ex(None, None, None, 0, 0, 1, C.byref(o)) == 1
+    with pytest.raises(ValueError):
+        rtbvh.trace_opts("bogus")
 
 
 def test_struct_layouts():
@@ -148,3 +177,21 @@ def test_radv_argument_errors():
     assert lib.rt_gpu_build_radv(None, fake, None, 16, 2, fake, 1 << 20, None) == 1
     assert lib.rt_gpu_build_radv(None, fake, None, 16, 0, C.c_void_p(72), 1 << 20, None) == 1
     assert lib.rt_gpu_build_radv(None, fake, None, 16, 0, fake, 100, None) == 2
+
+
+def test_tuner_search_space_is_valid():
+    """Every candidate the autotuner may pick is a shape the library accepts
+    (checked on the host: rt_trace_ex with n = 0 validates the shape only)."""
+    import autotune
+    import rtbvh
+    lib = rtbvh.lib()
+    b, r = rtbvh.rt_bvh_dev(), rtbvh.rt_rays_soa()
+    for blk in autotune.TRACE_BLOCKS:
+        for lds in autotune.TRACE_LDS:
+            for w in autotune.REFILL_RAYS_PER_WAVE:
+                for lanes in autotune.REFILL_LANES:
+                    o = rtbvh.trace_opts("refill", w, lanes, blk, lds)
+                    assert lib.rt_trace_ex(None, C.byref(b), C.byref(r), 0, 0, C.byref(o), None,
+                                           None, None, None) == 0
+    for bits in autotune.SORT_BITS:
+        rtbvh.build_requirements(100_000, rtbvh.build_cfg("radix", 4, bits))
