@@ -75,14 +75,19 @@ def build_time_4k(iters=200, warm=20):
 class Frame:
     builder = "radix"
 
-    def __init__(self):
+    def __init__(self, tuner=None):
         self.pos, self.idx = rt.gen_sphere(NU, NV, 1.0, 0.02, SEED)
         torch.cuda.synchronize()
-        rt.GpuBVH4.build(self.pos, self.idx, algorithm=self.builder)  # warm the build path
+        # radix digit width of the sort: measured when tuning (same tree for each)
+        self.sort_bits = (tuner.sort_bits(self.pos, self.idx, algorithm=self.builder,
+                                          tag="sphere") if tuner else rt.DEFAULT_SORT_BITS)
+        rt.GpuBVH4.build(self.pos, self.idx, algorithm=self.builder,
+                         sort_bits=self.sort_bits)  # warm the build path
         torch.cuda.synchronize()
         s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         s.record()
-        self.bvh = rt.GpuBVH4.build(self.pos, self.idx, algorithm=self.builder, sync=False)
+        self.bvh = rt.GpuBVH4.build(self.pos, self.idx, algorithm=self.builder, sync=False,
+                                    sort_bits=self.sort_bits)
         e.record()
         e.synchronize()
         self.build_ms = s.elapsed_time(e)
@@ -162,14 +167,14 @@ def batched_leg(meshes=256, seed=4, tuner=None):
     """Batched build and trace: `meshes` seeded random-displaced grids with
     triangle counts log-uniform in [64, 65536], each with its own ray set of
     4k-512k rays (origins uniform in the mesh bounds, directions uniform on
-    the sphere), closest-hit or any-hit 50/50.  Batched: meshes of at most
-    4096 triangles built in one persistent launch (larger ones one build
-    each), all ray sets traced in one launch replayed from a graph.  Plain:
-    one build and one trace launch per mesh, waiting for each.  Both trace
-    legs give identical results (checked).  With a tuner, the persistent grid,
-    the radix digit width of the larger builds and the batched trace's launch
-    shape are picked by measurement (the default-shape times are reported
-    beside the tuned ones)."""
+    the sphere), closest-hit or any-hit 50/50.  Batched: every mesh built in
+    one persistent launch (up to 4096 triangles in LDS, larger ones by one
+    workgroup each), all ray sets traced in one launch replayed from a graph.
+    Plain: one build (rt_gpu_build, the same implicit trees) and one trace
+    launch per mesh, waiting for each.  Both legs give identical results
+    (checked).  With a tuner, the persistent grid and the batched trace's
+    launch shape are picked by measurement (the default-shape times are
+    reported beside the tuned ones)."""
     rng = np.random.default_rng(seed)
     geo, rays, modes = [], [], []
     for k in range(meshes):
@@ -193,29 +198,23 @@ def batched_leg(meshes=256, seed=4, tuner=None):
     for k, t in zip(small, bb.launch()):
         trees[k] = t
     for k in big:
-        trees[k] = rt.GpuBVH4.build(*geo[k], algorithm="radix")
+        trees[k] = rt.GpuBVH4.build(*geo[k], algorithm="implicit")
     torch.cuda.synchronize()
 
-    def build_all(bits):
+    def build_all():
         s.record()
         bb.launch()
         for k in big:
-            rt.GpuBVH4.build(*geo[k], algorithm="radix", sync=False,
-                             sort_bits=bits.get(int(geo[k][1].shape[0]).bit_length()))
+            rt.GpuBVH4.build(*geo[k], algorithm="implicit", sync=False)
         e.record()
         e.synchronize()
         return s.elapsed_time(e)
 
-    build_ms_default = build_all({})
-    bits = {}
+    build_ms_default = build_all()
     if tuner is not None:
         tuner.build_batch(bb, tag="grids")
-        for k in big:  # one measurement per size class (log2 of the triangle count)
-            c = int(geo[k][1].shape[0]).bit_length()
-            if c not in bits:
-                bits[c] = tuner.sort_bits(*geo[k], algorithm="radix", tag="grid")
         torch.cuda.synchronize()
-    build_ms = build_all(bits)
+    build_ms = build_all()
     # batched trace, captured in a graph
     batch = rt.TraceBatch([(trees[k], rays[k], modes[k], None) for k in range(meshes)])
     batch.launch()
@@ -243,7 +242,7 @@ def batched_leg(meshes=256, seed=4, tuner=None):
     t0 = time.perf_counter()
     plain_out = []
     for k in range(meshes):
-        tk = rt.GpuBVH4.build(*geo[k], algorithm="auto")
+        tk = rt.GpuBVH4.build(*geo[k], algorithm="implicit")
         o = tk.intersect(rays[k]) if modes[k] == "closest" else tk.occluded(rays[k])
         torch.cuda.synchronize()
         plain_out.append(o)
@@ -265,8 +264,7 @@ def batched_leg(meshes=256, seed=4, tuner=None):
                         f"({tris} tris), 4k-512k random rays each ({nrays} rays), "
                         "closest/any 50/50",
             "build_ms": round(build_ms, 3), "build_ms_default_shape": round(build_ms_default, 3),
-            "small_meshes_one_launch": len(small),
-            "build_workgroups": bb.workgroups, "sort_bits_by_log2_tris": bits,
+            "meshes_one_launch": len(small), "build_workgroups": bb.workgroups,
             "trace_ms": round(trace_ms, 3), "trace_ms_default_shape": round(trace_ms_default, 3),
             "trace_shape": shape,
             "trace_mrays_per_s": round(nrays / trace_ms / 1e3, 1),
@@ -404,7 +402,7 @@ def main():
     tuner = None if a.no_tune else autotune.Tuner(
         log=lambda m: print(m, file=sys.stderr, flush=True))
     b4k_p50, b4k_p99, b4k_b2b = build_time_4k()
-    f = Frame()
+    f = Frame(tuner)
     if tuner is not None:
         f.tune(tuner)
     for _ in range(a.warmup):
@@ -456,6 +454,7 @@ def main():
                                "closest-hit + 16 shadow rays/pixel",
                    "tris": f.bvh.tri_count, "width": W, "height": H, "lights": NLIGHTS,
                    "rays_per_step": f.rays_per_step, "builder": f.builder,
+                   "sort_bits": f.sort_bits,
                    "schedule": f.shadow_opts.get("schedule", a.schedule),
                    "primary_trace": f.primary_opts or "default",
                    "shadow_trace": f.shadow_opts or "default",

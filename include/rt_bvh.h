@@ -183,16 +183,19 @@ rt_status rt_gpu_build_radv(rt_stream stream, const float* d_pos, const uint32_t
 
 /* Batched build: many meshes of 1..RT_BATCH_BUILD_MAX triangles built in ONE
  * persistent launch, each with exactly the output rt_gpu_build gives it alone
- * with RT_BUILD_IMPLICIT (the fast_bvh_builder layout; fused path).  No
- * reference counterpart (fast_bvh_builder.comp builds one mesh per dispatch,
- * src/main.cpp:871-885).  Larger meshes go through rt_gpu_build.
+ * with RT_BUILD_IMPLICIT (the fast_bvh_builder layout).  A mesh of at most
+ * 4096 triangles is built in LDS (the fused path); a larger one by one
+ * workgroup over its item scratch (rt_build_requirements' scratch bytes for
+ * RT_BUILD_IMPLICIT; NULL / 0 for n <= 4096).  No reference counterpart
+ * (fast_bvh_builder.comp builds one mesh per dispatch, src/main.cpp:871-885).
  *   1. rt_gpu_build_batch_plan (host) checks the items and writes the device
  *      table image (rt_gpu_build_batch_table_bytes(count) bytes) into host
  *      memory; order the items largest first for balance;
  *   2. the caller copies the table to device memory;
  *   3. rt_gpu_build_batch launches over it (no host work, no allocation:
- *      capturable in a graph).  workgroups = 0: min(count, 256).          */
-#define RT_BATCH_BUILD_MAX 4096u
+ *      capturable in a graph).  workgroups = 0: min(count, 256).
+ * Items must not share outputs or scratch.                                  */
+#define RT_BATCH_BUILD_MAX 65536u
 typedef struct rt_build_item {
     const float* pos;       /* as rt_gpu_build */
     const uint32_t* idx;    /* NULL: 9 floats per triangle */
@@ -203,6 +206,8 @@ typedef struct rt_build_item {
     uint32_t* codes;        /* optional */
     uint32_t n;
     uint32_t node_capacity;
+    void* scratch;          /* n > 4096: the RT_BUILD_IMPLICIT scratch of n */
+    uint64_t scratch_bytes;
 } rt_build_item;
 rt_status rt_gpu_build_batch_table_bytes(uint32_t count, size_t* bytes);
 rt_status rt_gpu_build_batch_plan(const rt_build_item* items, uint32_t count, void* table,

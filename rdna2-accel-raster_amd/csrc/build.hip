@@ -492,7 +492,9 @@ __device__ __forceinline__ void emit_leaf_box(const float* __restrict__ pos,
 }
 
 // Inner node i of level lv from child boxes (level lv+1, children 4i..4i+3).
-__device__ __forceinline__ void emit_inner(rt_node4* __restrict__ nodes, const LevelArgs& L,
+// LT: a level table with counts[] / offsets[] (LevelArgs, MedTree).
+template <typename LT>
+__device__ __forceinline__ void emit_inner(rt_node4* __restrict__ nodes, const LT& L,
                                            uint32_t lv, uint32_t i, const float* child_boxes,
                                            uint32_t child_stride_base, float box[6]) {
     NodeReg nr;
@@ -1131,11 +1133,176 @@ __global__ __launch_bounds__(kSmallBlock) void k_build_radv(
                       nullptr, nullptr, blockIdx.x, gridDim.x, sort != 0);
 }
 
+// ---------------------------------------------------------------------------
+// Medium path: 4096 < n <= 65536 built by ONE workgroup (inside the batched
+// build's persistent launch), with the item's scratch in global memory (the
+// buffers of the multi-launch path: key / value pairs, level boxes).  The
+// steps and the device functions are the multi-launch implicit build's, so
+// the output is byte-identical to rt_gpu_build(RT_BUILD_IMPLICIT):
+//   1. centroid bounds (wave min / max, then over the 16 waves)
+//   2. Morton code and index of every triangle
+//   3. four stable LSD passes of 8-bit digits: a digit histogram, its scan,
+//      then 4096-key tiles in order, each ranked with block_rank and written
+//      to the digit's running position
+//   4. sorted ids / codes, leaf records and leaf nodes (emit_leaf_box), then
+//      each inner level from the level below (emit_inner), boxes staged in
+//      the scratch between levels.
+// Global data written by one step and read by another crosses a
+// __syncthreads() (workgroup-scope release / acquire: every wave of the
+// workgroup runs on this CU).
+// ---------------------------------------------------------------------------
+constexpr uint32_t kMedMax = 65536;
+constexpr int kMedLevels = 8;  // levels of a 65536-triangle implicit tree
+constexpr int kMedItems = 4;   // keys per thread and tile
+constexpr uint32_t kMedTile = kSmallBlock * kMedItems;
+static_assert(RT_BATCH_BUILD_MAX == kMedMax, "the batched build covers the medium path");
+
+struct MedTree {
+    uint32_t levels, total_boxes;
+    uint32_t counts[kMedLevels];
+    uint32_t offsets[kMedLevels];
+    uint64_t keys0, vals0, keys1, vals1, boxes;  // byte offsets in the item's scratch
+};
+
+__device__ void build_medium(SmallLds& S, const MedTree& T, const float* __restrict__ pos,
+                             const uint32_t* __restrict__ idx, uint32_t n,
+                             rt_node4* __restrict__ nodes, rt_tri64* __restrict__ tris,
+                             uint32_t* __restrict__ ids_out, uint32_t* __restrict__ codes_out,
+                             rt_build_info* __restrict__ info, char* __restrict__ scratch) {
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, w = tid >> 6;
+    uint32_t* hist = S.u.lsd.key[0];        // 256 digit counts
+    uint32_t* run = S.u.lsd.key[0] + 256;   // running global start of each digit
+    float* bnd = reinterpret_cast<float*>(S.u.lsd.val[0]);  // lo[3], hi[3]
+
+    // 1. centroid bounds
+    float mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (uint32_t t = tid; t < n; t += kSmallBlock) {
+        float v[9], c[3];
+        tri_verts(pos, idx, t, v);
+        centroid(v, c);
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            mn[a] = fminf(mn[a], c[a]);
+            mx[a] = fmaxf(mx[a], c[a]);
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const float a0 = wave_min(mn[a]), a1 = wave_max(mx[a]);
+        if (lane == 0) {
+            S.red[w][a] = a0;
+            S.red[w][3 + a] = a1;
+        }
+    }
+    lds_barrier();
+    if (tid < 6) {
+        uint32_t o = f2o(S.red[0][tid]);
+        for (int ww = 1; ww < kSmallBlock / 64; ++ww) {
+            const uint32_t q = f2o(S.red[ww][tid]);
+            o = tid < 3 ? min(o, q) : max(o, q);
+        }
+        bnd[tid] = o2f(o);
+    }
+    lds_barrier();
+    float lo[3] = {bnd[0], bnd[1], bnd[2]}, hi[3] = {bnd[3], bnd[4], bnd[5]}, sc[3];
+    axis_scale(lo, hi, sc);
+
+    // 2. codes
+    uint32_t* ki = reinterpret_cast<uint32_t*>(scratch + T.keys0);
+    uint32_t* vi = reinterpret_cast<uint32_t*>(scratch + T.vals0);
+    uint32_t* ko = reinterpret_cast<uint32_t*>(scratch + T.keys1);
+    uint32_t* vo = reinterpret_cast<uint32_t*>(scratch + T.vals1);
+    for (uint32_t t = tid; t < n; t += kSmallBlock) {
+        float v[9], c[3];
+        tri_verts(pos, idx, t, v);
+        centroid(v, c);
+        ki[t] = morton(c, lo, sc);
+        vi[t] = t;
+    }
+    __syncthreads();
+
+    // 3. LSD passes (four: the codes end in the first buffers)
+    for (uint32_t shift = 0; shift < 32; shift += 8) {
+        if (tid < 256) hist[tid] = 0;
+        lds_barrier();
+        for (uint32_t t = tid; t < n; t += kSmallBlock)
+            atomicAdd(&hist[(ki[t] >> shift) & 0xFFu], 1u);
+        lds_barrier();
+        uint32_t hv[1] = {tid < 256 ? hist[tid] : 0u};
+        digit_excl_scan<256>(hv, S.wsum);
+        if (tid < 256) run[tid] = hv[0];
+        lds_barrier();
+        for (uint32_t base = 0; base < n; base += kMedTile) {
+            const uint32_t valid = min(kMedTile, n - base);
+            uint32_t key[kMedItems], val[kMedItems], dig[kMedItems], rk[kMedItems];
+#pragma unroll
+            for (int i = 0; i < kMedItems; ++i) {
+                const uint32_t lp = w * 64 * kMedItems + i * 64 + lane;
+                key[i] = lp < valid ? ki[base + lp] : 0xFFFFFFFFu;
+                val[i] = lp < valid ? vi[base + lp] : 0u;
+                dig[i] = (key[i] >> shift) & 0xFFu;
+            }
+            block_rank<kSmallBlock, kMedItems>(dig, rk, S.u.lsd.cnt, S.dstart, S.wsum);
+#pragma unroll
+            for (int i = 0; i < kMedItems; ++i) {
+                const uint32_t lp = w * 64 * kMedItems + i * 64 + lane;
+                if (lp < valid) {
+                    const uint32_t g = run[dig[i]] + rk[i] - S.dstart[dig[i]];
+                    ko[g] = key[i];
+                    vo[g] = val[i];
+                }
+            }
+            lds_barrier();
+            // advance each digit by its count in this tile (the padding keys
+            // all have digit 255 and rank after the valid ones)
+            if (tid < 256) {
+                const uint32_t next = tid < 255 ? S.dstart[tid + 1] : valid;
+                run[tid] += next - S.dstart[tid];
+            }
+            lds_barrier();
+        }
+        __syncthreads();
+        uint32_t* t0 = ki; ki = ko; ko = t0;
+        t0 = vi; vi = vo; vo = t0;
+    }
+
+    // 4. outputs
+    for (uint32_t q = tid; q < n; q += kSmallBlock) {
+        if (ids_out) ids_out[q] = vi[q];
+        if (codes_out) codes_out[q] = ki[q];
+    }
+    float* gb = reinterpret_cast<float*>(scratch + T.boxes);
+    const uint32_t lb = T.levels - 1;
+    for (uint32_t b = tid; b < T.counts[lb]; b += kSmallBlock) {
+        float box[6];
+        emit_leaf_box<false>(pos, idx, n, vi, nodes, tris, T.offsets[lb], b, box);
+#pragma unroll
+        for (int a = 0; a < 6; ++a) gb[6 * (size_t)(T.offsets[lb] + b) + a] = box[a];
+    }
+    __syncthreads();
+    for (int lv = (int)lb - 1; lv >= 0; --lv) {
+        for (uint32_t i = tid; i < T.counts[lv]; i += kSmallBlock) {
+            float box[6];
+            emit_inner(nodes, T, (uint32_t)lv, i, gb + 6 * (size_t)T.offsets[lv + 1], 0u, box);
+            if (lv > 0) {
+#pragma unroll
+                for (int a = 0; a < 6; ++a) gb[6 * (size_t)(T.offsets[lv] + i) + a] = box[a];
+            }
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        info->node_count = T.total_boxes;
+        info->depth = T.levels;
+    }
+}
+
 // Device table entry of the batched build: the caller's item and the level
-// table of its fused-path tree.
+// table of its tree (fused path for n <= 4096, medium path above).
 struct BatchBuild {
     rt_build_item item;
     SmallTree tree;
+    MedTree med;
 };
 
 // Persistent batched build: workgroup b builds items b, b + grid, b + 2 grid,
@@ -1146,9 +1313,13 @@ __global__ __launch_bounds__(kSmallBlock) void k_build_batch(const BatchBuild* _
     for (uint32_t k = blockIdx.x; k < count; k += gridDim.x) {
         const BatchBuild& b = table[k];
         const rt_build_item& it = b.item;
-        build_small<false>(S, b.tree, it.pos, it.idx, it.n, it.nodes, it.tris, it.sorted_ids,
-                           it.codes, it.info, 0u, 1u);
-        lds_barrier();  // LDS reused by the next mesh
+        if (it.n <= kSmallMax)
+            build_small<false>(S, b.tree, it.pos, it.idx, it.n, it.nodes, it.tris, it.sorted_ids,
+                               it.codes, it.info, 0u, 1u);
+        else
+            build_medium(S, b.med, it.pos, it.idx, it.n, it.nodes, it.tris, it.sorted_ids,
+                         it.codes, it.info, static_cast<char*>(it.scratch));
+        __syncthreads();  // LDS and scratch reused by the next mesh
     }
 }
 
@@ -1806,8 +1977,28 @@ extern "C" rt_status rt_gpu_build_batch_plan(const rt_build_item* items, uint32_
         if (!it.pos || !it.nodes || !it.tris || !it.info) return RT_ERR_ARG;
         const TreeLayout t = tree_layout(it.n);
         if (it.node_capacity < t.total_boxes) return RT_ERR_CAPACITY;
-        out[k].item = it;
-        out[k].tree = small_tree(t);
+        BatchBuild e = {};
+        e.item = it;
+        if (it.n <= kSmallMax) {
+            e.tree = small_tree(t);
+        } else {
+            // the multi-launch implicit build's scratch layout
+            const ScratchPlan p = plan_scratch(it.n, RT_BUILD_IMPLICIT);
+            if (!it.scratch || it.scratch_bytes < p.total) return RT_ERR_CAPACITY;
+            MedTree& m = e.med;
+            m.levels = t.levels;
+            m.total_boxes = t.total_boxes;
+            for (uint32_t l = 0; l < t.levels; ++l) {
+                m.counts[l] = t.counts[l];
+                m.offsets[l] = t.offsets[l];
+            }
+            m.keys0 = p.keys0;
+            m.vals0 = p.vals0;
+            m.keys1 = p.keys1;
+            m.vals1 = p.vals1;
+            m.boxes = p.boxes;
+        }
+        out[k] = e;
     }
     return RT_OK;
 }
@@ -1826,7 +2017,7 @@ extern "C" rt_status rt_gpu_build_radv(rt_stream stream, const float* d_pos,
                                        const uint32_t* d_idx, uint32_t n, uint32_t order,
                                        void* d_image, uint64_t image_bytes,
                                        uint32_t* d_sorted_ids) {
-    if (n == 0 || n > RT_BATCH_BUILD_MAX) return RT_ERR_CAPACITY;
+    if (n == 0 || n > kSmallMax) return RT_ERR_CAPACITY;
     if (!d_pos || !d_image || order > RT_RADV_MORTON) return RT_ERR_ARG;
     if (reinterpret_cast<uintptr_t>(d_image) % 64) return RT_ERR_ARG;
     const TreeLayout t = tree_layout(n);

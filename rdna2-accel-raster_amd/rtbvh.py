@@ -72,7 +72,8 @@ INST_GENERAL, INST_TRANSLATE, INST_IDENTITY = 0, 1, 2
 
 class rt_build_item(C.Structure):
     _fields_ = [("pos", vp), ("idx", vp), ("nodes", vp), ("tris", vp), ("info", vp),
-                ("sorted_ids", vp), ("codes", vp), ("n", u32), ("node_capacity", u32)]
+                ("sorted_ids", vp), ("codes", vp), ("n", u32), ("node_capacity", u32),
+                ("scratch", vp), ("scratch_bytes", C.c_uint64)]
 
 
 class rt_build_cfg(C.Structure):
@@ -403,7 +404,11 @@ class GpuBVH4:
         return (bits, box, tri) if counters else bits
 
 
-BATCH_BUILD_MAX = 4096
+# Batched builds take meshes of up to BATCH_BUILD_MAX triangles; those of at
+# most FUSED_BUILD_MAX are built in LDS (the fused path), larger ones by one
+# workgroup each over a scratch buffer.
+BATCH_BUILD_MAX = 65536
+FUSED_BUILD_MAX = 4096
 
 
 def build_radv(pos, idx=None, order="input", stream=None, out=None):
@@ -426,9 +431,10 @@ class BuildBatch:
     algorithm="implicit") of that mesh alone.
 
     meshes: list of (pos, idx or None).  The table is planned and uploaded
-    here, largest meshes first; launch() is a single kernel with
-    device-resident arguments (capturable in a graph).  .trees[i] is mesh
-    i's GpuBVH4 (filled by launch)."""
+    here, largest meshes first; meshes above FUSED_BUILD_MAX triangles get
+    their own scratch (one buffer for the batch).  launch() is a single
+    kernel with device-resident arguments (capturable in a graph).
+    .trees[i] is mesh i's GpuBVH4 (filled by launch)."""
 
     def __init__(self, meshes, keep_order=False, workgroups=0, device=None):
         dev = device or _dev()
@@ -437,6 +443,15 @@ class BuildBatch:
         self.trees = []
         arr = (rt_build_item * max(1, self.count))()
         order = sorted(range(self.count), key=lambda i: -_tri_count(*meshes[i]))
+        need = {}
+        for i in order:
+            n = _tri_count(*meshes[i])
+            if n > BATCH_BUILD_MAX:
+                raise ValueError(f"mesh {i}: {n} triangles > BATCH_BUILD_MAX")
+            if n > FUSED_BUILD_MAX:
+                need[i] = (build_requirements(n, build_cfg("implicit"))[1] + 255) // 256 * 256
+        self.scratch = torch.empty(max(256, sum(need.values())), dtype=torch.uint8, device=dev)
+        base, off = self.scratch.data_ptr(), 0
         for slot, i in enumerate(order):
             pos, idx = meshes[i]
             n = _tri_count(pos, idx)
@@ -449,8 +464,12 @@ class BuildBatch:
             if keep_order:
                 bvh.sorted_ids = torch.empty(n, dtype=torch.int32, device=dev)
                 bvh.codes = torch.empty(n, dtype=torch.int32, device=dev)
+            sp, sb = None, 0
+            if i in need:
+                sp, sb = vp(base + off), need[i]
+                off += sb
             arr[slot] = rt_build_item(ptr(pos), ptr(idx), ptr(nodes), ptr(tris), ptr(info),
-                                      ptr(bvh.sorted_ids), ptr(bvh.codes), n, cap)
+                                      ptr(bvh.sorted_ids), ptr(bvh.codes), n, cap, sp, sb)
             self.trees.append((i, bvh))
         self.trees = [b for _, b in sorted(self.trees, key=lambda x: x[0])]
         nbytes = C.c_size_t(0)

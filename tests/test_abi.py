@@ -195,3 +195,31 @@ def test_tuner_search_space_is_valid():
                                            None, None, None) == 0
     for bits in autotune.SORT_BITS:
         rtbvh.build_requirements(100_000, rtbvh.build_cfg("radix", 4, bits))
+
+
+def test_build_batch_plan_on_host():
+    """rt_gpu_build_batch_plan is host-only: meshes up to 65536 triangles;
+    those above 4096 need the implicit build's scratch (RT_ERR_CAPACITY
+    without it), an 80-B item record."""
+    import rtbvh
+    lib = rtbvh.lib()
+    assert C.sizeof(rtbvh.rt_build_item) == 80
+    fake = C.c_void_p(256)
+    nbytes = C.c_size_t(0)
+    assert lib.rt_gpu_build_batch_table_bytes(3, C.byref(nbytes)) == 0
+    table = (C.c_uint8 * nbytes.value)()
+
+    def item(n, scratch_bytes):
+        cap = rtbvh.build_requirements(n, rtbvh.build_cfg("implicit"))[0]
+        return rtbvh.rt_build_item(fake, None, fake, fake, fake, None, None, n, cap,
+                                   fake if scratch_bytes else None, scratch_bytes)
+
+    need = rtbvh.build_requirements(65536, rtbvh.build_cfg("implicit"))[1]
+    items = (rtbvh.rt_build_item * 3)(item(65536, need), item(4096, 0), item(7, 0))
+    assert lib.rt_gpu_build_batch_plan(items, 3, table, nbytes) == 0
+    items[0] = item(65536, need - 1)
+    assert lib.rt_gpu_build_batch_plan(items, 3, table, nbytes) == 2
+    items[0] = item(65537, 1 << 30)
+    assert lib.rt_gpu_build_batch_plan(items, 3, table, nbytes) == 2
+    items[0] = item(5000, rtbvh.build_requirements(5000, rtbvh.build_cfg("implicit"))[1])
+    assert lib.rt_gpu_build_batch_plan(items, 3, table, nbytes) == 0

@@ -167,3 +167,79 @@ def test_build_batch_graph_replays(rt, O):
         torch.cuda.synchronize()
         for t, (rn, rt_) in zip(trees, ref):
             assert torch.equal(t.nodes, rn) and torch.equal(t.tris, rt_)
+
+
+def _medium_meshes(rt, O):
+    """Meshes above 4096 triangles (one workgroup each in the batch) mixed
+    with small ones: indexed spheres, unindexed random triangles, a terrain
+    grid with long runs of equal codes, and the 65536 maximum."""
+    rng = np.random.default_rng(9)
+    out = []
+    for k, n in enumerate([65536, 4097, 5000, 16384, 16385, 30001, 100, 4096]):
+        if k % 2 == 0:
+            nu = max(2, int(np.sqrt(n / 2)))
+            pos, idx = O.sphere(nu, max(2, (n + 2 * nu - 1) // (2 * nu)), 1.0, 0.02, k)
+            idx = idx[:n]
+            out.append((torch.from_numpy(pos).cuda(),
+                        torch.from_numpy(np.ascontiguousarray(idx).view(np.int32)).cuda()))
+        else:
+            xyz = rng.random((n, 9), dtype=np.float32) * 2.0 - 1.0
+            out.append((torch.from_numpy(xyz).cuda(), None))
+    pos, idx = O.terrain(20_000)
+    out.append((torch.from_numpy(pos).cuda(),
+                torch.from_numpy(np.ascontiguousarray(idx).view(np.int32)).cuda()))
+    return out
+
+
+def test_build_batch_medium_equals_alone(rt, O):
+    """Meshes of 4097..65536 triangles built by one workgroup each inside the
+    persistent launch: nodes, leaf records, sorted ids and codes equal the
+    multi-launch implicit build of the same mesh alone, and the oracle's."""
+    meshes = _medium_meshes(rt, O)
+    for wg in (0, 2):
+        bb = rt.BuildBatch(meshes, keep_order=True, workgroups=wg)
+        trees = bb.launch()
+        torch.cuda.synchronize()
+        for (pos, idx), got in zip(meshes, trees):
+            want = rt.GpuBVH4.build(pos, idx, algorithm="implicit", keep_order=True)
+            torch.cuda.synchronize()
+            n = got.tri_count
+            assert [int(v) for v in got.info[:2].cpu()] == [want.node_count, want.depth]
+            np.testing.assert_array_equal(to_np(got.nodes)[: want.node_count * 128],
+                                          to_np(want.nodes)[: want.node_count * 128])
+            np.testing.assert_array_equal(to_np(got.tris)[: n * 64], to_np(want.tris)[: n * 64])
+            np.testing.assert_array_equal(u32(got.sorted_ids), u32(want.sorted_ids)[:n])
+            np.testing.assert_array_equal(u32(got.codes), u32(want.codes)[:n])
+    for k in (2, len(meshes) - 1):  # unindexed random, terrain ties
+        pos, idx = meshes[k]
+        p = to_np(pos)
+        i = None if idx is None else to_np(idx).view(np.uint32)
+        nodes_c, tris_c, ids_c, codes_c = O.lbvh_build(p.reshape(-1, 9) if i is None else p, i)
+        np.testing.assert_array_equal(u32(trees[k].codes), codes_c)
+        np.testing.assert_array_equal(u32(trees[k].sorted_ids), ids_c)
+        np.testing.assert_array_equal(to_np(trees[k].nodes)[: len(nodes_c) * 128],
+                                      nodes_c.view(np.uint8).reshape(-1))
+        np.testing.assert_array_equal(to_np(trees[k].tris)[: len(tris_c) * 64],
+                                      tris_c.view(np.uint8).reshape(-1))
+
+
+def test_build_batch_medium_graph_replays(rt, O):
+    meshes = _medium_meshes(rt, O)
+    bb = rt.BuildBatch(meshes)
+    trees = bb.launch()
+    torch.cuda.synchronize()
+    ref = [(t.nodes.clone(), t.tris.clone()) for t in trees]
+    s = torch.cuda.Stream()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g, stream=s):
+        bb.launch(stream=s)
+    for rep in range(3):
+        for t in trees:
+            t.nodes.fill_(0)
+            t.tris.fill_(0)
+        bb.scratch.fill_(0xFF)
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        for t, (rn, rt_) in zip(trees, ref):
+            assert torch.equal(t.nodes, rn) and torch.equal(t.tris, rt_)
