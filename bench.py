@@ -221,6 +221,8 @@ def batched_leg(meshes=256, seed=4, tuner=None):
     torch.cuda.synchronize()
 
     def graph_ms(shape, reps=5):
+        batch.launch(**shape)  # plans this shape's chunk table outside the capture
+        torch.cuda.synchronize()
         gs = torch.cuda.Stream()
         g = torch.cuda.CUDAGraph()
         with torch.cuda.graph(g, stream=gs):
@@ -333,25 +335,48 @@ def ao_frame_leg(frames=100, w=1920, h=1080):
             "ao_mean": round(float(out.mean()), 4)}
 
 
-def serving_leg(seed=7):
+def serving_leg(seed=7, tuner=None):
     """Serving: 32 resident scenes (10k-2.5M tris, log-uniform), requests of
     4k-2M rays (shadow, AO, closest) or 4k-64k-triangle deform-and-rebuilds.
     Capacity: 300 requests all waiting at t = 0, served (batched) and plain
-    (one launch per request, waiting for each).  Latency: 800 Poisson
-    arrivals at half the served capacity, p50 / p99 over all requests."""
+    (one launch per request, waiting for each; the library's default launch
+    shape, and again with the served path's shape).  Latency: 800 Poisson
+    arrivals at half the served capacity, p50 / p99 over all requests.  With
+    a tuner, the served trace schedule / shape is measured on a batch of 64
+    sample requests (not those served)."""
     import serve as S
     scenes = S.make_scenes(32)
     budget = 4 << 30
+    shape = {}
+    if tuner is not None:
+        sample = [r for r in S.make_requests(96, 0, scenes, seed=seed + 100)
+                  if r.kind != "rebuild"][:64]
+        items = []
+        for r in sample:
+            sc = scenes[r.scene]
+            r.rays = S._rays_for(r, sc)
+            items.append((sc.bvh, r.rays, "any" if r.kind != "closest" else "closest", None))
+        shape = tuner.trace_batch(rt.TraceBatch(items), tag="serving")
+        torch.cuda.synchronize()
+    S.run_plain(scenes, S.make_requests(40, 0, scenes, seed=seed + 200))  # warm every path
+    S.Server(scenes, budget, trace_shape=shape).run(S.make_requests(40, 0, scenes,
+                                                                    seed=seed + 300))
     reqs = S.make_requests(300, 0, scenes, seed=seed)
-    sat = S.Server(scenes, budget).run(reqs)
+    sat = S.Server(scenes, budget, trace_shape=shape).run(reqs)
     plain = S.run_plain(scenes, S.clone_requests(reqs))
+    plain_shaped = S.run_plain(scenes, S.clone_requests(reqs), trace_shape=shape)
     qps = round(0.5 * len(reqs) / sat["span_s"])
-    lat = S.Server(scenes, budget).run(S.make_requests(800, qps, scenes, seed=seed + 1))
+    lat = S.Server(scenes, budget, trace_shape=shape).run(
+        S.make_requests(800, qps, scenes, seed=seed + 1))
     return {"workload": f"32 resident scenes ({sum(x.tris for x in scenes)} tris), 4k-2M-ray "
                         "shadow/AO/closest requests and 4k-64k-tri rebuilds",
+            "trace_shape": shape or "default",
             "capacity_mrays_per_s": sat["mrays_per_s"],
             "plain_mrays_per_s": plain["mrays_per_s"],
+            "plain_same_shape_mrays_per_s": plain_shaped["mrays_per_s"],
             "speedup": round(sat["mrays_per_s"] / plain["mrays_per_s"], 2),
+            "speedup_vs_plain_same_shape": round(sat["mrays_per_s"] /
+                                                 plain_shaped["mrays_per_s"], 2),
             "qps": qps, "p50_ms": lat["p50_ms"], "p99_ms": lat["p99_ms"],
             "latency_mrays_per_s": lat["mrays_per_s"],
             "p99_ms_by_kind": {k: lat.get(f"p99_ms_{k}") for k in S.KINDS}}
@@ -476,7 +501,7 @@ def main():
     if not a.no_extra:
         out["ao_frame"] = ao_frame_leg()
         out["batched"] = batched_leg(tuner=tuner)
-        out["serving"] = serving_leg()
+        out["serving"] = serving_leg(tuner=tuner)
         out["p99_request_latency_ms"] = out["serving"]["p99_ms"]
     if tuner is not None:
         out["tuning"] = tuner.report()

@@ -280,11 +280,18 @@ rt_status rt_trace_batch_plan(const rt_trace_item* items, uint32_t count, uint32
 rt_status rt_trace_batch(rt_stream stream, const rt_trace_item* d_items,
                          const uint32_t* d_chunk_start, uint32_t count, uint32_t total_chunks,
                          uint32_t depth);
-/* rt_trace_batch with the launch shape of opts (block, lds_stack; schedule
- * must be RT_SCHED_SIMPLE, the other fields are unused).                    */
+/* rt_trace_batch with the schedule and launch shape of opts (same results
+ * for every choice).  RT_SCHED_SIMPLE: d_chunk_start from rt_trace_batch_plan
+ * (64-ray chunks, one per wave).  RT_SCHED_REFILL: d_chunk_start from
+ * rt_trace_batch_plan_ex with chunk_rays = opts->rays_per_wave; each wave
+ * walks one chunk, refilling finished lanes (refill_lanes as rt_trace_ex). */
 rt_status rt_trace_batch_ex(rt_stream stream, const rt_trace_item* d_items,
                             const uint32_t* d_chunk_start, uint32_t count, uint32_t total_chunks,
                             uint32_t depth, const rt_trace_opts* opts);
+/* rt_trace_batch_plan over chunks of chunk_rays rays (a multiple of 64, at
+ * most 4096): item i owns chunks [chunk_start[i], chunk_start[i+1]).      */
+rt_status rt_trace_batch_plan_ex(const rt_trace_item* items, uint32_t count, uint32_t chunk_rays,
+                                 uint32_t* chunk_start, uint32_t* depth);
 
 /* ---- two-level scene (the reference traces through a TLAS of instances:
  * src/main.cpp:920-1000, fullframe_benchmark.cpp:207-219) ------------------

@@ -11,7 +11,8 @@ configuration's bit for bit (a mismatch raises: it would be a kernel bug).
                                entries = fewer private-memory spills, fewer
                                resident waves), and for refill rays_per_wave
                                (the wave's ray tile) and refill_lanes
-  batched trace (rt_trace_batch_ex)   block, lds_stack
+  batched trace (rt_trace_batch_ex)   the same knobs (the refill chunk is a
+                               wave's share of one item)
   device build (rt_gpu_build)  sort_bits: digit width of the Morton radix
                                sort, 4 / 6 / 8 / 10 bits = 8 / 5 / 4 / 3 passes
   batched build (rt_gpu_build_batch)  workgroups of the persistent launch
@@ -142,10 +143,7 @@ class Tuner:
 
         def run(c):
             nonlocal out
-            if mode == "closest":
-                out = fn(sample, out=out, **c)
-            else:
-                out = fn(sample, out=out, **c)
+            out = fn(sample, out=out, **c)
             return out
 
         def check(c):
@@ -162,10 +160,12 @@ class Tuner:
         return self._pick(key, [best] + refill, run, check)
 
     def trace_batch(self, batch, tag=""):
-        """Launch shape (block, lds_stack) for TraceBatch.launch."""
+        """Schedule and launch shape for TraceBatch.launch: the best simple
+        shape (block, lds_stack), then the refill schedule around it."""
         key = (f"batch|{tag}|n{batch.count}|d{batch.depth}"
                f"|c{_size_class(batch.total_chunks)}")
-        cands = [{"block": b, "lds_stack": l} for b in TRACE_BLOCKS for l in TRACE_LDS]
+        cands = [{"schedule": "simple", "block": b, "lds_stack": l}
+                 for b in TRACE_BLOCKS for l in TRACE_LDS]
         cands.sort(key=lambda c: (c["block"], c["lds_stack"]) != (256, 20))
 
         def run(c):
@@ -174,7 +174,11 @@ class Tuner:
         def check(c):
             return _clone(run(c))
 
-        return self._pick(key, cands, run, check)
+        best = self._pick(key + "|simple", cands, run, check)
+        refill = [{"schedule": "refill", "block": best["block"],
+                   "lds_stack": best["lds_stack"], "rays_per_wave": w, "refill_lanes": r}
+                  for w in REFILL_RAYS_PER_WAVE for r in REFILL_LANES]
+        return self._pick(key, [best] + refill, run, check)
 
     # -- build -----------------------------------------------------------------
     def sort_bits(self, pos, idx=None, algorithm="radix", tag=""):

@@ -390,27 +390,33 @@ __global__ __launch_bounds__(kBlock) void k_trace_scene(const rt_instance* __res
 // are written once at the end.  Same per-ray arithmetic and order as k_trace.
 constexpr uint32_t kMaxPerWave = 4096;  // rays per wave: 128 occlusion words in LDS
 
+// One wave's refill loop over rays [wbeg, end) of one ray set (wbeg a
+// multiple of 64): whenever at least `refill` lanes are idle (or all are),
+// the idle lanes take the next rays of the range (ballot + mbcnt
+// compaction); each lane advances its ray by one node per iteration.  Any-hit
+// bits collect in wbits (the wave's LDS words, zeroed here) and are flushed
+// at the end.  Pointers are used as global (address space 1) views.
 template <int PRIV, bool ANY, bool COUNT>
-__global__ __launch_bounds__(kBlock) void k_trace_persist(
-    const rt_node4* __restrict__ nodes, const rt_tri64* __restrict__ tris, uint32_t node_count,
-    rt_rays_soa rays, uint32_t n, uint32_t per_wave, uint32_t refill, uint32_t lsu,
-    uint32_t* __restrict__ prim_out, float* __restrict__ t_out, uint32_t* __restrict__ bits_out,
-    uint32_t* __restrict__ box_out, uint32_t* __restrict__ tri_out) {
-    extern __shared__ int32_t lds_stack[];  // dynamic: ls * blockDim.x entries
-    // any hit: each wave's occlusion words (its range is whole 32-ray words)
-    __shared__ uint32_t lds_bits[ANY ? (kBlock / 64) * (kMaxPerWave / 32) : 1];
-    const uint32_t tid = threadIdx.x, lane = tid & 63u;
-    const int bs = (int)blockDim.x, ls = (int)lsu;
-    int32_t* col = lds_stack + tid;
-    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64u - lane));
-    const uint32_t wave = (blockIdx.x * blockDim.x + tid) >> 6;
-    const uint64_t wbeg = (uint64_t)wave * per_wave;
-    uint32_t cur = (uint32_t)min<uint64_t>(wbeg, n);
-    const uint32_t end = (uint32_t)min<uint64_t>(wbeg + per_wave, n);
-    const uint32_t wbeg32 = cur;  // a multiple of 64 (per_wave is)
-    uint32_t* wbits = lds_bits + (ANY ? (tid >> 6) * (kMaxPerWave / 32) : 0);
+__device__ __forceinline__ void refill_wave(
+    const rt_node4* __restrict__ nodes_, const rt_tri64* __restrict__ tris_, uint32_t node_count,
+    const rt_rays_soa& rays_, uint32_t wbeg32, uint32_t end, uint32_t refill, int32_t* col,
+    int ls, int bs, uint32_t* wbits, uint32_t* prim_out_, float* t_out_, uint32_t* bits_out_,
+    uint32_t* box_out_, uint32_t* tri_out_) {
+    const auto nodes = glb(nodes_);
+    const auto tris = glb(tris_);
+    const auto rox = glb(rays_.ox), roy = glb(rays_.oy), roz = glb(rays_.oz);
+    const auto rtmin = glb(rays_.tmin), rdx = glb(rays_.dx), rdy = glb(rays_.dy);
+    const auto rdz = glb(rays_.dz), rtmax = glb(rays_.tmax);
+    const auto prim_out = glb(prim_out_);
+    const auto t_out = glb(t_out_);
+    const auto bits_out = glb(bits_out_);
+    const auto box_out = glb(box_out_);
+    const auto tri_out = glb(tri_out_);
+    const uint32_t lane = threadIdx.x & 63u;
+    uint32_t cur = wbeg32;
     if (ANY)
         for (uint32_t i = lane; i < kMaxPerWave / 32; i += 64) wbits[i] = 0u;
+    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64u - lane));
 
     bool active = false;
     uint32_t r = 0;
@@ -428,8 +434,8 @@ __global__ __launch_bounds__(kBlock) void k_trace_persist(
                 const uint32_t rr = cur + __popcll(idle & lt);
                 if (rr < end) {
                     r = rr;
-                    ox = rays.ox[r]; oy = rays.oy[r]; oz = rays.oz[r]; tmin = rays.tmin[r];
-                    dx = rays.dx[r]; dy = rays.dy[r]; dz = rays.dz[r]; hitT = rays.tmax[r];
+                    ox = rox[r]; oy = roy[r]; oz = roz[r]; tmin = rtmin[r];
+                    dx = rdx[r]; dy = rdy[r]; dz = rdz[r]; hitT = rtmax[r];
                     ix = fabsf(dx) > 1e-6f ? 1.0f / dx : 1e6f;
                     iy = fabsf(dy) > 1e-6f ? 1.0f / dy : 1e6f;
                     iz = fabsf(dz) > 1e-6f ? 1.0f / dz : 1e6f;
@@ -448,11 +454,11 @@ __global__ __launch_bounds__(kBlock) void k_trace_persist(
         if (!finished) {
             int32_t ni;
         RT_SPOP(ni);
-            const float4* np = reinterpret_cast<const float4*>(nodes + ni);
-            const float4 mnx = np[0], mxx = np[1], mny = np[2], mxy = np[3], mnz = np[4],
-                         mxz = np[5];
-            const int4 ch = reinterpret_cast<const int4*>(np)[6];
-            const int4 cn = reinterpret_cast<const int4*>(np)[7];
+            const auto np = f4(nodes + ni);
+            const f4v mnx = np[0], mxx = np[1], mny = np[2], mxy = np[3], mnz = np[4],
+                      mxz = np[5];
+            const i4v ch = i4(nodes + ni)[6];
+            const i4v cn = i4(nodes + ni)[7];
             if (COUNT) boxTests += 4;
             const float bminx[4] = {mnx.x, mnx.y, mnx.z, mnx.w};
             const float bmaxx[4] = {mxx.x, mxx.y, mxx.z, mxx.w};
@@ -520,6 +526,62 @@ __global__ __launch_bounds__(kBlock) void k_trace_persist(
         const uint32_t words = (end - wbeg32 + 31) >> 5;
         for (uint32_t i = lane; i < words; i += 64) bits_out[(wbeg32 >> 5) + i] = wbits[i];
     }
+}
+
+template <int PRIV, bool ANY, bool COUNT>
+__global__ __launch_bounds__(kBlock) void k_trace_persist(
+    const rt_node4* __restrict__ nodes, const rt_tri64* __restrict__ tris, uint32_t node_count,
+    rt_rays_soa rays, uint32_t n, uint32_t per_wave, uint32_t refill, uint32_t lsu,
+    uint32_t* __restrict__ prim_out, float* __restrict__ t_out, uint32_t* __restrict__ bits_out,
+    uint32_t* __restrict__ box_out, uint32_t* __restrict__ tri_out) {
+    extern __shared__ int32_t lds_stack[];  // dynamic: ls * blockDim.x entries
+    // any hit: each wave's occlusion words (its range is whole 32-ray words)
+    __shared__ uint32_t lds_bits[ANY ? (kBlock / 64) * (kMaxPerWave / 32) : 1];
+    const uint32_t tid = threadIdx.x;
+    const uint32_t wave = (blockIdx.x * blockDim.x + tid) >> 6;
+    const uint64_t wbeg = (uint64_t)wave * per_wave;
+    const uint32_t beg = (uint32_t)min<uint64_t>(wbeg, n);  // a multiple of 64 (per_wave is)
+    const uint32_t end = (uint32_t)min<uint64_t>(wbeg + per_wave, n);
+    refill_wave<PRIV, ANY, COUNT>(nodes, tris, node_count, rays, beg, end, refill,
+                                  lds_stack + tid, (int)lsu, (int)blockDim.x,
+                                  lds_bits + (ANY ? (tid >> 6) * (kMaxPerWave / 32) : 0),
+                                  prim_out, t_out, bits_out, box_out, tri_out);
+}
+
+// Batched trace, refill form: item i owns the chunks [chunk[i], chunk[i+1])
+// of per_wave rays each (rt_trace_batch_plan_ex with chunk_rays = per_wave);
+// each wave takes one chunk (tree, rays and mode wave-uniform) and walks it
+// with the refill loop of k_trace_persist.
+template <int PRIV>
+__global__ __launch_bounds__(kBlock) void k_trace_batch_refill(
+    const rt_trace_item* __restrict__ items, const uint32_t* __restrict__ chunk, uint32_t count,
+    uint32_t chunks, uint32_t per_wave, uint32_t refill, uint32_t ls) {
+    extern __shared__ int32_t lds_stack[];  // dynamic: ls * blockDim.x entries
+    __shared__ uint32_t lds_bits[(kBlock / 64) * (kMaxPerWave / 32)];
+    const uint32_t tid = threadIdx.x, bs = blockDim.x;
+    const uint32_t c = __builtin_amdgcn_readfirstlane(blockIdx.x * (bs / 64) + (tid >> 6));
+    if (c >= chunks) return;
+    uint32_t lo = 0, hi = count;  // last item with chunk[item] <= c
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (chunk[mid] <= c) lo = mid;
+        else hi = mid;
+    }
+    const rt_trace_item& it = items[lo];
+    const uint64_t b0 = (uint64_t)(c - chunk[lo]) * per_wave;
+    const uint32_t beg = (uint32_t)min<uint64_t>(b0, it.count);
+    const uint32_t end = (uint32_t)min<uint64_t>(b0 + per_wave, it.count);
+    uint32_t* wbits = lds_bits + (tid >> 6) * (kMaxPerWave / 32);
+    if (it.flags & RT_ANY)
+        refill_wave<PRIV, true, false>(it.bvh.nodes, it.bvh.tris, it.bvh.node_count, it.rays,
+                                       beg, end, refill, lds_stack + tid, (int)ls, (int)bs,
+                                       wbits, it.hits.prim, nullptr, it.occlusion_bits, nullptr,
+                                       nullptr);
+    else
+        refill_wave<PRIV, false, false>(it.bvh.nodes, it.bvh.tris, it.bvh.node_count, it.rays,
+                                        beg, end, refill, lds_stack + tid, (int)ls, (int)bs,
+                                        wbits, it.hits.prim, it.hits.t, nullptr, nullptr,
+                                        nullptr);
 }
 
 struct TraceCfg {
@@ -632,9 +694,11 @@ extern "C" rt_status rt_trace(rt_stream stream, const rt_bvh_dev* bvh, const rt_
                        tri_tests);
 }
 
-extern "C" rt_status rt_trace_batch_plan(const rt_trace_item* items, uint32_t count,
-                                         uint32_t* chunk_start, uint32_t* depth) {
+extern "C" rt_status rt_trace_batch_plan_ex(const rt_trace_item* items, uint32_t count,
+                                            uint32_t chunk_rays, uint32_t* chunk_start,
+                                            uint32_t* depth) {
     if ((count && !items) || !chunk_start) return RT_ERR_ARG;
+    if (chunk_rays == 0 || chunk_rays % 64 || chunk_rays > kMaxPerWave) return RT_ERR_ARG;
     uint64_t c = 0;
     uint32_t deep = 1;
     for (uint32_t i = 0; i < count; ++i) {
@@ -652,7 +716,7 @@ extern "C" rt_status rt_trace_batch_plan(const rt_trace_item* items, uint32_t co
         if (d > kMaxDepth) return RT_ERR_CAPACITY;
         deep = d > deep ? d : deep;
         chunk_start[i] = (uint32_t)c;
-        c += ceil_div(it.count, 64);
+        c += ceil_div(it.count, chunk_rays);
         if (c > 0xFFFFFFFFull / kBlock) return RT_ERR_CAPACITY;
     }
     chunk_start[count] = (uint32_t)c;
@@ -660,19 +724,42 @@ extern "C" rt_status rt_trace_batch_plan(const rt_trace_item* items, uint32_t co
     return RT_OK;
 }
 
+extern "C" rt_status rt_trace_batch_plan(const rt_trace_item* items, uint32_t count,
+                                         uint32_t* chunk_start, uint32_t* depth) {
+    return rt_trace_batch_plan_ex(items, count, 64u, chunk_start, depth);
+}
+
 extern "C" rt_status rt_trace_batch_ex(rt_stream stream, const rt_trace_item* d_items,
                                        const uint32_t* d_chunk_start, uint32_t count,
                                        uint32_t total_chunks, uint32_t depth,
                                        const rt_trace_opts* opts) {
-    TraceCfg cfg{false, 0u, 0u, (uint32_t)kBlock, kLdsStack};
-    if (opts && opts->schedule != RT_SCHED_SIMPLE) return RT_ERR_ARG;
+    TraceCfg cfg{false, 1024u, 16u, (uint32_t)kBlock, kLdsStack};
+    if (opts) {
+        if (opts->schedule > RT_SCHED_REFILL) return RT_ERR_ARG;
+        cfg.refill = opts->schedule == RT_SCHED_REFILL;
+        if (opts->rays_per_wave) cfg.per_wave = opts->rays_per_wave;
+        if (opts->refill_lanes) cfg.refill_lanes = opts->refill_lanes;
+    }
     if (!shape_opts(opts, cfg)) return RT_ERR_ARG;
+    if (cfg.per_wave % 64 || cfg.per_wave > kMaxPerWave || cfg.refill_lanes > 64)
+        return RT_ERR_ARG;
     if (total_chunks == 0) return RT_OK;
     if (!d_items || !d_chunk_start || count == 0) return RT_ERR_ARG;
     if (depth == 0 || depth > kMaxDepth) return RT_ERR_ARG;
     hipStream_t s = as_stream(stream);
     const dim3 grid(ceil_div(total_chunks, cfg.block / 64)), block(cfg.block);
     const size_t shm = lds_bytes(cfg);
+    if (cfg.refill) {
+        if (small_priv(depth, cfg.lds))
+            hipLaunchKernelGGL(k_trace_batch_refill<kPrivSmall>, grid, block, shm, s, d_items,
+                               d_chunk_start, count, total_chunks, cfg.per_wave,
+                               cfg.refill_lanes, cfg.lds);
+        else
+            hipLaunchKernelGGL(k_trace_batch_refill<kPrivLarge>, grid, block, shm, s, d_items,
+                               d_chunk_start, count, total_chunks, cfg.per_wave,
+                               cfg.refill_lanes, cfg.lds);
+        return launch_status("k_trace_batch_refill");
+    }
     if (small_priv(depth, cfg.lds))
         hipLaunchKernelGGL(k_trace_batch<kPrivSmall>, grid, block, shm, s, d_items, d_chunk_start,
                            count, total_chunks, cfg.lds);

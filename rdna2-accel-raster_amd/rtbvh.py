@@ -145,6 +145,7 @@ _SIGS = {
     "rt_trace_batch_plan": [vp, u32, vp, vp],
     "rt_trace_batch": [vp, vp, vp, u32, u32, u32],
     "rt_trace_batch_ex": [vp, vp, vp, u32, u32, u32, vp],
+    "rt_trace_batch_plan_ex": [vp, u32, u32, vp, vp],
     "rt_trace_scene": [vp, vp, u32, u32, vp, u32, u32, vp, vp, vp],
     "rt_gen_ao_rays_scene": [vp, vp, u32, vp, vp, vp, vp, u32, u32, u32, f32, f32, f32, vp],
     "rt_ao_resolve": [vp, vp, vp, u32, vp],
@@ -436,7 +437,9 @@ class BuildBatch:
     kernel with device-resident arguments (capturable in a graph).
     .trees[i] is mesh i's GpuBVH4 (filled by launch)."""
 
-    def __init__(self, meshes, keep_order=False, workgroups=0, device=None):
+    def __init__(self, meshes, keep_order=False, workgroups=0, device=None, stream=None):
+        """stream: allocate on it and upload the table stream-ordered from
+        pinned memory (no host wait), for use from a serving loop."""
         dev = device or _dev()
         self.count = len(meshes)
         self.workgroups = workgroups
@@ -476,7 +479,13 @@ class BuildBatch:
         call("rt_gpu_build_batch_table_bytes", u32(self.count), C.byref(nbytes))
         host = (C.c_uint8 * max(1, nbytes.value))()
         call("rt_gpu_build_batch_plan", arr, u32(self.count), host, nbytes)
-        self.d_table = torch.frombuffer(bytearray(bytes(host)), dtype=torch.uint8).to(dev)
+        h_table = torch.frombuffer(bytearray(bytes(host)), dtype=torch.uint8)
+        if stream is None:
+            self.d_table = h_table.to(dev)
+        else:
+            self._pinned = h_table.pin_memory()
+            with torch.cuda.stream(stream):
+                self.d_table = self._pinned.to(dev, non_blocking=True)
 
     def launch(self, stream=None):
         call("rt_gpu_build_batch", stream_handle(stream), ptr(self.d_table), u32(self.count),
@@ -523,29 +532,48 @@ class TraceBatch:
             else:
                 raise ValueError(f"mode {mode!r}")
             self._keep.append((bvh, rays))
-        chunk = (u32 * (self.count + 1))()
-        depth = u32(0)
-        call("rt_trace_batch_plan", arr, u32(self.count), chunk, C.byref(depth))
-        self.depth = depth.value
-        self.total_chunks = chunk[self.count]
+        self._arr, self._dev, self._pinned = arr, dev, []
+        self._tables = {}  # rays per chunk -> (device chunk table, total chunks)
         h_items = torch.frombuffer(bytearray(bytes(arr)), dtype=torch.uint8)
-        h_chunk = torch.tensor(list(chunk), dtype=torch.int64).to(torch.int32)
-        if stream is None:
-            self.d_items, self.d_chunk = h_items.to(dev), h_chunk.to(dev)
-        else:
-            # stream-ordered upload from pinned copies (no host wait); the
-            # pinned buffers live as long as this batch
-            self._pinned = (h_items.pin_memory(), h_chunk.pin_memory())
-            with torch.cuda.stream(stream):
-                self.d_items = self._pinned[0].to(dev, non_blocking=True)
-                self.d_chunk = self._pinned[1].to(dev, non_blocking=True)
+        self.d_items = self._upload(h_items, stream)
+        self._table(64, stream)
 
-    def launch(self, stream=None, block=None, lds_stack=None):
-        """One launch over every item; block / lds_stack: launch shape (same
-        results for every choice; default TRACE_TUNING's)."""
-        o = trace_opts("simple", block=block, lds_stack=lds_stack)
-        call("rt_trace_batch_ex", stream_handle(stream), ptr(self.d_items), ptr(self.d_chunk),
-             u32(self.count), u32(self.total_chunks), u32(self.depth), C.byref(o))
+    def _upload(self, h, stream):
+        if stream is None:
+            return h.to(self._dev)
+        # stream-ordered upload from a pinned copy (no host wait); the pinned
+        # buffers live as long as this batch
+        p = h.pin_memory()
+        self._pinned.append(p)
+        with torch.cuda.stream(stream):
+            return p.to(self._dev, non_blocking=True)
+
+    def _table(self, rays_per_chunk, stream=None):
+        """Chunk table for chunks of rays_per_chunk rays (planned once)."""
+        if rays_per_chunk not in self._tables:
+            chunk = (u32 * (self.count + 1))()
+            depth = u32(0)
+            call("rt_trace_batch_plan_ex", self._arr, u32(self.count), u32(rays_per_chunk),
+                 chunk, C.byref(depth))
+            self.depth = depth.value
+            h = torch.tensor(list(chunk), dtype=torch.int64).to(torch.int32)
+            self._tables[rays_per_chunk] = (self._upload(h, stream), chunk[self.count])
+        return self._tables[rays_per_chunk]
+
+    @property
+    def total_chunks(self):
+        return self._tables[64][1]
+
+    def launch(self, stream=None, schedule="simple", block=None, lds_stack=None,
+               rays_per_wave=None, refill_lanes=None):
+        """One launch over every item (same results for every schedule and
+        shape; defaults TRACE_TUNING's).  schedule "refill": each wave walks
+        rays_per_wave rays of one item, refilling finished lanes."""
+        o = trace_opts(schedule, rays_per_wave=rays_per_wave, refill_lanes=refill_lanes,
+                       block=block, lds_stack=lds_stack)
+        d_chunk, total = self._table(o.rays_per_wave if schedule == "refill" else 64, stream)
+        call("rt_trace_batch_ex", stream_handle(stream), ptr(self.d_items), ptr(d_chunk),
+             u32(self.count), u32(total), u32(self.depth), C.byref(o))
         return self.outputs
 
 

@@ -13,16 +13,20 @@ Scheduling (one host thread):
     in the budget and fewer than `max_inflight` requests are in flight
     (FIFO: a large head is never overtaken);
   * all ray requests admitted in a tick get their rays generated and are
-    traced together in ONE rt_trace_batch launch; rebuilds rotate over
-    `rebuild_streams` streams of their own; ticks rotate over `streams`
-    streams, so work admitted while earlier work is still on the device
-    overlaps with it;
+    traced together in ONE rt_trace_batch launch (schedule and launch shape
+    `trace_shape`, e.g. the tuner's choice); all rebuilds admitted in a tick
+    are built together in ONE rt_gpu_build_batch launch (one workgroup per
+    mesh) on the next of `rebuild_streams` streams; ticks rotate over
+    `streams` streams, so work admitted while earlier work is still on the
+    device overlaps with it;
   * completion is polled through events; a finished request releases its
     bytes and its latency (arrival to observed completion) is recorded.
 
 The plain path (`run_plain`) serves the same requests one at a time with one
-trace launch per request and a wait after each, the results of both paths
-are compared by `same_results`.
+trace launch (or one rt_gpu_build) per request and a wait after each, with
+the library's default launch shape; the results of both paths are compared
+by `same_results`.  Rebuilt trees are the implicit (fast_bvh_builder) layout
+on both paths.
 
 Synthetic rays of each kind (origins uniform in the scene's bounds scaled by
 1.2, directions uniform on the sphere, seeded per request): closest-hit with
@@ -134,7 +138,7 @@ def make_requests(count, qps, scenes, seed=9, rays=(4_096, 2_000_000),
 def request_bytes(r):
     """Device bytes a request holds from admission to completion."""
     if r.kind == "rebuild":
-        cap, scratch = rt.build_requirements(r.n, rt.build_cfg("auto"))
+        cap, scratch = rt.build_requirements(r.n, rt.build_cfg("implicit"))
         return r.n * 36 + cap * rt.NODE_BYTES + r.n * rt.TRI_BYTES + scratch + 64
     out = r.n * 8 if r.kind == "closest" else ((r.n + 31) // 32) * 4
     return r.n * 32 + out
@@ -146,23 +150,28 @@ def _rays_for(r, scene, stream=None):
     return rt.gen_random_rays(r.n, r.seed, lo, hi, TMAX[r.kind], stream=stream)
 
 
+def _grid(r, stream=None):
+    return rt.gen_dynamic_grid(r.n, (r.seed % 200) * 0.05, 2.0, stream=stream)
+
+
 def _rebuild(r, stream=None):
-    xyz = rt.gen_dynamic_grid(r.n, (r.seed % 200) * 0.05, 2.0, stream=stream)
-    bvh = rt.GpuBVH4.build(xyz, None, algorithm="auto", stream=stream, sync=False)
+    xyz = _grid(r, stream)
+    bvh = rt.GpuBVH4.build(xyz, None, algorithm="implicit", stream=stream, sync=False)
     return xyz, bvh
 
 
-def _trace_alone(r, scene, stream=None):
+def _trace_alone(r, scene, stream=None, shape=None):
     if r.kind == "closest":
-        return scene.bvh.intersect(r.rays, stream=stream)
-    return scene.bvh.occluded(r.rays, stream=stream)
+        return scene.bvh.intersect(r.rays, stream=stream, **(shape or {}))
+    return scene.bvh.occluded(r.rays, stream=stream, **(shape or {}))
 
 
 # --------------------------------------------------------------- server ----
 class Server:
     def __init__(self, scenes, budget_bytes=2 << 30, streams=3, rebuild_streams=4,
-                 max_inflight=256, max_batch=128, keep_results=False):
+                 max_inflight=256, max_batch=128, keep_results=False, trace_shape=None):
         self.scenes = scenes
+        self.trace_shape = dict(trace_shape or {})
         self.keep_results = keep_results
         self.budget = budget_bytes
         self.max_inflight = max_inflight
@@ -204,7 +213,7 @@ class Server:
                     items.append((sc.bvh, r.rays, "any" if r.kind != "closest" else "closest",
                                   None))
                 batch = rt.TraceBatch(items, stream=s)
-                outs = batch.launch(stream=s)
+                outs = batch.launch(stream=s, **self.trace_shape)
                 self.launches += 1
                 for r, o in zip(ray_reqs, outs):
                     r.out = o
@@ -212,17 +221,21 @@ class Server:
                 ev = torch.cuda.Event()
                 ev.record(s)
             groups.append((ev, ray_reqs))
-        # rebuilds: a small build is a chain of short kernels, so each goes to
-        # the next of several streams and independent rebuilds overlap
-        for r in (r for r in took if r.kind == "rebuild"):
+        # rebuilds: every mesh of the tick in one persistent batched build
+        # (one workgroup each) on a stream of its own, beside the traces
+        reb = [r for r in took if r.kind == "rebuild"]
+        if reb:
             s = self.rebuild_streams[self.rebuilds % len(self.rebuild_streams)]
             self.rebuilds += 1
             with torch.cuda.stream(s):
-                xyz, r.bvh = _rebuild(r, stream=s)
-                r.keep.append(xyz)
+                grids = [_grid(r, stream=s) for r in reb]
+                bb = rt.BuildBatch([(g, None) for g in grids], stream=s)
+                for r, g, t in zip(reb, grids, bb.launch(stream=s)):
+                    r.bvh = t
+                    r.keep.extend((g, bb))
                 ev = torch.cuda.Event()
                 ev.record(s)
-            groups.append((ev, [r]))
+            groups.append((ev, reb))
         return groups
 
     def run(self, requests):
@@ -277,10 +290,12 @@ def summarize(finished, wall, server=None):
     return out
 
 
-def run_plain(scenes, requests, keep_results=False):
+def run_plain(scenes, requests, keep_results=False, trace_shape=None):
     """One request at a time, one trace launch each, waiting for each
-    (arrival times ignored: the plain path's best case).  Returns stats;
-    with keep_results each request's results stay in r.out / r.bvh."""
+    (arrival times ignored: the plain path's best case).  trace_shape: the
+    rt_trace_ex schedule / launch shape of every trace (default: the
+    library's).  Returns stats; with keep_results each request's results
+    stay in r.out / r.bvh."""
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for r in requests:
@@ -291,7 +306,7 @@ def run_plain(scenes, requests, keep_results=False):
         else:
             sc = scenes[r.scene]
             r.rays = _rays_for(r, sc)
-            r.out = _trace_alone(r, sc)
+            r.out = _trace_alone(r, sc, shape=trace_shape)
         torch.cuda.synchronize()
         r.arrival, r.done = a, time.perf_counter() - t0
         if not keep_results:

@@ -98,8 +98,16 @@ def test_trace_launch_shape_checks():
         assert lib.rt_trace_ex(None, C.byref(b), C.byref(r), 0, 0, C.byref(o), None, None, None,
                                None) == want, (block, lds)
         assert lib.rt_trace_batch_ex(None, None, None, 0, 0, 1, C.byref(o)) == want
-    o = rtbvh.rt_trace_opts(1, 0, 0, 0, 0)  # the batch path has one schedule
-    assert lib.rt_trace_batch_ex(None, None, None, 0, 0, 1, C.byref(o)) == 1
+    o = rtbvh.rt_trace_opts(1, 0, 0, 0, 0)  # refill with default chunks
+    assert lib.rt_trace_batch_ex(None, None, None, 0, 0, 1, C.byref(o)) == 0
+    for bad in (rtbvh.rt_trace_opts(2, 0, 0, 0, 0), rtbvh.rt_trace_opts(1, 100, 0, 0, 0),
+                rtbvh.rt_trace_opts(1, 8192, 0, 0, 0), rtbvh.rt_trace_opts(1, 256, 65, 0, 0)):
+        assert lib.rt_trace_batch_ex(None, None, None, 0, 0, 1, C.byref(bad)) == 1
+    # chunk tables of other chunk sizes: multiples of 64 up to 4096
+    chunk = (C.c_uint32 * 1)()
+    assert lib.rt_trace_batch_plan_ex(None, 0, 256, chunk, None) == 0 and chunk[0] == 0
+    for bad in (0, 100, 8192):
+        assert lib.rt_trace_batch_plan_ex(None, 0, bad, chunk, None) == 1
     with pytest.raises(ValueError):
         rtbvh.trace_opts("bogus")
 

@@ -59,7 +59,10 @@ def test_batch_equals_alone(rt, O):
             np.testing.assert_array_equal(_bits(u32(got), n), _bits(u32(bits), n))
 
 
-def test_batch_matches_oracle(rt, O):
+@pytest.mark.parametrize("sched", [dict(), dict(schedule="refill", rays_per_wave=512,
+                                                  refill_lanes=8, block=64)],
+                         ids=["simple", "refill"])
+def test_batch_matches_oracle(rt, O, sched):
     """One small tree from the oracle's software_bvh build, two items (closest
     and any) in one batch: hit IDs, t and occlusion bits equal the CPU path."""
     pos, idx = O.wave_grid(48, 48, 1.0, 0.0, 0.1, 4.0, 4.0)
@@ -69,7 +72,7 @@ def test_batch_matches_oracle(rt, O):
     r2 = O.random_rays(30_000, 8, (-1.2, -1.2, -0.3), (1.2, 1.2, 0.3), tmax=0.3)
     batch = rt.TraceBatch([(bvh, rt.Rays.from_numpy(r1.packed()), "closest", None),
                            (bvh, rt.Rays.from_numpy(r2.packed()), "any", None)])
-    (prim, t), bits = batch.launch()
+    (prim, t), bits = batch.launch(**sched)
     torch.cuda.synchronize()
     pc, tc = O.trace_bvh4(n4, tris, r1)
     np.testing.assert_array_equal(u32(prim), pc)

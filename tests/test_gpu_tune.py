@@ -53,7 +53,12 @@ def test_batch_launch_shapes(rt, O):
     ref = [tuple(x.clone() for x in o) if isinstance(o, tuple) else o.clone()
            for o in batch.launch()]
     torch.cuda.synchronize()
-    for shape in SHAPES:
+    refill = [dict(schedule="refill", block=64, lds_stack=8, rays_per_wave=64, refill_lanes=1),
+              dict(schedule="refill", block=256, lds_stack=20, rays_per_wave=4096,
+                   refill_lanes=64),
+              dict(schedule="refill", block=128, lds_stack=16, rays_per_wave=256,
+                   refill_lanes=8)]
+    for shape in SHAPES + refill:
         outs = batch.launch(**shape)
         torch.cuda.synchronize()
         for (bvh, rays, mode, _), a, b in zip(items, outs, ref):
@@ -141,3 +146,4 @@ def test_tuner_trace_batch(rt, O):
     tuner = autotune.Tuner(reps=1)
     shape = tuner.trace_batch(batch, tag="t")
     assert shape["block"] in (64, 128, 256) and 4 <= shape["lds_stack"] <= 48
+    assert shape["schedule"] in ("simple", "refill")
