@@ -403,6 +403,9 @@ def main():
                          "or one ray/thread)")
     ap.add_argument("--no-tune", action="store_true",
                     help="fixed launch shapes instead of the autotuner's measured choices")
+    ap.add_argument("--tune-cache", default=None,
+                    help="JSON file of tuned choices: read if present (no re-measurement), "
+                         "written after tuning")
     ap.add_argument("--rays-per-wave", type=int, default=1024)
     ap.add_argument("--refill-lanes", type=int, default=16)
     ap.add_argument("--builder", choices=["radix", "implicit"], default="radix")
@@ -425,7 +428,7 @@ def main():
         copy_probe()
 
     tuner = None if a.no_tune else autotune.Tuner(
-        log=lambda m: print(m, file=sys.stderr, flush=True))
+        cache_path=a.tune_cache, log=lambda m: print(m, file=sys.stderr, flush=True))
     b4k_p50, b4k_p99, b4k_b2b = build_time_4k()
     f = Frame(tuner)
     if tuner is not None:

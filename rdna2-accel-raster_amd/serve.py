@@ -9,6 +9,10 @@ charged against `budget_bytes` from admission to completion.
 
 Scheduling (one host thread):
   * a request becomes visible at its arrival time and waits FIFO;
+  * a tick is started only while fewer than `max_ticks` earlier ticks are
+    still on the device, so requests arriving while the GPU is busy gather
+    into the next tick's batch (continuous batching) instead of each going
+    out alone;
   * each tick admits from the head of the queue while the head's bytes fit
     in the budget and fewer than `max_inflight` requests are in flight
     (FIFO: a large head is never overtaken);
@@ -169,8 +173,10 @@ def _trace_alone(r, scene, stream=None, shape=None):
 # --------------------------------------------------------------- server ----
 class Server:
     def __init__(self, scenes, budget_bytes=2 << 30, streams=3, rebuild_streams=4,
-                 max_inflight=256, max_batch=128, keep_results=False, trace_shape=None):
+                 max_inflight=256, max_batch=128, keep_results=False, trace_shape=None,
+                 max_ticks=None):
         self.scenes = scenes
+        self.max_ticks = max_ticks or streams
         self.trace_shape = dict(trace_shape or {})
         self.keep_results = keep_results
         self.budget = budget_bytes
@@ -185,7 +191,7 @@ class Server:
 
     def _admit(self, queue, inflight, now):
         took = []
-        busy = sum(len(reqs) for _, reqs in inflight)
+        busy = sum(len(reqs) for _, _, reqs in inflight)
         while queue and busy + len(took) < self.max_inflight and len(took) < self.max_batch:
             r = queue[0]
             r.bytes = request_bytes(r)
@@ -248,12 +254,13 @@ class Server:
             now = time.perf_counter() - t0
             while pending and pending[0].arrival <= now:
                 queue.append(pending.popleft())
-            took = self._admit(queue, inflight, now) if queue else []
+            ticks = len({k for k, _, _ in inflight})
+            took = self._admit(queue, inflight, now) if queue and ticks < self.max_ticks else []
             if took:
-                inflight.extend(self._launch(took, tick))
+                inflight.extend((tick, ev, reqs) for ev, reqs in self._launch(took, tick))
                 tick += 1
             still = []
-            for ev, reqs in inflight:
+            for k, ev, reqs in inflight:
                 if ev.query():
                     t = time.perf_counter() - t0
                     for r in reqs:
@@ -264,7 +271,7 @@ class Server:
                             r.rays = r.out = r.bvh = None
                         finished.append(r)
                 else:
-                    still.append((ev, reqs))
+                    still.append((k, ev, reqs))
             inflight = still
         wall = time.perf_counter() - t0
         return summarize(finished, wall, self)
