@@ -164,6 +164,22 @@ rt_status rt_gpu_refit(rt_stream stream, const float* d_pos, const uint32_t* d_i
                        rt_node4* d_nodes, uint32_t node_count, rt_tri64* d_tris,
                        void* d_scratch, size_t scratch_bytes);
 
+/* Refit of ANY Box4 tree (RT_BUILD_RADIX or RT_BUILD_IMPLICIT device builds,
+ * an uploaded SoftwareBVH4): same topology, leaf records re-read from their
+ * triangle_id at the new positions, every slot box recomputed bottom-up (a
+ * leaf range: union of its records' boxes; an inner slot: union of the
+ * child's occupied slots).  Inner children must have larger node indices
+ * than their parents (pre-order or level order, as every builder here and
+ * SoftwareBVH4::fromBVH2 number them).  node_count = the tree's node count
+ * (rt_build_info.node_count, not the capacity: every node is visited),
+ * depth = its node levels (rt_build_info.depth; 0 = 64);
+ * scratch >= 4 * node_count bytes.  No
+ * reference counterpart (the reference rebuilds deforming meshes per frame,
+ * src/main.cpp:871-885).                                                    */
+rt_status rt_gpu_refit_bvh4(rt_stream stream, const float* d_pos, const uint32_t* d_idx,
+                            rt_node4* d_nodes, uint32_t node_count, rt_tri64* d_tris,
+                            uint32_t depth, void* d_scratch, size_t scratch_bytes);
+
 /* The RADV acceleration-structure image fast_bvh_builder.comp writes
  * (shaders/fast_bvh_builder.comp:91-257, formats radv_bvh.hpp:22-152):
  * 192-B header with the pre-root links at 0xB4 / 0xBC, box32 nodes (child

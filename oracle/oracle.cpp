@@ -1189,6 +1189,58 @@ int or_lbvh_refit(const float* pos, const uint32_t* idx, uint32_t n, or_bvh4_nod
     return 0;
 }
 
+// Refit of any Box4 tree whose inner children have larger indices than their
+// parents (pre-order numbering of the radix / SoftwareBVH4 trees, level order
+// of the implicit one): every leaf record re-read from its triangle_id, every
+// slot box recomputed -- a leaf range as the union of its records' boxes
+// (canonical +0 zeros, as the builds), an inner slot as the union of the
+// child's occupied slots -- visiting nodes from the last to the first, so a
+// child is final before its parent reads it.  Topology (child, count) and
+// empty slots are unchanged.  Returns 1 if a child index does not exceed its
+// parent's.  No reference counterpart: the reference rebuilds deforming
+// meshes every frame (src/main.cpp:871-885).
+int or_bvh4_refit(const float* pos, const uint32_t* idx, or_bvh4_node* nodes,
+                  uint32_t node_count, or_tri64* tris, uint32_t ntris) {
+    for (uint32_t s = 0; s < ntris; ++s) {
+        const float *p0, *p1, *p2;
+        tri_vertices(pos, idx, tris[s].triangle_id, &p0, &p1, &p2);
+        for (int c = 0; c < 3; ++c) {
+            tris[s].v[c] = p0[c];
+            tris[s].v[3 + c] = p1[c];
+            tris[s].v[6 + c] = p2[c];
+        }
+    }
+    for (int64_t i = (int64_t)node_count - 1; i >= 0; --i) {
+        or_bvh4_node& nd = nodes[i];
+        for (int k = 0; k < 4; ++k) {
+            float u[6] = {FLT_MAX, FLT_MAX, FLT_MAX, -FLT_MAX, -FLT_MAX, -FLT_MAX};
+            if (nd.count[k] > 0) {
+                const uint32_t first = (uint32_t)~nd.child[k];
+                for (int32_t q = 0; q < nd.count[k]; ++q) {
+                    const float* v = tris[first + q].v;
+                    for (int a = 0; a < 3; ++a) {
+                        u[a] = std::min(u[a], fmin3(v[a], v[3 + a], v[6 + a]) + 0.0f);
+                        u[3 + a] = std::max(u[3 + a], fmax3(v[a], v[3 + a], v[6 + a]) + 0.0f);
+                    }
+                }
+            } else if (nd.count[k] < 0) {
+                if (nd.child[k] <= i || (uint32_t)nd.child[k] >= node_count) return 1;
+                const or_bvh4_node& c = nodes[nd.child[k]];
+                for (int q = 0; q < 4; ++q) {
+                    if (c.count[q] == 0) continue;
+                    u[0] = std::min(u[0], c.minX[q]); u[3] = std::max(u[3], c.maxX[q]);
+                    u[1] = std::min(u[1], c.minY[q]); u[4] = std::max(u[4], c.maxY[q]);
+                    u[2] = std::min(u[2], c.minZ[q]); u[5] = std::max(u[5], c.maxZ[q]);
+                }
+            } else {
+                continue;
+            }
+            set_slot(nd, k, u, nd.child[k], nd.count[k]);
+        }
+    }
+    return 0;
+}
+
 // shaders/fast_bvh_builder.comp:91-257 byte image: 136-B header (+ pre-root
 // links at 0xB4/0xBC) at 0, Box32 nodes at 192 + 128 i (children
 // (rel >> 3) | type, rel measured from offset 192), 64-B triangle nodes after

@@ -24,6 +24,12 @@
  *   or_lbvh_build          -> invariants: permutation, sortedness, parent box
  *                             = union of children, root = scene bounds,
  *                             layout == compute_tree_layout (_ref).
+ *   or_lbvh_refit,
+ *   or_bvh4_refit          -> on unchanged positions each reproduces the
+ *                             builder's own tree bytes (implicit, radix,
+ *                             software_bvh); after a deformation the two agree
+ *                             on implicit trees and every slot box equals the
+ *                             brute-force union of its records / child slots.
  *   or_radv_encode         -> static_assert offsets of radv_bvh.hpp:106-152,
  *                             header constants of fast_bvh_builder.comp:220-256,
  *                             and the N+1-ray verification of src/main.cpp.
@@ -157,6 +163,8 @@ int or_rlbvh_build(const float* pos, const uint32_t* idx, uint32_t n, uint32_t m
                    uint32_t* codes_sorted);
 int or_lbvh_refit(const float* pos, const uint32_t* idx, uint32_t ntris,
                   or_bvh4_node* nodes, uint32_t node_count, or_tri64* tris);
+int or_bvh4_refit(const float* pos, const uint32_t* idx, or_bvh4_node* nodes,
+                  uint32_t node_count, or_tri64* tris, uint32_t ntris);
 int or_radv_encode(const float* xyz9, uint32_t n, const uint32_t* order,
                    uint8_t* out, uint64_t out_bytes);
 

@@ -435,6 +435,19 @@ def lbvh_refit(pos, idx, nodes, tris):
     return nodes, tris
 
 
+def bvh4_refit(pos, idx, nodes, tris):
+    """Generic refit (or_bvh4_refit) of any Box4 tree with children after
+    their parents: returns new (nodes, tris)."""
+    pos = _f(pos)
+    nodes = nodes.copy()
+    tris = tris.copy()
+    rc = lib().or_bvh4_refit(_p(pos), _p(None if idx is None else _u(idx)), _p(nodes),
+                             u32(len(nodes)), _p(tris), u32(len(tris)))
+    if rc:
+        raise ValueError("tree is not in parent-before-child order")
+    return nodes, tris
+
+
 def radv_encode(xyz9, order=None):
     xyz9 = _f(xyz9)
     n = xyz9.size // 9

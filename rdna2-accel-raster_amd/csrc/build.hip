@@ -1297,6 +1297,99 @@ __device__ void build_medium(SmallLds& S, const MedTree& T, const float* __restr
     }
 }
 
+// ---------------------------------------------------------------------------
+// Generic refit of any Box4 tree whose inner children have larger indices
+// than their parents (radix pre-order, implicit level order, an uploaded
+// SoftwareBVH4): topology kept, records and boxes from new positions.
+//   k_refit_leaf_slots  thread per node: every leaf-range slot rewrites its
+//                       records (vertices re-read by triangle_id) and gets
+//                       the union of their boxes; the node's round is reset.
+//   k_refit_round(r)    thread per unfinished node: if every inner child
+//                       finished in an earlier round (done < r), the node's
+//                       inner slots get the union of the child's occupied
+//                       slots and done = r.  A node of height h finishes in
+//                       round h, so `depth` rounds finish the root.  Reads of
+//                       a flag written in the same round see either the old
+//                       or the new value, and neither is < r: the kernel
+//                       boundaries order every box a node reads.
+// ---------------------------------------------------------------------------
+constexpr uint32_t kNotDone = 0xFFFFFFFFu;
+
+__device__ __forceinline__ void set_slot_box(rt_node4* nd, int k, const float b[6]) {
+    nd->minX[k] = b[0];
+    nd->minY[k] = b[1];
+    nd->minZ[k] = b[2];
+    nd->maxX[k] = b[3];
+    nd->maxY[k] = b[4];
+    nd->maxZ[k] = b[5];
+}
+
+__global__ __launch_bounds__(256) void k_refit_leaf_slots(const float* __restrict__ pos,
+                                                          const uint32_t* __restrict__ idx,
+                                                          rt_node4* __restrict__ nodes,
+                                                          uint32_t node_count,
+                                                          rt_tri64* __restrict__ tris,
+                                                          uint32_t* __restrict__ done) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= node_count) return;
+    done[i] = kNotDone;
+    const int4 ch = reinterpret_cast<const int4*>(nodes + i)[6];
+    const int4 cn = reinterpret_cast<const int4*>(nodes + i)[7];
+    const int32_t chv[4] = {ch.x, ch.y, ch.z, ch.w};
+    const int32_t cnv[4] = {cn.x, cn.y, cn.z, cn.w};
+    for (int k = 0; k < 4; ++k) {
+        if (cnv[k] <= 0) continue;
+        float u[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+        const uint32_t first = (uint32_t)~chv[k];
+        for (int32_t q = 0; q < cnv[k]; ++q) {
+            const uint32_t s = first + (uint32_t)q;
+            const uint32_t id = tris[s].triangle_id;
+            float v[9], bb[6];
+            tri_verts(pos, idx, id, v);
+            store_tri(tris + s, v, id);
+            tri_box(v, bb);
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                u[a] = fminf(u[a], bb[a]);
+                u[3 + a] = fmaxf(u[3 + a], bb[3 + a]);
+            }
+        }
+        set_slot_box(nodes + i, k, u);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_refit_round(rt_node4* __restrict__ nodes,
+                                                     uint32_t node_count,
+                                                     uint32_t* __restrict__ done, uint32_t r) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= node_count || done[i] != kNotDone) return;
+    const int4 ch = reinterpret_cast<const int4*>(nodes + i)[6];
+    const int4 cn = reinterpret_cast<const int4*>(nodes + i)[7];
+    const int32_t chv[4] = {ch.x, ch.y, ch.z, ch.w};
+    const int32_t cnv[4] = {cn.x, cn.y, cn.z, cn.w};
+    for (int k = 0; k < 4; ++k) {
+        if (cnv[k] >= 0) continue;
+        const uint32_t d = done[chv[k]];
+        if (d == kNotDone || d >= r) return;  // a child not finished before this round
+    }
+    for (int k = 0; k < 4; ++k) {
+        if (cnv[k] >= 0) continue;
+        const rt_node4& c = nodes[chv[k]];
+        float u[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+        for (int q = 0; q < 4; ++q) {
+            if (c.count[q] == 0) continue;
+            u[0] = fminf(u[0], c.minX[q]);
+            u[1] = fminf(u[1], c.minY[q]);
+            u[2] = fminf(u[2], c.minZ[q]);
+            u[3] = fmaxf(u[3], c.maxX[q]);
+            u[4] = fmaxf(u[4], c.maxY[q]);
+            u[5] = fmaxf(u[5], c.maxZ[q]);
+        }
+        set_slot_box(nodes + i, k, u);
+    }
+    done[i] = r;
+}
+
 // Device table entry of the batched build: the caller's item and the level
 // table of its tree (fused path for n <= 4096, medium path above).
 struct BatchBuild {
@@ -1958,6 +2051,25 @@ extern "C" rt_status rt_gpu_refit(rt_stream stream, const float* d_pos, const ui
     if (!d_scratch || scratch_bytes < p.total) return RT_ERR_CAPACITY;
     float* gboxes = reinterpret_cast<float*>(static_cast<char*>(d_scratch) + p.boxes);
     return emit_levels(s, d_pos, d_idx, n, nullptr, d_nodes, d_tris, gboxes, t, true);
+}
+
+extern "C" rt_status rt_gpu_refit_bvh4(rt_stream stream, const float* d_pos,
+                                       const uint32_t* d_idx, rt_node4* d_nodes,
+                                       uint32_t node_count, rt_tri64* d_tris, uint32_t depth,
+                                       void* d_scratch, size_t scratch_bytes) {
+    if (node_count == 0) return RT_OK;
+    if (!d_pos || !d_nodes || !d_tris) return RT_ERR_ARG;
+    if (depth > kMaxDepth) return RT_ERR_ARG;
+    if (!d_scratch || scratch_bytes < (size_t)node_count * 4) return RT_ERR_CAPACITY;
+    hipStream_t s = as_stream(stream);
+    uint32_t* done = static_cast<uint32_t*>(d_scratch);
+    const dim3 grid(ceil_div(node_count, 256)), block(256);
+    hipLaunchKernelGGL(k_refit_leaf_slots, grid, block, 0, s, d_pos, d_idx, d_nodes, node_count,
+                       d_tris, done);
+    const uint32_t rounds = depth ? depth : kMaxDepth;
+    for (uint32_t r = 0; r < rounds; ++r)
+        hipLaunchKernelGGL(k_refit_round, grid, block, 0, s, d_nodes, node_count, done, r);
+    return launch_status("k_refit_round");
 }
 
 extern "C" rt_status rt_gpu_build_batch_table_bytes(uint32_t count, size_t* bytes) {

@@ -126,6 +126,7 @@ _SIGS = {
     "rt_build_requirements": [u32, vp, vp, vp],
     "rt_gpu_build": [vp, vp, vp, u32, vp, vp, u32, vp, vp, vp, vp, vp, C.c_size_t],
     "rt_gpu_refit": [vp, vp, vp, u32, vp, u32, vp, vp, C.c_size_t],
+    "rt_gpu_refit_bvh4": [vp, vp, vp, vp, u32, vp, u32, vp, C.c_size_t],
     "rt_convert_tris48": [vp, vp, vp, u32, vp],
     "rt_trace": [vp, vp, vp, u32, u32, vp, vp, vp, vp],
     "rt_trace_ex": [vp, vp, vp, u32, u32, vp, vp, vp, vp, vp],
@@ -318,8 +319,26 @@ class GpuBVH4:
         bvh.pos, bvh.idx = pos, idx
         return bvh
 
-    def refit(self, pos, idx=None, stream=None):
-        """Same topology, new vertex positions (rt_gpu_refit, implicit trees)."""
+    def refit(self, pos, idx=None, stream=None, generic=None):
+        """Same topology, new vertex positions.  Implicit trees use the
+        implicit refit (rt_gpu_refit); any other tree (radix, uploaded
+        SoftwareBVH4), or generic=True, the generic one (rt_gpu_refit_bvh4),
+        which needs the node count and depth (sync=True builds have them)."""
+        if generic is None:
+            generic = getattr(self, "algorithm", None) != "implicit"
+        if generic:
+            if getattr(self, "info", None) is not None:
+                # the real node count and depth (a sync=False build holds the
+                # capacity, whose tail nodes are uninitialised)
+                self.node_count, self.depth = [int(v) for v in self.info[:2].cpu().tolist()]
+            need = max(256, self.node_count * 4)
+            if self._scratch is None or self._scratch.numel() < need:
+                self._scratch = torch.empty(need, dtype=torch.uint8, device=pos.device)
+            call("rt_gpu_refit_bvh4", stream_handle(stream), ptr(pos), ptr(idx), ptr(self.nodes),
+                 u32(self.node_count), ptr(self.tris), u32(self.depth), ptr(self._scratch),
+                 C.c_size_t(self._scratch.numel()))
+            self.pos, self.idx = pos, idx
+            return self
         need = build_requirements(self.tri_count, build_cfg("implicit"))[1]
         if self._scratch is None or self._scratch.numel() < need:
             self._scratch = torch.empty(max(need, 256), dtype=torch.uint8, device=pos.device)

@@ -231,3 +231,14 @@ def test_build_batch_plan_on_host():
     assert lib.rt_gpu_build_batch_plan(items, 3, table, nbytes) == 2
     items[0] = item(5000, rtbvh.build_requirements(5000, rtbvh.build_cfg("implicit"))[1])
     assert lib.rt_gpu_build_batch_plan(items, 3, table, nbytes) == 0
+
+
+def test_generic_refit_argument_checks():
+    import rtbvh
+    lib = rtbvh.lib()
+    fake = C.c_void_p(256)
+    assert lib.rt_gpu_refit_bvh4(None, None, None, None, 0, None, 0, None, 0) == 0  # empty
+    assert lib.rt_gpu_refit_bvh4(None, None, None, fake, 10, fake, 5, fake, 40) == 1
+    assert lib.rt_gpu_refit_bvh4(None, fake, None, fake, 10, fake, 65, fake, 40) == 1
+    assert lib.rt_gpu_refit_bvh4(None, fake, None, fake, 10, fake, 5, fake, 39) == 2
+    assert lib.rt_gpu_refit_bvh4(None, fake, None, fake, 10, fake, 5, None, 40) == 2

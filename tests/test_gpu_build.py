@@ -197,3 +197,69 @@ def test_radv_image_indexed(rt, O):
     want = O.lbvh_build(pos, idx)[2]
     np.testing.assert_array_equal(u32(ids), want)
     np.testing.assert_array_equal(to_np(img), O.radv_encode(xyz, order=want))
+
+
+def _deformed(pos, seed):
+    rng = np.random.default_rng(seed)
+    return (pos * np.float32(1.07) + rng.normal(0, 0.01, pos.shape).astype(np.float32)).astype(
+        np.float32)
+
+
+@pytest.mark.parametrize("n,maxp", [(5, 4), (1000, 1), (60_000, 4), (250_000, 4)])
+@pytest.mark.parametrize("sync", [True, False])
+def test_generic_refit_radix_matches_oracle(rt, O, n, maxp, sync):
+    """rt_gpu_refit_bvh4 on a radix tree: unchanged positions give the built
+    bytes back; deformed positions give the oracle refit's bytes."""
+    nu = 50
+    nv = max(2, n // (2 * nu) + 1)
+    pos, idx = O.sphere(nu, nv, 1.0, 0.02, 9)
+    idx = np.ascontiguousarray(idx[:n])
+    nodes_c, tris_c, _, _ = O.rlbvh_build(pos, idx, max_prims=maxp)
+    dpos = torch.from_numpy(pos).cuda()
+    didx = torch.from_numpy(idx.view(np.int32)).cuda()
+    bvh = rt.GpuBVH4.build(dpos, didx, algorithm="radix", max_leaf_prims=maxp, sync=sync)
+    bvh.refit(dpos, didx)
+    torch.cuda.synchronize()
+    assert bvh.node_count == len(nodes_c)
+    np.testing.assert_array_equal(to_np(bvh.nodes)[: len(nodes_c) * 128],
+                                  nodes_c.view(np.uint8).reshape(-1))
+    np.testing.assert_array_equal(to_np(bvh.tris)[: n * 64], tris_c.view(np.uint8).reshape(-1))
+    new = _deformed(pos, n)
+    bvh.refit(torch.from_numpy(new).cuda(), didx)
+    torch.cuda.synchronize()
+    want_n, want_t = O.bvh4_refit(new, idx, nodes_c, tris_c)
+    np.testing.assert_array_equal(to_np(bvh.nodes)[: len(nodes_c) * 128],
+                                  want_n.view(np.uint8).reshape(-1))
+    np.testing.assert_array_equal(to_np(bvh.tris)[: n * 64], want_t.view(np.uint8).reshape(-1))
+    if n <= 1000:  # the refitted tree traces the deformed mesh exactly
+        r = O.random_rays(20_000, 3, (-1.3, -1.3, -1.3), (1.3, 1.3, 1.3), tmax=1e30)
+        prim, t = bvh.intersect(rt.Rays.from_numpy(r.packed()))
+        pc, tc = O.trace_bvh4(want_n, want_t, r)
+        np.testing.assert_array_equal(u32(prim), pc)
+        np.testing.assert_array_equal(to_np(t).view(np.uint32), tc.view(np.uint32))
+
+
+def test_generic_refit_implicit_and_uploaded(rt, O):
+    """The generic refit equals the implicit-tree refit on an implicit tree,
+    and the oracle on an uploaded SoftwareBVH4 tree."""
+    xyz, _ = O.dynamic_grid(10_000, 0.2, 1.5)
+    d = torch.from_numpy(xyz).cuda()
+    a = rt.GpuBVH4.build(d, None, algorithm="implicit")
+    b = rt.GpuBVH4.build(d, None, algorithm="implicit")
+    moved = torch.from_numpy(_deformed(xyz.reshape(-1, 3), 5).reshape(xyz.shape)).cuda()
+    a.refit(moved, None)                 # implicit refit
+    b.refit(moved, None, generic=True)   # generic refit
+    torch.cuda.synchronize()
+    assert torch.equal(a.nodes, b.nodes) and torch.equal(a.tris, b.tris)
+    pos, idx = O.wave_grid(40, 40, 1.0, 0.0, 0.1, 4.0, 4.0)
+    n4, t64 = O.software_bvh4(pos, idx)
+    up = rt.GpuBVH4.from_bvh4(n4, tris64=t64, depth=O.bvh4_depth(n4))
+    new = _deformed(pos, 6)
+    didx = torch.from_numpy(np.ascontiguousarray(idx).view(np.int32)).cuda()
+    up.refit(torch.from_numpy(new).cuda(), didx)
+    torch.cuda.synchronize()
+    want_n, want_t = O.bvh4_refit(new, idx, n4, t64)
+    np.testing.assert_array_equal(to_np(up.nodes)[: len(n4) * 128],
+                                  want_n.view(np.uint8).reshape(-1))
+    np.testing.assert_array_equal(to_np(up.tris)[: len(t64) * 64],
+                                  want_t.view(np.uint8).reshape(-1))
