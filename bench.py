@@ -133,6 +133,24 @@ class Frame:
         return self.npix + self.npix * NLIGHTS
 
 
+def refit_time_ms(frame, reps=5):
+    """Generic refit (rt_gpu_refit_bvh4) of the 2.5M radix tree at the same
+    positions (which leaves every byte of the tree as built), per refit."""
+    b = frame.bvh
+    scratch = torch.empty(max(256, b.node_count * 4), dtype=torch.uint8, device="cuda")
+    args = (rt.stream_handle(), rt.ptr(frame.pos), rt.ptr(frame.idx), rt.ptr(b.nodes),
+            rt.u32(b.node_count), rt.ptr(b.tris), rt.u32(b.depth), rt.ptr(scratch),
+            rt.C.c_size_t(scratch.numel()))
+    rt.call("rt_gpu_refit_bvh4", *args)
+    s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    s.record()
+    for _ in range(reps):
+        rt.call("rt_gpu_refit_bvh4", *args)
+    e.record()
+    e.synchronize()
+    return s.elapsed_time(e) / reps
+
+
 def algorithmic_bytes_shadow(frame):
     """Bytes one shadow any-hit launch must move at minimum: the whole tree in
     the RADV-equivalent size (192 + 128 boxes + 64 N) plus 32 B per SoA ray
@@ -431,6 +449,7 @@ def main():
         cache_path=a.tune_cache, log=lambda m: print(m, file=sys.stderr, flush=True))
     b4k_p50, b4k_p99, b4k_b2b = build_time_4k()
     f = Frame(tuner)
+    refit_ms = refit_time_ms(f)
     if tuner is not None:
         f.tune(tuner)
     for _ in range(a.warmup):
@@ -498,6 +517,7 @@ def main():
         "build_us_4k_back_to_back": round(b4k_b2b, 2),
         "builds_per_s_4k": round(1e6 / b4k_b2b),
         "build_ms_2p5m": round(f.build_ms, 3),
+        "refit_ms_2p5m": round(refit_ms, 3),
         "frame_latency_ms_p50": round(percentile(step_ms, 0.5), 4),
         "frame_latency_ms_p99": round(percentile(step_ms, 0.99), 4),
     }
