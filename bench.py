@@ -297,12 +297,13 @@ def batched_leg(meshes=256, seed=4, tuner=None):
             "identical_results": same}
 
 
-def ao_frame_leg(frames=100, w=1920, h=1080):
+def ao_frame_leg(frames=100, w=1920, h=1080, tuner=None):
     """Dynamic AO frame: a static 999,698-triangle seeded random-displaced
     heightfield and a 4,096-triangle deforming grid (t_k = 0.05 k) as a
     two-level scene.  Per frame: regenerate the grid and rebuild its tree
     (fused path, same buffers), 1920x1080 primary closest hit, 1 SPP cosine
-    AO rays (tmax 0.5), any hit, AO resolve, 7x7 bilateral filter."""
+    AO rays (tmax 0.5), any hit, AO resolve, 7x7 bilateral filter.  With a
+    tuner, both scene traces use its measured launch shapes."""
     tp, ti = rt.gen_heightfield(707, 707, 4.0, 0.3, 31)
     terrain = rt.GpuBVH4.build(tp, ti, algorithm="radix")
     gx = rt.gen_dynamic_grid(4096, 0.0, 2.0)
@@ -322,16 +323,18 @@ def ao_frame_leg(frames=100, w=1920, h=1080):
     depth = torch.empty(npix, dtype=torch.float32, device="cuda")
     out = torch.empty(npix, dtype=torch.float32, device="cuda")
 
+    shapes = [{}, {}]
+
     def frame(k):
         rt.gen_dynamic_grid(4096, 0.05 * k, 2.0, out=gx)
         rt.call("rt_gpu_build", rt.stream_handle(), rt.ptr(gx), None, rt.u32(4096),
                 rt.C.byref(cfg), rt.ptr(grid.nodes), rt.u32(grid.node_count), rt.ptr(grid.tris),
                 rt.ptr(grid.info), None, None, None, rt.C.c_size_t(0))
         rt.gen_camera_rays(w, h, eye, look, up, 60.0, 0.0, 1e30, out=primary)
-        scene.intersect(primary, out=(prim, t, inst))
+        scene.intersect(primary, out=(prim, t, inst), **shapes[0])
         scene.gen_ao_rays(primary, prim, inst, t, w, h, frame=k, offset=0.03, tmin=0.01,
                           tmax=0.5, out=ao_rays)
-        scene.occluded(ao_rays, out=bits)
+        scene.occluded(ao_rays, out=bits, **shapes[1])
         rt.ao_resolve(bits, prim, out=ao)
         torch.where(prim == -1, torch.full_like(t, 1e30), t, out=depth)
         rt.blur7(ao, depth, w, h, 0.05, out=out)
@@ -339,6 +342,10 @@ def ao_frame_leg(frames=100, w=1920, h=1080):
     for k in range(3):
         frame(k)
     torch.cuda.synchronize()
+    if tuner is not None:
+        shapes[0] = tuner.trace_scene(scene, primary, "closest", tag="ao-primary")
+        shapes[1] = tuner.trace_scene(scene, ao_rays, "any", tag="ao")
+        torch.cuda.synchronize()
     s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     s.record()
     for k in range(frames):
@@ -350,6 +357,7 @@ def ao_frame_leg(frames=100, w=1920, h=1080):
                         "frame (two-level scene), 1920x1080 primary + 1 SPP AO (tmax 0.5) + 7x7 "
                         "bilateral", "frames": frames, "ms_per_frame": round(ms, 3),
             "fps": round(1000.0 / ms, 1), "mrays_per_s": round(2 * npix / ms / 1e3, 1),
+            "trace_shapes": shapes,
             "ao_mean": round(float(out.mean()), 4)}
 
 
@@ -522,7 +530,7 @@ def main():
         "frame_latency_ms_p99": round(percentile(step_ms, 0.99), 4),
     }
     if not a.no_extra:
-        out["ao_frame"] = ao_frame_leg()
+        out["ao_frame"] = ao_frame_leg(tuner=tuner)
         out["batched"] = batched_leg(tuner=tuner)
         out["serving"] = serving_leg(tuner=tuner)
         out["p99_request_latency_ms"] = out["serving"]["p99_ms"]

@@ -333,6 +333,12 @@ rt_status rt_trace_scene(rt_stream stream, const rt_instance* d_instances, uint3
                          uint32_t max_depth, const rt_rays_soa* rays, uint32_t n,
                          uint32_t flags, rt_hits_soa* hits, uint32_t* d_inst,
                          uint32_t* occl_bits);
+/* rt_trace_scene with the launch shape of opts (block, lds_stack; schedule
+ * must be RT_SCHED_SIMPLE); same results for every shape.                   */
+rt_status rt_trace_scene_ex(rt_stream stream, const rt_instance* d_instances, uint32_t count,
+                            uint32_t max_depth, const rt_rays_soa* rays, uint32_t n,
+                            uint32_t flags, const rt_trace_opts* opts, rt_hits_soa* hits,
+                            uint32_t* d_inst, uint32_t* occl_bits);
 
 /* ---- frame post-processing (shaders/rq_ao.comp, shaders/bilateral_blur.comp) */
 /* AO rays of rt_gen_ao_rays for a two-level scene: the hit normal comes from

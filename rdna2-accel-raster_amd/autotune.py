@@ -13,6 +13,7 @@ configuration's bit for bit (a mismatch raises: it would be a kernel bug).
                                (the wave's ray tile) and refill_lanes
   batched trace (rt_trace_batch_ex)   the same knobs (the refill chunk is a
                                wave's share of one item)
+  two-level scene (rt_trace_scene_ex)  block, lds_stack
   device build (rt_gpu_build)  sort_bits: digit width of the Morton radix
                                sort, 4 / 6 / 8 / 10 bits = 8 / 5 / 4 / 3 passes
   batched build (rt_gpu_build_batch)  workgroups of the persistent launch
@@ -158,6 +159,27 @@ class Tuner:
                    "lds_stack": best["lds_stack"], "rays_per_wave": w, "refill_lanes": r}
                   for w in REFILL_RAYS_PER_WAVE for r in REFILL_LANES]
         return self._pick(key, [best] + refill, run, check)
+
+    def trace_scene(self, scene, rays, mode, tag=""):
+        """Launch shape (block, lds_stack) for Scene.intersect / occluded."""
+        n = len(rays)
+        m = min(n, self.sample_rays)
+        sample = rays if m == n else rt.Rays(rays.data[:, :m])
+        key = f"scene|{mode}|{tag}|i{scene.count}|d{scene.depth}|r{_size_class(m)}"
+        fn = scene.intersect if mode == "closest" else scene.occluded
+        out = None
+
+        def run(c):
+            nonlocal out
+            out = fn(sample, out=out, **c)
+            return out
+
+        def check(c):
+            return _clone(run(c))
+
+        cands = [{"block": b, "lds_stack": l} for b in TRACE_BLOCKS for l in TRACE_LDS]
+        cands.sort(key=lambda c: (c["block"], c["lds_stack"]) != (256, 20))
+        return self._pick(key, cands, run, check)
 
     def trace_batch(self, batch, tag=""):
         """Schedule and launch shape for TraceBatch.launch: the best simple

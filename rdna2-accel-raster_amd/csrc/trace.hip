@@ -775,10 +775,14 @@ extern "C" rt_status rt_trace_batch(rt_stream stream, const rt_trace_item* d_ite
     return rt_trace_batch_ex(stream, d_items, d_chunk_start, count, total_chunks, depth, nullptr);
 }
 
-extern "C" rt_status rt_trace_scene(rt_stream stream, const rt_instance* d_instances,
-                                    uint32_t count, uint32_t max_depth, const rt_rays_soa* rays,
-                                    uint32_t n, uint32_t flags, rt_hits_soa* hits,
-                                    uint32_t* d_inst, uint32_t* occl_bits) {
+extern "C" rt_status rt_trace_scene_ex(rt_stream stream, const rt_instance* d_instances,
+                                       uint32_t count, uint32_t max_depth,
+                                       const rt_rays_soa* rays, uint32_t n, uint32_t flags,
+                                       const rt_trace_opts* opts, rt_hits_soa* hits,
+                                       uint32_t* d_inst, uint32_t* occl_bits) {
+    TraceCfg cfg{false, 0u, 0u, (uint32_t)kBlock, kLdsStack};
+    if (opts && opts->schedule != RT_SCHED_SIMPLE) return RT_ERR_ARG;
+    if (!shape_opts(opts, cfg)) return RT_ERR_ARG;
     if (!rays || (flags & ~RT_ANY)) return RT_ERR_ARG;
     if (n == 0) return RT_OK;
     if ((count && !d_instances) || !rays->ox || !rays->oy || !rays->oz || !rays->tmin ||
@@ -791,7 +795,6 @@ extern "C" rt_status rt_trace_scene(rt_stream stream, const rt_instance* d_insta
     if (depth > kMaxDepth) return RT_ERR_CAPACITY;
     uint32_t* prim = hits ? hits->prim : nullptr;
     float* t = hits ? hits->t : nullptr;
-    const TraceCfg cfg{false, 0u, 0u, (uint32_t)kBlock, kLdsStack};
     const dim3 grid(ceil_div(n, cfg.block)), block(cfg.block);
     const size_t shm = lds_bytes(cfg);
     hipStream_t s = as_stream(stream);
@@ -807,4 +810,12 @@ extern "C" rt_status rt_trace_scene(rt_stream stream, const rt_instance* d_insta
     }
 #undef RT_S
     return launch_status("k_trace_scene");
+}
+
+extern "C" rt_status rt_trace_scene(rt_stream stream, const rt_instance* d_instances,
+                                    uint32_t count, uint32_t max_depth, const rt_rays_soa* rays,
+                                    uint32_t n, uint32_t flags, rt_hits_soa* hits,
+                                    uint32_t* d_inst, uint32_t* occl_bits) {
+    return rt_trace_scene_ex(stream, d_instances, count, max_depth, rays, n, flags, nullptr,
+                             hits, d_inst, occl_bits);
 }

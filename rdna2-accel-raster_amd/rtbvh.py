@@ -148,6 +148,7 @@ _SIGS = {
     "rt_trace_batch_ex": [vp, vp, vp, u32, u32, u32, vp],
     "rt_trace_batch_plan_ex": [vp, u32, u32, vp, vp],
     "rt_trace_scene": [vp, vp, u32, u32, vp, u32, u32, vp, vp, vp],
+    "rt_trace_scene_ex": [vp, vp, u32, u32, vp, u32, u32, vp, vp, vp, vp],
     "rt_gen_ao_rays_scene": [vp, vp, u32, vp, vp, vp, vp, u32, u32, u32, f32, f32, f32, vp],
     "rt_ao_resolve": [vp, vp, vp, u32, vp],
     "rt_blur7": [vp, vp, vp, u32, u32, f32, vp],
@@ -620,8 +621,10 @@ class Scene:
         self.depth = depth
         self.d_table = torch.frombuffer(bytearray(bytes(arr)), dtype=torch.uint8).to(dev)
 
-    def intersect(self, rays, stream=None, out=None):
-        """Closest hit: (prim, t, instance index) tensors (int32, f32, int32)."""
+    def intersect(self, rays, stream=None, out=None, block=None, lds_stack=None,
+                  schedule="simple"):
+        """Closest hit: (prim, t, instance index) tensors (int32, f32, int32).
+        block / lds_stack: launch shape (same results for every choice)."""
         n = len(rays)
         dev = rays.data.device
         prim, t, inst = out if out else (torch.empty(n, dtype=torch.int32, device=dev),
@@ -629,18 +632,22 @@ class Scene:
                                          torch.empty(n, dtype=torch.int32, device=dev))
         hits = rt_hits_soa(ptr(prim), ptr(t))
         r = rays.soa()
-        call("rt_trace_scene", stream_handle(stream), ptr(self.d_table), u32(self.count),
-             u32(self.depth), C.byref(r), u32(n), u32(RT_CLOSEST), C.byref(hits), ptr(inst),
-             None)
+        o = trace_opts(schedule, block=block, lds_stack=lds_stack)
+        call("rt_trace_scene_ex", stream_handle(stream), ptr(self.d_table), u32(self.count),
+             u32(self.depth), C.byref(r), u32(n), u32(RT_CLOSEST), C.byref(o), C.byref(hits),
+             ptr(inst), None)
         return prim, t, inst
 
-    def occluded(self, rays, stream=None, out=None):
+    def occluded(self, rays, stream=None, out=None, block=None, lds_stack=None,
+                 schedule="simple"):
         n = len(rays)
         bits = out if out is not None else torch.empty(max(1, (n + 31) // 32),
                                                        dtype=torch.int32, device=rays.data.device)
         r = rays.soa()
-        call("rt_trace_scene", stream_handle(stream), ptr(self.d_table), u32(self.count),
-             u32(self.depth), C.byref(r), u32(n), u32(RT_ANY), None, None, ptr(bits))
+        o = trace_opts(schedule, block=block, lds_stack=lds_stack)
+        call("rt_trace_scene_ex", stream_handle(stream), ptr(self.d_table), u32(self.count),
+             u32(self.depth), C.byref(r), u32(n), u32(RT_ANY), C.byref(o), None, None,
+             ptr(bits))
         return bits
 
     def gen_ao_rays(self, primary, prim, inst, t, w, h, frame=0, offset=0.003, tmin=0.001,

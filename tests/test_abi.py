@@ -170,6 +170,10 @@ def test_scene_and_post_entry_points_on_host():
                               None) == 1  # counters: not a scene flag
     assert lib.rt_trace_scene(None, fake, 1, 0, C.byref(r), 8, 1, None, None, None) == 1
     assert lib.rt_trace_scene(None, fake, 1, 65, C.byref(r), 8, 0, C.byref(h), None, None) == 2
+    for o, want in ((rtbvh.rt_trace_opts(0, 0, 0, 64, 8), 0), (rtbvh.rt_trace_opts(1, 0, 0, 0, 0), 1),
+                    (rtbvh.rt_trace_opts(0, 0, 0, 96, 0), 1), (rtbvh.rt_trace_opts(0, 0, 0, 0, 49), 1)):
+        assert lib.rt_trace_scene_ex(None, fake, 1, 0, C.byref(r), 0, 0, C.byref(o), C.byref(h),
+                                     None, None) == want
     assert lib.rt_blur7(None, fake, fake, 4, 4, C.c_float(0.05), fake) == 1  # in place
     assert lib.rt_blur7(None, None, fake, 4, 4, C.c_float(0.05), fake) == 1
     assert lib.rt_ao_resolve(None, None, None, 0, None) == 0

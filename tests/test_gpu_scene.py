@@ -54,10 +54,13 @@ def scene(rt, O):
     return dev, oracle, meshes
 
 
-def test_scene_closest_and_any(rt, O, scene):
+@pytest.mark.parametrize("shape", [dict(), dict(block=64, lds_stack=4),
+                                   dict(block=128, lds_stack=32), dict(block=256, lds_stack=48)],
+                         ids=["default", "b64-l4", "b128-l32", "b256-l48"])
+def test_scene_closest_and_any(rt, O, scene, shape):
     dev, orc, _ = scene
     r = O.random_rays(150_001, 12, (-3.2, -3.2, -0.5), (3.2, 3.2, 2.0), tmax=1e30)
-    prim, t, inst = dev.intersect(rt.Rays.from_numpy(r.packed()))
+    prim, t, inst = dev.intersect(rt.Rays.from_numpy(r.packed()), **shape)
     torch.cuda.synchronize()
     pc, tc, ic = O.trace_scene(orc, r)
     np.testing.assert_array_equal(u32(prim), pc)
@@ -65,7 +68,7 @@ def test_scene_closest_and_any(rt, O, scene):
     np.testing.assert_array_equal(u32(inst), ic)
     assert len(set(ic[pc != O.MISS].tolist())) == 3  # every instance is hit somewhere
     r.tmax[:] = 0.6
-    bits = dev.occluded(rt.Rays.from_numpy(r.packed()))
+    bits = dev.occluded(rt.Rays.from_numpy(r.packed()), **shape)
     torch.cuda.synchronize()
     np.testing.assert_array_equal(_bits(u32(bits), len(r)), O.trace_scene(orc, r, mode=O.ANY))
 
