@@ -35,12 +35,15 @@ struct Child {
 };
 
 __device__ __forceinline__ void cx(Child& a, Child& b) {
-    // software_bvh_bvh4.comp:151: swap when st[i] < st[j]
-    if (a.key < b.key) {
-        Child t = a;
-        a = b;
-        b = t;
-    }
+    // software_bvh_bvh4.comp:151: swap when st[i] < st[j].  Written as
+    // selects (one compare, two v_cndmask per field) rather than a branch:
+    // a divergent branch costs exec-mask bookkeeping and register moves.
+    const bool s = a.key < b.key;
+    const float ka = s ? b.key : a.key, kb = s ? a.key : b.key;
+    const int32_t ca = s ? b.child : a.child, cb = s ? a.child : b.child;
+    const int32_t na = s ? b.count : a.count, nb = s ? a.count : b.count;
+    a = Child{ka, ca, na};
+    b = Child{kb, cb, nb};
 }
 
 // Global-address-space views.  Kernel pointer arguments are global to the
