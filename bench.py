@@ -178,7 +178,43 @@ def cpu_baseline(frame, threads):
     return {"value": round(n / dt / 1e6, 3), "unit": "Mrays/s", "cores": threads, "kind": "port",
             "sample": f"one whole frame of the same workload and tree: {len(prim_rays)} primary "
                       f"closest-hit + {len(sh_rays)} shadow any-hit rays; {dt:.1f} s wall on "
-                      f"{threads} threads"}
+                      f"{threads} threads",
+            "build": cpu_build_baseline(frame, O)}
+
+
+def cpu_build_baseline(frame, O):
+    """SoftwareBVH::build + SoftwareBVH4::fromBVH2 on one host core: the
+    reference's own header code compiled with its CMake flags
+    (oracle/_ref/libref_shipped.so, kind "reference") where that library is
+    present, else the oracle's restatement (kind "port").  The 4,096-triangle
+    deforming grid (median of 5) and the 2.5 M-triangle sphere of the frame
+    (one build)."""
+    xyz = O.dynamic_grid(4096, 0.0, 2.0)[0].reshape(-1, 3)
+    gidx = np.arange(xyz.shape[0], dtype=np.uint32).reshape(-1, 3)
+    spos = frame.pos.cpu().numpy()
+    sidx = frame.idx.cpu().numpy().view(np.uint32)
+    kind = "reference" if O.ref(shipped=True) is not None else "port"
+
+    def build(p, i):
+        if kind == "reference":
+            O.ref_build(p, i, 4, shipped=True)
+        else:
+            O.software_bvh4(p, i)
+
+    times = []
+    for _ in range(6):
+        t0 = time.perf_counter()
+        build(xyz, gidx)
+        times.append(time.perf_counter() - t0)
+    t4k = sorted(times[1:])[2]
+    t0 = time.perf_counter()
+    build(spos, sidx)
+    t25 = time.perf_counter() - t0
+    return {"kind": kind, "cores": 1, "build_us_4k": round(t4k * 1e6, 1),
+            "build_ms_2p5m": round(t25 * 1e3, 1),
+            "sample": "SoftwareBVH::build + fromBVH2, max 4 triangles per leaf: the 4,096-"
+                      "triangle grid (median of 5 after 1 warm-up) and the 2,500,000-triangle "
+                      "sphere (one build), host arrays in, Box4 nodes out"}
 
 
 def batched_leg(meshes=256, seed=4, tuner=None):
