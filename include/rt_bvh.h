@@ -366,6 +366,17 @@ rt_status rt_gen_wave_grid(rt_stream stream, uint32_t nx, uint32_t ny, float ext
                            float amp, float fx, float fy, float* d_pos, uint32_t* d_idx);
 rt_status rt_gen_heightfield(rt_stream stream, uint32_t nx, uint32_t ny, float extent, float amp,
                              uint32_t seed, float* d_pos, uint32_t* d_idx);
+/* common/mesh_gen.hpp:29-83 MeshGenerator::createTessellatedTerrain(target,
+ * size): res = floor(sqrt(target / 2)) + 1 (at least 2) vertices per side
+ * (rt_terrain_resolution), 2 (res - 1)^2 triangles.  d_pos: 3 res^2 floats
+ * (x, height, z); d_idx: 6 (res - 1)^2 indices.  fused != 0 evaluates the
+ * three multiply-adds as the reference's -O3 -march=native build contracts
+ * them (the build behind the reference's published results); heights use the device sinf /
+ * cosf, within a few ulp of the host library's.                             */
+rt_status rt_terrain_resolution(uint32_t target, uint32_t* res, uint32_t* nverts,
+                                uint32_t* ntris);
+rt_status rt_gen_terrain(rt_stream stream, uint32_t target, float size, int fused, float* d_pos,
+                         uint32_t* d_idx);
 /* include/radv_bvh.hpp:224-283 generate_dynamic_grid_mesh: 2 triangles per
  * quad, so d_xyz9 receives 2 * floor(n / 2) triangles (9 floats each); an
  * odd n leaves the last of n slots untouched.                              */

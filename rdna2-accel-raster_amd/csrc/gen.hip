@@ -25,6 +25,47 @@ __device__ __forceinline__ void put_ray(const rt_rays_out& o, size_t r, float ox
     o.dx[r] = dx; o.dy[r] = dy; o.dz[r] = dz; o.tmax[r] = tx;
 }
 
+// common/mesh_gen.hpp:29-83 createTessellatedTerrain: vertex (x, y) of a res
+// x res grid at (-size/2 + x step, h, -size/2 + y step) with
+// h = 5 sin(.05 px) cos(.05 pz) + 2 sin(.125 px + .085 pz).  FUSED evaluates
+// the three multiply-adds as the reference's -O3 -march=native build
+// contracts them (fmaf); otherwise one rounding per operator.  sinf / cosf
+// are the device's (within a few ulp of the host's).
+template <bool FUSED>
+__global__ void k_terrain_verts(uint32_t res, float step, float half, float* __restrict__ pos) {
+    const uint32_t v = blockIdx.x * kB + threadIdx.x;
+    if (v >= res * res) return;
+    const uint32_t x = v % res, y = v / res;
+    float px, pz, py;
+    if (FUSED) {
+        px = fmaf((float)x, step, -half);
+        pz = fmaf((float)y, step, -half);
+        const float fx = px * 0.05f, fz = pz * 0.05f;
+        const float arg = fmaf(fx, 2.5f, fz * 1.7f);
+        py = fmaf(sinf(fx) * cosf(fz), 5.0f, sinf(arg) * 2.0f);
+    } else {
+        px = -half + (float)x * step;
+        pz = -half + (float)y * step;
+        const float fx = px * 0.05f, fz = pz * 0.05f;
+        py = sinf(fx) * cosf(fz) * 5.0f + sinf(fx * 2.5f + fz * 1.7f) * 2.0f;
+    }
+    pos[3 * (size_t)v] = px;
+    pos[3 * (size_t)v + 1] = py;
+    pos[3 * (size_t)v + 2] = pz;
+}
+
+// Two triangles per quad, wound (i0, i2, i1), (i1, i2, i3) (mesh_gen.hpp:70-80).
+__global__ void k_terrain_idx(uint32_t res, uint32_t* __restrict__ idx) {
+    const uint32_t q = blockIdx.x * kB + threadIdx.x;
+    const uint32_t w = res - 1;
+    if (q >= w * w) return;
+    const uint32_t x = q % w, y = q / w;
+    const uint32_t i0 = y * res + x, i1 = i0 + 1, i2 = i0 + res, i3 = i2 + 1;
+    uint32_t* o = idx + 6 * (size_t)q;
+    o[0] = i0; o[1] = i2; o[2] = i1;
+    o[3] = i1; o[4] = i2; o[5] = i3;
+}
+
 // UV sphere, vertex (i, j): theta = pi j / nv, phi = 2 pi i / nu, radius
 // r (1 + disp (2u - 1)); pole rows share the radius of their first vertex.
 __global__ void k_sphere_verts(uint32_t nu, uint32_t nv, float radius, float disp, uint32_t seed,
@@ -424,6 +465,40 @@ extern "C" rt_status rt_gen_dynamic_grid(rt_stream stream, uint32_t n, float t, 
     hipLaunchKernelGGL(k_dynamic_grid, dim3(ceil_div(quads, kB)), dim3(kB), 0, as_stream(stream),
                        cols, rows, t, extent, d_xyz9);
     return launch_status("rt_gen_dynamic_grid");
+}
+
+// common/mesh_gen.hpp:31-34: res = floor(sqrt(target / 2)) + 1, at least 2.
+static uint32_t terrain_res(uint32_t target) {
+    const uint32_t res = (uint32_t)std::sqrt((double)(target / 2)) + 1u;
+    return res < 2u ? 2u : res;
+}
+
+extern "C" rt_status rt_terrain_resolution(uint32_t target, uint32_t* res, uint32_t* nverts,
+                                           uint32_t* ntris) {
+    const uint32_t r = terrain_res(target);
+    if ((uint64_t)r * r > 0xFFFFFFFFull) return RT_ERR_CAPACITY;
+    if (res) *res = r;
+    if (nverts) *nverts = r * r;
+    if (ntris) *ntris = (r - 1) * (r - 1) * 2;
+    return RT_OK;
+}
+
+extern "C" rt_status rt_gen_terrain(rt_stream stream, uint32_t target, float size, int fused,
+                                    float* d_pos, uint32_t* d_idx) {
+    if (!d_pos || !d_idx) return RT_ERR_ARG;
+    const uint32_t res = terrain_res(target);
+    if ((uint64_t)res * res > 0xFFFFFFFFull / 3) return RT_ERR_CAPACITY;
+    const float step = size / (float)(res - 1), half = size * 0.5f;
+    hipStream_t s = as_stream(stream);
+    const uint32_t nv = res * res, nq = (res - 1) * (res - 1);
+    if (fused)
+        hipLaunchKernelGGL(k_terrain_verts<true>, dim3(ceil_div(nv, kB)), dim3(kB), 0, s, res,
+                           step, half, d_pos);
+    else
+        hipLaunchKernelGGL(k_terrain_verts<false>, dim3(ceil_div(nv, kB)), dim3(kB), 0, s, res,
+                           step, half, d_pos);
+    hipLaunchKernelGGL(k_terrain_idx, dim3(ceil_div(nq, kB)), dim3(kB), 0, s, res, d_idx);
+    return launch_status("rt_gen_terrain");
 }
 
 extern "C" rt_status rt_gen_camera_rays(rt_stream stream, uint32_t w, uint32_t h,

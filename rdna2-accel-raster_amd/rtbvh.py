@@ -131,6 +131,8 @@ _SIGS = {
     "rt_trace": [vp, vp, vp, u32, u32, vp, vp, vp, vp],
     "rt_trace_ex": [vp, vp, vp, u32, u32, vp, vp, vp, vp, vp],
     "rt_gen_sphere": [vp, u32, u32, f32, f32, u32, vp, vp],
+    "rt_terrain_resolution": [u32, vp, vp, vp],
+    "rt_gen_terrain": [vp, u32, f32, C.c_int, vp, vp],
     "rt_gen_wave_grid": [vp, u32, u32, f32, f32, f32, f32, f32, vp, vp],
     "rt_gen_heightfield": [vp, u32, u32, f32, f32, u32, vp, vp],
     "rt_gen_dynamic_grid": [vp, u32, f32, f32, vp],
@@ -693,6 +695,20 @@ def _depth_host(nodes4):
 
 
 # --------------------------------------------------------------- generators --
+def gen_terrain(target, size=100.0, fused=False, stream=None):
+    """createTessellatedTerrain (rt_gen_terrain): (pos (V, 3) float32, idx (T,
+    3) int32) on the device; fused=True evaluates it as the reference's
+    -O3 -march=native build does."""
+    res, nv, nt = u32(0), u32(0), u32(0)
+    call("rt_terrain_resolution", u32(target), C.byref(res), C.byref(nv), C.byref(nt))
+    dev = _dev()
+    pos = torch.empty((nv.value, 3), dtype=torch.float32, device=dev)
+    idx = torch.empty((nt.value, 3), dtype=torch.int32, device=dev)
+    call("rt_gen_terrain", stream_handle(stream), u32(target), f32(size), C.c_int(1 if fused else 0),
+         ptr(pos), ptr(idx))
+    return pos, idx
+
+
 def gen_sphere(nu, nv, radius=1.0, disp=0.02, seed=7, stream=None):
     dev = _dev()
     pos = torch.empty(((nv + 1) * nu, 3), dtype=torch.float32, device=dev)

@@ -246,3 +246,14 @@ def test_generic_refit_argument_checks():
     assert lib.rt_gpu_refit_bvh4(None, fake, None, fake, 10, fake, 65, fake, 40) == 1
     assert lib.rt_gpu_refit_bvh4(None, fake, None, fake, 10, fake, 5, fake, 39) == 2
     assert lib.rt_gpu_refit_bvh4(None, fake, None, fake, 10, fake, 5, None, 40) == 2
+
+
+def test_terrain_resolution_matches_oracle(O):
+    import rtbvh
+    lib = rtbvh.lib()
+    for target in (0, 2, 7, 968, 1000, 100_000, 1_000_000, 10_000_000):
+        res, nv, nt = C.c_uint32(0), C.c_uint32(0), C.c_uint32(0)
+        assert lib.rt_terrain_resolution(target, C.byref(res), C.byref(nv), C.byref(nt)) == 0
+        assert res.value == O.lib().or_terrain_res(target)
+        assert nv.value == res.value ** 2 and nt.value == 2 * (res.value - 1) ** 2
+    assert lib.rt_gen_terrain(None, 1000, C.c_float(100.0), 0, None, None) == 1

@@ -263,3 +263,23 @@ def test_generic_refit_implicit_and_uploaded(rt, O):
                                   want_n.view(np.uint8).reshape(-1))
     np.testing.assert_array_equal(to_np(up.tris)[: len(t64) * 64],
                                   want_t.view(np.uint8).reshape(-1))
+
+
+@pytest.mark.parametrize("target", [2, 1000, 100_000])
+@pytest.mark.parametrize("fused", [False, True])
+def test_terrain_generator(rt, O, target, fused):
+    """rt_gen_terrain = createTessellatedTerrain: resolution and indices exact,
+    x / z exact (no transcendental), heights within 4e-6 of the oracle: the
+    device sinf / cosf against the host library's, each within a few ulp,
+    scaled by the summands 5 sin cos and 2 sin (|h| <= 7, ulp(4..8) = 4.8e-7),
+    so the bound is absolute, not relative to a height near zero."""
+    pos, idx = rt.gen_terrain(target, 100.0, fused=fused)
+    torch.cuda.synchronize()
+    pc, ic = O.terrain(target, fused=fused)
+    p = to_np(pos)
+    assert p.shape == pc.shape
+    np.testing.assert_array_equal(u32(idx), ic)
+    np.testing.assert_array_equal(p[:, 0], pc[:, 0])
+    np.testing.assert_array_equal(p[:, 2], pc[:, 2])
+    err = np.abs(p[:, 1].astype(np.float64) - pc[:, 1])
+    assert err.max() <= 4e-6, err.max()
