@@ -78,6 +78,14 @@ __device__ __forceinline__ RT_GLOBAL T* glb(T* p) {
     return (RT_GLOBAL T*)p;
 }
 
+// Keeps a node's child-index vector live before the "any child hit" branch:
+// the compiler otherwise sinks that load into the branch, so a visit with a
+// hit child pays a second memory round trip after the slab tests.  With the
+// use here the load is issued with the other seven of the node.
+__device__ __forceinline__ void keep_loaded(const i4v& v) {
+    asm volatile("" : : "v"(v.x), "v"(v.y), "v"(v.z), "v"(v.w));
+}
+
 // The stack pointer is a separate local (a register); only the overflow
 // array lives in private memory (keeping both in one aggregate would put the
 // pointer in scratch too).
@@ -109,25 +117,46 @@ struct Stack {
             dst = reinterpret_cast<volatile int32_t*>(st.priv)[sp - ls]; \
     } while (0)
 
-// Leaf range [first, first + cnt): Moller-Trumbore on each 64-B record.
-template <bool ANY, typename TP>
-__device__ __forceinline__ bool test_leaf(TP __restrict__ tris, uint32_t first,
-                                          uint32_t cnt, float ox, float oy, float oz, float dx,
-                                          float dy, float dz, float tmin, float& hitT,
-                                          uint32_t& hitPrim) {
-    for (uint32_t q = 0; q < cnt; ++q) {
-        const auto tp = f4(tris + first + q);
+// The leaf children of one node, in their sorted order, as ONE loop of
+// single-triangle steps: pend holds the sorted positions p (bit p) whose
+// child is a leaf range.  Per ray this is the reference's sequence (each
+// leaf child's range in sorted order, triangles in record order, the count
+// added when a leaf child is started, any hit stopping at the first accepted
+// triangle); per wave the loop runs max over lanes of the lane's triangle
+// count instead of, position by position, the max over lanes of that
+// position's count (divergent lanes share the one loop body).
+template <bool ANY, bool COUNT, typename TP>
+__device__ __forceinline__ bool test_leaves(TP __restrict__ tris, const Child (&e)[4],
+                                            uint32_t pend, float ox, float oy, float oz,
+                                            float dx, float dy, float dz, float tmin,
+                                            float& hitT, uint32_t& hitPrim,
+                                            uint32_t& triTests) {
+    uint32_t first = 0, left = 0;
+    for (;;) {
+        if (left == 0) {
+            if (pend == 0) return false;
+            const uint32_t k = __builtin_ctz(pend);
+            pend &= pend - 1u;
+            const int32_t c = k == 0 ? e[0].child : k == 1 ? e[1].child
+                            : k == 2 ? e[2].child : e[3].child;
+            const int32_t n = k == 0 ? e[0].count : k == 1 ? e[1].count
+                            : k == 2 ? e[2].count : e[3].count;
+            first = (uint32_t)~c;
+            left = (uint32_t)n;
+            if (COUNT) triTests += left;
+        }
+        const auto tp = f4(tris + first);
         const f4v a = tp[0], b = tp[1], c = tp[2];
         float t;
-        if (!mt_hit(a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w, c.x, ox, oy, oz, dx, dy, dz, &t))
-            continue;
-        if (t >= tmin && t < hitT) {
+        if (mt_hit(a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w, c.x, ox, oy, oz, dx, dy, dz, &t) &&
+            t >= tmin && t < hitT) {
             hitT = t;
-            hitPrim = tris[first + q].triangle_id;
+            hitPrim = tris[first].triangle_id;
             if (ANY) return true;
         }
+        ++first;
+        --left;
     }
-    return false;
 }
 
 // One ray's traversal (the body shared by k_trace, k_trace_batch and
@@ -144,8 +173,6 @@ __device__ __forceinline__ void trace_ray(NP __restrict__ nodes, TP __restrict__
     const float ix = fabsf(dx) > 1e-6f ? 1.0f / dx : 1e6f;
     const float iy = fabsf(dy) > 1e-6f ? 1.0f / dy : 1e6f;
     const float iz = fabsf(dz) > 1e-6f ? 1.0f / dz : 1e6f;
-    bool done = false;
-
     Stack<PRIV> st;
     int sp = 0;
     if (start) RT_SPUSH(0);
@@ -192,24 +219,25 @@ __device__ __forceinline__ void trace_ray(NP __restrict__ nodes, TP __restrict__
         for (int k = 0; k < 4; ++k)
             e[k] = hit[k] ? Child{fmaxf(te[k], -FLT_MAX), chv[k], cnv[k]}
                           : Child{-INFINITY, 0, 0};
+        keep_loaded(ch);
         if (!(hit[0] || hit[1] || hit[2] || hit[3])) continue;
         cx(e[0], e[1]); cx(e[0], e[2]); cx(e[0], e[3]);
         cx(e[1], e[2]); cx(e[1], e[3]);
         cx(e[2], e[3]);
+        // :158-189 in sorted order: inner children pushed, leaf children
+        // tested (padding, count 0, skipped).  The pushes do not depend on
+        // the leaf tests, so all are made first and the leaves then run as
+        // one loop (for any hit a push after the accepted leaf is never
+        // popped: the ray ends).
+        uint32_t pend = 0;
 #pragma unroll
-        for (int p = 0; p < 4; ++p) {  // :158-189; padding (count 0) is skipped
-            if (e[p].count > 0) {
-                if (COUNT) triTests += (uint32_t)e[p].count;
-                if (test_leaf<ANY>(tris, (uint32_t)(~e[p].child), (uint32_t)e[p].count, ox, oy,
-                                   oz, dx, dy, dz, tmin, hitT, hitPrim)) {
-                    done = true;  // any hit: first accepted triangle ends the ray
-                    break;
-                }
-            } else if (e[p].count < 0) {
-                RT_SPUSH(e[p].child);
-            }
+        for (int p = 0; p < 4; ++p) {
+            if (e[p].count < 0) RT_SPUSH(e[p].child);
+            pend |= (e[p].count > 0 ? 1u : 0u) << p;
         }
-        if (ANY && done) break;
+        if (pend && test_leaves<ANY, COUNT>(tris, e, pend, ox, oy, oz, dx, dy, dz, tmin, hitT,
+                                            hitPrim, triTests))
+            break;  // any hit: the first accepted triangle ends the ray
     }
 }
 
@@ -490,23 +518,20 @@ __device__ __forceinline__ void refill_wave(
             for (int k = 0; k < 4; ++k)
                 e[k] = hit[k] ? Child{fmaxf(te[k], -FLT_MAX), chv[k], cnv[k]}
                               : Child{-INFINITY, 0, 0};
+            keep_loaded(ch);
             if (hit[0] || hit[1] || hit[2] || hit[3]) {
                 cx(e[0], e[1]); cx(e[0], e[2]); cx(e[0], e[3]);
                 cx(e[1], e[2]); cx(e[1], e[3]);
                 cx(e[2], e[3]);
+                uint32_t pend = 0;  // as trace_ray: pushes, then the leaves as one loop
 #pragma unroll
                 for (int p = 0; p < 4; ++p) {
-                    if (e[p].count > 0) {
-                        if (COUNT) triTests += (uint32_t)e[p].count;
-                        if (test_leaf<ANY>(tris, (uint32_t)(~e[p].child), (uint32_t)e[p].count,
-                                           ox, oy, oz, dx, dy, dz, tmin, hitT, hitPrim)) {
-                            finished = true;
-                            break;
-                        }
-                    } else if (e[p].count < 0) {
-                        RT_SPUSH(e[p].child);
-                    }
+                    if (e[p].count < 0) RT_SPUSH(e[p].child);
+                    pend |= (e[p].count > 0 ? 1u : 0u) << p;
                 }
+                if (pend && test_leaves<ANY, COUNT>(tris, e, pend, ox, oy, oz, dx, dy, dz, tmin,
+                                                    hitT, hitPrim, triTests))
+                    finished = true;
             }
             if (sp == 0) finished = true;
         }
