@@ -293,6 +293,23 @@ def batched_leg(meshes=256, seed=4, tuner=None):
     trace_ms_default = graph_ms({})
     shape = tuner.trace_batch(batch, tag="grids") if tuner is not None else {}
     trace_ms = graph_ms(shape)
+    # the whole batch -- every build and every trace -- as one captured graph
+    graph_ms_total = None
+    if not big:
+        gs = torch.cuda.Stream()
+        g = torch.cuda.CUDAGraph()
+        torch.cuda.synchronize()
+        with torch.cuda.graph(g, stream=gs):
+            bb.launch(stream=gs)
+            batch.launch(stream=gs, **shape)
+        g.replay()
+        torch.cuda.synchronize()
+        s.record()
+        for _ in range(5):
+            g.replay()
+        e.record()
+        e.synchronize()
+        graph_ms_total = s.elapsed_time(e) / 5
     # plain: one build + one trace launch per mesh, waiting for each
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -328,9 +345,72 @@ def batched_leg(meshes=256, seed=4, tuner=None):
             "plain_trace_mrays_per_s": round(nrays / plain_trace_ms / 1e3, 1),
             "plain_build_and_trace_ms": round(plain_ms, 3),
             "batched_build_and_trace_ms": round(build_ms + trace_ms, 3),
+            "graph_build_and_trace_ms": graph_ms_total and round(graph_ms_total, 3),
             "speedup_build_and_trace": round(plain_ms / (build_ms + trace_ms), 2),
             "speedup_trace": round(plain_trace_ms / trace_ms, 2),
             "identical_results": same}
+
+
+def batched_2p5m_leg(frame, requests=64, seed=11, tuner=None):
+    """Batched vs plain at 2.5 M triangles: `requests` ray sets against the
+    2.5M-triangle sphere of the headline frame, 4k-2M rays each
+    (log-uniform), kind shadow (any hit, tmax 0.5), AO (any hit, tmax 0.05)
+    or closest hit in turn; origins uniform in 1.2x the mesh bounds,
+    directions uniform on the sphere.  Batched: one rt_trace_batch launch
+    replayed from a graph (tuned shape).  Plain: one rt_trace per request
+    with the default shape, waiting for each.  Results compared bit for
+    bit."""
+    rng = np.random.default_rng(seed)
+    lo = tuple(-1.2 * 1.02 for _ in range(3))
+    hi = tuple(1.2 * 1.02 for _ in range(3))
+    kinds = ("shadow", "ao", "closest")
+    tmax = {"shadow": 0.5, "ao": 0.05, "closest": 1e30}
+    items, rays = [], []
+    for k in range(requests):
+        n = int(np.exp(rng.uniform(np.log(4096), np.log(2_000_000))))
+        kind = kinds[k % 3]
+        r = rt.gen_random_rays(n, 9000 + k, lo, hi, tmax[kind])
+        rays.append((r, kind))
+        items.append((frame.bvh, r, "closest" if kind == "closest" else "any", None))
+    nrays = sum(len(r) for r, _ in rays)
+    batch = rt.TraceBatch(items)
+    shape = tuner.trace_batch(batch, tag="sphere-2p5m") if tuner is not None else {}
+    batch.launch(**shape)
+    torch.cuda.synchronize()
+    gs = torch.cuda.Stream()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g, stream=gs):
+        batch.launch(stream=gs, **shape)
+    g.replay()
+    torch.cuda.synchronize()
+    s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    s.record()
+    for _ in range(3):
+        g.replay()
+    e.record()
+    e.synchronize()
+    batched_ms = s.elapsed_time(e) / 3
+    plain = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for r, kind in rays:
+        plain.append(frame.bvh.intersect(r) if kind == "closest" else frame.bvh.occluded(r))
+        torch.cuda.synchronize()
+    plain_ms = (time.perf_counter() - t0) * 1e3
+    same = True
+    for (r, kind), o, p in zip(rays, batch.outputs, plain):
+        n = len(r)
+        if kind == "closest":
+            same &= bool(torch.equal(o[0][:n], p[0][:n]) and torch.equal(o[1][:n], p[1][:n]))
+        else:
+            w = (n + 31) // 32
+            same &= bool(torch.equal(o[:w], p[:w]))
+    return {"workload": f"{requests} ray sets ({nrays} rays, 4k-2M each, shadow / AO / closest) "
+                        "against the 2.5M-triangle sphere", "trace_shape": shape or "default",
+            "batched_ms": round(batched_ms, 3),
+            "batched_mrays_per_s": round(nrays / batched_ms / 1e3, 1),
+            "plain_ms": round(plain_ms, 3), "plain_mrays_per_s": round(nrays / plain_ms / 1e3, 1),
+            "speedup": round(plain_ms / batched_ms, 2), "identical_results": same}
 
 
 def ao_frame_leg(frames=100, w=1920, h=1080, tuner=None):
@@ -568,6 +648,7 @@ def main():
     if not a.no_extra:
         out["ao_frame"] = ao_frame_leg(tuner=tuner)
         out["batched"] = batched_leg(tuner=tuner)
+        out["batched_2p5m"] = batched_2p5m_leg(f, tuner=tuner)
         out["serving"] = serving_leg(tuner=tuner)
         out["p99_request_latency_ms"] = out["serving"]["p99_ms"]
     if tuner is not None:
