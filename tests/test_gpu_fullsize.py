@@ -1,0 +1,111 @@
+"""Parity at the benchmark's full sizes, through properties that do not
+depend on size and through sampled rays against the oracle:
+
+  * the 2.5M-triangle displaced sphere of the shadow configuration, radix
+    build: Morton codes non-decreasing, sorted ids a permutation, every leaf
+    record holding its triangle's vertices and id, every inner slot's box
+    containing the child's occupied slots, the root containing every vertex;
+  * 200,000 rays of the frame (camera rays and their shadow rays towards the
+    16 lights) traced on that tree: closest-hit ids / t and any-hit bits
+    bit-identical to the oracle's traversal of the same tree;
+  * the 65,536-triangle maximum of the batched build equal to the
+    multi-launch build of the same mesh.
+"""
+import numpy as np
+import pytest
+import torch
+
+from conftest import u32, to_np
+
+pytestmark = pytest.mark.gpu
+
+NU, NV, W, H, NL = 1250, 1000, 1920, 1080, 16
+
+
+@pytest.fixture(scope="module")
+def sphere(rt):
+    pos, idx = rt.gen_sphere(NU, NV, 1.0, 0.02, 7)
+    bvh = rt.GpuBVH4.build(pos, idx, algorithm="radix", keep_order=True)
+    torch.cuda.synchronize()
+    return pos, idx, bvh
+
+
+def test_full_size_build_properties(rt, sphere):
+    pos, idx, bvh = sphere
+    n = idx.shape[0]
+    assert n == 2 * NU * NV
+    codes = bvh.codes[:n].to(torch.int64)
+    assert bool((codes[1:] >= codes[:-1]).all()), "codes sorted"
+    ids = bvh.sorted_ids[:n].to(torch.int64)
+    assert torch.equal(torch.sort(ids).values, torch.arange(n, device=ids.device)), "permutation"
+    rec_i = bvh.tris[: n * 64].view(torch.int32).view(n, 16)
+    rec_f = bvh.tris[: n * 64].view(torch.float32).view(n, 16)
+    assert torch.equal(rec_i[:, 12].to(torch.int64), ids), "record ids in sorted order"
+    verts = pos[idx.to(torch.int64)[ids]].reshape(n, 9)
+    assert torch.equal(rec_f[:, :9], verts), "record vertices"
+    nc = bvh.node_count
+    raw = bvh.nodes[: nc * 128]
+    node_i = raw.view(torch.int32).view(nc, 32)
+    b = raw.view(torch.float32).view(nc, 32)[:, :24].reshape(nc, 6, 4)  # minX maxX ... maxZ
+    child, count = node_i[:, 24:28].to(torch.int64), node_i[:, 28:32]
+    inner = count < 0
+    pi, pk = inner.nonzero(as_tuple=True)
+    c = child[pi, pk]
+    occ = (count[c] != 0)
+    big = torch.tensor(float("inf"), device=b.device)
+    cmin = torch.where(occ.unsqueeze(1), b[c][:, [0, 2, 4], :], big).amin(2)
+    cmax = torch.where(occ.unsqueeze(1), b[c][:, [1, 3, 5], :], -big).amax(2)
+    pmin = b[pi, :, pk][:, [0, 2, 4]]
+    pmax = b[pi, :, pk][:, [1, 3, 5]]
+    assert bool((pmin <= cmin).all() and (pmax >= cmax).all()), "slot boxes contain children"
+    root_occ = count[0] != 0
+    rmin = torch.where(root_occ, b[0][[0, 2, 4]], big).amin(1)
+    rmax = torch.where(root_occ, b[0][[1, 3, 5]], -big).amax(1)
+    assert bool((pos.amin(0) >= rmin).all() and (pos.amax(0) <= rmax).all()), "root bounds"
+
+
+def test_full_size_frame_rays_match_oracle(rt, O, sphere):
+    pos, idx, bvh = sphere
+    prim_rays = rt.gen_camera_rays(W, H, (0.0, 0.0, 1.25), (0.0, 0.0, 0.0), (0.0, 1.0, 0.0),
+                                   60.0, 0.0, 1e30)
+    prim, t = bvh.intersect(prim_rays)
+    lo, hi = pos.amin(0).cpu().double().numpy(), pos.amax(0).cpu().double().numpy()
+    c, ext = (lo + hi) / 2, hi - lo
+    lights = torch.tensor(rt.lights_in_box(c - 1.5 * ext, c + 1.5 * ext, NL, 1337),
+                          dtype=torch.float32, device="cuda")
+    shadow = rt.gen_shadow_rays(pos, idx, prim_rays, prim, t, lights, 1e-4)
+    torch.cuda.synchronize()
+    nodes = to_np(bvh.nodes)[: bvh.node_count * 128].view(O.NODE4)
+    tris = to_np(bvh.tris)[: bvh.tri_count * 64].view(O.TRI64)
+    rng = np.random.default_rng(3)
+    # closest hit: 100,000 camera rays spread over the frame
+    sel = np.sort(rng.choice(W * H, 100_000, replace=False))
+    r = O.Rays.stack(to_np(prim_rays.data)[:, sel])
+    pc, tc = O.trace_bvh4(nodes, tris, r)
+    np.testing.assert_array_equal(u32(prim)[sel], pc)
+    np.testing.assert_array_equal(to_np(t)[sel].view(np.uint32), tc.view(np.uint32))
+    # any hit: 100,000 shadow rays spread over all 16 lights, gathered into a
+    # ray set of their own and traced alone
+    sel = np.sort(rng.choice(W * H * NL, 100_000, replace=False))
+    sh = rt.Rays(shadow.data[:, torch.from_numpy(sel).cuda()].contiguous())
+    bits = bvh.occluded(sh)
+    torch.cuda.synchronize()
+    ra = O.Rays.stack(to_np(sh.data))
+    pa, _ = O.trace_bvh4(nodes, tris, ra, mode=O.ANY)
+    b = u32(bits)
+    i = np.arange(len(sel))
+    np.testing.assert_array_equal(((b[i >> 5] >> (i & 31)) & 1).astype(bool), pa != O.MISS)
+    assert 0.05 < (pa != O.MISS).mean() < 0.95  # the sample holds both outcomes
+
+
+def test_full_size_batched_build_maximum(rt, O):
+    nx, ny = 256, 128  # 65,536 triangles
+    pos, idx = rt.gen_heightfield(nx, ny, 1.0, 0.05, 77)
+    assert idx.shape[0] == rt.BATCH_BUILD_MAX
+    bb = rt.BuildBatch([(pos, idx)], keep_order=True)
+    got = bb.launch()[0]
+    want = rt.GpuBVH4.build(pos, idx, algorithm="implicit", keep_order=True)
+    torch.cuda.synchronize()
+    assert torch.equal(got.nodes[: want.node_count * 128], want.nodes[: want.node_count * 128])
+    assert torch.equal(got.tris, want.tris[: got.tris.numel()])
+    assert torch.equal(got.codes, want.codes[: got.codes.numel()])
