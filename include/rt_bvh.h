@@ -273,6 +273,31 @@ rt_status rt_trace_ex(rt_stream stream, const rt_bvh_dev* bvh, const rt_rays_soa
                       uint32_t n, uint32_t flags, const rt_trace_opts* opts, rt_hits_soa* hits,
                       uint32_t* occl_bits, uint32_t* box_tests, uint32_t* tri_tests);
 
+/* BVH2 traversal of a SoftwareBVH as built (software_bvh.hpp:11-22,35-175:
+ * GpuBvhNode 32 B nodes, GpuTriangle 48 B triangles in leaf order), the
+ * loop and predicates of benchmarks/exp0_probe/shaders/
+ * software_bvh_procedural.comp:47-171: popped node's box tested against the
+ * closest t (tFar >= max(0, tNear), tNear < closest), leaf triangles in
+ * record order (|a| < 1e-8 rejected, accept t > tmin and t < closest),
+ * inner node: both children's boxes tested, hit ones pushed far first.
+ * order (optional) maps a leaf position to the original triangle index (the
+ * index SoftwareBVH drops); NULL reports the leaf position.  depth = node
+ * levels (rt_bvh2_depth on the host array; 0 = unknown, at most 64).
+ * flags / outputs as rt_trace; opts: block and lds_stack only.            */
+typedef struct rt_bvh2_dev {
+    const void* nodes;     /* GpuBvhNode[node_count] */
+    const float* tris48;   /* GpuTriangle[tri_count] */
+    const uint32_t* order; /* optional: original index of each leaf position */
+    uint32_t node_count;
+    uint32_t tri_count;
+    uint32_t depth;
+    uint32_t reserved;
+} rt_bvh2_dev;
+rt_status rt_bvh2_depth(const void* host_nodes, uint32_t node_count, uint32_t* depth);
+rt_status rt_trace_bvh2(rt_stream stream, const rt_bvh2_dev* bvh, const rt_rays_soa* rays,
+                        uint32_t n, uint32_t flags, const rt_trace_opts* opts, rt_hits_soa* hits,
+                        uint32_t* occl_bits, uint32_t* box_tests, uint32_t* tri_tests);
+
 /* Batched trace: many (tree, ray set, mode) items in ONE launch, each with
  * the results rt_trace gives it alone.  No reference counterpart (the
  * reference dispatches one ray set per submit, microbench_main.cpp:344);

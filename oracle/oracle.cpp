@@ -576,6 +576,124 @@ void or_trace_bvh4(const or_bvh4_node* nodes, uint32_t node_count, const or_tri6
     });
 }
 
+// software_bvh_procedural.comp:47-82 intersectAABB: slabs against the ray,
+// hit iff tFar >= max(0, tNear) and tNear < tCurrent (the ray's tmin is not
+// used by the box test).
+static inline bool bvh2_box(const float bmin[3], const float bmax[3], const float o[3],
+                            const float inv[3], float tcur, float* tbox) {
+    float tn = -FLT_MAX, tf = FLT_MAX;
+    float t0[3], t1[3];
+    for (int c = 0; c < 3; ++c) {
+        t0[c] = (bmin[c] - o[c]) * inv[c];
+        t1[c] = (bmax[c] - o[c]) * inv[c];
+    }
+    tn = std::max(std::max(std::min(t0[0], t1[0]), std::min(t0[1], t1[1])),
+                  std::min(t0[2], t1[2]));
+    tf = std::min(std::min(std::max(t0[0], t1[0]), std::max(t0[1], t1[1])),
+                  std::max(t0[2], t1[2]));
+    *tbox = tn;
+    return tf >= std::max(0.0f, tn) && tn < tcur;
+}
+
+// software_bvh_procedural.comp:60-82 intersectTriangle on a 48-B GpuTriangle
+// (v0, v1, v2 as float4): Moller-Trumbore with |a| < 1e-8 rejected.
+static inline bool bvh2_tri(const float* t48, const float o[3], const float d[3], float* tout) {
+    const float *v0 = t48, *v1 = t48 + 4, *v2 = t48 + 8;
+    float e1x = v1[0] - v0[0], e1y = v1[1] - v0[1], e1z = v1[2] - v0[2];
+    float e2x = v2[0] - v0[0], e2y = v2[1] - v0[1], e2z = v2[2] - v0[2];
+    float hx = d[1] * e2z - d[2] * e2y;
+    float hy = d[2] * e2x - d[0] * e2z;
+    float hz = d[0] * e2y - d[1] * e2x;
+    float a = e1x * hx + e1y * hy + e1z * hz;
+    if (std::fabs(a) < 1e-8f) return false;
+    float f = 1.0f / a;
+    float sx = o[0] - v0[0], sy = o[1] - v0[1], sz = o[2] - v0[2];
+    float u = f * (sx * hx + sy * hy + sz * hz);
+    if (u < 0.0f || u > 1.0f) return false;
+    float qx = sy * e1z - sz * e1y;
+    float qy = sz * e1x - sx * e1z;
+    float qz = sx * e1y - sy * e1x;
+    float v = f * (d[0] * qx + d[1] * qy + d[2] * qz);
+    if (v < 0.0f || u + v > 1.0f) return false;
+    *tout = f * (e2x * qx + e2y * qy + e2z * qz);
+    return true;
+}
+
+// BVH2 traversal of a SoftwareBVH (GpuBvhNode + GpuTriangle arrays), the
+// loop of software_bvh_procedural.comp:119-171: the popped node's box is
+// tested (culled when it misses or starts beyond the closest t), a leaf
+// tests its triangles in record order (accept t > tmin and t < closest t:
+// the shader's t > 0.01 with the ray's tmin), an inner node tests both
+// children's boxes and pushes the hit ones far first (left popped first
+// when tLeft < tRight).  inv = 1 / d with |d| <= 1e-8 replaced by +-1e-8
+// (:108-110).  The shader starts from closestT = 1000 and writes a hit flag;
+// here closest t starts at the ray's tmax and the hit's leaf position (or
+// order[position], the original index) and t are written.  Any hit (mode 1)
+// stops at the first accepted triangle.  Counters: box tests = boxes tested
+// (popped nodes + children of inner nodes), triangle tests = triangles
+// tested.  The stack is unbounded (the shader's holds 32).
+void or_trace_bvh2(const or_bvh2_node* nodes, uint32_t node_count, const float* tris48,
+                   const uint32_t* order, const float* ox, const float* oy, const float* oz,
+                   const float* tmn, const float* dx, const float* dy, const float* dz,
+                   const float* tmx, uint32_t nrays, int mode, int nthreads,
+                   uint32_t* out_prim, float* out_t, uint32_t* out_box, uint32_t* out_tri) {
+    parallel_chunks(nrays, nthreads, [&](uint32_t b, uint32_t e) {
+        std::vector<uint32_t> stack(256);
+        for (uint32_t r = b; r < e; ++r) {
+            const float o[3] = {ox[r], oy[r], oz[r]};
+            const float d[3] = {dx[r], dy[r], dz[r]};
+            float inv[3];
+            for (int c = 0; c < 3; ++c) {
+                const float dd = std::fabs(d[c]) > 1e-8f ? d[c] : (d[c] >= 0.0f ? 1e-8f : -1e-8f);
+                inv[c] = 1.0f / dd;
+            }
+            float closest = tmx[r];
+            uint32_t prim = 0xFFFFFFFFu, boxT = 0, triT = 0;
+            size_t sp = 0;
+            bool done = false;
+            if (node_count > 0) stack[sp++] = 0;
+            while (sp > 0 && !done) {
+                const or_bvh2_node& nd = nodes[stack[--sp]];
+                float tb;
+                ++boxT;
+                if (!bvh2_box(nd.aabbMin, nd.aabbMax, o, inv, closest, &tb)) continue;
+                if (nd.leftChild < 0) {
+                    const uint32_t first = (uint32_t)~nd.leftChild, cnt = (uint32_t)nd.rightChild;
+                    for (uint32_t i = 0; i < cnt; ++i) {
+                        ++triT;
+                        float t;
+                        if (!bvh2_tri(tris48 + 12 * (size_t)(first + i), o, d, &t)) continue;
+                        if (t > tmn[r] && t < closest) {
+                            closest = t;
+                            prim = order ? order[first + i] : first + i;
+                            if (mode == OR_ANY) { done = true; break; }
+                        }
+                    }
+                } else {
+                    const uint32_t L = (uint32_t)nd.leftChild, R = (uint32_t)nd.rightChild;
+                    float tl, tr;
+                    boxT += 2;
+                    const bool hl = bvh2_box(nodes[L].aabbMin, nodes[L].aabbMax, o, inv, closest, &tl);
+                    const bool hr = bvh2_box(nodes[R].aabbMin, nodes[R].aabbMax, o, inv, closest, &tr);
+                    if (sp + 2 > stack.size()) stack.resize(stack.size() * 2);
+                    if (hl && hr) {
+                        if (tl < tr) { stack[sp++] = R; stack[sp++] = L; }
+                        else { stack[sp++] = L; stack[sp++] = R; }
+                    } else if (hl) {
+                        stack[sp++] = L;
+                    } else if (hr) {
+                        stack[sp++] = R;
+                    }
+                }
+            }
+            out_prim[r] = prim;
+            if (out_t) out_t[r] = closest;
+            if (out_box) out_box[r] = boxT;
+            if (out_tri) out_tri[r] = triT;
+        }
+    });
+}
+
 // Every triangle against every ray with the same predicate.  Closest: the
 // smallest t; at equal t the lowest triangle_id.  Any: first triangle whose t
 // lies in [tmin, tmax).

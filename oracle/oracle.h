@@ -19,6 +19,11 @@
  *                             rates; known-answer rays (probe_main.cpp,
  *                             src/main.cpp); brute force on small meshes.
  *   or_exp0_rays           -> pinned through the same exp0 statistics.
+ *   or_trace_bvh2          -> brute force over all triangles with the same
+ *                             predicates; a hit is a hit of the BVH4
+ *                             traversal of the same mesh up to the two
+ *                             shaders' predicate differences (|a| 1e-8 vs 1e-7,
+ *                             t > tmin vs t >= tmin); known-answer rays.
  *   or_morton30            -> bit-interleave closed form, per-axis brute force.
  *   or_sort_pairs          -> std::stable_sort on the same keys.
  *   or_lbvh_build          -> invariants: permutation, sortedness, parent box
@@ -98,6 +103,11 @@ void or_trace_bvh4(const or_bvh4_node* nodes, uint32_t node_count, const or_tri6
                    const float* ox, const float* oy, const float* oz, const float* tmin,
                    const float* dx, const float* dy, const float* dz, const float* tmax,
                    uint32_t nrays, int mode, uint32_t stack_limit, int nthreads,
+                   uint32_t* out_prim, float* out_t, uint32_t* out_box, uint32_t* out_tri);
+void or_trace_bvh2(const or_bvh2_node* nodes, uint32_t node_count, const float* tris48,
+                   const uint32_t* order, const float* ox, const float* oy, const float* oz,
+                   const float* tmin, const float* dx, const float* dy, const float* dz,
+                   const float* tmax, uint32_t nrays, int mode, int nthreads,
                    uint32_t* out_prim, float* out_t, uint32_t* out_box, uint32_t* out_tri);
 void or_trace_brute(const or_tri64* tris, uint32_t ntris,
                     const float* ox, const float* oy, const float* oz, const float* tmin,

@@ -211,6 +211,25 @@ def trace_bvh4(nodes, tris, rays, mode=CLOSEST, stack_limit=0, threads=None, cou
     return prim, t
 
 
+def trace_bvh2(nodes2, tris48, rays, order=None, mode=CLOSEST, threads=None, counters=False):
+    """BVH2 traversal of a SoftwareBVH (software_bvh_procedural.comp): returns
+    (prim, t[, box, tri]); prim = order[leaf position] (or the position)."""
+    nodes2 = np.ascontiguousarray(nodes2)
+    tris48 = _f(tris48)
+    n = len(rays)
+    prim = np.zeros(n, np.uint32)
+    t = np.zeros(n, np.float32)
+    box = np.zeros(n, np.uint32) if counters else None
+    tri = np.zeros(n, np.uint32) if counters else None
+    o = None if order is None else np.ascontiguousarray(order, np.uint32)
+    lib().or_trace_bvh2(_p(nodes2), u32(len(nodes2)), _p(tris48), _p(o), _p(rays.ox),
+                        _p(rays.oy), _p(rays.oz), _p(rays.tmin), _p(rays.dx), _p(rays.dy),
+                        _p(rays.dz), _p(rays.tmax), u32(n), C.c_int(mode),
+                        C.c_int(threads or os.cpu_count() or 1), _p(prim), _p(t), _p(box),
+                        _p(tri))
+    return (prim, t, box, tri) if counters else (prim, t)
+
+
 def trace_brute(tris, rays, mode=CLOSEST, threads=None):
     tris = np.ascontiguousarray(tris)
     n = len(rays)

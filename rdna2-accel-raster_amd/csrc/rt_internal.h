@@ -68,14 +68,14 @@ __host__ __device__ inline float hash01(uint32_t seed, uint32_t i) {
 __device__ __forceinline__ bool mt_hit(float v0x, float v0y, float v0z, float v1x, float v1y,
                                        float v1z, float v2x, float v2y, float v2z, float ox,
                                        float oy, float oz, float dx, float dy, float dz,
-                                       float* tout) {
+                                       float* tout, float eps = 1e-7f) {
     float e1x = v1x - v0x, e1y = v1y - v0y, e1z = v1z - v0z;
     float e2x = v2x - v0x, e2y = v2y - v0y, e2z = v2z - v0z;
     float hx = dy * e2z - dz * e2y;
     float hy = dz * e2x - dx * e2z;
     float hz = dx * e2y - dy * e2x;
     float a = e1x * hx + e1y * hy + e1z * hz;
-    if (fabsf(a) < 1e-7f) return false;
+    if (fabsf(a) < eps) return false;
     float f = 1.0f / a;
     float sx = ox - v0x, sy = oy - v0y, sz = oz - v0z;
     float u = f * (sx * hx + sy * hy + sz * hz);

@@ -17,6 +17,8 @@
 // tuner can trade occupancy against private-memory spills; nothing is ever
 // dropped (the shader drops on overflow, :185-187): 3 * depth + 1 entries
 // bound the live stack and the private size is chosen from the tree depth.
+#include <vector>
+
 #include "rt_internal.h"
 
 namespace rt {
@@ -612,6 +614,116 @@ __global__ __launch_bounds__(kBlock) void k_trace_batch_refill(
                                         nullptr);
 }
 
+// ---------------------------------------------------------------------------
+// BVH2 traversal of a SoftwareBVH (GpuBvhNode 32 B + GpuTriangle 48 B), the
+// loop of benchmarks/exp0_probe/shaders/software_bvh_procedural.comp:119-171:
+// the popped node's box is tested against the current closest t, a leaf
+// tests its triangles in record order (|a| < 1e-8 rejected, accept
+// t > tmin and t < closest), an inner node tests both children's boxes and
+// pushes the hit ones far first.  Box test (:47-58): hit iff
+// tFar >= max(0, tNear) and tNear < closest.  Same arithmetic order as the
+// shader; the stack holds depth + 1 entries at most (LDS part + private).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ bool box2(const f4v& lo, const f4v& hi, float ox, float oy, float oz,
+                                     float ix, float iy, float iz, float tcur, float& tnear) {
+    const float t0x = (lo.x - ox) * ix, t1x = (hi.x - ox) * ix;
+    const float t0y = (lo.y - oy) * iy, t1y = (hi.y - oy) * iy;
+    const float t0z = (lo.z - oz) * iz, t1z = (hi.z - oz) * iz;
+    const float tn = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fminf(t0z, t1z));
+    const float tf = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fmaxf(t0z, t1z));
+    tnear = tn;
+    return tf >= fmaxf(0.0f, tn) && tn < tcur;
+}
+
+template <int PRIV, bool ANY, bool COUNT>
+__global__ __launch_bounds__(kBlock) void k_trace_bvh2(
+    const f4v* __restrict__ nodes, const f4v* __restrict__ tris, const uint32_t* __restrict__ order,
+    uint32_t node_count, rt_rays_soa rays, uint32_t n, uint32_t lsu,
+    uint32_t* __restrict__ prim_out, float* __restrict__ t_out, uint32_t* __restrict__ bits_out,
+    uint32_t* __restrict__ box_out, uint32_t* __restrict__ tri_out) {
+    extern __shared__ int32_t lds_stack[];  // dynamic: ls * blockDim.x entries
+    const uint32_t tid = threadIdx.x;
+    const int bs = (int)blockDim.x, ls = (int)lsu;
+    int32_t* col = lds_stack + tid;
+    const uint32_t r = blockIdx.x * blockDim.x + tid;
+    const bool valid = r < n;
+    float ox = 0.f, oy = 0.f, oz = 0.f, tmin = 1.f, dx = 0.f, dy = 0.f, dz = 1.f, tmax = 0.f;
+    if (valid) {
+        ox = rays.ox[r]; oy = rays.oy[r]; oz = rays.oz[r]; tmin = rays.tmin[r];
+        dx = rays.dx[r]; dy = rays.dy[r]; dz = rays.dz[r]; tmax = rays.tmax[r];
+    }
+    // :108-110
+    const float ix = 1.0f / (fabsf(dx) > 1e-8f ? dx : (dx >= 0.0f ? 1e-8f : -1e-8f));
+    const float iy = 1.0f / (fabsf(dy) > 1e-8f ? dy : (dy >= 0.0f ? 1e-8f : -1e-8f));
+    const float iz = 1.0f / (fabsf(dz) > 1e-8f ? dz : (dz >= 0.0f ? 1e-8f : -1e-8f));
+    float closest = tmax;
+    uint32_t prim = RT_MISS, boxT = 0, triT = 0;
+    Stack<PRIV> st;
+    int sp = 0;
+    if (valid && node_count > 0) RT_SPUSH(0);
+    bool done = false;
+    while (sp > 0 && !done) {
+        int32_t ni;
+        RT_SPOP(ni);
+        const f4v lo = nodes[2 * ni], hi = nodes[2 * ni + 1];
+        if (COUNT) ++boxT;
+        float tb;
+        if (!box2(lo, hi, ox, oy, oz, ix, iy, iz, closest, tb)) continue;
+        const int32_t left = __float_as_int(lo.w), right = __float_as_int(hi.w);
+        if (left < 0) {
+            const uint32_t first = (uint32_t)~left, cnt = (uint32_t)right;
+            for (uint32_t i = 0; i < cnt; ++i) {
+                if (COUNT) ++triT;
+                const f4v* tp = tris + 3 * (size_t)(first + i);
+                const f4v a = tp[0], b = tp[1], c = tp[2];
+                float t;
+                if (!mt_hit(a.x, a.y, a.z, b.x, b.y, b.z, c.x, c.y, c.z, ox, oy, oz, dx, dy, dz,
+                            &t, 1e-8f))
+                    continue;
+                if (t > tmin && t < closest) {
+                    closest = t;
+                    prim = order ? order[first + i] : first + i;
+                    if (ANY) {
+                        done = true;
+                        break;
+                    }
+                }
+            }
+        } else {
+            const f4v llo = nodes[2 * left], lhi = nodes[2 * left + 1];
+            const f4v rlo = nodes[2 * right], rhi = nodes[2 * right + 1];
+            if (COUNT) boxT += 2;
+            float tl, tr;
+            const bool hl = box2(llo, lhi, ox, oy, oz, ix, iy, iz, closest, tl);
+            const bool hr = box2(rlo, rhi, ox, oy, oz, ix, iy, iz, closest, tr);
+            if (hl && hr) {
+                if (tl < tr) {
+                    RT_SPUSH(right);
+                    RT_SPUSH(left);
+                } else {
+                    RT_SPUSH(left);
+                    RT_SPUSH(right);
+                }
+            } else if (hl) {
+                RT_SPUSH(left);
+            } else if (hr) {
+                RT_SPUSH(right);
+            }
+        }
+    }
+    if (ANY) {
+        write_wave_bits(bits_out, r - (tid & 63u), n, valid && prim != RT_MISS);
+        if (valid && prim_out) prim_out[r] = prim;
+    } else if (valid) {
+        prim_out[r] = prim;
+        t_out[r] = closest;
+    }
+    if (COUNT && valid) {
+        box_out[r] = boxT;
+        tri_out[r] = triT;
+    }
+}
+
 struct TraceCfg {
     bool refill;
     uint32_t per_wave;      // multiple of 64, <= kMaxPerWave
@@ -713,6 +825,82 @@ extern "C" rt_status rt_trace_ex(rt_stream stream, const rt_bvh_dev* bvh, const 
                                     tri_tests);
     return launch_v<kPrivLarge>(s, bvh, rays, n, flags, cfg, hits, occl_bits, box_tests,
                                 tri_tests);
+}
+
+extern "C" rt_status rt_bvh2_depth(const void* nodes, uint32_t node_count, uint32_t* depth) {
+    if (!depth || (node_count && !nodes)) return RT_ERR_ARG;
+    *depth = 0;
+    if (node_count == 0) return RT_OK;
+    struct N2 {
+        float mn[3];
+        int32_t l;
+        float mx[3];
+        int32_t r;
+    };
+    const N2* nd = static_cast<const N2*>(nodes);
+    std::vector<std::pair<uint32_t, uint32_t>> st{{0u, 1u}};
+    uint32_t deep = 0, visited = 0;
+    while (!st.empty()) {
+        const auto [i, d] = st.back();
+        st.pop_back();
+        if (i >= node_count || ++visited > node_count) return RT_ERR_ARG;  // not a tree
+        deep = d > deep ? d : deep;
+        if (nd[i].l >= 0) {
+            st.push_back({(uint32_t)nd[i].l, d + 1});
+            st.push_back({(uint32_t)nd[i].r, d + 1});
+        }
+    }
+    *depth = deep;
+    return RT_OK;
+}
+
+extern "C" rt_status rt_trace_bvh2(rt_stream stream, const rt_bvh2_dev* bvh,
+                                   const rt_rays_soa* rays, uint32_t n, uint32_t flags,
+                                   const rt_trace_opts* opts, rt_hits_soa* hits,
+                                   uint32_t* occl_bits, uint32_t* box_tests,
+                                   uint32_t* tri_tests) {
+    if (!bvh || !rays) return RT_ERR_ARG;
+    if (flags & ~(RT_ANY | RT_COUNTERS)) return RT_ERR_ARG;
+    TraceCfg cfg{false, 0u, 0u, (uint32_t)kBlock, kLdsStack};
+    if (opts && opts->schedule != RT_SCHED_SIMPLE) return RT_ERR_ARG;
+    if (!shape_opts(opts, cfg)) return RT_ERR_ARG;
+    if (n == 0) return RT_OK;
+    if (!rays->ox || !rays->oy || !rays->oz || !rays->tmin || !rays->dx || !rays->dy ||
+        !rays->dz || !rays->tmax)
+        return RT_ERR_ARG;
+    const bool any = flags & RT_ANY, cnt = flags & RT_COUNTERS;
+    if (any && !occl_bits) return RT_ERR_ARG;
+    if (!any && (!hits || !hits->prim || !hits->t)) return RT_ERR_ARG;
+    if (cnt && (!box_tests || !tri_tests)) return RT_ERR_ARG;
+    if (bvh->node_count > 0 && (!bvh->nodes || !bvh->tris48)) return RT_ERR_ARG;
+    // a BVH2 stack holds at most depth + 1 entries; depth 0 = unknown (<= 64)
+    const uint32_t depth = bvh->depth ? bvh->depth : kMaxDepth;
+    if (depth > kMaxDepth) return RT_ERR_CAPACITY;
+    const bool small = depth + 1u <= cfg.lds + (uint32_t)kPrivSmall;
+    uint32_t* prim = hits ? hits->prim : nullptr;
+    float* t = hits ? hits->t : nullptr;
+    const f4v* nd = static_cast<const f4v*>(bvh->nodes);
+    const f4v* tr = reinterpret_cast<const f4v*>(bvh->tris48);
+    const dim3 grid(ceil_div(n, cfg.block)), block(cfg.block);
+    const size_t shm = lds_bytes(cfg);
+    hipStream_t s = as_stream(stream);
+#define RT_B2(P, A, C)                                                                         \
+    hipLaunchKernelGGL((k_trace_bvh2<P, A, C>), grid, block, shm, s, nd, tr, bvh->order,       \
+                       bvh->node_count, *rays, n, cfg.lds, prim, t, occl_bits, box_tests,      \
+                       tri_tests)
+    if (small) {
+        if (any && cnt) RT_B2(kPrivSmall, true, true);
+        else if (any) RT_B2(kPrivSmall, true, false);
+        else if (cnt) RT_B2(kPrivSmall, false, true);
+        else RT_B2(kPrivSmall, false, false);
+    } else {
+        if (any && cnt) RT_B2(kPrivLarge, true, true);
+        else if (any) RT_B2(kPrivLarge, true, false);
+        else if (cnt) RT_B2(kPrivLarge, false, true);
+        else RT_B2(kPrivLarge, false, false);
+    }
+#undef RT_B2
+    return launch_status("k_trace_bvh2");
 }
 
 extern "C" rt_status rt_trace(rt_stream stream, const rt_bvh_dev* bvh, const rt_rays_soa* rays,

@@ -57,6 +57,11 @@ class rt_hits_soa(C.Structure):
     _fields_ = [("prim", vp), ("t", vp)]
 
 
+class rt_bvh2_dev(C.Structure):
+    _fields_ = [("nodes", vp), ("tris48", vp), ("order", vp), ("node_count", u32),
+                ("tri_count", u32), ("depth", u32), ("reserved", u32)]
+
+
 class rt_trace_item(C.Structure):
     _fields_ = [("bvh", rt_bvh_dev), ("rays", rt_rays_soa), ("count", u32), ("flags", u32),
                 ("hits", rt_hits_soa), ("occlusion_bits", vp)]
@@ -130,6 +135,8 @@ _SIGS = {
     "rt_convert_tris48": [vp, vp, vp, u32, vp],
     "rt_trace": [vp, vp, vp, u32, u32, vp, vp, vp, vp],
     "rt_trace_ex": [vp, vp, vp, u32, u32, vp, vp, vp, vp, vp],
+    "rt_bvh2_depth": [vp, u32, vp],
+    "rt_trace_bvh2": [vp, vp, vp, u32, u32, vp, vp, vp, vp, vp],
     "rt_gen_sphere": [vp, u32, u32, f32, f32, u32, vp, vp],
     "rt_terrain_resolution": [u32, vp, vp, vp],
     "rt_gen_terrain": [vp, u32, f32, C.c_int, vp, vp],
@@ -424,6 +431,78 @@ class GpuBVH4:
         o = trace_opts(schedule, **tune)
         call("rt_trace_ex", stream_handle(stream), C.byref(b), C.byref(r), u32(n), u32(flags),
              C.byref(o), None, ptr(bits), ptr(box), ptr(tri))
+        return (bits, box, tri) if counters else bits
+
+
+class GpuBVH2:
+    """Device copy of a SoftwareBVH as its builder lays it out (GpuBvhNode
+    32-B nodes, GpuTriangle 48-B triangles in leaf order), traversed by
+    rt_trace_bvh2 with the loop of the reference's software BVH shader
+    (software_bvh_procedural.comp:47-171).  order: original triangle index
+    of each leaf position (reported as the hit id; None reports the
+    position)."""
+
+    def __init__(self, nodes, tris48, order, node_count, tri_count, depth):
+        self.nodes, self.tris48, self.order = nodes, tris48, order
+        self.node_count, self.tri_count, self.depth = node_count, tri_count, depth
+
+    @staticmethod
+    def from_bvh2(nodes2, tris48, order=None, device=None):
+        dev = device or _dev()
+        nb = np.ascontiguousarray(nodes2).view(np.uint8).reshape(-1)
+        if nb.size % 32:
+            raise ValueError("nodes2: 32-byte GpuBvhNode records expected")
+        ncount = nb.size // 32
+        depth = u32(0)
+        call("rt_bvh2_depth", nb.ctypes.data_as(vp) if ncount else None, u32(ncount),
+             C.byref(depth))
+        t = np.ascontiguousarray(tris48, dtype=np.float32).reshape(-1)
+        if t.size % 12:
+            raise ValueError("tris48: 48-byte GpuTriangle records expected")
+        nodes = torch.from_numpy(nb.copy()).to(dev) if ncount else torch.empty(
+            32, dtype=torch.uint8, device=dev)
+        tris = torch.from_numpy(t.copy()).to(dev) if t.size else torch.empty(
+            12, dtype=torch.float32, device=dev)
+        o = None
+        if order is not None:
+            o = torch.from_numpy(np.ascontiguousarray(order, dtype=np.uint32).view(np.int32)).to(dev)
+        return GpuBVH2(nodes, tris, o, ncount, t.size // 12, depth.value)
+
+    def dev(self):
+        return rt_bvh2_dev(ptr(self.nodes), ptr(self.tris48), ptr(self.order),
+                           u32(self.node_count), u32(self.tri_count), u32(self.depth), u32(0))
+
+    def _trace(self, rays, flags, outs, counters, stream, tune):
+        n = len(rays)
+        box = tri = None
+        if counters:
+            box = torch.empty(n, dtype=torch.int32, device=rays.data.device)
+            tri = torch.empty(n, dtype=torch.int32, device=rays.data.device)
+            flags |= RT_COUNTERS
+        b = self.dev()
+        r = rays.soa()
+        o = trace_opts("simple", **tune)
+        hits, bits = outs
+        call("rt_trace_bvh2", stream_handle(stream), C.byref(b), C.byref(r), u32(n), u32(flags),
+             C.byref(o), C.byref(hits) if hits is not None else None, ptr(bits), ptr(box),
+             ptr(tri))
+        return box, tri
+
+    def intersect(self, rays, counters=False, stream=None, **tune):
+        """Closest hit per ray: (prim uint32-as-int32, t float32[, box, tri]).
+        tune: block, lds_stack."""
+        n = len(rays)
+        prim = torch.empty(n, dtype=torch.int32, device=rays.data.device)
+        t = torch.empty(n, dtype=torch.float32, device=rays.data.device)
+        box, tri = self._trace(rays, RT_CLOSEST, (rt_hits_soa(ptr(prim), ptr(t)), None),
+                               counters, stream, tune)
+        return (prim, t, box, tri) if counters else (prim, t)
+
+    def occluded(self, rays, counters=False, stream=None, **tune):
+        """Any-hit occlusion bits: int32 tensor of ceil(n/32) words."""
+        n = len(rays)
+        bits = torch.empty(max(1, (n + 31) // 32), dtype=torch.int32, device=rays.data.device)
+        box, tri = self._trace(rays, RT_ANY, (None, bits), counters, stream, tune)
         return (bits, box, tri) if counters else bits
 
 

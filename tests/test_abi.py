@@ -257,3 +257,45 @@ def test_terrain_resolution_matches_oracle(O):
         assert res.value == O.lib().or_terrain_res(target)
         assert nv.value == res.value ** 2 and nt.value == 2 * (res.value - 1) ** 2
     assert lib.rt_gen_terrain(None, 1000, C.c_float(100.0), 0, None, None) == 1
+
+
+def _bvh2_depth_py(n2):
+    """Node levels of a BVH2 by an explicit walk from the root."""
+    deep, st = 0, [(0, 1)]
+    while st:
+        i, d = st.pop()
+        deep = max(deep, d)
+        if n2["leftChild"][i] >= 0:
+            st += [(int(n2["leftChild"][i]), d + 1), (int(n2["rightChild"][i]), d + 1)]
+    return deep
+
+
+def test_bvh2_depth_and_argument_checks(O):
+    import rtbvh
+    lib = rtbvh.lib()
+    pos, idx = O.sphere(40, 30, 1.0, 0.05, 3)
+    for mp in (1, 4, 8):
+        n2, _, _ = O.bvh2_build(pos, idx, mp)
+        d = C.c_uint32(0)
+        assert lib.rt_bvh2_depth(n2.ctypes.data_as(C.c_void_p), len(n2), C.byref(d)) == 0
+        assert d.value == _bvh2_depth_py(n2) > 1
+    bad = n2.copy()
+    bad["rightChild"][0] = len(bad) + 5  # child index out of range
+    assert lib.rt_bvh2_depth(bad.ctypes.data_as(C.c_void_p), len(bad), C.byref(d)) == 1
+    assert lib.rt_bvh2_depth(None, 0, C.byref(d)) == 0 and d.value == 0
+    fake = C.c_void_p(256)
+    b = rtbvh.rt_bvh2_dev(fake, fake, None, 10, 10, 65, 0)
+    r = rtbvh.rt_rays_soa(*([fake] * 8))
+    h = rtbvh.rt_hits_soa(fake, fake)
+    assert lib.rt_trace_bvh2(None, C.byref(b), C.byref(r), 0, 0, None, C.byref(h), None, None,
+                             None) == 0  # no rays: nothing to do
+    assert lib.rt_trace_bvh2(None, C.byref(b), C.byref(r), 5, 0, None, C.byref(h), None, None,
+                             None) == 2  # deeper than 64 levels
+    b.depth = 8
+    assert lib.rt_trace_bvh2(None, C.byref(b), C.byref(r), 5, 1, None, None, None, None,
+                             None) == 1  # any hit without bits
+    assert lib.rt_trace_bvh2(None, C.byref(b), C.byref(r), 5, 8, None, C.byref(h), None, None,
+                             None) == 1  # unknown flag
+    o = rtbvh.rt_trace_opts(1, 0, 0, 0, 0)  # refill schedule: not offered here
+    assert lib.rt_trace_bvh2(None, C.byref(b), C.byref(r), 5, 0, C.byref(o), C.byref(h), None,
+                             None, None) == 1
