@@ -27,18 +27,13 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "rdna2-accel-raster_amd"))
 import rtbvh as rt  # noqa: E402
 import autotune  # noqa: E402
+from timing import median, percentile  # noqa: E402
 
 METRIC = "Mrays/s any-hit+closest-hit @2.5M tris; BVH build µs @4K tris; p99 req latency"
 HBM_PEAK_GBS = 8000.0
 W, H, NLIGHTS = 1920, 1080, 16
 NU, NV, SEED = 1250, 1000, 7
 EPS = 1e-4
-
-
-def percentile(v, q):
-    """common/vk_timer.cpp:106-107 convention: sorted[floor(q (n - 1))]."""
-    s = sorted(v)
-    return s[int(q * (len(s) - 1))]
 
 
 def build_time_4k(iters=200, warm=20):
@@ -69,7 +64,7 @@ def build_time_4k(iters=200, warm=20):
     e.record()
     e.synchronize()
     back_to_back = s.elapsed_time(e) * 1000.0 / iters
-    return percentile(times, 0.5), percentile(times, 0.99), back_to_back
+    return median(times), percentile(times, 0.99), back_to_back
 
 
 class Frame:
@@ -469,12 +464,32 @@ def ao_frame_leg(frames=100, w=1920, h=1080, tuner=None):
     e.record()
     e.synchronize()
     ms = s.elapsed_time(e) / frames
+    # image quality of the filtered frame against its 1-SPP input
+    # (ImageMetrics::compute on the device), and that call's own time
+    a2, b2 = out.view(h, w), ao.view(h, w)
+    scratch = torch.empty(rt.image_metrics_scratch_bytes(w, h) // 8, dtype=torch.float64,
+                          device="cuda")
+    q = torch.empty(3, dtype=torch.float64, device="cuda")
+    for _ in range(5):
+        rt.image_metrics(a2, b2, out=q, scratch=scratch)
+    reps = 200
+    s.record()
+    for _ in range(reps):
+        rt.image_metrics(a2, b2, out=q, scratch=scratch)
+    e.record()
+    e.synchronize()
+    met_us = s.elapsed_time(e) * 1000.0 / reps
+    mse, psnr, ssim = q.cpu().tolist()
     return {"workload": "dynamic AO: 999,698-tri heightfield + 4,096-tri grid rebuilt per "
                         "frame (two-level scene), 1920x1080 primary + 1 SPP AO (tmax 0.5) + 7x7 "
                         "bilateral", "frames": frames, "ms_per_frame": round(ms, 3),
             "fps": round(1000.0 / ms, 1), "mrays_per_s": round(2 * npix / ms / 1e3, 1),
             "trace_shapes": shapes,
-            "ao_mean": round(float(out.mean()), 4)}
+            "ao_mean": round(float(out.mean()), 4),
+            "quality_filtered_vs_1spp": {"mse": round(mse, 6), "psnr_db": round(psnr, 3),
+                                         "ssim": round(ssim, 5)},
+            "image_metrics_us": round(met_us, 2),
+            "image_metrics_gbs": round(2 * 4 * npix / (met_us * 1e3), 1)}
 
 
 def serving_leg(seed=7, tuner=None):
@@ -642,7 +657,7 @@ def main():
         "builds_per_s_4k": round(1e6 / b4k_b2b),
         "build_ms_2p5m": round(f.build_ms, 3),
         "refit_ms_2p5m": round(refit_ms, 3),
-        "frame_latency_ms_p50": round(percentile(step_ms, 0.5), 4),
+        "frame_latency_ms_p50": round(median(step_ms), 4),
         "frame_latency_ms_p99": round(percentile(step_ms, 0.99), 4),
     }
     if not a.no_extra:

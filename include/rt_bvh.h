@@ -384,6 +384,23 @@ rt_status rt_ao_resolve(rt_stream stream, const uint32_t* d_bits, const uint32_t
 rt_status rt_blur7(rt_stream stream, const float* d_in, const float* d_depth, uint32_t w,
                    uint32_t h, float depth_thr, float* d_out);
 
+/* ---- image quality (common/image_metrics.hpp:16-104 ImageMetrics::compute) --
+ * d_out = {mse, psnr, ssim} (3 doubles, device memory) of two w x h float
+ * images: mse = mean of (double(a) - double(b))^2; psnr = 100 when
+ * mse < 1e-12, else 10 log10(1 / mse); ssim = mean over the full 8 x 8 blocks
+ * of the block SSIM (double means, (n - 1)-normalised variances and
+ * covariance, C1 = 0.01^2, C2 = 0.03^2) clamped to [0, 1], 1.0 when no full
+ * block fits; no pixels: {0, 100, 1}.  d_block_ssim (optional): each block's
+ * clamped SSIM, row-major over the (w / 8) x (h / 8) blocks, equal to the
+ * reference's per-block value bit for bit.  The totals add the same terms as
+ * the reference in another (fixed) order.  Scratch: rt_image_metrics_scratch_
+ * bytes, 8-byte aligned; the call replaces ImageMetrics::compute's host
+ * loops at fullframe_benchmark.cpp:1045 and the experiment mains. */
+rt_status rt_image_metrics_scratch_bytes(uint32_t w, uint32_t h, size_t* bytes);
+rt_status rt_image_metrics(rt_stream stream, const float* d_a, const float* d_b, uint32_t w,
+                           uint32_t h, double* d_out, double* d_block_ssim, void* d_scratch,
+                           size_t scratch_bytes);
+
 /* ---- generators (synthetic inputs made on the device) -------------------- */
 rt_status rt_gen_sphere(rt_stream stream, uint32_t nu, uint32_t nv, float radius, float disp,
                         uint32_t seed, float* d_pos, uint32_t* d_idx);

@@ -1512,12 +1512,99 @@ double or_rms(const float* a, const float* b, uint64_t n) {
     return n ? std::sqrt(s / (double)n) : 0.0;
 }
 
-// common/vk_timer.cpp:104-107: sorted[floor(q * (n - 1))] (q as float).
+// common/vk_timer.cpp:105-108: sorted[floor(q * (n - 1))] (q as float).
 float or_percentile(float* v, uint32_t n, float q) {
     if (n == 0) return 0.0f;
     std::sort(v, v + n);
     size_t i = (size_t)(q * (float)(n - 1));
     return v[i];
+}
+
+// GpuTimer::runBenchmark statistics (common/vk_timer.cpp:96-121) of n
+// samples (sorted in place).  out: median = sorted[n / 2] (:103), min, max,
+// p5 and p95 = sorted[floor(q (n - 1))] with q as float (:105-108), mean =
+// float(double sum / n) (:110-111), standard deviation over n with the float
+// mean (:113-118), then p99 by the rule of p5 / p95.  n = 0: all zero.
+void or_timer_stats(float* v, uint32_t n, float out[8]) {
+    for (int i = 0; i < 8; ++i) out[i] = 0.0f;
+    if (n == 0) return;
+    std::sort(v, v + n);
+    const auto pq = [&](float q) { return v[(size_t)(q * (float)(n - 1))]; };
+    out[0] = v[n / 2];
+    out[1] = v[0];
+    out[2] = v[n - 1];
+    out[3] = pq(0.05f);
+    out[4] = pq(0.95f);
+    double sum = 0.0;
+    for (uint32_t i = 0; i < n; ++i) sum += v[i];
+    const float avg = (float)(sum / n);
+    out[5] = avg;
+    double var = 0.0;
+    for (uint32_t i = 0; i < n; ++i) {
+        const double d = v[i] - avg;
+        var += d * d;
+    }
+    out[6] = (float)std::sqrt(var / n);
+    out[7] = pq(0.99f);
+}
+
+// ImageMetrics::compute (common/image_metrics.hpp:16-104) of two w x h
+// float images.  out: mse = sum of (double(a) - double(b))^2 in pixel order
+// / pixel count; psnr = 100 when mse < 1e-12, else 10 log10(1 / mse);
+// ssim = mean over the full 8 x 8 blocks (row-major block order) of the
+// block SSIM clamped to [0, 1]: block means and (n - 1)-normalised variances
+// and covariance in double, each summed over the block's pixels in row-major
+// order, C1 = 0.01^2, C2 = 0.03^2; 1.0 when no full block fits.  No pixels:
+// mse 0, psnr 100, ssim 1.  block (optional): each block's clamped SSIM.
+void or_image_metrics(const float* a, const float* b, uint32_t w, uint32_t h, double out[3],
+                      double* block) {
+    out[0] = 0.0;
+    out[1] = 100.0;
+    out[2] = 1.0;
+    const size_t npix = (size_t)w * h;
+    if (npix == 0) return;
+    double sq = 0.0;
+    for (size_t i = 0; i < npix; ++i) {
+        const double d = (double)a[i] - (double)b[i];
+        sq += d * d;
+    }
+    out[0] = sq / (double)npix;
+    out[1] = out[0] < 1e-12 ? 100.0 : 10.0 * std::log10(1.0 / out[0]);
+    const uint32_t bw = w / 8, bh = h / 8;
+    if (bw == 0 || bh == 0) return;
+    const double c1 = (0.01 * 1.0) * (0.01 * 1.0), c2 = (0.03 * 1.0) * (0.03 * 1.0);
+    double total = 0.0;
+    for (uint32_t by = 0; by < bh; ++by)
+        for (uint32_t bx = 0; bx < bw; ++bx) {
+            const float* pa = a + (size_t)by * 8 * w + bx * 8;
+            const float* pb = b + (size_t)by * 8 * w + bx * 8;
+            double ma = 0.0, mb = 0.0;
+            for (int y = 0; y < 8; ++y)
+                for (int x = 0; x < 8; ++x) {
+                    ma += pa[(size_t)y * w + x];
+                    mb += pb[(size_t)y * w + x];
+                }
+            ma /= 64;
+            mb /= 64;
+            double va = 0.0, vb = 0.0, cab = 0.0;
+            for (int y = 0; y < 8; ++y)
+                for (int x = 0; x < 8; ++x) {
+                    const double da = pa[(size_t)y * w + x] - ma;
+                    const double db = pb[(size_t)y * w + x] - mb;
+                    va += da * da;
+                    vb += db * db;
+                    cab += da * db;
+                }
+            va /= 63;
+            vb /= 63;
+            cab /= 63;
+            const double num = (2.0 * ma * mb + c1) * (2.0 * cab + c2);
+            const double den = (ma * ma + mb * mb + c1) * (va + vb + c2);
+            const double s = std::min(std::max(num / den, 0.0), 1.0);
+            if (block) block[(size_t)by * bw + bx] = s;
+            total += s;
+        }
+    out[2] = total / (double)((size_t)bw * bh);
 }
 
 }  // extern "C"

@@ -185,6 +185,9 @@ void or_blur7(const float* in, const float* depth, uint32_t w, uint32_t h,
               float depth_thr, float* out);
 double or_rms(const float* a, const float* b, uint64_t n);
 float or_percentile(float* v, uint32_t n, float q);
+void or_timer_stats(float* v, uint32_t n, float out[8]);
+void or_image_metrics(const float* a, const float* b, uint32_t w, uint32_t h, double out[3],
+                      double* block);
 
 #ifdef __cplusplus
 }

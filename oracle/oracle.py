@@ -503,6 +503,38 @@ def percentile(v, q):
     return float(lib().or_percentile(_p(v), u32(len(v)), f32(q)))
 
 
+TIMER_FIELDS = ("median", "min", "max", "p5", "p95", "avg", "stddev", "p99")
+
+
+def timer_stats(v):
+    """GpuTimer::runBenchmark statistics (common/vk_timer.cpp:96-121) + p99."""
+    v = _f(v).reshape(-1).copy()
+    out = np.zeros(8, np.float32)
+    lib().or_timer_stats(_p(v), u32(len(v)), _p(out))
+    return dict(zip(TIMER_FIELDS, (float(x) for x in out)))
+
+
+def image_metrics(a, b, blocks=False):
+    """ImageMetrics::compute (common/image_metrics.hpp:16-104): (mse, psnr,
+    ssim[, per-block clamped SSIM]) of two (h, w) float32 images."""
+    a, b = _f(a), _f(b)
+    h, w = a.shape
+    out = np.zeros(3, np.float64)
+    blk = np.zeros(max(1, (w // 8) * (h // 8)), np.float64) if blocks else None
+    lib().or_image_metrics(_p(a), _p(b), u32(w), u32(h), _p(out), _p(blk))
+    res = tuple(float(x) for x in out)
+    return res + (blk[: (w // 8) * (h // 8)],) if blocks else res
+
+
+def ref_image_metrics(a, b):
+    """The reference's own ImageMetrics::compute (oracle/_ref)."""
+    a, b = _f(a), _f(b)
+    h, w = a.shape
+    out = np.zeros(3, np.float64)
+    ref().ref_image_metrics(_p(a), _p(b), u32(w), u32(h), _p(out))
+    return tuple(float(x) for x in out)
+
+
 # ------------------------------------------------------------ reference -----
 def ref_terrain(target, size=100.0, shipped=False):
     r = ref(shipped)

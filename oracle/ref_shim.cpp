@@ -1,11 +1,12 @@
 // ref_shim.cpp -- extern "C" entry points over the reference's own header-only
 // CPU path, compiled from the headers where they lie under the reference
 // checkout (common/software_bvh.hpp, common/mesh_gen.hpp,
-// include/radv_bvh.hpp).  Test infrastructure: builds oracle/_ref/libref.so,
+// common/image_metrics.hpp, include/radv_bvh.hpp).  Test infrastructure: builds oracle/_ref/libref.so,
 // which the tests use to pin the restatement in oracle.cpp.  Nothing here is
 // copied from the reference; it only calls the reference's functions.
 #include "software_bvh.hpp"
 #include "radv_bvh.hpp"
+#include "image_metrics.hpp"
 
 #include <cstring>
 
@@ -96,6 +97,17 @@ int ref_sizes(uint32_t* out) {
     out[6] = sizeof(rdna2::FastBvhPushConstants);
     out[7] = offsetof(rdna2::RadvBvhTriangleNode, triangle_id);
     return 8;
+}
+
+// ImageMetrics::compute (image_metrics.hpp:16): out = mse, psnr, ssim
+int ref_image_metrics(const float* a, const float* b, uint32_t w, uint32_t h, double* out) {
+    const size_t n = (size_t)w * h;
+    std::vector<float> va(a, a + n), vb(b, b + n);
+    const ImageQualityMetrics m = ImageMetrics::compute(va, vb, w, h);
+    out[0] = m.mse;
+    out[1] = m.psnr;
+    out[2] = m.ssim;
+    return 0;
 }
 
 }  // extern "C"

@@ -161,6 +161,8 @@ _SIGS = {
     "rt_gen_ao_rays_scene": [vp, vp, u32, vp, vp, vp, vp, u32, u32, u32, f32, f32, f32, vp],
     "rt_ao_resolve": [vp, vp, vp, u32, vp],
     "rt_blur7": [vp, vp, vp, u32, u32, f32, vp],
+    "rt_image_metrics_scratch_bytes": [u32, u32, vp],
+    "rt_image_metrics": [vp, vp, vp, u32, u32, vp, vp, vp, C.c_size_t],
     "rt_device_sync": [],
 }
 
@@ -753,6 +755,34 @@ def blur7(img, depth, w, h, thr=0.05, stream=None, out=None):
     call("rt_blur7", stream_handle(stream), ptr(img), ptr(depth), u32(w), u32(h), f32(thr),
          ptr(o))
     return o
+
+
+def image_metrics_scratch_bytes(w, h):
+    b = C.c_size_t(0)
+    call("rt_image_metrics_scratch_bytes", u32(w), u32(h), C.byref(b))
+    return b.value
+
+
+def image_metrics(a, b, blocks=False, stream=None, out=None, scratch=None):
+    """ImageMetrics::compute on the device (rt_image_metrics): a float64
+    device tensor {mse, psnr, ssim} of two (h, w) float32 images; with
+    blocks=True also each full 8x8 block's clamped SSIM ((h//8, w//8))."""
+    if a.shape != b.shape or a.dim() != 2:
+        raise ValueError("two (h, w) images of one shape expected")
+    if a.dtype != torch.float32 or b.dtype != torch.float32:
+        raise ValueError("float32 images expected")
+    a, b = a.contiguous(), b.contiguous()
+    h, w = a.shape
+    dev = a.device
+    res = out if out is not None else torch.empty(3, dtype=torch.float64, device=dev)
+    blk = torch.empty((h // 8, w // 8), dtype=torch.float64, device=dev) if blocks else None
+    need = image_metrics_scratch_bytes(w, h)
+    if scratch is None or scratch.numel() * scratch.element_size() < need:
+        scratch = torch.empty(max(1, need // 8), dtype=torch.float64, device=dev)
+    call("rt_image_metrics", stream_handle(stream), ptr(a), ptr(b), u32(w), u32(h), ptr(res),
+         ptr(blk) if blocks and blk.numel() else None, ptr(scratch),
+         C.c_size_t(scratch.numel() * scratch.element_size()))
+    return (res, blk) if blocks else res
 
 
 def _depth_host(nodes4):

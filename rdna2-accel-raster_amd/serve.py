@@ -46,15 +46,14 @@ import numpy as np
 import torch
 
 import rtbvh as rt
+from timing import median, percentile
 
 KINDS = ("shadow", "ao", "closest", "rebuild")
 TMAX = {"shadow": 0.5, "ao": 0.05, "closest": 1e30}
 
 
-def percentile(v, q):
-    """common/vk_timer.cpp:106-107 convention: sorted[floor(q (n - 1))]."""
-    s = sorted(v)
-    return s[int(q * (len(s) - 1))] if s else float("nan")
+# p50 = median (sorted[n // 2]) and p99 = sorted[floor(0.99 (n - 1))]: the
+# statistics of common/vk_timer.cpp:96-108 (timing.py).
 
 
 # --------------------------------------------------------------- scenes ----
@@ -283,7 +282,7 @@ def summarize(finished, wall, server=None):
     span = max(r.done for r in finished) - min(r.arrival for r in finished) if finished else 0.0
     out = {"requests": len(finished), "rays": rays,
            "rebuilds": sum(1 for r in finished if r.kind == "rebuild"),
-           "p50_ms": round(percentile(lat, 0.5), 3), "p99_ms": round(percentile(lat, 0.99), 3),
+           "p50_ms": round(median(lat), 3), "p99_ms": round(percentile(lat, 0.99), 3),
            "mean_ms": round(float(np.mean(lat)), 3) if lat else None,
            "span_s": round(span, 4), "wall_s": round(wall, 4),
            "mrays_per_s": round(rays / span / 1e6, 1) if span > 0 else None}

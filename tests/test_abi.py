@@ -299,3 +299,21 @@ def test_bvh2_depth_and_argument_checks(O):
     o = rtbvh.rt_trace_opts(1, 0, 0, 0, 0)  # refill schedule: not offered here
     assert lib.rt_trace_bvh2(None, C.byref(b), C.byref(r), 5, 0, C.byref(o), C.byref(h), None,
                              None, None) == 1
+
+
+def test_image_metrics_host_checks():
+    import rtbvh
+    lib = rtbvh.lib()
+    b = C.c_size_t(0)
+    assert lib.rt_image_metrics_scratch_bytes(1920, 1080, C.byref(b)) == 0
+    # one (squared error, SSIM) double pair per wave of 64 threads, one block each
+    assert b.value == 16 * -(-(1920 // 8) * (1080 // 8) // 64)
+    assert lib.rt_image_metrics_scratch_bytes(0, 0, C.byref(b)) == 0 and b.value == 16
+    assert lib.rt_image_metrics_scratch_bytes(4, 4, None) == 1
+    fake = C.c_void_p(256)
+    assert lib.rt_image_metrics(None, fake, fake, 8, 8, None, None, fake, 1 << 20) == 1
+    assert lib.rt_image_metrics(None, None, fake, 8, 8, fake, None, fake, 1 << 20) == 1
+    assert lib.rt_image_metrics(None, fake, fake, 8, 8, fake, None, None, 1 << 20) == 1
+    assert lib.rt_image_metrics(None, fake, fake, 8, 8, fake, None, fake, 8) == 2
+    assert lib.rt_image_metrics(None, fake, fake, 8, 8, fake, C.c_void_p(260), fake,
+                                1 << 20) == 1  # block output not 8-byte aligned
