@@ -15,6 +15,11 @@
  *   shaders/fast_bvh_builder.comp:74-83,91  FastBvhPushConstants + vkCmdDispatch(1,1,1)
  *                                           -> rt_gpu_build / rt_gpu_build_radv
  *   include/radv_bvh.hpp:193-215            compute_tree_layout -> rt_tree_layout
+ *   include/radv_bvh.hpp:311-390            dump_bvh_structure -> rt_radv_describe
+ *   rayQueryEXT over the fast_bvh_builder image -> rt_trace_radv
+ *   benchmarks/exp0_probe/shaders/software_bvh_procedural.comp:47-171
+ *                                           BVH2 traversal -> rt_trace_bvh2
+ *   common/image_metrics.hpp:16-104         ImageMetrics::compute -> rt_image_metrics
  *   benchmarks/exp0_probe/shaders/software_bvh_bvh4.comp:22-46,60-196
  *                                           SSBO traversal ABI -> rt_trace
  *   benchmarks/exp0_probe/shaders/software_bvh.comp:5-15
@@ -197,6 +202,28 @@ rt_status rt_gpu_build_radv(rt_stream stream, const float* d_pos, const uint32_t
                             uint32_t n, uint32_t order, void* d_image, uint64_t image_bytes,
                             uint32_t* d_sorted_ids);
 
+/* Decoded header of a RADV image in host memory: the RadvAccelStructHeader
+ * fields (include/radv_bvh.hpp:59-82, the header part of dump_bvh_structure,
+ * radv_bvh.hpp:311-390), the box32 node count (leaf_nodes_offset -
+ * bvh_offset) / 128 and the root node's child ids.  RT_ERR_ARG when bytes <
+ * 136 or the sections do not fit (bvh_offset not 64-aligned, the box section
+ * not a whole number of 128-B nodes, triangles past the end); the header
+ * fields are filled either way.                                             */
+typedef struct rt_radv_info {
+    uint32_t bvh_offset, root_flags;
+    float aabb[6]; /* min xyz, max xyz */
+    uint32_t update_dispatch_size[3];
+    uint32_t geometry_type, geometry_count;
+    uint32_t leaf_node_offsets_offset, build_flags, type;
+    uint64_t compacted_size, serialization_size, size;
+    uint32_t copy_dispatch_size[3];
+    uint32_t box_count;
+    uint64_t leaf_nodes_offset, primitive_count, instance_count;
+    uint32_t root_children[4];
+} rt_radv_info;
+rt_status rt_radv_describe(const void* host_image, uint64_t bytes, rt_radv_info* info);
+
+
 /* Batched build: many meshes of 1..RT_BATCH_BUILD_MAX triangles built in ONE
  * persistent launch, each with exactly the output rt_gpu_build gives it alone
  * with RT_BUILD_IMPLICIT (the fast_bvh_builder layout).  A mesh of at most
@@ -272,6 +299,29 @@ typedef struct rt_trace_opts {
 rt_status rt_trace_ex(rt_stream stream, const rt_bvh_dev* bvh, const rt_rays_soa* rays,
                       uint32_t n, uint32_t flags, const rt_trace_opts* opts, rt_hits_soa* hits,
                       uint32_t* occl_bits, uint32_t* box_tests, uint32_t* tri_tests);
+
+/* Software traversal of a RADV image in device memory: the consumer the
+ * reference's hardware ray queries are for (rayQueryEXT over the
+ * fast_bvh_builder output, src/main.cpp:871-885 then the a17 shaders), with
+ * rt_trace's predicates and near-first order over each box32 node's four
+ * slots; a triangle child is a one-triangle leaf, its triangle_id the hit id.
+ * On the image of a tree, results and counters equal rt_trace on the same
+ * tree as Box4 nodes (rt_gpu_build RT_BUILD_IMPLICIT for the Morton image).
+ * Children of other node types, and ids outside their section, are skipped.
+ * image 16-byte aligned; box_count / tri_count from rt_radv_describe (or
+ * rt_tree_layout for this library's images); depth = box levels (0 =
+ * unknown, at most 64).  opts: block and lds_stack only.                    */
+typedef struct rt_radv_dev {
+    const void* image;
+    uint64_t bytes;
+    uint32_t bvh_offset;
+    uint32_t box_count;
+    uint32_t tri_count;
+    uint32_t depth;
+} rt_radv_dev;
+rt_status rt_trace_radv(rt_stream stream, const rt_radv_dev* img, const rt_rays_soa* rays,
+                        uint32_t n, uint32_t flags, const rt_trace_opts* opts, rt_hits_soa* hits,
+                        uint32_t* occl_bits, uint32_t* box_tests, uint32_t* tri_tests);
 
 /* BVH2 traversal of a SoftwareBVH as built (software_bvh.hpp:11-22,35-175:
  * GpuBvhNode 32 B nodes, GpuTriangle 48 B triangles in leaf order), the

@@ -1459,6 +1459,61 @@ int or_radv_encode(const float* xyz9, uint32_t n, const uint32_t* order, uint8_t
     return 0;
 }
 
+// Decoder of a RADV acceleration-structure image (include/radv_bvh.hpp:22-152,
+// the header at 0, Box32 nodes from bvh_offset, triangle nodes from
+// leaf_nodes_offset) into Box4 arrays: Box32 node i (at bvh_offset + 128 i)
+// becomes Box4 node i; a child id (offset >> 3) | type (offset from
+// bvh_offset, radv_bvh.hpp:39-49) of type box32 becomes an inner slot (count
+// -1, child = offset / 128), of type triangle a one-triangle leaf slot
+// (count 1, child = ~j for the triangle node j at leaf_nodes_offset + 64 j),
+// 0xFFFFFFFF an all-zero empty slot; slot boxes are copied.  Triangle nodes
+// are copied whole (the 64-B leaf record).  Returns 1 on a malformed image
+// (sizes, alignment, a child outside its section or of another type).
+int or_radv_decode(const uint8_t* img, uint64_t bytes, or_bvh4_node* nodes, uint32_t cap,
+                   uint32_t* node_count, or_tri64* tris, uint32_t tcap, uint32_t* tri_count) {
+    if (bytes < 136) return 1;
+    uint32_t bvh;
+    uint64_t leaf, prims;
+    std::memcpy(&bvh, img + 0x00, 4);
+    std::memcpy(&leaf, img + 0x60, 8);
+    std::memcpy(&prims, img + 0x68, 8);
+    if (bvh % 64 || leaf < bvh || (leaf - bvh) % 128 || leaf + 64 * prims > bytes) return 1;
+    const uint64_t boxes = (leaf - bvh) / 128;
+    if (boxes > cap || prims > tcap) return 1;
+    for (uint64_t i = 0; i < boxes; ++i) {
+        const uint8_t* b = img + bvh + 128 * i;
+        or_bvh4_node& nd = nodes[i];
+        std::memset(&nd, 0, sizeof nd);
+        for (int c = 0; c < 4; ++c) {
+            uint32_t id;
+            float bb[6];
+            std::memcpy(&id, b + 4 * c, 4);
+            std::memcpy(bb, b + 16 + 24 * c, 24);
+            if (id == 0xFFFFFFFFu) continue;
+            const uint64_t off = (uint64_t)(id & ~7u) << 3;
+            const uint32_t type = id & 7u;
+            int32_t child, count;
+            if (type == 5) {
+                if (off % 128 || off / 128 >= boxes) return 1;
+                child = (int32_t)(off / 128);
+                count = -1;
+            } else if (type == 0) {
+                const uint64_t tbase = leaf - bvh;
+                if (off < tbase || (off - tbase) % 64 || (off - tbase) / 64 >= prims) return 1;
+                child = (int32_t)~(uint32_t)((off - tbase) / 64);
+                count = 1;
+            } else {
+                return 1;
+            }
+            set_slot(nd, c, bb, child, count);  // AABB = min xyz, max xyz
+        }
+    }
+    std::memcpy(tris, img + leaf, 64 * prims);
+    *node_count = (uint32_t)boxes;
+    *tri_count = (uint32_t)prims;
+    return 0;
+}
+
 // ---------------------------------------------------------------------------
 // Post-processing and metrics
 // ---------------------------------------------------------------------------

@@ -175,6 +175,8 @@ int or_lbvh_refit(const float* pos, const uint32_t* idx, uint32_t ntris,
                   or_bvh4_node* nodes, uint32_t node_count, or_tri64* tris);
 int or_bvh4_refit(const float* pos, const uint32_t* idx, or_bvh4_node* nodes,
                   uint32_t node_count, or_tri64* tris, uint32_t ntris);
+int or_radv_decode(const uint8_t* img, uint64_t bytes, or_bvh4_node* nodes, uint32_t cap,
+                   uint32_t* node_count, or_tri64* tris, uint32_t tcap, uint32_t* tri_count);
 int or_radv_encode(const float* xyz9, uint32_t n, const uint32_t* order,
                    uint8_t* out, uint64_t out_bytes);
 

@@ -479,6 +479,20 @@ def radv_encode(xyz9, order=None):
     return out
 
 
+def radv_decode(img):
+    """RADV image (uint8 array) -> (Box4 nodes NODE4, 64-B leaves TRI64)."""
+    img = np.ascontiguousarray(img, dtype=np.uint8).reshape(-1)
+    cap = max(1, img.size // 128)
+    nodes = np.zeros(cap, NODE4)
+    tris = np.zeros(max(1, img.size // 64), TRI64)
+    nc, nt = u32(0), u32(0)
+    rc = lib().or_radv_decode(_p(img), C.c_uint64(img.size), _p(nodes), u32(cap), C.byref(nc),
+                              _p(tris), u32(len(tris)), C.byref(nt))
+    if rc:
+        raise ValueError("malformed RADV image")
+    return nodes[: nc.value].copy(), tris[: nt.value].copy()
+
+
 # ------------------------------------------------------------- post ----------
 def ao_resolve(bits, prim):
     npix = len(prim)
@@ -524,6 +538,28 @@ def image_metrics(a, b, blocks=False):
     lib().or_image_metrics(_p(a), _p(b), u32(w), u32(h), _p(out), _p(blk))
     res = tuple(float(x) for x in out)
     return res + (blk[: (w // 8) * (h // 8)],) if blocks else res
+
+
+RADV_HEADER_FIELDS = ("bvh_offset", "root_flags", "update_dispatch_size0",
+                      "update_dispatch_size1", "update_dispatch_size2", "compacted_size",
+                      "serialization_size", "copy_dispatch_size0", "copy_dispatch_size1",
+                      "copy_dispatch_size2", "size", "geometry_type", "geometry_count",
+                      "leaf_nodes_offset", "primitive_count", "instance_count",
+                      "leaf_node_offsets_offset", "build_flags", "type")
+
+
+def ref_radv_header(img):
+    """The header and root child ids of a RADV image read through the
+    reference's own RadvAccelStructHeader / RadvBvhBox32Node (oracle/_ref)."""
+    img = np.ascontiguousarray(img, dtype=np.uint8).reshape(-1)
+    out = np.zeros(len(RADV_HEADER_FIELDS), np.uint64)
+    aabb = np.zeros(6, np.float32)
+    root = np.zeros(4, np.uint32)
+    ref().ref_radv_header(_p(img), _p(out), _p(aabb), _p(root))
+    d = dict(zip(RADV_HEADER_FIELDS, (int(x) for x in out)))
+    d["aabb"] = aabb
+    d["root_children"] = root
+    return d
 
 
 def ref_image_metrics(a, b):

@@ -99,6 +99,30 @@ int ref_sizes(uint32_t* out) {
     return 8;
 }
 
+// RadvAccelStructHeader (radv_bvh.hpp:59-82) read through the reference's
+// own struct: out = bvh_offset, root_flags, update_dispatch_size[3],
+// compacted_size, serialization_size, copy_dispatch_size[3], size,
+// geometry_type, geometry_count, leaf_nodes_offset, primitive_count,
+// instance_count, leaf_node_offsets_offset, build_flags, type; aabb = min xyz,
+// max xyz; root = the root Box32 node's child ids.
+int ref_radv_header(const uint8_t* img, uint64_t* out, float* aabb, uint32_t* root) {
+    rdna2::RadvAccelStructHeader h;
+    std::memcpy(&h, img, sizeof h);
+    const uint64_t v[] = {h.bvh_offset, h.root_flags, h.update_dispatch_size[0],
+                          h.update_dispatch_size[1], h.update_dispatch_size[2], h.compacted_size,
+                          h.serialization_size, h.copy_dispatch_size[0], h.copy_dispatch_size[1],
+                          h.copy_dispatch_size[2], h.size, h.geometry_type, h.geometry_count,
+                          h.leaf_nodes_offset, h.primitive_count, h.instance_count,
+                          h.leaf_node_offsets_offset, h.build_flags, h.type};
+    std::memcpy(out, v, sizeof v);
+    aabb[0] = h.aabb.min.x; aabb[1] = h.aabb.min.y; aabb[2] = h.aabb.min.z;
+    aabb[3] = h.aabb.max.x; aabb[4] = h.aabb.max.y; aabb[5] = h.aabb.max.z;
+    rdna2::RadvBvhBox32Node r;
+    std::memcpy(&r, img + h.bvh_offset, sizeof r);
+    for (int i = 0; i < 4; ++i) root[i] = r.children[i];
+    return (int)(sizeof v / sizeof v[0]);
+}
+
 // ImageMetrics::compute (image_metrics.hpp:16): out = mse, psnr, ssim
 int ref_image_metrics(const float* a, const float* b, uint32_t w, uint32_t h, double* out) {
     const size_t n = (size_t)w * h;

@@ -17,6 +17,7 @@
 // tuner can trade occupancy against private-memory spills; nothing is ever
 // dropped (the shader drops on overflow, :185-187): 3 * depth + 1 entries
 // bound the live stack and the private size is chosen from the tree depth.
+#include <cstring>
 #include <vector>
 
 #include "rt_internal.h"
@@ -161,16 +162,64 @@ __device__ __forceinline__ bool test_leaves(TP __restrict__ tris, const Child (&
     }
 }
 
-// One ray's traversal (the body shared by k_trace, k_trace_batch and
-// k_trace_scene).  col is the thread's LDS stack column (ls entries, stride
-// bs).  Returns with hitT / hitPrim (closest: nearest accepted hit; any:
-// first accepted hit).
-template <int PRIV, bool ANY, bool COUNT, typename NP, typename TP>
+// A RADV Box32 node (include/radv_bvh.hpp:103-116: 4 child ids, then 4
+// AABBs of min xyz, max xyz) as the four slots of a Box4 node.  A child id
+// (offset >> 3) | type, offset from the BVH base (radv_bvh.hpp:39-49), of
+// type box32 is an inner slot (count -1, child = offset / 128); of type
+// triangle a one-triangle leaf slot (count 1, child = ~j for the triangle node
+// j of the leaf section, which starts 128 boxes bytes after the base).  Empty
+// ids (0xFFFFFFFF), other types, offsets outside their section and box
+// children not after their parent (the root-first layout of every tree this
+// library and fast_bvh_builder write) are empty slots, so a malformed image
+// cannot send a load out of the image or form a cycle.
+template <typename NP>
+__device__ __forceinline__ i4v radv_node(NP p, int32_t self, uint32_t boxes, uint32_t ntris,
+                                         float (&mnx)[4],
+                                         float (&mxx)[4], float (&mny)[4], float (&mxy)[4],
+                                         float (&mnz)[4], float (&mxz)[4], int32_t (&chv)[4],
+                                         int32_t (&cnv)[4]) {
+    const auto q = f4(p);
+    const i4v ids = i4(p)[0];
+    const f4v q1 = q[1], q2 = q[2], q3 = q[3], q4 = q[4], q5 = q[5], q6 = q[6];
+    const float f[24] = {q1.x, q1.y, q1.z, q1.w, q2.x, q2.y, q2.z, q2.w,
+                         q3.x, q3.y, q3.z, q3.w, q4.x, q4.y, q4.z, q4.w,
+                         q5.x, q5.y, q5.z, q5.w, q6.x, q6.y, q6.z, q6.w};
+    const int32_t idv[4] = {ids.x, ids.y, ids.z, ids.w};
+    const uint64_t tbase = (uint64_t)boxes << 7;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        mnx[k] = f[6 * k];
+        mny[k] = f[6 * k + 1];
+        mnz[k] = f[6 * k + 2];
+        mxx[k] = f[6 * k + 3];
+        mxy[k] = f[6 * k + 4];
+        mxz[k] = f[6 * k + 5];
+        const uint32_t id = (uint32_t)idv[k], type = id & 7u;
+        const uint64_t off = (uint64_t)(id & ~7u) << 3;
+        const bool box = type == 5u && (off & 127u) == 0 && (off >> 7) < boxes &&
+                         (off >> 7) > (uint64_t)self;
+        const bool tri = type == 0u && off >= tbase && ((off - tbase) >> 6) < ntris &&
+                         (off & 63u) == 0;
+        chv[k] = box ? (int32_t)(off >> 7) : tri ? (int32_t)~(uint32_t)((off - tbase) >> 6) : 0;
+        cnv[k] = box ? -1 : tri ? 1 : 0;
+    }
+    return ids;
+}
+
+// One ray's traversal (the body shared by k_trace, k_trace_batch,
+// k_trace_scene and k_trace_radv).  col is the thread's LDS stack column (ls
+// entries, stride bs).  Returns with hitT / hitPrim (closest: nearest
+// accepted hit; any: first accepted hit).  RADV: nodes are the Box32 nodes of
+// a RADV image (rboxes of them) and tris its triangle nodes (rtris); as the
+// image is caller data, a ray also stops after rboxes pops (a tree pops each
+// node at most once) and pushes beyond the stack's capacity are dropped.
+template <int PRIV, bool ANY, bool COUNT, typename NP, typename TP, bool RADV = false>
 __device__ __forceinline__ void trace_ray(NP __restrict__ nodes, TP __restrict__ tris, bool start,
                                           float ox, float oy, float oz, float tmin, float dx,
                                           float dy, float dz, int32_t* col, int ls, int bs,
                                           float& hitT, uint32_t& hitPrim, uint32_t& boxTests,
-                                          uint32_t& triTests) {
+                                          uint32_t& triTests, uint32_t rboxes = 0,
+                                          uint32_t rtris = 0) {
     // software_bvh_bvh4.comp:96-100
     const float ix = fabsf(dx) > 1e-6f ? 1.0f / dx : 1e6f;
     const float iy = fabsf(dy) > 1e-6f ? 1.0f / dy : 1e6f;
@@ -178,24 +227,34 @@ __device__ __forceinline__ void trace_ray(NP __restrict__ nodes, TP __restrict__
     Stack<PRIV> st;
     int sp = 0;
     if (start) RT_SPUSH(0);
+    uint32_t pops = 0;
 
     while (sp > 0) {
+        if (RADV && ++pops > rboxes) break;
         int32_t ni;
         RT_SPOP(ni);
-        const auto np = f4(nodes + ni);
-        const f4v mnx = np[0], mxx = np[1], mny = np[2], mxy = np[3], mnz = np[4], mxz = np[5];
-        const i4v ch = i4(nodes + ni)[6];
-        const i4v cn = i4(nodes + ni)[7];
+        float bminx[4], bmaxx[4], bminy[4], bmaxy[4], bminz[4], bmaxz[4];
+        int32_t chv[4], cnv[4];
+        i4v ch;
+        if constexpr (RADV) {
+            ch = radv_node(nodes + ni, ni, rboxes, rtris, bminx, bmaxx, bminy, bmaxy, bminz,
+                           bmaxz, chv, cnv);
+        } else {
+            const auto np = f4(nodes + ni);
+            const f4v mnx = np[0], mxx = np[1], mny = np[2], mxy = np[3], mnz = np[4],
+                      mxz = np[5];
+            ch = i4(nodes + ni)[6];
+            const i4v cn = i4(nodes + ni)[7];
+            bminx[0] = mnx.x; bminx[1] = mnx.y; bminx[2] = mnx.z; bminx[3] = mnx.w;
+            bmaxx[0] = mxx.x; bmaxx[1] = mxx.y; bmaxx[2] = mxx.z; bmaxx[3] = mxx.w;
+            bminy[0] = mny.x; bminy[1] = mny.y; bminy[2] = mny.z; bminy[3] = mny.w;
+            bmaxy[0] = mxy.x; bmaxy[1] = mxy.y; bmaxy[2] = mxy.z; bmaxy[3] = mxy.w;
+            bminz[0] = mnz.x; bminz[1] = mnz.y; bminz[2] = mnz.z; bminz[3] = mnz.w;
+            bmaxz[0] = mxz.x; bmaxz[1] = mxz.y; bmaxz[2] = mxz.z; bmaxz[3] = mxz.w;
+            chv[0] = ch.x; chv[1] = ch.y; chv[2] = ch.z; chv[3] = ch.w;
+            cnv[0] = cn.x; cnv[1] = cn.y; cnv[2] = cn.z; cnv[3] = cn.w;
+        }
         if (COUNT) boxTests += 4;
-
-        const float bminx[4] = {mnx.x, mnx.y, mnx.z, mnx.w};
-        const float bmaxx[4] = {mxx.x, mxx.y, mxx.z, mxx.w};
-        const float bminy[4] = {mny.x, mny.y, mny.z, mny.w};
-        const float bmaxy[4] = {mxy.x, mxy.y, mxy.z, mxy.w};
-        const float bminz[4] = {mnz.x, mnz.y, mnz.z, mnz.w};
-        const float bmaxz[4] = {mxz.x, mxz.y, mxz.z, mxz.w};
-        const int32_t chv[4] = {ch.x, ch.y, ch.z, ch.w};
-        const int32_t cnv[4] = {cn.x, cn.y, cn.z, cn.w};
 
         float te[4];
         bool hit[4];
@@ -234,7 +293,7 @@ __device__ __forceinline__ void trace_ray(NP __restrict__ nodes, TP __restrict__
         uint32_t pend = 0;
 #pragma unroll
         for (int p = 0; p < 4; ++p) {
-            if (e[p].count < 0) RT_SPUSH(e[p].child);
+            if (e[p].count < 0 && (!RADV || sp < ls + PRIV)) RT_SPUSH(e[p].child);
             pend |= (e[p].count > 0 ? 1u : 0u) << p;
         }
         if (pend && test_leaves<ANY, COUNT>(tris, e, pend, ox, oy, oz, dx, dy, dz, tmin, hitT,
@@ -281,6 +340,47 @@ __global__ __launch_bounds__(kBlock) void k_trace(const rt_node4* __restrict__ n
     trace_ray<PRIV, ANY, COUNT>(nodes, tris, valid && node_count > 0, ox, oy, oz, tmin, dx, dy,
                                 dz, lds_stack + tid, (int)ls, (int)bs, hitT, hitPrim, boxTests,
                                 triTests);
+    if (ANY) {
+        write_wave_bits(bits_out, r - (tid & 63u), n, valid && hitPrim != RT_MISS);
+        if (valid && prim_out) prim_out[r] = hitPrim;
+    } else if (valid) {
+        prim_out[r] = hitPrim;
+        t_out[r] = hitT;
+    }
+    if (COUNT && valid) {
+        box_out[r] = boxTests;
+        tri_out[r] = triTests;
+    }
+}
+
+// k_trace over a RADV image: nodes = the Box32 nodes (image + bvh_offset),
+// tris = the triangle nodes (image + leaf_nodes_offset).
+template <int PRIV, bool ANY, bool COUNT>
+__global__ __launch_bounds__(kBlock) void k_trace_radv(const rt_node4* __restrict__ nodes,
+                                                       const rt_tri64* __restrict__ tris,
+                                                       uint32_t boxes, uint32_t ntris,
+                                                       rt_rays_soa rays, uint32_t n, uint32_t ls,
+                                                       uint32_t* __restrict__ prim_out,
+                                                       float* __restrict__ t_out,
+                                                       uint32_t* __restrict__ bits_out,
+                                                       uint32_t* __restrict__ box_out,
+                                                       uint32_t* __restrict__ tri_out) {
+    extern __shared__ int32_t lds_stack[];
+    const uint32_t tid = threadIdx.x, bs = blockDim.x;
+    const uint32_t r = blockIdx.x * bs + tid;
+    const bool valid = r < n;
+    float ox = 0.0f, oy = 0.0f, oz = 0.0f, tmin = 1.0f;
+    float dx = 0.0f, dy = 0.0f, dz = 1.0f, tmax = 0.0f;
+    if (valid) {
+        ox = rays.ox[r]; oy = rays.oy[r]; oz = rays.oz[r]; tmin = rays.tmin[r];
+        dx = rays.dx[r]; dy = rays.dy[r]; dz = rays.dz[r]; tmax = rays.tmax[r];
+    }
+    float hitT = tmax;
+    uint32_t hitPrim = RT_MISS;
+    uint32_t boxTests = 0, triTests = 0;
+    trace_ray<PRIV, ANY, COUNT, const rt_node4*, const rt_tri64*, true>(
+        nodes, tris, valid && boxes > 0, ox, oy, oz, tmin, dx, dy, dz, lds_stack + tid, (int)ls,
+        (int)bs, hitT, hitPrim, boxTests, triTests, boxes, ntris);
     if (ANY) {
         write_wave_bits(bits_out, r - (tid & 63u), n, valid && hitPrim != RT_MISS);
         if (valid && prim_out) prim_out[r] = hitPrim;
@@ -825,6 +925,97 @@ extern "C" rt_status rt_trace_ex(rt_stream stream, const rt_bvh_dev* bvh, const 
                                     tri_tests);
     return launch_v<kPrivLarge>(s, bvh, rays, n, flags, cfg, hits, occl_bits, box_tests,
                                 tri_tests);
+}
+
+extern "C" rt_status rt_radv_describe(const void* host_image, uint64_t bytes,
+                                     rt_radv_info* info) {
+    if (!host_image || !info) return RT_ERR_ARG;
+    if (bytes < 136) return RT_ERR_ARG;
+    const uint8_t* b = static_cast<const uint8_t*>(host_image);
+    rt_radv_info o{};
+    const auto g32 = [&](size_t off) { uint32_t v; std::memcpy(&v, b + off, 4); return v; };
+    const auto g64 = [&](size_t off) { uint64_t v; std::memcpy(&v, b + off, 8); return v; };
+    // include/radv_bvh.hpp:59-82 RadvAccelStructHeader field offsets
+    o.bvh_offset = g32(0x00);
+    o.root_flags = g32(0x04);
+    std::memcpy(o.aabb, b + 0x08, 24);
+    for (int i = 0; i < 3; ++i) o.update_dispatch_size[i] = g32(0x20 + 4 * i);
+    o.compacted_size = g64(0x30);
+    o.serialization_size = g64(0x38);
+    for (int i = 0; i < 3; ++i) o.copy_dispatch_size[i] = g32(0x40 + 4 * i);
+    o.size = g64(0x50);
+    o.geometry_type = g32(0x58);
+    o.geometry_count = g32(0x5C);
+    o.leaf_nodes_offset = g64(0x60);
+    o.primitive_count = g64(0x68);
+    o.instance_count = g64(0x70);
+    o.leaf_node_offsets_offset = g32(0x78);
+    o.build_flags = g32(0x7C);
+    o.type = g32(0x80);
+    *info = o;
+    if (o.bvh_offset % 64 || o.leaf_nodes_offset < o.bvh_offset ||
+        (o.leaf_nodes_offset - o.bvh_offset) % 128 ||
+        (o.leaf_nodes_offset - o.bvh_offset) / 128 > 0xFFFFFFFFull ||
+        o.primitive_count > 0xFFFFFFFFull ||
+        o.leaf_nodes_offset + 64 * o.primitive_count > bytes)
+        return RT_ERR_ARG;
+    info->box_count = (uint32_t)((o.leaf_nodes_offset - o.bvh_offset) / 128);
+    if (o.bvh_offset + 128 <= bytes && info->box_count > 0)
+        for (int i = 0; i < 4; ++i) info->root_children[i] = g32(o.bvh_offset + 4 * i);
+    return RT_OK;
+}
+
+extern "C" rt_status rt_trace_radv(rt_stream stream, const rt_radv_dev* img,
+                                   const rt_rays_soa* rays, uint32_t n, uint32_t flags,
+                                   const rt_trace_opts* opts, rt_hits_soa* hits,
+                                   uint32_t* occl_bits, uint32_t* box_tests,
+                                   uint32_t* tri_tests) {
+    if (!img || !rays) return RT_ERR_ARG;
+    if (flags & ~(RT_ANY | RT_COUNTERS)) return RT_ERR_ARG;
+    TraceCfg cfg{false, 0u, 0u, (uint32_t)kBlock, kLdsStack};
+    if (opts && opts->schedule != RT_SCHED_SIMPLE) return RT_ERR_ARG;
+    if (!shape_opts(opts, cfg)) return RT_ERR_ARG;
+    // the sections must lie inside the image, 16-byte aligned for the loads
+    const uint64_t leaf = (uint64_t)img->bvh_offset + 128ull * img->box_count;
+    if (leaf + 64ull * img->tri_count > img->bytes) return RT_ERR_CAPACITY;
+    if (img->box_count && (!img->image || img->bvh_offset % 64 ||
+                           reinterpret_cast<uintptr_t>(img->image) % 16))
+        return RT_ERR_ARG;
+    if (n == 0) return RT_OK;
+    if (!rays->ox || !rays->oy || !rays->oz || !rays->tmin || !rays->dx || !rays->dy ||
+        !rays->dz || !rays->tmax)
+        return RT_ERR_ARG;
+    const bool any = flags & RT_ANY, cnt = flags & RT_COUNTERS;
+    if (any && !occl_bits) return RT_ERR_ARG;
+    if (!any && (!hits || !hits->prim || !hits->t)) return RT_ERR_ARG;
+    if (cnt && (!box_tests || !tri_tests)) return RT_ERR_ARG;
+    const uint32_t depth = img->depth ? img->depth : kMaxDepth;
+    if (depth > kMaxDepth) return RT_ERR_CAPACITY;
+    const uint8_t* base = static_cast<const uint8_t*>(img->image);
+    const rt_node4* nd = reinterpret_cast<const rt_node4*>(base + img->bvh_offset);
+    const rt_tri64* tr = reinterpret_cast<const rt_tri64*>(base + leaf);
+    uint32_t* prim = hits ? hits->prim : nullptr;
+    float* t = hits ? hits->t : nullptr;
+    const dim3 grid(ceil_div(n, cfg.block)), block(cfg.block);
+    const size_t shm = lds_bytes(cfg);
+    hipStream_t s = as_stream(stream);
+#define RT_R(P, A, C)                                                                          \
+    hipLaunchKernelGGL((k_trace_radv<P, A, C>), grid, block, shm, s, nd, tr, img->box_count,   \
+                       img->tri_count, *rays, n, cfg.lds, prim, t, occl_bits, box_tests,       \
+                       tri_tests)
+    if (small_priv(depth, cfg.lds)) {
+        if (any && cnt) RT_R(kPrivSmall, true, true);
+        else if (any) RT_R(kPrivSmall, true, false);
+        else if (cnt) RT_R(kPrivSmall, false, true);
+        else RT_R(kPrivSmall, false, false);
+    } else {
+        if (any && cnt) RT_R(kPrivLarge, true, true);
+        else if (any) RT_R(kPrivLarge, true, false);
+        else if (cnt) RT_R(kPrivLarge, false, true);
+        else RT_R(kPrivLarge, false, false);
+    }
+#undef RT_R
+    return launch_status("k_trace_radv");
 }
 
 extern "C" rt_status rt_bvh2_depth(const void* nodes, uint32_t node_count, uint32_t* depth) {

@@ -62,6 +62,22 @@ class rt_bvh2_dev(C.Structure):
                 ("tri_count", u32), ("depth", u32), ("reserved", u32)]
 
 
+class rt_radv_info(C.Structure):
+    _fields_ = [("bvh_offset", u32), ("root_flags", u32), ("aabb", C.c_float * 6),
+                ("update_dispatch_size", u32 * 3), ("geometry_type", u32),
+                ("geometry_count", u32), ("leaf_node_offsets_offset", u32),
+                ("build_flags", u32), ("type", u32), ("compacted_size", C.c_uint64),
+                ("serialization_size", C.c_uint64), ("size", C.c_uint64),
+                ("copy_dispatch_size", u32 * 3), ("box_count", u32),
+                ("leaf_nodes_offset", C.c_uint64), ("primitive_count", C.c_uint64),
+                ("instance_count", C.c_uint64), ("root_children", u32 * 4)]
+
+
+class rt_radv_dev(C.Structure):
+    _fields_ = [("image", vp), ("bytes", C.c_uint64), ("bvh_offset", u32), ("box_count", u32),
+                ("tri_count", u32), ("depth", u32)]
+
+
 class rt_trace_item(C.Structure):
     _fields_ = [("bvh", rt_bvh_dev), ("rays", rt_rays_soa), ("count", u32), ("flags", u32),
                 ("hits", rt_hits_soa), ("occlusion_bits", vp)]
@@ -135,6 +151,8 @@ _SIGS = {
     "rt_convert_tris48": [vp, vp, vp, u32, vp],
     "rt_trace": [vp, vp, vp, u32, u32, vp, vp, vp, vp],
     "rt_trace_ex": [vp, vp, vp, u32, u32, vp, vp, vp, vp, vp],
+    "rt_radv_describe": [vp, C.c_uint64, vp],
+    "rt_trace_radv": [vp, vp, vp, u32, u32, vp, vp, vp, vp, vp],
     "rt_bvh2_depth": [vp, u32, vp],
     "rt_trace_bvh2": [vp, vp, vp, u32, u32, vp, vp, vp, vp, vp],
     "rt_gen_sphere": [vp, u32, u32, f32, f32, u32, vp, vp],
@@ -755,6 +773,97 @@ def blur7(img, depth, w, h, thr=0.05, stream=None, out=None):
     call("rt_blur7", stream_handle(stream), ptr(img), ptr(depth), u32(w), u32(h), f32(thr),
          ptr(o))
     return o
+
+
+RADV_NODE_TYPES = {0: "triangle", 4: "box16", 5: "box32", 6: "instance", 7: "aabb"}
+
+
+def radv_describe(image):
+    """Decoded header of a RADV image (rt_radv_describe; the header part of
+    dump_bvh_structure, include/radv_bvh.hpp:311-390): a dict of the
+    RadvAccelStructHeader fields, box_count and the root's children as
+    (id, type name, offset from the BVH base) or None for an empty slot.
+    image: uint8 numpy array or tensor (a device tensor is copied).  Raises
+    RtError when the sections do not fit the image."""
+    if isinstance(image, torch.Tensor):
+        image = image.cpu().numpy()
+    b = np.ascontiguousarray(image, dtype=np.uint8).reshape(-1)
+    info = rt_radv_info()
+    call("rt_radv_describe", b.ctypes.data_as(vp), C.c_uint64(b.size), C.byref(info))
+    d = {name: getattr(info, name) for name, _ in rt_radv_info._fields_}
+    for k in ("aabb", "update_dispatch_size", "copy_dispatch_size", "root_children"):
+        d[k] = list(d[k])
+    d["root"] = [None if c == MISS else (c, RADV_NODE_TYPES.get(c & 7, "?"), (c & ~7) << 3)
+                 for c in d["root_children"]]
+    return d
+
+
+def format_radv(image):
+    """radv_describe as text: header offsets and values, root children."""
+    d = radv_describe(image)
+    lines = [f"bvh_offset 0x{d['bvh_offset']:x}  root_flags 0x{d['root_flags']:x}",
+             f"aabb min {tuple(d['aabb'][:3])} max {tuple(d['aabb'][3:])}",
+             f"size {d['size']}  compacted {d['compacted_size']}  serialization "
+             f"{d['serialization_size']}",
+             f"geometry_type {d['geometry_type']}  geometry_count {d['geometry_count']}  "
+             f"type {d['type']}  build_flags 0x{d['build_flags']:x}",
+             f"leaf_nodes_offset 0x{d['leaf_nodes_offset']:x}  primitive_count "
+             f"{d['primitive_count']}  box nodes {d['box_count']}"]
+    for i, c in enumerate(d["root"]):
+        lines.append(f"root child {i}: " + ("empty" if c is None else
+                                            f"id 0x{c[0]:08x} {c[1]} at +0x{c[2]:x}"))
+    return "\n".join(lines)
+
+
+class RadvBVH:
+    """A RADV image in device memory traversed in software (rt_trace_radv):
+    closest / any hit with rt_trace's predicates; hits report triangle_id."""
+
+    def __init__(self, image, info=None, depth=None):
+        if image.dtype != torch.uint8 or not image.is_cuda:
+            raise ValueError("a uint8 device tensor expected")
+        self.image = image.contiguous()
+        self.info = info if info is not None else radv_describe(self.image)
+        n = self.info["primitive_count"]
+        if depth is None:
+            # images of this library / fast_bvh_builder: compute_tree_layout levels
+            counts, _, tb, _ = tree_layout(n)
+            depth = len(counts) if tb == self.info["box_count"] else 0
+        self.depth = depth
+
+    def dev(self):
+        return rt_radv_dev(ptr(self.image), C.c_uint64(self.image.numel()),
+                           u32(self.info["bvh_offset"]), u32(self.info["box_count"]),
+                           u32(self.info["primitive_count"]), u32(self.depth))
+
+    def _trace(self, rays, flags, hits, bits, counters, stream, tune):
+        n = len(rays)
+        box = tri = None
+        if counters:
+            box = torch.empty(n, dtype=torch.int32, device=rays.data.device)
+            tri = torch.empty(n, dtype=torch.int32, device=rays.data.device)
+            flags |= RT_COUNTERS
+        d = self.dev()
+        r = rays.soa()
+        o = trace_opts("simple", **tune)
+        call("rt_trace_radv", stream_handle(stream), C.byref(d), C.byref(r), u32(n), u32(flags),
+             C.byref(o), C.byref(hits) if hits is not None else None, ptr(bits), ptr(box),
+             ptr(tri))
+        return box, tri
+
+    def intersect(self, rays, counters=False, stream=None, **tune):
+        n = len(rays)
+        prim = torch.empty(n, dtype=torch.int32, device=rays.data.device)
+        t = torch.empty(n, dtype=torch.float32, device=rays.data.device)
+        box, tri = self._trace(rays, RT_CLOSEST, rt_hits_soa(ptr(prim), ptr(t)), None, counters,
+                               stream, tune)
+        return (prim, t, box, tri) if counters else (prim, t)
+
+    def occluded(self, rays, counters=False, stream=None, **tune):
+        n = len(rays)
+        bits = torch.empty(max(1, (n + 31) // 32), dtype=torch.int32, device=rays.data.device)
+        box, tri = self._trace(rays, RT_ANY, None, bits, counters, stream, tune)
+        return (bits, box, tri) if counters else bits
 
 
 def image_metrics_scratch_bytes(w, h):
