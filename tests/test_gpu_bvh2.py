@@ -28,12 +28,14 @@ def terrain2(O):
     return n2, t48, order
 
 
-@pytest.mark.parametrize("coherent", [True, False])
-def test_bvh2_exp0_rays_match_oracle(rt, O, terrain2, coherent):
+@pytest.mark.parametrize("coherent,tmin", [(True, 0.01), (False, 0.01), (False, 0.001)])
+def test_bvh2_exp0_rays_match_oracle(rt, O, terrain2, coherent, tmin):
+    """tmin 0.01: software_bvh_procedural.comp; 0.001: its LDS-stack variant
+    software_bvh_lds.comp (the same loop, t > 0.001)."""
     n2, t48, order = terrain2
     g = rt.GpuBVH2.from_bvh2(n2, t48, order)
     assert 10 <= g.depth <= 64
-    r = O.exp0_rays(1 << 16, coherent)
+    r = O.exp0_rays(1 << 16, coherent, tmin=tmin)
     rays = rt.Rays.from_numpy(r.packed())
     prim, t, box, tri = g.intersect(rays, counters=True)
     bits, abox, atri = g.occluded(rays, counters=True)
