@@ -285,16 +285,28 @@ rt_status rt_trace(rt_stream stream, const rt_bvh_dev* bvh, const rt_rays_soa* r
  *   lds_stack        traversal-stack entries per lane kept in LDS, 4..48
  *                    (0 = 20); deeper entries spill to private memory.  More
  *                    entries mean fewer spills and fewer resident waves.
+ *   any_order        (rt_trace_ex, RT_ANY without RT_COUNTERS or hit ids,
+ *                    RT_SCHED_SIMPLE) the order children are visited in:
+ *                    RT_ORDER_NEAR (0, the reference's near-first order),
+ *                    RT_ORDER_FAR (farthest hit child first), RT_ORDER_SLOT
+ *                    (slot order, no sort).  The occlusion bits are the same
+ *                    for every order (until an accepted hit ends a ray, every
+ *                    node whose box meets [tmin, tmax] is visited); only the
+ *                    work differs.  Other entry points take 0 only.
  * opts = NULL: RT_SCHED_SIMPLE (RT_TRACE_PERSISTENT in flags: RT_SCHED_REFILL). */
 #define RT_SCHED_SIMPLE 0u
 #define RT_SCHED_REFILL 1u
+#define RT_ORDER_NEAR 0u
+#define RT_ORDER_FAR 1u
+#define RT_ORDER_SLOT 2u
 typedef struct rt_trace_opts {
     uint32_t schedule;
     uint32_t rays_per_wave; /* multiple of 64, at most 4096; 0 = 1024 */
     uint32_t refill_lanes;  /* 1..64; 0 = 16 */
     uint32_t block;         /* 64 / 128 / 256; 0 = 256 */
     uint32_t lds_stack;     /* 4..48; 0 = 20 */
-    uint32_t reserved[3];
+    uint32_t any_order;     /* RT_ORDER_*; 0 = near first */
+    uint32_t reserved[2];
 } rt_trace_opts;
 rt_status rt_trace_ex(rt_stream stream, const rt_bvh_dev* bvh, const rt_rays_soa* rays,
                       uint32_t n, uint32_t flags, const rt_trace_opts* opts, rt_hits_soa* hits,

@@ -158,7 +158,11 @@ class Tuner:
         refill = [{"schedule": "refill", "block": best["block"],
                    "lds_stack": best["lds_stack"], "rays_per_wave": w, "refill_lanes": r}
                   for w in REFILL_RAYS_PER_WAVE for r in REFILL_LANES]
-        return self._pick(key, [best] + refill, run, check)
+        # occlusion bits do not depend on the child order: far-first and slot
+        # order at the best simple shape compete too
+        orders = [] if mode == "closest" else [
+            dict(best, any_order=o) for o in rt.ANY_ORDERS if o != "near"]
+        return self._pick(key, [best] + refill + orders, run, check)
 
     def trace_scene(self, scene, rays, mode, tag=""):
         """Launch shape (block, lds_stack) for Scene.intersect / occluded."""

@@ -37,6 +37,21 @@ struct Child {
     int32_t count;
 };
 
+// Order constants of an any-hit query that returns only occlusion bits
+// (rt_trace_opts.any_order): near-first (the reference's), far-first, slot.
+constexpr int kOrderNear = 0, kOrderFar = 1, kOrderSlot = 2;
+
+// Compare-exchange putting the smaller key first (the far-first order's
+// ascending sort; pushes then go nearest first, so the farthest pops first).
+__device__ __forceinline__ void cx_up(Child& a, Child& b) {
+    const bool s = b.key < a.key;
+    const float ka = s ? b.key : a.key, kb = s ? a.key : b.key;
+    const int32_t ca = s ? b.child : a.child, cb = s ? a.child : b.child;
+    const int32_t na = s ? b.count : a.count, nb = s ? a.count : b.count;
+    a = Child{ka, ca, na};
+    b = Child{kb, cb, nb};
+}
+
 __device__ __forceinline__ void cx(Child& a, Child& b) {
     // software_bvh_bvh4.comp:151: swap when st[i] < st[j].  Written as
     // selects (one compare, two v_cndmask per field) rather than a branch:
@@ -213,7 +228,13 @@ __device__ __forceinline__ i4v radv_node(NP p, int32_t self, uint32_t boxes, uin
 // a RADV image (rboxes of them) and tris its triangle nodes (rtris); as the
 // image is caller data, a ray also stops after rboxes pops (a tree pops each
 // node at most once) and pushes beyond the stack's capacity are dropped.
-template <int PRIV, bool ANY, bool COUNT, typename NP, typename TP, bool RADV = false>
+// ORDER (any hit without counters or hit ids only): kOrderNear is the
+// reference's order; kOrderFar pops the farthest hit child first; kOrderSlot
+// skips the sort.  The occlusion bit of a ray does not depend on the order:
+// until an accepted hit ends the ray, every node whose box meets [tmin, tmax]
+// is visited, whatever the order.
+template <int PRIV, bool ANY, bool COUNT, typename NP, typename TP, bool RADV = false,
+          int ORDER = kOrderNear>
 __device__ __forceinline__ void trace_ray(NP __restrict__ nodes, TP __restrict__ tris, bool start,
                                           float ox, float oy, float oz, float tmin, float dx,
                                           float dy, float dz, int32_t* col, int ls, int bs,
@@ -282,9 +303,18 @@ __device__ __forceinline__ void trace_ray(NP __restrict__ nodes, TP __restrict__
                           : Child{-INFINITY, 0, 0};
         keep_loaded(ch);
         if (!(hit[0] || hit[1] || hit[2] || hit[3])) continue;
-        cx(e[0], e[1]); cx(e[0], e[2]); cx(e[0], e[3]);
-        cx(e[1], e[2]); cx(e[1], e[3]);
-        cx(e[2], e[3]);
+        if constexpr (ORDER == kOrderNear) {
+            cx(e[0], e[1]); cx(e[0], e[2]); cx(e[0], e[3]);
+            cx(e[1], e[2]); cx(e[1], e[3]);
+            cx(e[2], e[3]);
+        } else if constexpr (ORDER == kOrderFar) {
+            static_assert(ANY && !COUNT, "another order is for occlusion bits only");
+            cx_up(e[0], e[1]); cx_up(e[0], e[2]); cx_up(e[0], e[3]);
+            cx_up(e[1], e[2]); cx_up(e[1], e[3]);
+            cx_up(e[2], e[3]);
+        } else {
+            static_assert(ANY && !COUNT, "another order is for occlusion bits only");
+        }
         // :158-189 in sorted order: inner children pushed, leaf children
         // tested (padding, count 0, skipped).  The pushes do not depend on
         // the leaf tests, so all are made first and the leaves then run as
@@ -313,7 +343,7 @@ __device__ __forceinline__ void write_wave_bits(BP bits_out, uint32_t r0, uint32
     if (lane == 32 && r0 + 32 < n) bits_out[(r0 >> 5) + 1] = (uint32_t)(m >> 32);
 }
 
-template <int PRIV, bool ANY, bool COUNT>
+template <int PRIV, bool ANY, bool COUNT, int ORDER = kOrderNear>
 __global__ __launch_bounds__(kBlock) void k_trace(const rt_node4* __restrict__ nodes,
                                                   const rt_tri64* __restrict__ tris,
                                                   uint32_t node_count, rt_rays_soa rays,
@@ -337,9 +367,9 @@ __global__ __launch_bounds__(kBlock) void k_trace(const rt_node4* __restrict__ n
     float hitT = tmax;
     uint32_t hitPrim = RT_MISS;
     uint32_t boxTests = 0, triTests = 0;
-    trace_ray<PRIV, ANY, COUNT>(nodes, tris, valid && node_count > 0, ox, oy, oz, tmin, dx, dy,
-                                dz, lds_stack + tid, (int)ls, (int)bs, hitT, hitPrim, boxTests,
-                                triTests);
+    trace_ray<PRIV, ANY, COUNT, const rt_node4*, const rt_tri64*, false, ORDER>(
+        nodes, tris, valid && node_count > 0, ox, oy, oz, tmin, dx, dy, dz, lds_stack + tid,
+        (int)ls, (int)bs, hitT, hitPrim, boxTests, triTests);
     if (ANY) {
         write_wave_bits(bits_out, r - (tid & 63u), n, valid && hitPrim != RT_MISS);
         if (valid && prim_out) prim_out[r] = hitPrim;
@@ -830,15 +860,19 @@ struct TraceCfg {
     uint32_t refill_lanes;  // 1..64
     uint32_t block;         // 64, 128 or 256
     uint32_t lds;           // LDS stack entries per lane, kLdsMin..kLdsMax
+    uint32_t order;         // any-hit child order (kOrderNear / Far / Slot)
 };
 
-// Checks the launch-shape fields of opts (0 = default) into cfg.
-bool shape_opts(const rt_trace_opts* o, TraceCfg& cfg) {
+// Checks the launch-shape fields of opts (0 = default) into cfg; any_order
+// only where the entry point offers it (with_order).
+bool shape_opts(const rt_trace_opts* o, TraceCfg& cfg, bool with_order = false) {
     if (!o) return true;
     if (o->block) cfg.block = o->block;
     if (o->lds_stack) cfg.lds = o->lds_stack;
+    if (o->any_order && !with_order) return false;
+    cfg.order = o->any_order;
     return (cfg.block == 64 || cfg.block == 128 || cfg.block == 256) && cfg.lds >= kLdsMin &&
-           cfg.lds <= kLdsMax;
+           cfg.lds <= kLdsMax && cfg.order <= (uint32_t)kOrderSlot;
 }
 
 size_t lds_bytes(const TraceCfg& cfg) { return (size_t)cfg.lds * cfg.block * 4; }
@@ -875,6 +909,17 @@ rt_status launch_v(hipStream_t s, const rt_bvh_dev* b, const rt_rays_soa* rays, 
         return launch_status("k_trace_persist");
     }
     dim3 grid(ceil_div(n, cfg.block)), block(cfg.block);
+    if (cfg.order == kOrderFar) {
+        hipLaunchKernelGGL((k_trace<PRIV, true, false, kOrderFar>), grid, block, shm, s, b->nodes,
+                           b->tris, b->node_count, *rays, n, cfg.lds, prim, t, bits, box, tri);
+        return launch_status("k_trace");
+    }
+    if (cfg.order == kOrderSlot) {
+        hipLaunchKernelGGL((k_trace<PRIV, true, false, kOrderSlot>), grid, block, shm, s,
+                           b->nodes, b->tris, b->node_count, *rays, n, cfg.lds, prim, t, bits,
+                           box, tri);
+        return launch_status("k_trace");
+    }
 #define RT_L(A, C)                                                                            \
     hipLaunchKernelGGL((k_trace<PRIV, A, C>), grid, block, shm, s, b->nodes, b->tris,         \
                        b->node_count, *rays, n, cfg.lds, prim, t, bits, box, tri)
@@ -904,8 +949,13 @@ extern "C" rt_status rt_trace_ex(rt_stream stream, const rt_bvh_dev* bvh, const 
         if (opts->rays_per_wave) cfg.per_wave = opts->rays_per_wave;
         if (opts->refill_lanes) cfg.refill_lanes = opts->refill_lanes;
     }
-    if (!shape_opts(opts, cfg)) return RT_ERR_ARG;
+    if (!shape_opts(opts, cfg, true)) return RT_ERR_ARG;
     if (cfg.per_wave % 64 || cfg.per_wave > kMaxPerWave || cfg.refill_lanes > 64)
+        return RT_ERR_ARG;
+    // another child order: occlusion bits only (no counters, no hit ids), one
+    // ray per thread
+    if (cfg.order != kOrderNear &&
+        (!(flags & RT_ANY) || (flags & RT_COUNTERS) || (hits && hits->prim) || cfg.refill))
         return RT_ERR_ARG;
     if (n == 0) return RT_OK;
     if (!rays->ox || !rays->oy || !rays->oz || !rays->tmin || !rays->dx || !rays->dy ||

@@ -113,7 +113,12 @@ BUILD_ALGOS = {"auto": 0, "radix": 1, "implicit": 2}
 
 class rt_trace_opts(C.Structure):
     _fields_ = [("schedule", u32), ("rays_per_wave", u32), ("refill_lanes", u32),
-                ("block", u32), ("lds_stack", u32), ("reserved", u32 * 3)]
+                ("block", u32), ("lds_stack", u32), ("any_order", u32), ("reserved", u32 * 2)]
+
+
+# Child order of an any-hit query returning occlusion bits only (same bits
+# for every order; the work differs).
+ANY_ORDERS = {"near": 0, "far": 1, "slot": 2}
 
 
 # Traversal schedule used when a call does not choose one: "simple" (one ray
@@ -128,7 +133,7 @@ LDS_STACK_RANGE = (4, 48)
 
 
 def trace_opts(schedule=None, rays_per_wave=None, refill_lanes=None, block=None,
-               lds_stack=None):
+               lds_stack=None, any_order=None):
     s = schedule or DEFAULT_SCHEDULE
     if s not in ("simple", "refill"):
         raise ValueError(f"schedule {s!r}")
@@ -136,7 +141,8 @@ def trace_opts(schedule=None, rays_per_wave=None, refill_lanes=None, block=None,
                          rays_per_wave or TRACE_TUNING["rays_per_wave"],
                          refill_lanes or TRACE_TUNING["refill_lanes"],
                          block or TRACE_TUNING["block"],
-                         lds_stack or TRACE_TUNING["lds_stack"])
+                         lds_stack or TRACE_TUNING["lds_stack"],
+                         ANY_ORDERS[any_order or "near"])
 
 
 _lib = None

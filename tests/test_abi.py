@@ -205,6 +205,10 @@ def test_tuner_search_space_is_valid():
                     o = rtbvh.trace_opts("refill", w, lanes, blk, lds)
                     assert lib.rt_trace_ex(None, C.byref(b), C.byref(r), 0, 0, C.byref(o), None,
                                            None, None, None) == 0
+            for order in rtbvh.ANY_ORDERS:  # any hit, bits only, simple schedule
+                o = rtbvh.trace_opts("simple", block=blk, lds_stack=lds, any_order=order)
+                assert lib.rt_trace_ex(None, C.byref(b), C.byref(r), 0, rtbvh.RT_ANY,
+                                       C.byref(o), None, None, None, None) == 0
     for bits in autotune.SORT_BITS:
         rtbvh.build_requirements(100_000, rtbvh.build_cfg("radix", 4, bits))
 
@@ -317,3 +321,32 @@ def test_image_metrics_host_checks():
     assert lib.rt_image_metrics(None, fake, fake, 8, 8, fake, None, fake, 8) == 2
     assert lib.rt_image_metrics(None, fake, fake, 8, 8, fake, C.c_void_p(260), fake,
                                 1 << 20) == 1  # block output not 8-byte aligned
+
+
+def test_any_order_checks():
+    """any_order: rt_trace_ex only, any hit without counters or hit ids, the
+    simple schedule; values 0..2; other entry points take 0 only."""
+    import rtbvh
+    lib = rtbvh.lib()
+    assert rtbvh.rt_trace_opts.any_order.offset == 20
+    b = rtbvh.rt_bvh_dev()
+    r = rtbvh.rt_rays_soa()
+    fake = C.c_void_p(256)
+    hits = rtbvh.rt_hits_soa(fake, fake)
+    for order in (0, 1, 2):
+        o = rtbvh.rt_trace_opts(0, 0, 0, 0, 0, order)
+        assert lib.rt_trace_ex(None, C.byref(b), C.byref(r), 0, 1, C.byref(o), None, None, None,
+                               None) == 0
+    o = rtbvh.rt_trace_opts(0, 0, 0, 0, 0, 3)
+    assert lib.rt_trace_ex(None, C.byref(b), C.byref(r), 0, 1, C.byref(o), None, None, None,
+                           None) == 1
+    far = rtbvh.rt_trace_opts(0, 0, 0, 0, 0, 1)
+    for flags, h in ((0, None), (1 | 0x100, None), (1, C.byref(hits)), (1 | 0x200, None)):
+        assert lib.rt_trace_ex(None, C.byref(b), C.byref(r), 0, flags, C.byref(far), h, None,
+                               None, None) == 1, flags
+    refill = rtbvh.rt_trace_opts(1, 0, 0, 0, 0, 1)
+    assert lib.rt_trace_ex(None, C.byref(b), C.byref(r), 0, 1, C.byref(refill), None, None,
+                           None, None) == 1
+    assert lib.rt_trace_batch_ex(None, None, None, 0, 0, 1, C.byref(far)) == 1
+    assert lib.rt_trace_scene_ex(None, fake, 1, 0, C.byref(r), 0, 0, C.byref(far), None, None,
+                                 None) == 1
