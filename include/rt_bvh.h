@@ -293,6 +293,14 @@ rt_status rt_trace(rt_stream stream, const rt_bvh_dev* bvh, const rt_rays_soa* r
  *                    for every order (until an accepted hit ends a ray, every
  *                    node whose box meets [tmin, tmax] is visited); only the
  *                    work differs.  Other entry points take 0 only.
+ *   grid_width,      (rt_trace_ex, RT_SCHED_SIMPLE) wave tiles: the n rays are
+ *   tile_width       rows of grid_width consecutive rays (an image, or
+ *                    several stacked: n a multiple of grid_width, grid_width
+ *                    a multiple of tile_width); each wave traces a tile of
+ *                    tile_width (8, 16, 32) rays across and 64 / tile_width
+ *                    rows down instead of 64 consecutive rays.  Results are
+ *                    written per ray as before (same values); only which rays
+ *                    share a wave changes.  0 / 0 = linear waves.
  * opts = NULL: RT_SCHED_SIMPLE (RT_TRACE_PERSISTENT in flags: RT_SCHED_REFILL). */
 #define RT_SCHED_SIMPLE 0u
 #define RT_SCHED_REFILL 1u
@@ -306,7 +314,8 @@ typedef struct rt_trace_opts {
     uint32_t block;         /* 64 / 128 / 256; 0 = 256 */
     uint32_t lds_stack;     /* 4..48; 0 = 20 */
     uint32_t any_order;     /* RT_ORDER_*; 0 = near first */
-    uint32_t reserved[2];
+    uint32_t grid_width;    /* rays per row of a ray grid; 0 = linear waves */
+    uint32_t tile_width;    /* 8 / 16 / 32 with grid_width */
 } rt_trace_opts;
 rt_status rt_trace_ex(rt_stream stream, const rt_bvh_dev* bvh, const rt_rays_soa* rays,
                       uint32_t n, uint32_t flags, const rt_trace_opts* opts, rt_hits_soa* hits,

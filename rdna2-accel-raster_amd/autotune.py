@@ -35,6 +35,7 @@ TRACE_BLOCKS = (64, 128, 256)
 TRACE_LDS = (8, 12, 16, 20, 24, 32)
 REFILL_RAYS_PER_WAVE = (256, 1024, 4096)
 REFILL_LANES = (8, 16, 32)
+TILE_WIDTHS = (8, 16, 32)  # wave tiles of a ray grid: 8x8, 16x4, 32x2 rays
 SORT_BITS = rt.SORT_BITS
 BUILD_WORKGROUPS = (64, 128, 256, 512, 1024)
 
@@ -130,15 +131,19 @@ class Tuner:
         return best
 
     # -- trace -----------------------------------------------------------------
-    def trace(self, bvh, rays, mode, tag=""):
+    def trace(self, bvh, rays, mode, tag="", grid_width=None):
         """Schedule and launch shape for bvh.intersect / bvh.occluded on rays
         like these (a leading slice of at most sample_rays is measured).
-        Returns the keyword arguments for intersect / occluded."""
+        grid_width: the rays are rows of that many rays (an image), which
+        adds wave tiles of the grid to the candidates (the slice is then
+        whole rows).  Returns the keyword arguments for intersect / occluded."""
         n = len(rays)
         m = min(n, self.sample_rays)
+        if grid_width:
+            m = max(grid_width, m // grid_width * grid_width)
         sample = rays if m == n else rt.Rays(rays.data[:, :m])
         key = (f"trace|{mode}|{tag}|d{bvh.depth}|t{_size_class(bvh.tri_count)}"
-               f"|r{_size_class(m)}")
+               f"|r{_size_class(m)}" + (f"|g{grid_width}" if grid_width else ""))
         fn = bvh.intersect if mode == "closest" else bvh.occluded
         out = None
 
@@ -162,7 +167,9 @@ class Tuner:
         # order at the best simple shape compete too
         orders = [] if mode == "closest" else [
             dict(best, any_order=o) for o in rt.ANY_ORDERS if o != "near"]
-        return self._pick(key, [best] + refill + orders, run, check)
+        tiles = [dict(best, tile=(grid_width, tw)) for tw in TILE_WIDTHS
+                 if grid_width and grid_width % tw == 0]
+        return self._pick(key, [best] + refill + orders + tiles, run, check)
 
     def trace_scene(self, scene, rays, mode, tag=""):
         """Launch shape (block, lds_stack) for Scene.intersect / occluded."""

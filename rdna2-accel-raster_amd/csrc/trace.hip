@@ -343,7 +343,47 @@ __device__ __forceinline__ void write_wave_bits(BP bits_out, uint32_t r0, uint32
     if (lane == 32 && r0 + 32 < n) bits_out[(r0 >> 5) + 1] = (uint32_t)(m >> 32);
 }
 
-template <int PRIV, bool ANY, bool COUNT, int ORDER = kOrderNear>
+// Wave tiles of a ray grid (rt_trace_opts.grid_width / tile_width): the rays
+// are rows of `width` consecutive rays (rows = n / width); wave w takes the
+// tile of 2^tw_log2 rays across and 64 / 2^tw_log2 rows down, tiles in
+// row-major order, so a wave's rays are a compact patch of the image instead
+// of 64 rays of one row.
+struct TileMap {
+    uint32_t width;    // rays per row, a multiple of the tile width
+    uint32_t rows;     // n / width
+    uint32_t tw_log2;  // tile width 8, 16 or 32 as log2
+};
+
+// Ray of global thread g in the tile order; valid = inside the grid.
+__device__ __forceinline__ uint32_t tile_ray(uint32_t g, const TileMap& T, bool& valid) {
+    const uint32_t tile = g >> 6, lane = g & 63u;
+    const uint32_t per_row = T.width >> T.tw_log2;
+    const uint32_t trow = tile / per_row, tcol = tile - trow * per_row;
+    const uint32_t y = (trow << (6u - T.tw_log2)) + (lane >> T.tw_log2);
+    const uint32_t x = (tcol << T.tw_log2) + (lane & ((1u << T.tw_log2) - 1u));
+    valid = y < T.rows;
+    return valid ? y * T.width + x : 0u;
+}
+
+// Occlusion bits of a tiled wave: lanes 8k..8k+7 hold 8 consecutive rays of
+// one row starting at a multiple of 8 (tile widths and the row width are
+// multiples of 8), i.e. one byte of the bit array, stored by lane 8k.  The
+// lane storing the last byte of the grid also clears the bytes after it up to
+// the end of the last 32-bit word (as the linear kernel leaves them zero).
+template <typename BP>
+__device__ __forceinline__ void write_tile_bits(BP bits_out, uint32_t r, uint32_t n, bool valid,
+                                                bool occluded) {
+    const uint64_t m = __ballot(valid && occluded);
+    const uint32_t lane = threadIdx.x & 63u;
+    if (valid && (lane & 7u) == 0) {
+        auto bytes = reinterpret_cast<uint8_t*>(bits_out);
+        bytes[r >> 3] = (uint8_t)(m >> lane);
+        if (r + 8 == n)
+            for (uint32_t b = n >> 3; b < ((n + 31) >> 5) * 4; ++b) bytes[b] = 0;
+    }
+}
+
+template <int PRIV, bool ANY, bool COUNT, int ORDER = kOrderNear, bool TILE = false>
 __global__ __launch_bounds__(kBlock) void k_trace(const rt_node4* __restrict__ nodes,
                                                   const rt_tri64* __restrict__ tris,
                                                   uint32_t node_count, rt_rays_soa rays,
@@ -352,11 +392,18 @@ __global__ __launch_bounds__(kBlock) void k_trace(const rt_node4* __restrict__ n
                                                   float* __restrict__ t_out,
                                                   uint32_t* __restrict__ bits_out,
                                                   uint32_t* __restrict__ box_out,
-                                                  uint32_t* __restrict__ tri_out) {
+                                                  uint32_t* __restrict__ tri_out,
+                                                  TileMap tmap) {
     extern __shared__ int32_t lds_stack[];  // dynamic: ls * blockDim.x entries
     const uint32_t tid = threadIdx.x, bs = blockDim.x;
-    const uint32_t r = blockIdx.x * bs + tid;
-    const bool valid = r < n;
+    bool valid;
+    uint32_t r;
+    if constexpr (TILE) {
+        r = tile_ray(blockIdx.x * bs + tid, tmap, valid);
+    } else {
+        r = blockIdx.x * bs + tid;
+        valid = r < n;
+    }
 
     float ox = 0.0f, oy = 0.0f, oz = 0.0f, tmin = 1.0f;
     float dx = 0.0f, dy = 0.0f, dz = 1.0f, tmax = 0.0f;
@@ -371,7 +418,10 @@ __global__ __launch_bounds__(kBlock) void k_trace(const rt_node4* __restrict__ n
         nodes, tris, valid && node_count > 0, ox, oy, oz, tmin, dx, dy, dz, lds_stack + tid,
         (int)ls, (int)bs, hitT, hitPrim, boxTests, triTests);
     if (ANY) {
-        write_wave_bits(bits_out, r - (tid & 63u), n, valid && hitPrim != RT_MISS);
+        if constexpr (TILE)
+            write_tile_bits(bits_out, r, n, valid, hitPrim != RT_MISS);
+        else
+            write_wave_bits(bits_out, r - (tid & 63u), n, valid && hitPrim != RT_MISS);
         if (valid && prim_out) prim_out[r] = hitPrim;
     } else if (valid) {
         prim_out[r] = hitPrim;
@@ -861,6 +911,8 @@ struct TraceCfg {
     uint32_t block;         // 64, 128 or 256
     uint32_t lds;           // LDS stack entries per lane, kLdsMin..kLdsMax
     uint32_t order;         // any-hit child order (kOrderNear / Far / Slot)
+    uint32_t grid_width;    // ray grid width for wave tiles; 0 = linear
+    uint32_t tile_width;    // 8, 16 or 32 (with grid_width)
 };
 
 // Checks the launch-shape fields of opts (0 = default) into cfg; any_order
@@ -869,8 +921,14 @@ bool shape_opts(const rt_trace_opts* o, TraceCfg& cfg, bool with_order = false) 
     if (!o) return true;
     if (o->block) cfg.block = o->block;
     if (o->lds_stack) cfg.lds = o->lds_stack;
-    if (o->any_order && !with_order) return false;
+    if ((o->any_order || o->tile_width || o->grid_width) && !with_order) return false;
     cfg.order = o->any_order;
+    cfg.grid_width = o->grid_width;
+    cfg.tile_width = o->tile_width;
+    if (cfg.tile_width || cfg.grid_width) {
+        if (cfg.tile_width != 8 && cfg.tile_width != 16 && cfg.tile_width != 32) return false;
+        if (cfg.grid_width == 0 || cfg.grid_width % cfg.tile_width) return false;
+    }
     return (cfg.block == 64 || cfg.block == 128 || cfg.block == 256) && cfg.lds >= kLdsMin &&
            cfg.lds <= kLdsMax && cfg.order <= (uint32_t)kOrderSlot;
 }
@@ -908,25 +966,33 @@ rt_status launch_v(hipStream_t s, const rt_bvh_dev* b, const rt_rays_soa* rays, 
 #undef RT_P
         return launch_status("k_trace_persist");
     }
-    dim3 grid(ceil_div(n, cfg.block)), block(cfg.block);
-    if (cfg.order == kOrderFar) {
-        hipLaunchKernelGGL((k_trace<PRIV, true, false, kOrderFar>), grid, block, shm, s, b->nodes,
-                           b->tris, b->node_count, *rays, n, cfg.lds, prim, t, bits, box, tri);
-        return launch_status("k_trace");
+    TileMap tm{cfg.grid_width, 0u, 0u};
+    uint64_t threads = n;
+    if (cfg.tile_width) {
+        tm.rows = n / cfg.grid_width;
+        tm.tw_log2 = cfg.tile_width == 8 ? 3u : cfg.tile_width == 16 ? 4u : 5u;
+        const uint64_t th = 64u >> tm.tw_log2;
+        threads = (uint64_t)(cfg.grid_width / cfg.tile_width) * ((tm.rows + th - 1) / th) * 64u;
     }
-    if (cfg.order == kOrderSlot) {
-        hipLaunchKernelGGL((k_trace<PRIV, true, false, kOrderSlot>), grid, block, shm, s,
-                           b->nodes, b->tris, b->node_count, *rays, n, cfg.lds, prim, t, bits,
-                           box, tri);
-        return launch_status("k_trace");
+    dim3 grid(ceil_div(threads, cfg.block)), block(cfg.block);
+#define RT_L(A, C, O, T)                                                                      \
+    hipLaunchKernelGGL((k_trace<PRIV, A, C, O, T>), grid, block, shm, s, b->nodes, b->tris,   \
+                       b->node_count, *rays, n, cfg.lds, prim, t, bits, box, tri, tm)
+    if (cfg.tile_width) {
+        if (cfg.order == kOrderFar) RT_L(true, false, kOrderFar, true);
+        else if (cfg.order == kOrderSlot) RT_L(true, false, kOrderSlot, true);
+        else if (any && cnt) RT_L(true, true, kOrderNear, true);
+        else if (any) RT_L(true, false, kOrderNear, true);
+        else if (cnt) RT_L(false, true, kOrderNear, true);
+        else RT_L(false, false, kOrderNear, true);
+    } else {
+        if (cfg.order == kOrderFar) RT_L(true, false, kOrderFar, false);
+        else if (cfg.order == kOrderSlot) RT_L(true, false, kOrderSlot, false);
+        else if (any && cnt) RT_L(true, true, kOrderNear, false);
+        else if (any) RT_L(true, false, kOrderNear, false);
+        else if (cnt) RT_L(false, true, kOrderNear, false);
+        else RT_L(false, false, kOrderNear, false);
     }
-#define RT_L(A, C)                                                                            \
-    hipLaunchKernelGGL((k_trace<PRIV, A, C>), grid, block, shm, s, b->nodes, b->tris,         \
-                       b->node_count, *rays, n, cfg.lds, prim, t, bits, box, tri)
-    if (any && cnt) RT_L(true, true);
-    else if (any) RT_L(true, false);
-    else if (cnt) RT_L(false, true);
-    else RT_L(false, false);
 #undef RT_L
     return launch_status("k_trace");
 }
@@ -957,6 +1023,8 @@ extern "C" rt_status rt_trace_ex(rt_stream stream, const rt_bvh_dev* bvh, const 
     if (cfg.order != kOrderNear &&
         (!(flags & RT_ANY) || (flags & RT_COUNTERS) || (hits && hits->prim) || cfg.refill))
         return RT_ERR_ARG;
+    // wave tiles: one ray per thread, whole rows
+    if (cfg.tile_width && (cfg.refill || n % cfg.grid_width)) return RT_ERR_ARG;
     if (n == 0) return RT_OK;
     if (!rays->ox || !rays->oy || !rays->oz || !rays->tmin || !rays->dx || !rays->dy ||
         !rays->dz || !rays->tmax)

@@ -113,7 +113,8 @@ BUILD_ALGOS = {"auto": 0, "radix": 1, "implicit": 2}
 
 class rt_trace_opts(C.Structure):
     _fields_ = [("schedule", u32), ("rays_per_wave", u32), ("refill_lanes", u32),
-                ("block", u32), ("lds_stack", u32), ("any_order", u32), ("reserved", u32 * 2)]
+                ("block", u32), ("lds_stack", u32), ("any_order", u32), ("grid_width", u32),
+                ("tile_width", u32)]
 
 
 # Child order of an any-hit query returning occlusion bits only (same bits
@@ -133,16 +134,18 @@ LDS_STACK_RANGE = (4, 48)
 
 
 def trace_opts(schedule=None, rays_per_wave=None, refill_lanes=None, block=None,
-               lds_stack=None, any_order=None):
+               lds_stack=None, any_order=None, tile=None):
+    """tile: (grid_width, tile_width) wave tiles of a ray grid, or None."""
     s = schedule or DEFAULT_SCHEDULE
     if s not in ("simple", "refill"):
         raise ValueError(f"schedule {s!r}")
+    gw, tw = tile if tile else (0, 0)
     return rt_trace_opts(1 if s == "refill" else 0,
                          rays_per_wave or TRACE_TUNING["rays_per_wave"],
                          refill_lanes or TRACE_TUNING["refill_lanes"],
                          block or TRACE_TUNING["block"],
                          lds_stack or TRACE_TUNING["lds_stack"],
-                         ANY_ORDERS[any_order or "near"])
+                         ANY_ORDERS[any_order or "near"], gw, tw)
 
 
 _lib = None

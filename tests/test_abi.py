@@ -350,3 +350,32 @@ def test_any_order_checks():
     assert lib.rt_trace_batch_ex(None, None, None, 0, 0, 1, C.byref(far)) == 1
     assert lib.rt_trace_scene_ex(None, fake, 1, 0, C.byref(r), 0, 0, C.byref(far), None, None,
                                  None) == 1
+
+
+def test_tile_checks():
+    """grid_width / tile_width: rt_trace_ex only, tile width 8 / 16 / 32
+    dividing the grid width, whole rows, simple schedule."""
+    import rtbvh
+    lib = rtbvh.lib()
+    assert rtbvh.rt_trace_opts.grid_width.offset == 24
+    assert rtbvh.rt_trace_opts.tile_width.offset == 28
+    b = rtbvh.rt_bvh_dev()
+    r = rtbvh.rt_rays_soa()
+    fake = C.c_void_p(256)
+    r5 = rtbvh.rt_rays_soa(*([fake] * 8))
+    h = rtbvh.rt_hits_soa(fake, fake)
+    for gw, tw, ok in [(0, 0, True), (64, 8, True), (64, 16, True), (64, 32, True),
+                       (64, 64, False), (64, 4, False), (0, 8, False), (24, 16, False),
+                       (64, 0, False)]:
+        o = rtbvh.rt_trace_opts(0, 0, 0, 0, 0, 0, gw, tw)
+        assert lib.rt_trace_ex(None, C.byref(b), C.byref(r), 0, 0, C.byref(o), None, None, None,
+                               None) == (0 if ok else 1), (gw, tw)
+    o = rtbvh.rt_trace_opts(0, 0, 0, 0, 0, 0, 64, 8)
+    # 100 rays are not whole rows of 64
+    assert lib.rt_trace_ex(None, C.byref(b), C.byref(r5), 100, 0, C.byref(o), C.byref(h), None,
+                           None, None) == 1
+    o.schedule = 1
+    assert lib.rt_trace_ex(None, C.byref(b), C.byref(r), 0, 0, C.byref(o), None, None, None,
+                           None) == 1
+    o.schedule = 0
+    assert lib.rt_trace_batch_ex(None, None, None, 0, 0, 1, C.byref(o)) == 1
