@@ -7,7 +7,12 @@
 //   a host-built SoftwareBVH4 uploaded as SSBOs -> rtgpu::GpuBVH4::fromHost(...)
 //   software_bvh_bvh4.comp closest hit          -> GpuBVH4::intersect
 //   TerminateOnFirstHit ray queries (rq_ao.comp) -> GpuBVH4::occluded
-//   (no reference counterpart)                  -> GpuBVH4::refit, BuildBatch
+//   (no reference counterpart)                  -> GpuBVH4::refit
+//   a host-built SoftwareBVH + software_bvh_procedural.comp (BVH2 loop)
+//                                               -> rtgpu::GpuBVH2::fromHost, intersect
+//   fast_bvh_builder image + ray queries over it -> rtgpu::RadvBVH::build, intersect
+//   dump_bvh_structure (radv_bvh.hpp:311-390)    -> RadvBVH::describe
+//   ImageMetrics::compute (image_metrics.hpp:16) -> rtgpu::imageMetrics (device images)
 //
 // Header-only.  Device memory is owned by RAII arrays; every library status
 // other than RT_OK, and every failing HIP call, throws rtgpu::Error (the
@@ -283,5 +288,169 @@ private:
     DeviceArray<rt_tri64> tris_;
     uint32_t nodeCount_ = 0, triCount_ = 0, depth_ = 0;
 };
+
+// A host-built SoftwareBVH (software_bvh.hpp:11-22,35-175) traversed on the device with
+// the BVH2 loop of software_bvh_procedural.comp (rt_trace_bvh2): its GpuBvhNode and
+// GpuTriangle arrays upload unchanged.  leafOrder (optional): the original triangle index
+// of each leaf position, reported as the hit id (without it, the leaf position).
+class GpuBVH2 {
+public:
+    static GpuBVH2 fromHost(const void* nodes, uint32_t nodeCount, const float* tris48,
+                            const uint32_t* leafOrder, uint32_t triCount,
+                            hipStream_t s = nullptr) {
+        GpuBVH2 b;
+        b.s_ = s;
+        check(rt_bvh2_depth(nodes, nodeCount, &b.depth_), "rt_bvh2_depth");
+        b.nodes_.upload(static_cast<const uint8_t*>(nodes), (size_t)nodeCount * 32, s);
+        b.tris_.upload(tris48, (size_t)triCount * 12, s);
+        if (leafOrder) b.order_.upload(leafOrder, triCount, s);
+        b.nodeCount_ = nodeCount;
+        b.triCount_ = triCount;
+        return b;
+    }
+
+    void intersect(const Rays& rays, Hits& out, bool counters = false,
+                   const rt_trace_opts* opts = nullptr) const {
+        const size_t n = rays.size();
+        out.prim.resize(n);
+        out.t.resize(n);
+        if (counters) {
+            out.boxTests.resize(n);
+            out.triTests.resize(n);
+        }
+        const rt_bvh2_dev v = view();
+        const rt_rays_soa r = rays.view();
+        rt_hits_soa h = {out.prim.data(), out.t.data()};
+        check(rt_trace_bvh2(s_, &v, &r, (uint32_t)n, RT_CLOSEST | (counters ? RT_COUNTERS : 0u),
+                            opts, &h, nullptr, counters ? out.boxTests.data() : nullptr,
+                            counters ? out.triTests.data() : nullptr),
+              "rt_trace_bvh2 (closest)");
+    }
+
+    void occluded(const Rays& rays, DeviceArray<uint32_t>& bits,
+                  const rt_trace_opts* opts = nullptr) const {
+        const size_t n = rays.size();
+        bits.resize(n ? (n + 31) / 32 : 1);
+        const rt_bvh2_dev v = view();
+        const rt_rays_soa r = rays.view();
+        check(rt_trace_bvh2(s_, &v, &r, (uint32_t)n, RT_ANY, opts, nullptr, bits.data(),
+                            nullptr, nullptr),
+              "rt_trace_bvh2 (any)");
+    }
+
+    rt_bvh2_dev view() const {
+        return {nodes_.data(), tris_.data(), order_.size() ? order_.data() : nullptr,
+                nodeCount_, triCount_, depth_, 0u};
+    }
+    uint32_t depth() const { return depth_; }
+
+private:
+    hipStream_t s_ = nullptr;
+    DeviceArray<uint8_t> nodes_;
+    DeviceArray<float> tris_;
+    DeviceArray<uint32_t> order_;
+    uint32_t nodeCount_ = 0, triCount_ = 0, depth_ = 0;
+};
+
+// The RADV acceleration-structure image of fast_bvh_builder.comp built on the device
+// (rt_gpu_build_radv, at most 4,096 triangles as the reference builder) and traced in
+// software over its bytes (rt_trace_radv): the ray-query consumers of that image.
+class RadvBVH {
+public:
+    // order: RT_RADV_INPUT_ORDER (the reference builder's leaves) or RT_RADV_MORTON.
+    template <class Mesh>
+    static RadvBVH build(const Mesh& mesh, uint32_t order = RT_RADV_INPUT_ORDER,
+                         hipStream_t s = nullptr) {
+        const std::vector<float> pos = packed_positions(mesh);
+        const uint32_t n = (uint32_t)(mesh.indices.size() / 3);
+        RadvBVH b;
+        b.s_ = s;
+        DeviceArray<float> dpos;
+        DeviceArray<uint32_t> didx;
+        dpos.upload(pos.data(), pos.size(), s);
+        didx.upload(mesh.indices.data(), mesh.indices.size(), s);
+        uint32_t counts[32], offsets[32], levels = 0, boxes = 0;
+        uint64_t bytes = 0;
+        check(rt_tree_layout(n, counts, offsets, &levels, &boxes, &bytes), "rt_tree_layout");
+        b.image_.resize(bytes);
+        check(rt_gpu_build_radv(s, dpos.data(), didx.data(), n, order, b.image_.data(), bytes,
+                                nullptr),
+              "rt_gpu_build_radv");
+        hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");  // inputs freed next
+        b.info_ = b.describe();
+        b.depth_ = levels;
+        return b;
+    }
+
+    // The header of the image (downloads and decodes it; rt_radv_describe).
+    rt_radv_info describe() const {
+        const std::vector<uint8_t> h = image_.download(s_);
+        rt_radv_info info;
+        check(rt_radv_describe(h.data(), h.size(), &info), "rt_radv_describe");
+        return info;
+    }
+
+    void intersect(const Rays& rays, Hits& out, bool counters = false,
+                   const rt_trace_opts* opts = nullptr) const {
+        const size_t n = rays.size();
+        out.prim.resize(n);
+        out.t.resize(n);
+        if (counters) {
+            out.boxTests.resize(n);
+            out.triTests.resize(n);
+        }
+        const rt_radv_dev v = view();
+        const rt_rays_soa r = rays.view();
+        rt_hits_soa h = {out.prim.data(), out.t.data()};
+        check(rt_trace_radv(s_, &v, &r, (uint32_t)n, RT_CLOSEST | (counters ? RT_COUNTERS : 0u),
+                            opts, &h, nullptr, counters ? out.boxTests.data() : nullptr,
+                            counters ? out.triTests.data() : nullptr),
+              "rt_trace_radv (closest)");
+    }
+
+    void occluded(const Rays& rays, DeviceArray<uint32_t>& bits,
+                  const rt_trace_opts* opts = nullptr) const {
+        const size_t n = rays.size();
+        bits.resize(n ? (n + 31) / 32 : 1);
+        const rt_radv_dev v = view();
+        const rt_rays_soa r = rays.view();
+        check(rt_trace_radv(s_, &v, &r, (uint32_t)n, RT_ANY, opts, nullptr, bits.data(), nullptr,
+                            nullptr),
+              "rt_trace_radv (any)");
+    }
+
+    rt_radv_dev view() const {
+        return {image_.data(), image_.size(), info_.bvh_offset, info_.box_count,
+                (uint32_t)info_.primitive_count, depth_};
+    }
+    const DeviceArray<uint8_t>& image() const { return image_; }
+
+private:
+    hipStream_t s_ = nullptr;
+    DeviceArray<uint8_t> image_;
+    rt_radv_info info_{};
+    uint32_t depth_ = 0;
+};
+
+// ImageQualityMetrics of common/image_metrics.hpp:8-12.
+struct ImageQuality {
+    double mse = 0.0;
+    double psnr = 100.0;
+    double ssim = 1.0;
+};
+
+// ImageMetrics::compute(imgA, imgB, width, height) (image_metrics.hpp:16-104) of two
+// float images in device memory (rt_image_metrics); waits for the result.
+inline ImageQuality imageMetrics(const float* d_a, const float* d_b, uint32_t width,
+                                 uint32_t height, hipStream_t s = nullptr) {
+    size_t bytes = 0;
+    check(rt_image_metrics_scratch_bytes(width, height, &bytes), "rt_image_metrics_scratch_bytes");
+    DeviceArray<double> scratch(bytes / sizeof(double) + 1), out(3);
+    check(rt_image_metrics(s, d_a, d_b, width, height, out.data(), nullptr, scratch.data(),
+                           scratch.size() * sizeof(double)),
+          "rt_image_metrics");
+    const std::vector<double> h = out.download(s);  // waits: scratch freed next
+    return {h[0], h[1], h[2]};
+}
 
 }  // namespace rtgpu

@@ -9,8 +9,16 @@
 // Prints the CSV's software-path columns plus this GPU's traversal rate:
 //   Triangles,RayPattern,HitRate_SW_pct,AvgBoxTests_PerRay,AvgBoxTests_HitOnly,
 //   AvgTriTests_PerRay,AvgTriTests_HitOnly,GPU_MRays_per_sec
+// BVH2 rows (prefix "bvh2,"): the same terrain's SoftwareBVH (before the
+// collapse) traced with the loop of software_bvh_procedural.comp
+// (GpuBVH2, tmin 0.01) and of its LDS-stack variant software_bvh_lds.comp
+// (tmin 0.001): Triangles,RayPattern,Shader,HitRate_pct,AvgBoxTests,
+// AvgTriTests,GPU_MRays_per_sec.
 // Then builds the same terrain on the device (GpuBVH4::build, radix),
-// refits it at the same positions and checks the bytes are unchanged.
+// refits it at the same positions and checks the bytes are unchanged; for
+// the smallest terrain also the RADV image of its Morton order, traced over
+// its bytes (RadvBVH), against the implicit Box4 build of the same mesh, and
+// the image metrics of the two hit-distance images (identical: mse 0).
 //
 //   exp0_probe_gpu [targets...]   (default 1000 100000 1000000)
 //   exp0_probe_gpu --host-only    argument checks of the wrapper, no device work
@@ -50,6 +58,46 @@ SceneMesh terrain(uint32_t target, std::vector<float>& pos, std::vector<uint32_t
     return m;
 }
 
+std::string fmt(double v) {
+    char b[64];
+    std::snprintf(b, sizeof b, "%.6f", v);
+    return b;
+}
+
+// The RADV image of the mesh (Morton order) traced over its bytes equals the implicit
+// Box4 build of the same mesh; the image metrics of the two hit-distance images are
+// those of identical images.  Returns 1 on a mismatch.
+int radv_check(const SceneMesh& mesh, hipStream_t s) {
+    rtgpu::RadvBVH img = rtgpu::RadvBVH::build(mesh, RT_RADV_MORTON, s);
+    rtgpu::GpuBVH4 box = rtgpu::GpuBVH4::build(mesh, 4, RT_BUILD_IMPLICIT, s);
+    const rt_radv_info info = img.describe();
+    const uint32_t w = 256, h = 256, n = w * h;
+    std::vector<float> f[8];
+    for (auto& v : f) v.assign(n, 0.0f);
+    const float cam[3] = {0.0f, 50.0f, -45.0f}, look[3] = {0.0f, 0.0f, 5.0f};
+    or_exp0_rays(n, 0, cam, look, f[0].data(), f[1].data(), f[2].data(), f[4].data(),
+                 f[5].data(), f[6].data());
+    for (uint32_t r = 0; r < n; ++r) {
+        f[3][r] = 0.01f;
+        f[7][r] = 1000.0f;
+    }
+    const float* p[8];
+    for (int k = 0; k < 8; ++k) p[k] = f[k].data();
+    rtgpu::Rays rays(p, n, s);
+    rtgpu::Hits a, b;
+    img.intersect(rays, a, true);
+    box.intersect(rays, b, true);
+    const bool same = a.prim.download(s) == b.prim.download(s) &&
+                      a.boxTests.download(s) == b.boxTests.download(s) &&
+                      a.triTests.download(s) == b.triTests.download(s);
+    const rtgpu::ImageQuality q = rtgpu::imageMetrics(a.t.data(), b.t.data(), w, h, s);
+    std::printf("# RADV image: %u box nodes, %llu triangles, bvh_offset 0x%x; traced over its bytes "
+                "%s the implicit Box4 tree; metrics of the t images: mse %g psnr %g ssim %g\n",
+                info.box_count, (unsigned long long)info.primitive_count, info.bvh_offset,
+                same ? "equals" : "DIFFERS FROM", q.mse, q.psnr, q.ssim);
+    return same && q.mse == 0.0 && q.psnr == 100.0 ? 0 : 1;
+}
+
 int host_only() {
     // rt_status errors surface as rtgpu::Error before any device work
     int fails = 0;
@@ -82,6 +130,7 @@ int main(int argc, char** argv) {
     if (targets.empty()) targets = {1000u, 100000u, 1000000u};
     hipStream_t s;
     rtgpu::hip_check(hipStreamCreate(&s), "hipStreamCreate");
+    std::vector<std::string> bvh2_rows;
     std::printf("Triangles,RayPattern,HitRate_SW_pct,AvgBoxTests_PerRay,AvgBoxTests_HitOnly,"
                 "AvgTriTests_PerRay,AvgTriTests_HitOnly,GPU_MRays_per_sec\n");
     for (uint32_t target : targets) {
@@ -102,6 +151,8 @@ int main(int argc, char** argv) {
         const uint32_t depth = or_bvh4_depth(n4.data(), c4);
         rtgpu::GpuBVH4 bvh = rtgpu::GpuBVH4::fromHost(n4.data(), c4, t48.data(), order.data(),
                                                       ntri, depth, s);
+        rtgpu::GpuBVH2 bvh2 = rtgpu::GpuBVH2::fromHost(n2.data(), c2, t48.data(), order.data(),
+                                                       ntri, s);
         for (int coherent = 1; coherent >= 0; --coherent) {
             const uint32_t n = 1u << 20;
             std::vector<float> f[8];
@@ -148,6 +199,39 @@ int main(int argc, char** argv) {
                         coherent ? "Coherent" : "Incoherent", 100.0 * nh / n, sb / n,
                         nh ? sbh / nh : 0.0, st / n, nh ? sth / nh : 0.0,
                         10.0 * n / (ms * 1e-3) / 1e6);
+            // the BVH2 shaders on the same rays: tmin 0.01 (scratch stack) and 0.001 (LDS)
+            for (int lds = 0; lds < 2; ++lds) {
+                for (uint32_t r = 0; r < n; ++r) f[3][r] = lds ? 0.001f : 0.01f;
+                for (int k = 0; k < 8; ++k) p[k] = f[k].data();
+                rtgpu::Rays r2(p, n, s);
+                rtgpu::Hits h2;
+                bvh2.intersect(r2, h2, true);
+                const std::vector<uint32_t> p2 = h2.prim.download(s);
+                const std::vector<uint32_t> b2 = h2.boxTests.download(s);
+                const std::vector<uint32_t> q2 = h2.triTests.download(s);
+                double hits2 = 0, box2 = 0, tri2 = 0;
+                for (uint32_t r = 0; r < n; ++r) {
+                    hits2 += p2[r] != RT_MISS;
+                    box2 += b2[r];
+                    tri2 += q2[r];
+                }
+                bvh2.intersect(r2, h2);
+                rtgpu::hip_check(hipEventCreate(&e0), "hipEventCreate");
+                rtgpu::hip_check(hipEventCreate(&e1), "hipEventCreate");
+                rtgpu::hip_check(hipEventRecord(e0, s), "hipEventRecord");
+                for (int k = 0; k < 10; ++k) bvh2.intersect(r2, h2);
+                rtgpu::hip_check(hipEventRecord(e1, s), "hipEventRecord");
+                rtgpu::hip_check(hipEventSynchronize(e1), "hipEventSynchronize");
+                float ms2 = 0.0f;
+                rtgpu::hip_check(hipEventElapsedTime(&ms2, e0, e1), "hipEventElapsedTime");
+                (void)hipEventDestroy(e0);
+                (void)hipEventDestroy(e1);
+                bvh2_rows.push_back(std::to_string(ntri) + "," +
+                                    (coherent ? "Coherent" : "Incoherent") + "," +
+                                    (lds ? "Software_ALU_LDS" : "Software_ALU_Scratch") + "," +
+                                    fmt(100.0 * hits2 / n) + "," + fmt(box2 / n) + "," +
+                                    fmt(tri2 / n) + "," + fmt(10.0 * n / (ms2 * 1e-3) / 1e6));
+            }
         }
         // device build of the same mesh, refit at the same positions: unchanged bytes
         rtgpu::GpuBVH4 dev = rtgpu::GpuBVH4::build(mesh, 4, RT_BUILD_RADIX, s);
@@ -162,7 +246,9 @@ int main(int argc, char** argv) {
         std::printf("# device radix build: %u nodes, depth %u; refit at the same positions %s\n",
                     dev.nodeCount(), dev.depth(), same ? "unchanged" : "CHANGED");
         if (!same) return 1;
+        if (target == targets.front() && ntri <= 4096 && radv_check(mesh, s)) return 1;
     }
+    for (const std::string& row : bvh2_rows) std::printf("bvh2,%s\n", row.c_str());
     (void)hipStreamDestroy(s);
     return 0;
 }

@@ -28,7 +28,8 @@ def test_cpp_probe_reproduces_published_exp0():
     """Hit rate and mean box / triangle tests per ray of the C++ path on the
     GPU equal the reference's published software-path statistics within the
     tolerance of the Python path's test; a device build refitted at the same
-    positions keeps its bytes."""
+    positions keeps its bytes; the BVH2, RADV-image and image-metrics wrappers
+    run on the same terrains."""
     from test_oracle_known import EXP0
     r = subprocess.run([_probe(), "1000", "100000"], capture_output=True, text=True,
                        timeout=240)
@@ -46,3 +47,16 @@ def test_cpp_probe_reproduces_published_exp0():
         assert abs(tri - ref["tri"]) / ref["tri"] < 2e-3
         assert mrays > 0
     assert r.stdout.count("refit at the same positions unchanged") == 2
+    # BVH2 shaders on the same terrains and rays (GpuBVH2): hit rates within 0.05 points
+    # of the BVH4 path (the shaders' predicates differ only at |a| < 1e-7 vs 1e-8 and the
+    # t >= / > tmin edge), both stack variants measured
+    hit4 = {(int(row[0]), row[1]): float(row[2]) for row in rows}
+    b2 = [line.split(",")[1:] for line in r.stdout.splitlines() if line.startswith("bvh2,")]
+    assert len(b2) == 8
+    for ntri, pattern, shader, hit, box, tri, mrays in b2:
+        assert shader in ("Software_ALU_Scratch", "Software_ALU_LDS")
+        assert abs(float(hit) - hit4[(int(ntri), pattern)]) < 0.05
+        assert float(box) > 1 and float(tri) > 0 and float(mrays) > 0
+    # RADV image of the first terrain traced over its bytes = the implicit Box4 build
+    assert "traced over its bytes equals the implicit Box4 tree" in r.stdout
+    assert "mse 0 psnr 100" in r.stdout
