@@ -456,8 +456,8 @@ def ao_frame_leg(frames=100, w=1920, h=1080, tuner=None):
         frame(k)
     torch.cuda.synchronize()
     if tuner is not None:
-        shapes[0] = tuner.trace_scene(scene, primary, "closest", tag="ao-primary")
-        shapes[1] = tuner.trace_scene(scene, ao_rays, "any", tag="ao")
+        shapes[0] = tuner.trace_scene(scene, primary, "closest", tag="ao-primary", grid_width=w)
+        shapes[1] = tuner.trace_scene(scene, ao_rays, "any", tag="ao", grid_width=w)
         torch.cuda.synchronize()
     s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     s.record()

@@ -171,12 +171,16 @@ class Tuner:
                  if grid_width and grid_width % tw == 0]
         return self._pick(key, [best] + refill + orders + tiles, run, check)
 
-    def trace_scene(self, scene, rays, mode, tag=""):
-        """Launch shape (block, lds_stack) for Scene.intersect / occluded."""
+    def trace_scene(self, scene, rays, mode, tag="", grid_width=None):
+        """Launch shape (block, lds_stack, and wave tiles of the ray grid when
+        grid_width is given) for Scene.intersect / occluded."""
         n = len(rays)
         m = min(n, self.sample_rays)
+        if grid_width:
+            m = max(grid_width, m // grid_width * grid_width)
         sample = rays if m == n else rt.Rays(rays.data[:, :m])
-        key = f"scene|{mode}|{tag}|i{scene.count}|d{scene.depth}|r{_size_class(m)}"
+        key = (f"scene|{mode}|{tag}|i{scene.count}|d{scene.depth}|r{_size_class(m)}"
+               + (f"|g{grid_width}" if grid_width else ""))
         fn = scene.intersect if mode == "closest" else scene.occluded
         out = None
 
@@ -190,7 +194,12 @@ class Tuner:
 
         cands = [{"block": b, "lds_stack": l} for b in TRACE_BLOCKS for l in TRACE_LDS]
         cands.sort(key=lambda c: (c["block"], c["lds_stack"]) != (256, 20))
-        return self._pick(key, cands, run, check)
+        if not grid_width:
+            return self._pick(key, cands, run, check)
+        best = self._pick(key + "|shape", cands, run, check)
+        tiles = [dict(best, tile=(grid_width, tw)) for tw in TILE_WIDTHS
+                 if grid_width % tw == 0]
+        return self._pick(key, [best] + tiles, run, check)
 
     def trace_batch(self, batch, tag=""):
         """Schedule and launch shape for TraceBatch.launch: the best simple

@@ -535,18 +535,24 @@ __global__ __launch_bounds__(kBlock) void k_trace_batch(const rt_trace_item* __r
 // an any-hit ray stops at the first occluding instance.  Instance records
 // are wave-uniform (scalar loads); the map is applied as written, in the
 // order ((m0 x + m1 y) + m2 z) + t.
-template <int PRIV, bool ANY>
+template <int PRIV, bool ANY, bool TILE = false>
 __global__ __launch_bounds__(kBlock) void k_trace_scene(const rt_instance* __restrict__ inst,
                                                         uint32_t count, rt_rays_soa rays,
                                                         uint32_t n, uint32_t* __restrict__ prim_out,
                                                         float* __restrict__ t_out,
                                                         uint32_t* __restrict__ inst_out,
                                                         uint32_t* __restrict__ bits_out,
-                                                        uint32_t ls) {
+                                                        uint32_t ls, TileMap tmap) {
     extern __shared__ int32_t lds_stack[];  // dynamic: ls * blockDim.x entries
     const uint32_t tid = threadIdx.x, bs = blockDim.x;
-    const uint32_t r = blockIdx.x * bs + tid;
-    const bool valid = r < n;
+    bool valid;
+    uint32_t r;
+    if constexpr (TILE) {
+        r = tile_ray(blockIdx.x * bs + tid, tmap, valid);
+    } else {
+        r = blockIdx.x * bs + tid;
+        valid = r < n;
+    }
     float ox = 0.0f, oy = 0.0f, oz = 0.0f, tmin = 1.0f;
     float dx = 0.0f, dy = 0.0f, dz = 1.0f, tmax = 0.0f;
     if (valid) {
@@ -586,7 +592,10 @@ __global__ __launch_bounds__(kBlock) void k_trace_scene(const rt_instance* __res
         }
     }
     if (ANY) {
-        write_wave_bits(bits_out, r - (tid & 63u), n, valid && bprim != RT_MISS);
+        if constexpr (TILE)
+            write_tile_bits(bits_out, r, n, valid, bprim != RT_MISS);
+        else
+            write_wave_bits(bits_out, r - (tid & 63u), n, valid && bprim != RT_MISS);
     } else if (valid) {
         prim_out[r] = bprim;
         t_out[r] = best;
@@ -916,12 +925,15 @@ struct TraceCfg {
 };
 
 // Checks the launch-shape fields of opts (0 = default) into cfg; any_order
-// only where the entry point offers it (with_order).
-bool shape_opts(const rt_trace_opts* o, TraceCfg& cfg, bool with_order = false) {
+// and wave tiles only where the entry point offers them (allow: kWithOrder,
+// kWithTiles).
+constexpr uint32_t kWithOrder = 1u, kWithTiles = 2u;
+bool shape_opts(const rt_trace_opts* o, TraceCfg& cfg, uint32_t allow = 0u) {
     if (!o) return true;
     if (o->block) cfg.block = o->block;
     if (o->lds_stack) cfg.lds = o->lds_stack;
-    if ((o->any_order || o->tile_width || o->grid_width) && !with_order) return false;
+    if (o->any_order && !(allow & kWithOrder)) return false;
+    if ((o->tile_width || o->grid_width) && !(allow & kWithTiles)) return false;
     cfg.order = o->any_order;
     cfg.grid_width = o->grid_width;
     cfg.tile_width = o->tile_width;
@@ -934,6 +946,17 @@ bool shape_opts(const rt_trace_opts* o, TraceCfg& cfg, bool with_order = false) 
 }
 
 size_t lds_bytes(const TraceCfg& cfg) { return (size_t)cfg.lds * cfg.block * 4; }
+
+// Threads of a one-ray-per-thread launch over n rays: n, or with wave tiles
+// 64 per tile of the grid (partial tiles at the bottom included); fills tm.
+uint64_t launch_threads(const TraceCfg& cfg, uint32_t n, TileMap& tm) {
+    tm = TileMap{cfg.grid_width, 0u, 0u};
+    if (!cfg.tile_width) return n;
+    tm.rows = n / cfg.grid_width;
+    tm.tw_log2 = cfg.tile_width == 8 ? 3u : cfg.tile_width == 16 ? 4u : 5u;
+    const uint64_t th = 64u >> tm.tw_log2;
+    return (uint64_t)(cfg.grid_width / cfg.tile_width) * ((tm.rows + th - 1) / th) * 64u;
+}
 
 // Private overflow entries needed beyond the LDS part: at most 3 * depth + 1
 // entries are ever live.
@@ -966,15 +989,8 @@ rt_status launch_v(hipStream_t s, const rt_bvh_dev* b, const rt_rays_soa* rays, 
 #undef RT_P
         return launch_status("k_trace_persist");
     }
-    TileMap tm{cfg.grid_width, 0u, 0u};
-    uint64_t threads = n;
-    if (cfg.tile_width) {
-        tm.rows = n / cfg.grid_width;
-        tm.tw_log2 = cfg.tile_width == 8 ? 3u : cfg.tile_width == 16 ? 4u : 5u;
-        const uint64_t th = 64u >> tm.tw_log2;
-        threads = (uint64_t)(cfg.grid_width / cfg.tile_width) * ((tm.rows + th - 1) / th) * 64u;
-    }
-    dim3 grid(ceil_div(threads, cfg.block)), block(cfg.block);
+    TileMap tm;
+    dim3 grid(ceil_div(launch_threads(cfg, n, tm), cfg.block)), block(cfg.block);
 #define RT_L(A, C, O, T)                                                                      \
     hipLaunchKernelGGL((k_trace<PRIV, A, C, O, T>), grid, block, shm, s, b->nodes, b->tris,   \
                        b->node_count, *rays, n, cfg.lds, prim, t, bits, box, tri, tm)
@@ -1015,7 +1031,7 @@ extern "C" rt_status rt_trace_ex(rt_stream stream, const rt_bvh_dev* bvh, const 
         if (opts->rays_per_wave) cfg.per_wave = opts->rays_per_wave;
         if (opts->refill_lanes) cfg.refill_lanes = opts->refill_lanes;
     }
-    if (!shape_opts(opts, cfg, true)) return RT_ERR_ARG;
+    if (!shape_opts(opts, cfg, kWithOrder | kWithTiles)) return RT_ERR_ARG;
     if (cfg.per_wave % 64 || cfg.per_wave > kMaxPerWave || cfg.refill_lanes > 64)
         return RT_ERR_ARG;
     // another child order: occlusion bits only (no counters, no hit ids), one
@@ -1307,8 +1323,9 @@ extern "C" rt_status rt_trace_scene_ex(rt_stream stream, const rt_instance* d_in
                                        uint32_t* d_inst, uint32_t* occl_bits) {
     TraceCfg cfg{false, 0u, 0u, (uint32_t)kBlock, kLdsStack};
     if (opts && opts->schedule != RT_SCHED_SIMPLE) return RT_ERR_ARG;
-    if (!shape_opts(opts, cfg)) return RT_ERR_ARG;
+    if (!shape_opts(opts, cfg, kWithTiles)) return RT_ERR_ARG;
     if (!rays || (flags & ~RT_ANY)) return RT_ERR_ARG;
+    if (cfg.tile_width && n % cfg.grid_width) return RT_ERR_ARG;  // whole rows
     if (n == 0) return RT_OK;
     if ((count && !d_instances) || !rays->ox || !rays->oy || !rays->oz || !rays->tmin ||
         !rays->dx || !rays->dy || !rays->dz || !rays->tmax)
@@ -1320,18 +1337,22 @@ extern "C" rt_status rt_trace_scene_ex(rt_stream stream, const rt_instance* d_in
     if (depth > kMaxDepth) return RT_ERR_CAPACITY;
     uint32_t* prim = hits ? hits->prim : nullptr;
     float* t = hits ? hits->t : nullptr;
-    const dim3 grid(ceil_div(n, cfg.block)), block(cfg.block);
+    TileMap tm;
+    const dim3 grid(ceil_div(launch_threads(cfg, n, tm), cfg.block)), block(cfg.block);
     const size_t shm = lds_bytes(cfg);
     hipStream_t s = as_stream(stream);
-#define RT_S(P, A)                                                                             \
-    hipLaunchKernelGGL((k_trace_scene<P, A>), grid, block, shm, s, d_instances, count, *rays,  \
-                       n, prim, t, d_inst, occl_bits, cfg.lds)
+    const bool tiled = cfg.tile_width != 0;
+#define RT_S(P, A, T)                                                                          \
+    hipLaunchKernelGGL((k_trace_scene<P, A, T>), grid, block, shm, s, d_instances, count,      \
+                       *rays, n, prim, t, d_inst, occl_bits, cfg.lds, tm)
     if (small_priv(depth, cfg.lds)) {
-        if (any) RT_S(kPrivSmall, true);
-        else RT_S(kPrivSmall, false);
+        if (any) { if (tiled) RT_S(kPrivSmall, true, true); else RT_S(kPrivSmall, true, false); }
+        else if (tiled) RT_S(kPrivSmall, false, true);
+        else RT_S(kPrivSmall, false, false);
     } else {
-        if (any) RT_S(kPrivLarge, true);
-        else RT_S(kPrivLarge, false);
+        if (any) { if (tiled) RT_S(kPrivLarge, true, true); else RT_S(kPrivLarge, true, false); }
+        else if (tiled) RT_S(kPrivLarge, false, true);
+        else RT_S(kPrivLarge, false, false);
     }
 #undef RT_S
     return launch_status("k_trace_scene");

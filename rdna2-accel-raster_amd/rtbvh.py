@@ -732,9 +732,10 @@ class Scene:
         self.d_table = torch.frombuffer(bytearray(bytes(arr)), dtype=torch.uint8).to(dev)
 
     def intersect(self, rays, stream=None, out=None, block=None, lds_stack=None,
-                  schedule="simple"):
+                  schedule="simple", tile=None):
         """Closest hit: (prim, t, instance index) tensors (int32, f32, int32).
-        block / lds_stack: launch shape (same results for every choice)."""
+        block / lds_stack / tile (grid_width, tile_width): launch shape (same
+        results for every choice)."""
         n = len(rays)
         dev = rays.data.device
         prim, t, inst = out if out else (torch.empty(n, dtype=torch.int32, device=dev),
@@ -742,19 +743,19 @@ class Scene:
                                          torch.empty(n, dtype=torch.int32, device=dev))
         hits = rt_hits_soa(ptr(prim), ptr(t))
         r = rays.soa()
-        o = trace_opts(schedule, block=block, lds_stack=lds_stack)
+        o = trace_opts(schedule, block=block, lds_stack=lds_stack, tile=tile)
         call("rt_trace_scene_ex", stream_handle(stream), ptr(self.d_table), u32(self.count),
              u32(self.depth), C.byref(r), u32(n), u32(RT_CLOSEST), C.byref(o), C.byref(hits),
              ptr(inst), None)
         return prim, t, inst
 
     def occluded(self, rays, stream=None, out=None, block=None, lds_stack=None,
-                 schedule="simple"):
+                 schedule="simple", tile=None):
         n = len(rays)
         bits = out if out is not None else torch.empty(max(1, (n + 31) // 32),
                                                        dtype=torch.int32, device=rays.data.device)
         r = rays.soa()
-        o = trace_opts(schedule, block=block, lds_stack=lds_stack)
+        o = trace_opts(schedule, block=block, lds_stack=lds_stack, tile=tile)
         call("rt_trace_scene_ex", stream_handle(stream), ptr(self.d_table), u32(self.count),
              u32(self.depth), C.byref(r), u32(n), u32(RT_ANY), C.byref(o), None, None,
              ptr(bits))

@@ -379,3 +379,8 @@ def test_tile_checks():
                            None) == 1
     o.schedule = 0
     assert lib.rt_trace_batch_ex(None, None, None, 0, 0, 1, C.byref(o)) == 1
+    # the scene trace takes wave tiles (whole rows) but no child order
+    assert lib.rt_trace_scene_ex(None, fake, 1, 0, C.byref(r), 0, 0, C.byref(o), None, None,
+                                 None) == 0
+    assert lib.rt_trace_scene_ex(None, fake, 1, 0, C.byref(r5), 100, 0, C.byref(o),
+                                 C.byref(h), None, None) == 1

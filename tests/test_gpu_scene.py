@@ -122,3 +122,23 @@ def test_blur7_bit_exact(rt, O, w, h):
     torch.cuda.synchronize()
     np.testing.assert_array_equal(to_np(out).view(np.uint32),
                                   O.blur7(img, depth, w, h, 0.05).view(np.uint32))
+
+
+@pytest.mark.parametrize("tw", [8, 16, 32])
+def test_scene_wave_tiles(rt, O, scene, tw):
+    """Wave tiles of a 480-ray-wide grid (320 rows): per-ray results equal
+    the oracle's (closest: ids, t, instance; any: bits)."""
+    dev, orc, _ = scene
+    r = O.random_rays(480 * 320, 21, (-3.2, -3.2, -0.5), (3.2, 3.2, 2.0), tmax=1e30)
+    prim, t, inst = dev.intersect(rt.Rays.from_numpy(r.packed()), tile=(480, tw), block=64)
+    torch.cuda.synchronize()
+    pc, tc, ic = O.trace_scene(orc, r)
+    np.testing.assert_array_equal(u32(prim), pc)
+    np.testing.assert_array_equal(to_np(t).view(np.uint32), tc.view(np.uint32))
+    np.testing.assert_array_equal(u32(inst), ic)
+    r.tmax[:] = 0.6
+    bits = dev.occluded(rt.Rays.from_numpy(r.packed()), tile=(480, tw))
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(_bits(u32(bits), len(r)), O.trace_scene(orc, r, mode=O.ANY))
+    with pytest.raises(rt.RtError):
+        dev.occluded(rt.Rays.from_numpy(r.packed()), tile=(500, 8))  # not whole rows
