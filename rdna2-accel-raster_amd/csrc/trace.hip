@@ -116,11 +116,15 @@ struct Stack {
     int32_t priv[PRIV];
 };
 
+// sp counts the entries; spo = sp * bs is the LDS slot offset of entry sp,
+// kept by additions (a multiply per push / pop is a quarter-rate VALU op);
+// lso = (ls - 1) * bs is the last LDS slot's.
 #define RT_SPUSH(v)                                                      \
     do {                                                                 \
-        if (sp < ls) col[sp * bs] = (v);                                 \
+        if (sp < ls) col[spo] = (v);                                     \
         else st.priv[sp - ls] = (v);                                     \
         ++sp;                                                            \
+        spo += bs;                                                       \
     } while (0)
 
 // Pop: the LDS read uses a clamped, always-valid slot (a ds_read); private
@@ -130,7 +134,8 @@ struct Stack {
 #define RT_SPOP(dst)                                                     \
     do {                                                                 \
         --sp;                                                            \
-        dst = col[min(sp, ls - 1) * bs];                                 \
+        spo -= bs;                                                       \
+        dst = col[min(spo, lso)];                                        \
         if (sp >= ls)                                                    \
             dst = reinterpret_cast<volatile int32_t*>(st.priv)[sp - ls]; \
     } while (0)
@@ -245,8 +250,13 @@ __device__ __forceinline__ void trace_ray(NP __restrict__ nodes, TP __restrict__
     const float ix = fabsf(dx) > 1e-6f ? 1.0f / dx : 1e6f;
     const float iy = fabsf(dy) > 1e-6f ? 1.0f / dy : 1e6f;
     const float iz = fabsf(dz) > 1e-6f ? 1.0f / dz : 1e6f;
+    // keys of hit children are at least -FLT_MAX (below: missed children are
+    // keyed -inf); clamping tmin once gives every key that bound, as
+    // fmaxf(te, -FLT_MAX) per child would (max is exact)
+    const float tminc = fmaxf(tmin, -FLT_MAX);
     Stack<PRIV> st;
-    int sp = 0;
+    int sp = 0, spo = 0;
+    const int lso = (ls - 1) * bs;
     if (start) RT_SPUSH(0);
     uint32_t pops = 0;
 
@@ -287,7 +297,7 @@ __device__ __forceinline__ void trace_ray(NP __restrict__ nodes, TP __restrict__
             float tminX = fminf(t0x, t1x), tmaxX = fmaxf(t0x, t1x);
             float tminY = fminf(t0y, t1y), tmaxY = fmaxf(t0y, t1y);
             float tminZ = fminf(t0z, t1z), tmaxZ = fmaxf(t0z, t1z);
-            te[k] = fmaxf(fmaxf(tminX, tminY), fmaxf(tminZ, tmin));
+            te[k] = fmaxf(fmaxf(tminX, tminY), fmaxf(tminZ, tminc));
             float tx = fminf(fminf(tmaxX, tmaxY), fminf(tmaxZ, hitT));
             hit[k] = cnv[k] != 0 && tx >= te[k];
         }
@@ -299,8 +309,7 @@ __device__ __forceinline__ void trace_ray(NP __restrict__ nodes, TP __restrict__
         Child e[4];
 #pragma unroll
         for (int k = 0; k < 4; ++k)
-            e[k] = hit[k] ? Child{fmaxf(te[k], -FLT_MAX), chv[k], cnv[k]}
-                          : Child{-INFINITY, 0, 0};
+            e[k] = hit[k] ? Child{te[k], chv[k], cnv[k]} : Child{-INFINITY, 0, 0};
         keep_loaded(ch);
         if (!(hit[0] || hit[1] || hit[2] || hit[3])) continue;
         if constexpr (ORDER == kOrderNear) {
@@ -645,8 +654,10 @@ __device__ __forceinline__ void refill_wave(
     float ox = 0.f, oy = 0.f, oz = 0.f, tmin = 1.f, dx = 0.f, dy = 0.f, dz = 1.f;
     float ix = 0.f, iy = 0.f, iz = 0.f, hitT = 0.f;
     uint32_t hitPrim = RT_MISS, boxTests = 0, triTests = 0;
+    float tminc = -FLT_MAX;  // fmaxf(tmin, -FLT_MAX), as in trace_ray
     Stack<PRIV> st;
-    int sp = 0;
+    int sp = 0, spo = 0;
+    const int lso = (ls - 1) * bs;
 
     for (;;) {
         const uint64_t idle = __ballot(!active);
@@ -661,9 +672,11 @@ __device__ __forceinline__ void refill_wave(
                     ix = fabsf(dx) > 1e-6f ? 1.0f / dx : 1e6f;
                     iy = fabsf(dy) > 1e-6f ? 1.0f / dy : 1e6f;
                     iz = fabsf(dz) > 1e-6f ? 1.0f / dz : 1e6f;
+                    tminc = fmaxf(tmin, -FLT_MAX);
                     hitPrim = RT_MISS;
                     boxTests = triTests = 0;
                     sp = 0;
+                    spo = 0;
                     if (node_count > 0) RT_SPUSH(0);
                     active = true;
                 }
@@ -700,15 +713,14 @@ __device__ __forceinline__ void refill_wave(
                 float tminX = fminf(t0x, t1x), tmaxX = fmaxf(t0x, t1x);
                 float tminY = fminf(t0y, t1y), tmaxY = fmaxf(t0y, t1y);
                 float tminZ = fminf(t0z, t1z), tmaxZ = fmaxf(t0z, t1z);
-                te[k] = fmaxf(fmaxf(tminX, tminY), fmaxf(tminZ, tmin));
+                te[k] = fmaxf(fmaxf(tminX, tminY), fmaxf(tminZ, tminc));
                 float tx = fminf(fminf(tmaxX, tmaxY), fminf(tmaxZ, hitT));
                 hit[k] = cnv[k] != 0 && tx >= te[k];
             }
             Child e[4];  // in-place exchange sort, as in k_trace
 #pragma unroll
             for (int k = 0; k < 4; ++k)
-                e[k] = hit[k] ? Child{fmaxf(te[k], -FLT_MAX), chv[k], cnv[k]}
-                              : Child{-INFINITY, 0, 0};
+                e[k] = hit[k] ? Child{te[k], chv[k], cnv[k]} : Child{-INFINITY, 0, 0};
             keep_loaded(ch);
             if (hit[0] || hit[1] || hit[2] || hit[3]) {
                 cx(e[0], e[1]); cx(e[0], e[2]); cx(e[0], e[3]);
@@ -848,7 +860,8 @@ __global__ __launch_bounds__(kBlock) void k_trace_bvh2(
     float closest = tmax;
     uint32_t prim = RT_MISS, boxT = 0, triT = 0;
     Stack<PRIV> st;
-    int sp = 0;
+    int sp = 0, spo = 0;
+    const int lso = (ls - 1) * bs;
     if (valid && node_count > 0) RT_SPUSH(0);
     bool done = false;
     while (sp > 0 && !done) {
