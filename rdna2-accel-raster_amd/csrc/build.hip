@@ -1052,7 +1052,19 @@ __device__ void build_small(SmallLds& S, const SmallTree& T, const float* __rest
                 }
                 rank[i] = first[i];
             }
-            for (uint32_t j = 0; j < most; ++j) {
+            // the first kRankUnroll members of every bucket in one batch of
+            // independent LDS reads (reads past a bucket are masked; indices
+            // stay inside comp), larger buckets continue one member a trip
+            constexpr uint32_t kRankUnroll = 8;
+#pragma unroll
+            for (uint32_t j = 0; j < kRankUnroll; ++j) {
+#pragma unroll
+                for (int i = 0; i < kSmallItems; ++i) {
+                    const uint32_t c = S.u.msd.comp[min(first[i] + j, kSmallMax - 1u)];
+                    rank[i] += (j < len[i] && c < me[i]) ? 1u : 0u;
+                }
+            }
+            for (uint32_t j = kRankUnroll; j < most; ++j) {
 #pragma unroll
                 for (int i = 0; i < kSmallItems; ++i)
                     if (j < len[i]) rank[i] += S.u.msd.comp[first[i] + j] < me[i] ? 1u : 0u;
