@@ -3,7 +3,8 @@
 Workload (one step = one frame of the shadow any-hit configuration):
   mesh   UV sphere 1250 x 1000 quads = 2,500,000 FP32 triangles, radius 1,
          seeded random radial displacement +-2 % (generated on the device),
-         Box4 BVH built on the device once (inputs resident before timing)
+         Box4 BVH built on the device once (inputs resident before timing;
+         radix tree, leaves of at most 2-4 triangles as tuned on whole frames)
   step   1920x1080 pinhole camera rays (60 deg vertical FOV, every pixel hits)
          -> closest-hit trace -> 16 shadow rays per pixel towards 16 lights
          uniform in the box 3x the mesh bounds (tmax = light distance)
@@ -76,17 +77,7 @@ class Frame:
         # radix digit width of the sort: measured when tuning (same tree for each)
         self.sort_bits = (tuner.sort_bits(self.pos, self.idx, algorithm=self.builder,
                                           tag="sphere") if tuner else rt.DEFAULT_SORT_BITS)
-        rt.GpuBVH4.build(self.pos, self.idx, algorithm=self.builder,
-                         sort_bits=self.sort_bits)  # warm the build path
-        torch.cuda.synchronize()
-        s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        s.record()
-        self.bvh = rt.GpuBVH4.build(self.pos, self.idx, algorithm=self.builder, sync=False,
-                                    sort_bits=self.sort_bits)
-        e.record()
-        e.synchronize()
-        self.build_ms = s.elapsed_time(e)
-        self.bvh.node_count, self.bvh.depth = [int(v) for v in self.bvh.info[:2].cpu().tolist()]
+        self.build_tree(4)
         lo = self.pos.min(0).values.cpu().numpy().astype(np.float64)
         hi = self.pos.max(0).values.cpu().numpy().astype(np.float64)
         c, ext = (lo + hi) / 2, (hi - lo)
@@ -102,9 +93,38 @@ class Frame:
         self.ev = None
         self.primary_opts, self.shadow_opts = {}, {}
 
+    def build_tree(self, leaf):
+        """The frame's radix tree with leaves of at most `leaf` triangles
+        (device build, timed after a warm-up build)."""
+        rt.GpuBVH4.build(self.pos, self.idx, algorithm=self.builder, sort_bits=self.sort_bits,
+                         max_leaf_prims=leaf)  # warm the build path
+        torch.cuda.synchronize()
+        s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        s.record()
+        self.bvh = rt.GpuBVH4.build(self.pos, self.idx, algorithm=self.builder, sync=False,
+                                    sort_bits=self.sort_bits, max_leaf_prims=leaf)
+        e.record()
+        e.synchronize()
+        self.build_ms = s.elapsed_time(e)
+        self.bvh.node_count, self.bvh.depth = [int(v) for v in self.bvh.info[:2].cpu().tolist()]
+        self.leaf = leaf
+
     def tune(self, tuner):
-        """Schedule / launch shape of both traces, measured on this frame's
-        rays (every choice gives the same hits)."""
+        """The tree's leaf size (whole frames timed; a size that changes the
+        frame's masks is left out), then the schedule / launch shape of both
+        traces, measured on this frame's rays (every choice gives the same
+        hits)."""
+        trees = {self.leaf: (self.bvh, self.build_ms)}
+
+        def use(leaf):
+            if leaf not in trees:
+                self.build_tree(leaf)
+                trees[leaf] = (self.bvh, self.build_ms)
+            self.bvh, self.build_ms = trees[leaf]
+            self.leaf = leaf
+
+        use(tuner.frame_leaf_size(use, self.step, lambda: self.masks, tag="sphere-2p5m"))
+        trees.clear()
         self.step()
         torch.cuda.synchronize()
         self.primary_opts = tuner.trace(self.bvh, self.primary, "closest", tag="primary",
@@ -661,7 +681,8 @@ def main():
                    "schedule": f.shadow_opts.get("schedule", a.schedule),
                    "primary_trace": f.primary_opts or "default",
                    "shadow_trace": f.shadow_opts or "default",
-                   "bvh_nodes": f.bvh.node_count, "bvh_depth": f.bvh.depth},
+                   "bvh_nodes": f.bvh.node_count, "bvh_depth": f.bvh.depth,
+                   "max_leaf_prims": f.leaf},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic, "kernel": "k_trace_persist any-hit (shadow)"

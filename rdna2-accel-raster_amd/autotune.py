@@ -36,6 +36,7 @@ TRACE_LDS = (8, 12, 16, 20, 24, 32)
 REFILL_RAYS_PER_WAVE = (256, 1024, 4096)
 REFILL_LANES = (8, 16, 32)
 TILE_WIDTHS = (8, 16, 32)  # wave tiles of a ray grid: 8x8, 16x4, 32x2 rays
+LEAF_SIZES = (4, 3, 2)     # radix-tree leaf sizes of a frame's tree, the default first
 SORT_BITS = rt.SORT_BITS
 BUILD_WORKGROUPS = (64, 128, 256, 512, 1024)
 
@@ -101,9 +102,11 @@ class Tuner:
                 json.dump(self.cache, f, indent=1, sort_keys=True)
             os.replace(tmp, self.cache_path)
 
-    def _pick(self, key, candidates, run, check):
+    def _pick(self, key, candidates, run, check, skip_changed=False):
         """Times run(c) for every candidate c, checks check(c) == the first
-        candidate's output, caches and returns the fastest."""
+        candidate's output, caches and returns the fastest.  A candidate whose
+        output differs raises, or with skip_changed (knobs that may change
+        results, such as the tree's leaf size) is left out."""
         if key in self.cache:
             return self.cache[key]["choice"]
         t0 = time.perf_counter()
@@ -114,6 +117,9 @@ class Tuner:
             out = check(c)
             torch.cuda.synchronize()
             if not _same(out, ref):
+                if skip_changed:
+                    self.history.append((key, c, None))
+                    continue
                 raise AssertionError(f"tuner: candidate {c} changed the result of {key}")
             ms = time_ms(lambda: run(c), reps=self.reps)
             table.append((c, ms))
@@ -246,6 +252,25 @@ class Tuner:
                     t.sorted_ids.clone(), t.codes.clone(), torch.tensor([count, depth]))
 
         return self._pick(key, cands, run, check)
+
+    def frame_leaf_size(self, use, step, output, tag="", sizes=LEAF_SIZES):
+        """Leaf size (max_leaf_prims) of a frame's radix tree: use(size) makes
+        the frame trace a tree of that leaf size (built once, outside the
+        timing), step() runs one frame, output() is the frame's result.  A
+        leaf size whose frame result differs from the first size's (a tie
+        between two triangles at the same t resolved another way) is left
+        out."""
+
+        def run(c):
+            use(c)
+            return step()
+
+        def check(c):
+            use(c)
+            step()
+            return _clone(output())
+
+        return self._pick(f"leaf|{tag}", list(sizes), run, check, skip_changed=True)
 
     def build_batch(self, batch, tag=""):
         """Persistent-grid size for BuildBatch.launch."""

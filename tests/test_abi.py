@@ -211,6 +211,9 @@ def test_tuner_search_space_is_valid():
                                        C.byref(o), None, None, None, None) == 0
     for bits in autotune.SORT_BITS:
         rtbvh.build_requirements(100_000, rtbvh.build_cfg("radix", 4, bits))
+    assert autotune.LEAF_SIZES[0] == 4  # the default leaf size is the reference for the others
+    for leaf in autotune.LEAF_SIZES:
+        rtbvh.build_requirements(100_000, rtbvh.build_cfg("radix", leaf))
 
 
 def test_build_batch_plan_on_host():
