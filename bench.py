@@ -621,8 +621,11 @@ def main():
     if a.copy_probe:
         copy_probe()
 
+    # traces are tuned on their whole ray sets (the frame's 33 M shadow rays take
+    # ~12 ms a launch): sampled subsets rank near-equal shapes unreliably
     tuner = None if a.no_tune else autotune.Tuner(
-        cache_path=a.tune_cache, log=lambda m: print(m, file=sys.stderr, flush=True))
+        cache_path=a.tune_cache, sample_rays=1 << 26,
+        log=lambda m: print(m, file=sys.stderr, flush=True))
     b4k_p50, b4k_p99, b4k_b2b = build_time_4k()
     f = Frame(tuner)
     refit_ms = refit_time_ms(f)

@@ -37,6 +37,22 @@ REFILL_RAYS_PER_WAVE = (256, 1024, 4096)
 REFILL_LANES = (8, 16, 32)
 TILE_WIDTHS = (8, 16, 32)  # wave tiles of a ray grid: 8x8, 16x4, 32x2 rays
 LEAF_SIZES = (4, 3, 2)     # radix-tree leaf sizes of a frame's tree, the default first
+SAMPLE_ROW_BLOCK = 8       # rows per block of a grid sample (a multiple of every tile height)
+
+
+def _grid_sample(rays, width, m):
+    """About m rays of a grid of `width`-ray rows as blocks of SAMPLE_ROW_BLOCK
+    whole rows spread evenly over all rows (every light layer / image region
+    is represented, and the sample is a grid of the same width again)."""
+    rows = len(rays) // width
+    blocks = max(1, min(rows // SAMPLE_ROW_BLOCK, m // (width * SAMPLE_ROW_BLOCK)))
+    if blocks * SAMPLE_ROW_BLOCK >= rows:
+        return rays
+    step = rows // blocks
+    starts = torch.arange(blocks, device=rays.data.device) * step
+    row = (starts[:, None] + torch.arange(SAMPLE_ROW_BLOCK, device=rays.data.device)).reshape(-1)
+    col = (row[:, None] * width + torch.arange(width, device=rays.data.device)).reshape(-1)
+    return rt.Rays(rays.data[:, col].contiguous())
 SORT_BITS = rt.SORT_BITS
 BUILD_WORKGROUPS = (64, 128, 256, 512, 1024)
 
@@ -145,9 +161,9 @@ class Tuner:
         whole rows).  Returns the keyword arguments for intersect / occluded."""
         n = len(rays)
         m = min(n, self.sample_rays)
-        if grid_width:
-            m = max(grid_width, m // grid_width * grid_width)
-        sample = rays if m == n else rt.Rays(rays.data[:, :m])
+        sample = rays if m == n else (_grid_sample(rays, grid_width, m) if grid_width
+                                      else rt.Rays(rays.data[:, :m]))
+        m = len(sample)
         key = (f"trace|{mode}|{tag}|d{bvh.depth}|t{_size_class(bvh.tri_count)}"
                f"|r{_size_class(m)}" + (f"|g{grid_width}" if grid_width else ""))
         fn = bvh.intersect if mode == "closest" else bvh.occluded
@@ -182,9 +198,9 @@ class Tuner:
         grid_width is given) for Scene.intersect / occluded."""
         n = len(rays)
         m = min(n, self.sample_rays)
-        if grid_width:
-            m = max(grid_width, m // grid_width * grid_width)
-        sample = rays if m == n else rt.Rays(rays.data[:, :m])
+        sample = rays if m == n else (_grid_sample(rays, grid_width, m) if grid_width
+                                      else rt.Rays(rays.data[:, :m]))
+        m = len(sample)
         key = (f"scene|{mode}|{tag}|i{scene.count}|d{scene.depth}|r{_size_class(m)}"
                + (f"|g{grid_width}" if grid_width else ""))
         fn = scene.intersect if mode == "closest" else scene.occluded
