@@ -13,6 +13,9 @@ Scheduling (one host thread):
     still on the device, so requests arriving while the GPU is busy gather
     into the next tick's batch (continuous batching) instead of each going
     out alone;
+  * a tick started while the device is idle takes at most `idle_batch`
+    requests, so the first launch is not held back by a full tick's host-side
+    setup; ticks started while others run take up to `max_batch`;
   * each tick admits from the head of the queue while the head's bytes fit
     in the budget and fewer than `max_inflight` requests are in flight
     (FIFO: a large head is never overtaken);
@@ -173,7 +176,7 @@ def _trace_alone(r, scene, stream=None, shape=None):
 class Server:
     def __init__(self, scenes, budget_bytes=2 << 30, streams=3, rebuild_streams=4,
                  max_inflight=256, max_batch=128, keep_results=False, trace_shape=None,
-                 max_ticks=None):
+                 max_ticks=None, idle_batch=16):
         self.scenes = scenes
         self.max_ticks = max_ticks or streams
         self.trace_shape = dict(trace_shape or {})
@@ -181,6 +184,9 @@ class Server:
         self.budget = budget_bytes
         self.max_inflight = max_inflight
         self.max_batch = max_batch
+        # while the device is idle a tick takes at most idle_batch requests, so
+        # the first launch is not held back by a full tick's host-side setup
+        self.idle_batch = min(idle_batch, max_batch) if idle_batch else max_batch
         self.streams = [torch.cuda.Stream() for _ in range(streams)]
         self.rebuild_streams = [torch.cuda.Stream() for _ in range(rebuild_streams)]
         self.rebuilds = 0
@@ -191,7 +197,8 @@ class Server:
     def _admit(self, queue, inflight, now):
         took = []
         busy = sum(len(reqs) for _, _, reqs in inflight)
-        while queue and busy + len(took) < self.max_inflight and len(took) < self.max_batch:
+        limit = self.max_batch if inflight else self.idle_batch
+        while queue and busy + len(took) < self.max_inflight and len(took) < limit:
             r = queue[0]
             r.bytes = request_bytes(r)
             if self.used + r.bytes > self.budget:
