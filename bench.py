@@ -436,15 +436,17 @@ def ao_frame_leg(frames=100, w=1920, h=1080, tuner=None):
     two-level scene.  Per frame: regenerate the grid and rebuild its tree
     (fused path, same buffers), 1920x1080 primary closest hit, 1 SPP cosine
     AO rays (tmax 0.5), any hit, AO resolve, 7x7 bilateral filter.  With a
-    tuner, both scene traces use its measured launch shapes."""
+    tuner, the terrain tree's leaf size is timed on whole frames (a size that
+    changes the filtered image is left out) and both scene traces use its
+    measured launch shapes."""
     tp, ti = rt.gen_heightfield(707, 707, 4.0, 0.3, 31)
     terrain = rt.GpuBVH4.build(tp, ti, algorithm="radix")
     gx = rt.gen_dynamic_grid(4096, 0.0, 2.0)
     grid = rt.GpuBVH4.build(gx, None, algorithm="implicit")
     cfg = rt.build_cfg("implicit")
     m_grid = np.array([[1, 0, 0, 0], [0, 1, 0, 0], [0, 0, 1, 4.6]], np.float32)
-    scene = rt.Scene([(terrain, tp, ti, None, rt.INST_IDENTITY),
-                      (grid, gx, None, m_grid, rt.INST_TRANSLATE)])
+    state = {"leaf": 4, "scene": rt.Scene([(terrain, tp, ti, None, rt.INST_IDENTITY),
+                                           (grid, gx, None, m_grid, rt.INST_TRANSLATE)])}
     eye, look, up = (0.0, -6.0, 4.0), (0.0, 0.0, 0.2), (0.0, 0.0, 1.0)
     npix = w * h
     primary, ao_rays = rt.Rays.empty(npix), rt.Rays.empty(npix)
@@ -464,6 +466,7 @@ def ao_frame_leg(frames=100, w=1920, h=1080, tuner=None):
                 rt.C.byref(cfg), rt.ptr(grid.nodes), rt.u32(grid.node_count), rt.ptr(grid.tris),
                 rt.ptr(grid.info), None, None, None, rt.C.c_size_t(0))
         rt.gen_camera_rays(w, h, eye, look, up, 60.0, 0.0, 1e30, out=primary)
+        scene = state["scene"]
         scene.intersect(primary, out=(prim, t, inst), **shapes[0])
         scene.gen_ao_rays(primary, prim, inst, t, w, h, frame=k, offset=0.03, tmin=0.01,
                           tmax=0.5, out=ao_rays)
@@ -476,8 +479,21 @@ def ao_frame_leg(frames=100, w=1920, h=1080, tuner=None):
         frame(k)
     torch.cuda.synchronize()
     if tuner is not None:
-        shapes[0] = tuner.trace_scene(scene, primary, "closest", tag="ao-primary", grid_width=w)
-        shapes[1] = tuner.trace_scene(scene, ao_rays, "any", tag="ao", grid_width=w)
+        scenes = {4: state["scene"]}
+
+        def use(leaf):
+            if leaf not in scenes:
+                tb = rt.GpuBVH4.build(tp, ti, algorithm="radix", max_leaf_prims=leaf)
+                scenes[leaf] = rt.Scene([(tb, tp, ti, None, rt.INST_IDENTITY),
+                                         (grid, gx, None, m_grid, rt.INST_TRANSLATE)])
+            state["scene"], state["leaf"] = scenes[leaf], leaf
+
+        use(tuner.frame_leaf_size(use, lambda: frame(0), lambda: out, tag="ao-terrain"))
+        scenes.clear()
+        frame(0)
+        shapes[0] = tuner.trace_scene(state["scene"], primary, "closest", tag="ao-primary",
+                                      grid_width=w)
+        shapes[1] = tuner.trace_scene(state["scene"], ao_rays, "any", tag="ao", grid_width=w)
         torch.cuda.synchronize()
     s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     s.record()
@@ -521,7 +537,7 @@ def ao_frame_leg(frames=100, w=1920, h=1080, tuner=None):
                         "frame (two-level scene), 1920x1080 primary + 1 SPP AO (tmax 0.5) + 7x7 "
                         "bilateral", "frames": frames, "ms_per_frame": round(ms, 3),
             "fps": round(1000.0 / ms, 1), "mrays_per_s": round(2 * npix / ms / 1e3, 1),
-            "trace_shapes": shapes,
+            "trace_shapes": shapes, "terrain_max_leaf_prims": state["leaf"],
             "ao_mean": round(float(out.mean()), 4),
             "quality_filtered_vs_1spp": {"mse": round(mse, 6), "psnr_db": round(psnr, 3),
                                          "ssim": round(ssim, 5)},
