@@ -4,7 +4,7 @@ Workload (one step = one frame of the shadow any-hit configuration):
   mesh   UV sphere 1250 x 1000 quads = 2,500,000 FP32 triangles, radius 1,
          seeded random radial displacement +-2 % (generated on the device),
          Box4 BVH built on the device once (inputs resident before timing;
-         radix tree, leaves of at most 2-4 triangles as tuned on whole frames)
+         radix tree, leaves of at most 1-4 triangles as tuned on whole frames)
   step   1920x1080 pinhole camera rays (60 deg vertical FOV, every pixel hits)
          -> closest-hit trace -> 16 shadow rays per pixel towards 16 lights
          uniform in the box 3x the mesh bounds (tmax = light distance)

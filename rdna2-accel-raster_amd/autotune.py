@@ -36,7 +36,7 @@ TRACE_LDS = (8, 12, 16, 20, 24, 32)
 REFILL_RAYS_PER_WAVE = (256, 1024, 4096)
 REFILL_LANES = (8, 16, 32)
 TILE_WIDTHS = (8, 16, 32)  # wave tiles of a ray grid: 8x8, 16x4, 32x2 rays
-LEAF_SIZES = (4, 3, 2)     # radix-tree leaf sizes of a frame's tree, the default first
+LEAF_SIZES = (4, 3, 2, 1)  # radix-tree leaf sizes of a frame's tree, the default first
 SAMPLE_ROW_BLOCK = 8       # rows per block of a grid sample (a multiple of every tile height)
 
 

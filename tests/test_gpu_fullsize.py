@@ -22,10 +22,10 @@ pytestmark = pytest.mark.gpu
 NU, NV, W, H, NL = 1250, 1000, 1920, 1080, 16
 
 
-@pytest.fixture(scope="module", params=[4, 2], ids=["leaf4", "leaf2"])
+@pytest.fixture(scope="module", params=[4, 2, 1], ids=["leaf4", "leaf2", "leaf1"])
 def sphere(rt, request):
     """The frame's sphere and its radix tree, with leaves of at most 4 (the
-    default) or 2 triangles (a leaf size the frame's tuner may pick)."""
+    default), 2 or 1 triangles (leaf sizes the frame's tuner may pick)."""
     pos, idx = rt.gen_sphere(NU, NV, 1.0, 0.02, 7)
     bvh = rt.GpuBVH4.build(pos, idx, algorithm="radix", keep_order=True,
                            max_leaf_prims=request.param)
