@@ -110,27 +110,29 @@ class Frame:
         self.leaf = leaf
 
     def tune(self, tuner):
-        """The tree's leaf size (whole frames timed; a size that changes the
-        frame's masks is left out), then the schedule / launch shape of both
-        traces, measured on this frame's rays (every choice gives the same
-        hits)."""
-        trees = {self.leaf: (self.bvh, self.build_ms)}
+        """The tree's leaf size, timed on whole frames, each tree traced with
+        its own tuned schedule / launch shapes (leaf sizes 1 and 2 are within
+        a few per cent of each other under the default shapes, and their
+        order flips once the shapes are tuned).  A size that changes the
+        frame's masks is left out; every shape gives the same hits."""
+        trees = {}
 
         def use(leaf):
             if leaf not in trees:
-                self.build_tree(leaf)
-                trees[leaf] = (self.bvh, self.build_ms)
-            self.bvh, self.build_ms = trees[leaf]
+                if trees or leaf != self.leaf:  # the first call may reuse __init__'s tree
+                    self.build_tree(leaf)
+                self.step()
+                torch.cuda.synchronize()
+                self.primary_opts = tuner.trace(self.bvh, self.primary, "closest",
+                                                tag=f"primary-leaf{leaf}", grid_width=W)
+                self.shadow_opts = tuner.trace(self.bvh, self.shadow, "any",
+                                               tag=f"shadow-leaf{leaf}", grid_width=W)
+                trees[leaf] = (self.bvh, self.build_ms, self.primary_opts, self.shadow_opts)
+            self.bvh, self.build_ms, self.primary_opts, self.shadow_opts = trees[leaf]
             self.leaf = leaf
 
         use(tuner.frame_leaf_size(use, self.step, lambda: self.masks, tag="sphere-2p5m"))
         trees.clear()
-        self.step()
-        torch.cuda.synchronize()
-        self.primary_opts = tuner.trace(self.bvh, self.primary, "closest", tag="primary",
-                                        grid_width=W)
-        self.shadow_opts = tuner.trace(self.bvh, self.shadow, "any", tag="shadow",
-                                       grid_width=W)
 
     def step(self, events=None):
         rt.gen_camera_rays(W, H, (0.0, 0.0, 1.25), (0.0, 0.0, 0.0), (0.0, 1.0, 0.0), 60.0,

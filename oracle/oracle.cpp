@@ -27,7 +27,7 @@ inline float hash01(uint32_t seed, uint32_t i) {
 
 template <class F>
 void parallel_chunks(uint32_t n, int nthreads, F fn) {
-    const uint32_t chunk = 4096;  // BASELINE.md plan: 4,096-ray chunks
+    const uint32_t chunk = 4096;  // rays handed to a thread at a time
     if (nthreads <= 1 || n <= chunk) {
         fn(0u, n);
         return;
