@@ -4,7 +4,8 @@ Workload (one step = one frame of the shadow any-hit configuration):
   mesh   UV sphere 1250 x 1000 quads = 2,500,000 FP32 triangles, radius 1,
          seeded random radial displacement +-2 % (generated on the device),
          Box4 BVH built on the device once (inputs resident before timing;
-         radix tree, leaves of at most 1-4 triangles as tuned on whole frames)
+         radix tree, leaves of at most 1-4 triangles and 0 or 4 rounds of
+         treelet restructuring, as tuned on whole frames)
   step   1920x1080 pinhole camera rays (60 deg vertical FOV, every pixel hits)
          -> closest-hit trace -> 16 shadow rays per pixel towards 16 lights
          uniform in the box 3x the mesh bounds (tmax = light distance)
@@ -68,6 +69,24 @@ def build_time_4k(iters=200, warm=20):
     return median(times), percentile(times, 0.99), back_to_back
 
 
+FRAME_TREELET_ROUNDS = (0, 4)  # treelet restructuring rounds the frame's tree is tuned over
+
+
+def same_frame(a, b):
+    """Frame results (primary t, primary triangle, light masks) equal up to
+    ties: every primary t bit-identical, the masks identical wherever the
+    primary triangle is, and another triangle at under 1e-4 of the pixels
+    (two triangles meeting at exactly the hit t: which one a tree reports
+    decides the shadow rays' origin triangle)."""
+    (t1, p1, m1), (t0, p0, m0) = a, b
+    if not torch.equal(t1.view(torch.int32), t0.view(torch.int32)):
+        return False
+    same_prim = p1 == p0
+    if (~same_prim).float().mean().item() >= 1e-4:
+        return False
+    return torch.equal(m1[same_prim], m0[same_prim])
+
+
 class Frame:
     builder = "radix"
 
@@ -77,7 +96,7 @@ class Frame:
         # radix digit width of the sort: measured when tuning (same tree for each)
         self.sort_bits = (tuner.sort_bits(self.pos, self.idx, algorithm=self.builder,
                                           tag="sphere") if tuner else rt.DEFAULT_SORT_BITS)
-        self.build_tree(4)
+        self.build_tree(4, 0)
         lo = self.pos.min(0).values.cpu().numpy().astype(np.float64)
         hi = self.pos.max(0).values.cpu().numpy().astype(np.float64)
         c, ext = (lo + hi) / 2, (hi - lo)
@@ -93,45 +112,53 @@ class Frame:
         self.ev = None
         self.primary_opts, self.shadow_opts = {}, {}
 
-    def build_tree(self, leaf):
-        """The frame's radix tree with leaves of at most `leaf` triangles
-        (device build, timed after a warm-up build)."""
+    def build_tree(self, leaf, rounds=0):
+        """The frame's radix tree with leaves of at most `leaf` triangles and
+        `rounds` rounds of treelet restructuring (device build, timed after a
+        warm-up build)."""
         rt.GpuBVH4.build(self.pos, self.idx, algorithm=self.builder, sort_bits=self.sort_bits,
-                         max_leaf_prims=leaf)  # warm the build path
+                         max_leaf_prims=leaf, treelet_rounds=rounds)  # warm the build path
         torch.cuda.synchronize()
         s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         s.record()
         self.bvh = rt.GpuBVH4.build(self.pos, self.idx, algorithm=self.builder, sync=False,
-                                    sort_bits=self.sort_bits, max_leaf_prims=leaf)
+                                    sort_bits=self.sort_bits, max_leaf_prims=leaf,
+                                    treelet_rounds=rounds)
         e.record()
         e.synchronize()
         self.build_ms = s.elapsed_time(e)
         self.bvh.node_count, self.bvh.depth = [int(v) for v in self.bvh.info[:2].cpu().tolist()]
-        self.leaf = leaf
+        self.leaf, self.rounds = leaf, rounds
 
     def tune(self, tuner):
-        """The tree's leaf size, timed on whole frames, each tree traced with
-        its own tuned schedule / launch shapes (leaf sizes 1 and 2 are within
-        a few per cent of each other under the default shapes, and their
-        order flips once the shapes are tuned).  A size that changes the
-        frame's masks is left out; every shape gives the same hits."""
+        """The tree (leaf size x treelet rounds), timed on whole frames, each
+        tree traced with its own tuned schedule / launch shapes (leaf sizes 1
+        and 2 are within a few per cent of each other under the default
+        shapes, and their order flips once the shapes are tuned).  Every
+        shape gives the same hits; a tree that changes the frame's result
+        beyond ties (same_frame) is left out."""
         trees = {}
 
-        def use(leaf):
-            if leaf not in trees:
-                if trees or leaf != self.leaf:  # the first call may reuse __init__'s tree
-                    self.build_tree(leaf)
+        def use(c):
+            leaf, rounds = int(c[0]), int(c[1])
+            if (leaf, rounds) not in trees:
+                if trees or (leaf, rounds) != (self.leaf, self.rounds):  # __init__'s tree
+                    self.build_tree(leaf, rounds)
                 self.step()
                 torch.cuda.synchronize()
+                tag = f"leaf{leaf}-r{rounds}"
                 self.primary_opts = tuner.trace(self.bvh, self.primary, "closest",
-                                                tag=f"primary-leaf{leaf}", grid_width=W)
+                                                tag=f"primary-{tag}", grid_width=W)
                 self.shadow_opts = tuner.trace(self.bvh, self.shadow, "any",
-                                               tag=f"shadow-leaf{leaf}", grid_width=W)
-                trees[leaf] = (self.bvh, self.build_ms, self.primary_opts, self.shadow_opts)
-            self.bvh, self.build_ms, self.primary_opts, self.shadow_opts = trees[leaf]
-            self.leaf = leaf
+                                               tag=f"shadow-{tag}", grid_width=W)
+                trees[(leaf, rounds)] = (self.bvh, self.build_ms, self.primary_opts,
+                                         self.shadow_opts)
+            self.bvh, self.build_ms, self.primary_opts, self.shadow_opts = trees[(leaf, rounds)]
+            self.leaf, self.rounds = leaf, rounds
 
-        use(tuner.frame_leaf_size(use, self.step, lambda: self.masks, tag="sphere-2p5m"))
+        cands = [(leaf, r) for r in FRAME_TREELET_ROUNDS for leaf in autotune.LEAF_SIZES]
+        use(tuner.frame_leaf_size(use, self.step, lambda: (self.t, self.prim, self.masks),
+                                  tag="sphere-2p5m", sizes=cands, same=same_frame))
         trees.clear()
 
     def step(self, events=None):
@@ -447,7 +474,7 @@ def ao_frame_leg(frames=100, w=1920, h=1080, tuner=None):
     grid = rt.GpuBVH4.build(gx, None, algorithm="implicit")
     cfg = rt.build_cfg("implicit")
     m_grid = np.array([[1, 0, 0, 0], [0, 1, 0, 0], [0, 0, 1, 4.6]], np.float32)
-    state = {"leaf": 4, "scene": rt.Scene([(terrain, tp, ti, None, rt.INST_IDENTITY),
+    state = {"leaf": 4, "rounds": 0, "scene": rt.Scene([(terrain, tp, ti, None, rt.INST_IDENTITY),
                                            (grid, gx, None, m_grid, rt.INST_TRANSLATE)])}
     eye, look, up = (0.0, -6.0, 4.0), (0.0, 0.0, 0.2), (0.0, 0.0, 1.0)
     npix = w * h
@@ -481,16 +508,20 @@ def ao_frame_leg(frames=100, w=1920, h=1080, tuner=None):
         frame(k)
     torch.cuda.synchronize()
     if tuner is not None:
-        scenes = {4: state["scene"]}
+        scenes = {(4, 0): state["scene"]}
 
-        def use(leaf):
-            if leaf not in scenes:
-                tb = rt.GpuBVH4.build(tp, ti, algorithm="radix", max_leaf_prims=leaf)
-                scenes[leaf] = rt.Scene([(tb, tp, ti, None, rt.INST_IDENTITY),
-                                         (grid, gx, None, m_grid, rt.INST_TRANSLATE)])
-            state["scene"], state["leaf"] = scenes[leaf], leaf
+        def use(c):
+            leaf, rounds = int(c[0]), int(c[1])
+            if (leaf, rounds) not in scenes:
+                tb = rt.GpuBVH4.build(tp, ti, algorithm="radix", max_leaf_prims=leaf,
+                                      treelet_rounds=rounds)
+                scenes[(leaf, rounds)] = rt.Scene([(tb, tp, ti, None, rt.INST_IDENTITY),
+                                                   (grid, gx, None, m_grid, rt.INST_TRANSLATE)])
+            state["scene"], state["leaf"], state["rounds"] = scenes[(leaf, rounds)], leaf, rounds
 
-        use(tuner.frame_leaf_size(use, lambda: frame(0), lambda: out, tag="ao-terrain"))
+        cands = [(leaf, r) for r in FRAME_TREELET_ROUNDS for leaf in autotune.LEAF_SIZES]
+        use(tuner.frame_leaf_size(use, lambda: frame(0), lambda: out, tag="ao-terrain",
+                                  sizes=cands))
         scenes.clear()
         frame(0)
         shapes[0] = tuner.trace_scene(state["scene"], primary, "closest", tag="ao-primary",
@@ -540,6 +571,7 @@ def ao_frame_leg(frames=100, w=1920, h=1080, tuner=None):
                         "bilateral", "frames": frames, "ms_per_frame": round(ms, 3),
             "fps": round(1000.0 / ms, 1), "mrays_per_s": round(2 * npix / ms / 1e3, 1),
             "trace_shapes": shapes, "terrain_max_leaf_prims": state["leaf"],
+            "terrain_treelet_rounds": state["rounds"],
             "ao_mean": round(float(out.mean()), 4),
             "quality_filtered_vs_1spp": {"mse": round(mse, 6), "psnr_db": round(psnr, 3),
                                          "ssim": round(ssim, 5)},
@@ -703,7 +735,7 @@ def main():
                    "primary_trace": f.primary_opts or "default",
                    "shadow_trace": f.shadow_opts or "default",
                    "bvh_nodes": f.bvh.node_count, "bvh_depth": f.bvh.depth,
-                   "max_leaf_prims": f.leaf},
+                   "max_leaf_prims": f.leaf, "treelet_rounds": f.rounds},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic, "kernel": "k_trace_persist any-hit (shadow)"

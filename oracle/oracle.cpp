@@ -368,6 +368,143 @@ int32_t collapse(const or_bvh2_node* n2, int32_t idx, std::vector<or_bvh4_node>&
     return o;
 }
 
+// Treelet restructuring of a BVH2 whose leaves are triangle ranges (Karras &
+// Aila 2013; no reference counterpart), one round = a post-order pass.  At
+// inner node x: a treelet of up to 7 leaves grows from x's children by
+// expanding the inner treelet leaf of largest surface area (first in slot
+// order on ties); over its leaf subsets s (ascending), cost(s) = 1.2 * SA(box
+// of s) + the cheapest split of s (the part holding s's lowest leaf, scanned
+// from s - 1 down, first strict minimum), a single leaf costing its subtree's
+// SAH cost (leaf: SA * triangles; inner: 1.2 * SA + both children's costs).
+// If cost(all) is strictly below x's current cost, the treelet is rebuilt
+// breadth-first, reusing its inner nodes in the order they were expanded; the
+// rebuilt nodes' boxes are the unions of their children.
+struct TreeletOpt {
+    std::vector<or_bvh2_node>& t;
+    std::vector<float> cost;
+    static constexpr int kLeaves = 7;
+    static constexpr float kInner = 1.2f;
+
+    static float area(const float* lo, const float* hi) {
+        const float dx = hi[0] - lo[0], dy = hi[1] - lo[1], dz = hi[2] - lo[2];
+        return 2.0f * (dx * dy + dy * dz + dz * dx);
+    }
+    bool inner(int32_t u) const { return t[u].leftChild >= 0; }
+    float sah(int32_t u) const {
+        return inner(u) ? cost[u] : area(t[u].aabbMin, t[u].aabbMax) * (float)t[u].rightChild;
+    }
+    static float split(const float* C, uint32_t s, uint32_t& bp) {
+        const uint32_t low = s & (0u - s);
+        float best = INFINITY;
+        bp = 0;
+        for (uint32_t p = (s - 1) & s; p > 0; p = (p - 1) & s) {
+            if (!(p & low)) continue;
+            const float c = C[p] + C[s ^ p];
+            if (c < best) {
+                best = c;
+                bp = p;
+            }
+        }
+        return best;
+    }
+    void node(int32_t x) {
+        int32_t lf[kLeaves], in[kLeaves - 1];
+        lf[0] = t[x].leftChild;
+        lf[1] = t[x].rightChild;
+        in[0] = x;
+        int nl = 2, ni = 1;
+        float cx = kInner * area(t[x].aabbMin, t[x].aabbMax) + (sah(lf[0]) + sah(lf[1]));
+        while (nl < kLeaves) {
+            int best = -1;
+            float bs = -1.0f;
+            for (int q = 0; q < nl; ++q) {
+                if (!inner(lf[q])) continue;
+                const float a = area(t[lf[q]].aabbMin, t[lf[q]].aabbMax);
+                if (a > bs) {
+                    bs = a;
+                    best = q;
+                }
+            }
+            if (best < 0) break;
+            const int32_t e = lf[best];
+            in[ni++] = e;
+            lf[best] = t[e].leftChild;
+            lf[nl++] = t[e].rightChild;
+        }
+        if (nl >= 3) {
+            float C[1 << kLeaves];
+            for (int q = 0; q < nl; ++q) C[1u << q] = sah(lf[q]);
+            const uint32_t full = (1u << nl) - 1;
+            for (uint32_t s = 3; s <= full; ++s) {
+                if (!(s & (s - 1))) continue;
+                float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+                for (int q = 0; q < nl; ++q) {
+                    if (!((s >> q) & 1u)) continue;
+                    for (int a = 0; a < 3; ++a) {
+                        lo[a] = std::min(lo[a], t[lf[q]].aabbMin[a]);
+                        hi[a] = std::max(hi[a], t[lf[q]].aabbMax[a]);
+                    }
+                }
+                uint32_t bp;
+                const float sp = split(C, s, bp);
+                C[s] = kInner * area(lo, hi) + sp;
+            }
+            if (C[full] < cx) {
+                int32_t qn[kLeaves - 1], ql[kLeaves - 1], qr[kLeaves - 1];
+                uint32_t qs[kLeaves - 1];
+                qn[0] = x;
+                qs[0] = full;
+                int nq = 1, next = 1;
+                bool ok = true;
+                for (int i = 0; i < nq; ++i) {
+                    uint32_t p;
+                    split(C, qs[i], p);
+                    if (p == 0) {
+                        ok = false;
+                        break;
+                    }
+                    const uint32_t part[2] = {p, qs[i] ^ p};
+                    int32_t ch[2];
+                    for (int j = 0; j < 2; ++j) {
+                        if (!(part[j] & (part[j] - 1))) {
+                            ch[j] = lf[__builtin_ctz(part[j])];
+                        } else {
+                            ch[j] = in[next++];
+                            qn[nq] = ch[j];
+                            qs[nq] = part[j];
+                            ++nq;
+                        }
+                    }
+                    ql[i] = ch[0];
+                    qr[i] = ch[1];
+                }
+                if (ok) {
+                    for (int i = 0; i < nq; ++i) {
+                        t[qn[i]].leftChild = ql[i];
+                        t[qn[i]].rightChild = qr[i];
+                    }
+                    for (int i = nq - 1; i >= 1; --i) {
+                        or_bvh2_node& y = t[qn[i]];
+                        for (int a = 0; a < 3; ++a) {
+                            y.aabbMin[a] = std::min(t[ql[i]].aabbMin[a], t[qr[i]].aabbMin[a]);
+                            y.aabbMax[a] = std::max(t[ql[i]].aabbMax[a], t[qr[i]].aabbMax[a]);
+                        }
+                        cost[qn[i]] = C[qs[i]];
+                    }
+                    cx = C[full];
+                }
+            }
+        }
+        cost[x] = cx;
+    }
+    void post(int32_t x) {
+        if (!inner(x)) return;
+        post(t[x].leftChild);
+        post(t[x].rightChild);
+        node(x);
+    }
+};
+
 }  // namespace
 
 extern "C" {
@@ -1255,12 +1392,13 @@ struct KarrasBuilder {
 };
 
 // Full radix-tree build: Morton codes, stable sort, radix tree with
-// max_prims leaves (BVH2, pre-order), collapse to Box4, 64-B leaves in sorted
-// order.  Returns 1 when a capacity is too small.
-int or_rlbvh_build(const float* pos, const uint32_t* idx, uint32_t n, uint32_t max_prims,
-                   or_bvh2_node* n2_out, uint32_t cap2, uint32_t* count2, or_bvh4_node* n4_out,
-                   uint32_t cap4, uint32_t* count4, or_tri64* tris, uint32_t* sorted_ids,
-                   uint32_t* codes_sorted) {
+// max_prims leaves (BVH2, pre-order), `rounds` rounds of treelet
+// restructuring, collapse to Box4, 64-B leaves in sorted order.  Returns 1
+// when a capacity is too small.
+int or_rlbvh_build_opt(const float* pos, const uint32_t* idx, uint32_t n, uint32_t max_prims,
+                       uint32_t rounds, or_bvh2_node* n2_out, uint32_t cap2, uint32_t* count2,
+                       or_bvh4_node* n4_out, uint32_t cap4, uint32_t* count4, or_tri64* tris,
+                       uint32_t* sorted_ids, uint32_t* codes_sorted) {
     *count2 = 0;
     *count4 = 0;
     if (n == 0) return 0;
@@ -1288,6 +1426,10 @@ int or_rlbvh_build(const float* pos, const uint32_t* idx, uint32_t n, uint32_t m
     KarrasBuilder kb{keys.data(), max_prims, {}, box.data()};
     kb.nodes.reserve(2 * (size_t)n);
     kb.build(0, n - 1);
+    if (rounds) {
+        TreeletOpt opt{kb.nodes, std::vector<float>(kb.nodes.size(), 0.0f)};
+        for (uint32_t r = 0; r < rounds; ++r) opt.post(0);
+    }
     if (kb.nodes.size() > cap2) return 1;
     std::memcpy(n2_out, kb.nodes.data(), kb.nodes.size() * sizeof(or_bvh2_node));
     *count2 = (uint32_t)kb.nodes.size();
@@ -1295,6 +1437,14 @@ int or_rlbvh_build(const float* pos, const uint32_t* idx, uint32_t n, uint32_t m
     if (sorted_ids) std::memcpy(sorted_ids, vals.data(), n * sizeof(uint32_t));
     if (codes_sorted) std::memcpy(codes_sorted, keys.data(), n * sizeof(uint32_t));
     return 0;
+}
+
+int or_rlbvh_build(const float* pos, const uint32_t* idx, uint32_t n, uint32_t max_prims,
+                   or_bvh2_node* n2_out, uint32_t cap2, uint32_t* count2, or_bvh4_node* n4_out,
+                   uint32_t cap4, uint32_t* count4, or_tri64* tris, uint32_t* sorted_ids,
+                   uint32_t* codes_sorted) {
+    return or_rlbvh_build_opt(pos, idx, n, max_prims, 0, n2_out, cap2, count2, n4_out, cap4,
+                              count4, tris, sorted_ids, codes_sorted);
 }
 
 // Refit: same triangle order (taken from tris[i].triangle_id), new positions.

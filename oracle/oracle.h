@@ -167,6 +167,10 @@ void or_tree_layout(uint32_t n, uint32_t* counts, uint32_t* offsets, uint32_t* l
 int or_lbvh_build(const float* pos, const uint32_t* idx, uint32_t ntris,
                   or_bvh4_node* nodes, uint32_t node_cap, uint32_t* node_count,
                   or_tri64* tris, uint32_t* sorted_ids, uint32_t* codes_sorted);
+int or_rlbvh_build_opt(const float* pos, const uint32_t* idx, uint32_t n, uint32_t max_prims,
+                       uint32_t rounds, or_bvh2_node* n2, uint32_t cap2, uint32_t* count2,
+                       or_bvh4_node* n4, uint32_t cap4, uint32_t* count4, or_tri64* tris,
+                       uint32_t* sorted_ids, uint32_t* codes_sorted);
 int or_rlbvh_build(const float* pos, const uint32_t* idx, uint32_t n, uint32_t max_prims,
                    or_bvh2_node* n2, uint32_t cap2, uint32_t* count2, or_bvh4_node* n4,
                    uint32_t cap4, uint32_t* count4, or_tri64* tris, uint32_t* sorted_ids,

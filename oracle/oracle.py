@@ -424,8 +424,9 @@ def lbvh_build(pos, idx=None):
     return nodes[:cnt.value].copy(), tris, ids, codes
 
 
-def rlbvh_build(pos, idx=None, max_prims=4, with_bvh2=False):
-    """Radix-tree LBVH collapsed to Box4: (nodes4, tris64, sorted ids, codes)."""
+def rlbvh_build(pos, idx=None, max_prims=4, with_bvh2=False, treelet_rounds=0):
+    """Radix-tree LBVH collapsed to Box4: (nodes4, tris64, sorted ids, codes);
+    treelet_rounds rounds of SAH treelet restructuring before the collapse."""
     pos = _f(pos)
     n = idx.shape[0] if idx is not None else pos.size // 9
     cap = max(1, 2 * n)
@@ -435,9 +436,10 @@ def rlbvh_build(pos, idx=None, max_prims=4, with_bvh2=False):
     ids = np.zeros(n, np.uint32)
     codes = np.zeros(n, np.uint32)
     c2, c4 = u32(0), u32(0)
-    rc = lib().or_rlbvh_build(_p(pos), _p(None if idx is None else _u(idx)), u32(n),
-                              u32(max_prims), _p(n2), u32(cap), C.byref(c2), _p(n4), u32(cap),
-                              C.byref(c4), _p(tris), _p(ids), _p(codes))
+    rc = lib().or_rlbvh_build_opt(_p(pos), _p(None if idx is None else _u(idx)), u32(n),
+                                  u32(max_prims), u32(treelet_rounds), _p(n2), u32(cap),
+                                  C.byref(c2), _p(n4), u32(cap), C.byref(c4), _p(tris), _p(ids),
+                                  _p(codes))
     if rc:
         raise RuntimeError("or_rlbvh_build failed")
     out = (n4[:c4.value].copy(), tris, ids, codes)

@@ -121,17 +121,19 @@ class Tuner:
                 json.dump(self.cache, f, indent=1, sort_keys=True)
             os.replace(tmp, self.cache_path)
 
-    def _pick(self, key, candidates, run, check, skip_changed=False):
+    def _pick(self, key, candidates, run, check, skip_changed=False, same=None):
         """Times run(c) for every candidate c, checks check(c) == the first
-        candidate's output, caches and returns the fastest.  A candidate whose
-        output differs raises, or with skip_changed (knobs that may change
-        results, such as the tree's leaf size) is left out."""
+        candidate's output (or same(out, first) when given), caches and
+        returns the fastest.  A candidate whose output differs raises, or with
+        skip_changed (knobs that may change results, such as the tree's leaf
+        size) is left out."""
         if key in self.cache:
             return self.cache[key]["choice"]
         t0 = time.perf_counter()
         self._depth += 1  # a candidate's run may tune other knobs: time counted once
         try:
-            best, best_ms, table = self._measure(key, candidates, run, check, skip_changed)
+            best, best_ms, table = self._measure(key, candidates, run, check, skip_changed,
+                                                 same or _same)
         finally:
             self._depth -= 1
         if self._depth == 0:
@@ -145,14 +147,14 @@ class Tuner:
         self._save()
         return best
 
-    def _measure(self, key, candidates, run, check, skip_changed):
+    def _measure(self, key, candidates, run, check, skip_changed, same):
         ref = _clone(check(candidates[0]))
         torch.cuda.synchronize()
         best, best_ms, table = None, float("inf"), []
         for c in candidates:
             out = check(c)
             torch.cuda.synchronize()
-            if not _same(out, ref):
+            if not same(out, ref):
                 if skip_changed:
                     self.history.append((key, c, None))
                     continue
@@ -281,13 +283,12 @@ class Tuner:
 
         return self._pick(key, cands, run, check)
 
-    def frame_leaf_size(self, use, step, output, tag="", sizes=LEAF_SIZES):
-        """Leaf size (max_leaf_prims) of a frame's radix tree: use(size) makes
-        the frame trace a tree of that leaf size (built once, outside the
-        timing), step() runs one frame, output() is the frame's result.  A
-        leaf size whose frame result differs from the first size's (a tie
-        between two triangles at the same t resolved another way) is left
-        out."""
+    def frame_leaf_size(self, use, step, output, tag="", sizes=LEAF_SIZES, same=None):
+        """Tree of a frame: use(c) makes the frame trace the tree of candidate
+        c (a leaf size, or a (leaf size, treelet rounds) pair; built once,
+        outside the timing), step() runs one frame, output() is the frame's
+        result.  A candidate whose frame result differs from the first one's
+        (by same(out, first) when given, else bit for bit) is left out."""
 
         def run(c):
             use(c)
@@ -298,7 +299,7 @@ class Tuner:
             step()
             return _clone(output())
 
-        return self._pick(f"leaf|{tag}", list(sizes), run, check, skip_changed=True)
+        return self._pick(f"leaf|{tag}", list(sizes), run, check, skip_changed=True, same=same)
 
     def build_batch(self, batch, tag=""):
         """Persistent-grid size for BuildBatch.launch."""

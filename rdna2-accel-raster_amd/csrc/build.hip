@@ -1484,6 +1484,9 @@ struct RadixArrays {
     uint32_t* starts;      // n / kWindow + 1: k_rt_top climb starts
     int4* cand;            // m, -1 = empty
     uint32_t* size4;       // m: Box4 nodes in the subtree rooted here
+    float* cost;           // m: SAH cost of the subtree (treelet restructuring)
+    uint32_t* emin;        // m: lowest sorted position below the node
+    uint32_t* emax;        // m: highest sorted position below the node
     uint32_t n, m, maxp;
 };
 
@@ -1555,6 +1558,8 @@ __global__ __launch_bounds__(256) void k_rt_radix(RadixArrays A, rt_build_info* 
     const int32_t R = (hi == g + 1) ? (int32_t)(A.m + g + 1) : (int32_t)(g + 1);
     A.rf[i] = (uint32_t)lo;
     A.rl[i] = (uint32_t)hi;
+    A.emin[i] = (uint32_t)lo;
+    A.emax[i] = (uint32_t)hi;
     A.left[i] = L;
     A.right[i] = R;
     A.parent[L] = (int32_t)i;
@@ -1682,9 +1687,269 @@ __global__ __launch_bounds__(1024) void k_rt_top(RadixArrays A) {
     }
 }
 
+// ---------------------------------------------------------------------------
+// Treelet restructuring of the radix tree (Karras & Aila 2013, "Fast Parallel
+// Construction of High-Quality Bounding Volume Hierarchies"), bottom-up.  At
+// each inner node x (more than maxp triangles) a treelet of up to 7 leaves
+// grows from x's two children by expanding the inner treelet leaf of largest
+// surface area (the first in slot order on ties).  A dynamic program over the
+// leaf subsets finds the binary tree over those leaves of least SAH cost
+//   leaf range: SA * triangles;  inner node: 1.2 * SA + cost(left) + cost(right)
+// (each split counted once: the part holding the subset's lowest leaf, scanned
+// from s - 1 down, the first strict minimum kept).  When that cost is strictly
+// below the current one the treelet is rebuilt, reusing its inner nodes in
+// breadth-first order.  Leaf ranges, and so the triangle order, never change;
+// inner nodes keep their radix ranges in rf / rl (which then only say that
+// they are inner), and emin / emax follow the sorted positions below them.
+// A node's result depends only on its subtree, so the order in which threads
+// reach the nodes does not change the tree.
+// ---------------------------------------------------------------------------
+constexpr int kTreeletLeaves = 7;
+constexpr uint32_t kMaxTreeletRounds = 8;
+constexpr uint32_t kTreeletSets = 1u << kTreeletLeaves;
+constexpr uint32_t kTreeletBlock = 256;  // LDS: kTreeletSets floats per thread, 128 KiB
+constexpr float kCostInner = 1.2f;
+
+__device__ __forceinline__ uint32_t ext_lo(const RadixArrays& A, int32_t u) {
+    return (uint32_t)u < A.m ? A.emin[u] : (uint32_t)u - A.m;
+}
+__device__ __forceinline__ uint32_t ext_hi(const RadixArrays& A, int32_t u) {
+    return (uint32_t)u < A.m ? A.emax[u] : (uint32_t)u - A.m;
+}
+__device__ __forceinline__ bool ext_straddles(const RadixArrays& A, int32_t x) {
+    return (uint32_t)x < A.m && A.emin[x] / kWindow != A.emax[x] / kWindow;
+}
+
+// SAH cost of the subtree at u: a leaf range SA * triangles, an inner node its
+// stored cost.
+__device__ __forceinline__ float sah_cost(const RadixArrays& A, int32_t u) {
+    if (ninner(A, u)) return A.cost[u];
+    return box_sa(A.box + 6 * (size_t)u) * (float)(nlast(A, u) - nfirst(A, u) + 1);
+}
+
+// Cheapest split of leaf subset s; C is this thread's LDS column.  The parts
+// holding s's lowest leaf, in descending order, are low | sub for the proper
+// submasks sub of rest = s ^ low, descending; four are loaded at a time (the
+// scan is bound by LDS latency at one wave per SIMD) and compared in order.
+__device__ __forceinline__ float best_split(const float* C, uint32_t s, uint32_t& bp) {
+    const uint32_t low = s & (0u - s), rest = s ^ low;
+    float best = INFINITY;
+    bp = 0;
+    uint32_t sub = (rest - 1) & rest;
+    bool end = false;
+    while (!end) {
+        uint32_t q[4];
+        int k = 0;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            if (end) continue;
+            q[j] = sub;
+            k = j + 1;
+            if (sub == 0)
+                end = true;
+            else
+                sub = (sub - 1) & rest;
+        }
+        float a[4], b[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            if (j >= k) continue;
+            a[j] = C[(low | q[j]) * kTreeletBlock];
+            b[j] = C[(rest ^ q[j]) * kTreeletBlock];
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            if (j >= k) continue;
+            const float c = a[j] + b[j];
+            if (c < best) {
+                best = c;
+                bp = low | q[j];
+            }
+        }
+    }
+    return best;
+}
+
+__device__ void treelet_node(const RadixArrays& A, int32_t x, float* C) {
+    if (!ninner(A, x)) return;  // a leaf range: its cost is computed where used
+    int32_t lf[kTreeletLeaves], in[kTreeletLeaves - 1];
+    lf[0] = A.left[x];
+    lf[1] = A.right[x];
+    in[0] = x;
+    int nl = 2, ni = 1;
+    float cx = kCostInner * box_sa(A.box + 6 * (size_t)x) +
+               (sah_cost(A, lf[0]) + sah_cost(A, lf[1]));
+    while (nl < kTreeletLeaves) {
+        int best = -1;
+        float bs = -1.0f;
+        for (int q = 0; q < nl; ++q) {
+            if (!ninner(A, lf[q])) continue;
+            const float a = box_sa(A.box + 6 * (size_t)lf[q]);
+            if (a > bs) {
+                bs = a;
+                best = q;
+            }
+        }
+        if (best < 0) break;
+        const int32_t e = lf[best];
+        in[ni++] = e;
+        lf[best] = A.left[e];
+        lf[nl++] = A.right[e];
+    }
+    if (nl >= 3) {
+        float lb[kTreeletLeaves][6];
+#pragma unroll
+        for (int q = 0; q < kTreeletLeaves; ++q) {
+            if (q >= nl) continue;
+#pragma unroll
+            for (int a = 0; a < 6; ++a) lb[q][a] = A.box[6 * (size_t)lf[q] + a];
+            C[(1u << q) * kTreeletBlock] = sah_cost(A, lf[q]);
+        }
+        const uint32_t full = (1u << nl) - 1;
+        for (uint32_t s = 3; s <= full; ++s) {
+            if (!(s & (s - 1))) continue;  // one leaf: set above
+            float b[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+#pragma unroll
+            for (int q = 0; q < kTreeletLeaves; ++q) {
+                if (!((s >> q) & 1u)) continue;
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    b[a] = fminf(b[a], lb[q][a]);
+                    b[3 + a] = fmaxf(b[3 + a], lb[q][3 + a]);
+                }
+            }
+            uint32_t bp;
+            const float split = best_split(C, s, bp);
+            C[s * kTreeletBlock] = kCostInner * box_sa(b) + split;
+        }
+        const float cf = C[full * kTreeletBlock];
+        if (cf < cx) {
+            // plan the rebuild first (no stores): breadth-first over the subsets
+            int32_t qn[kTreeletLeaves - 1], ql[kTreeletLeaves - 1], qr[kTreeletLeaves - 1];
+            uint32_t qs[kTreeletLeaves - 1];
+            qn[0] = x;
+            qs[0] = full;
+            int nq = 1, next = 1;
+            bool ok = true;
+            for (int i = 0; i < nq && ok; ++i) {
+                uint32_t p;
+                best_split(C, qs[i], p);
+                if (p == 0) {
+                    ok = false;
+                    break;
+                }
+                const uint32_t part[2] = {p, qs[i] ^ p};
+                int32_t ch[2];
+                for (int j = 0; j < 2; ++j) {
+                    if (!(part[j] & (part[j] - 1))) {
+                        ch[j] = lf[__ffs(part[j]) - 1];
+                    } else {
+                        ch[j] = in[next++];
+                        qn[nq] = ch[j];
+                        qs[nq] = part[j];
+                        ++nq;
+                    }
+                }
+                ql[i] = ch[0];
+                qr[i] = ch[1];
+            }
+            if (ok) {
+                for (int i = 0; i < nq; ++i) {
+                    A.left[qn[i]] = ql[i];
+                    A.right[qn[i]] = qr[i];
+                    A.parent[ql[i]] = qn[i];
+                    A.parent[qr[i]] = qn[i];
+                }
+                for (int i = nq - 1; i >= 1; --i) {  // children before parents
+                    const int32_t y = qn[i];
+                    A.emin[y] = min(ext_lo(A, ql[i]), ext_lo(A, qr[i]));
+                    A.emax[y] = max(ext_hi(A, ql[i]), ext_hi(A, qr[i]));
+                    A.cost[y] = C[qs[i] * kTreeletBlock];
+                    finish_node(A, y);
+                }
+                cx = cf;
+            }
+        }
+    }
+    A.cost[x] = cx;
+    finish_node(A, x);
+}
+
+// Round set-up: arrival counters cleared, the nodes whose positions cross a
+// window boundary listed (top_count zeroed by k_tl_zero before).
+__global__ void k_tl_zero(RadixArrays A) {
+    if (threadIdx.x == 0) *A.top_count = 0;
+}
+__global__ __launch_bounds__(256) void k_tl_prep(RadixArrays A) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= A.m) return;
+    A.done[i] = 0;
+    if (A.emin[i] / kWindow != A.emax[i] / kWindow) A.top[atomicAdd(A.top_count, 1u)] = i;
+}
+
+// One restructuring round, pass 1: as k_rt_bottom_up over kWindow sorted
+// positions, with a node inside the window when all positions below it are;
+// the arrival counters are global (every node inside the window is reached
+// only by this workgroup, so workgroup scope suffices).
+__global__ __launch_bounds__(kTreeletBlock) void k_tl_window(RadixArrays A) {
+    __shared__ float C[kTreeletSets * kTreeletBlock];
+    float* mine = C + threadIdx.x;
+    const uint32_t lo = blockIdx.x * kWindow;
+    const uint32_t hi = min(lo + kWindow, A.n) - 1;
+    for (uint32_t p = lo + threadIdx.x; p <= hi; p += kTreeletBlock) {
+        int32_t x = A.parent[A.m + p];
+        // a climb inside the window passes fewer than kWindow nodes (bound kept
+        // so that no malformed link can hold a wave)
+        for (uint32_t steps = 0; x >= 0 && steps < kWindow; ++steps) {
+            if (!(A.emin[x] >= lo && A.emax[x] <= hi)) break;
+            const uint32_t old = __hip_atomic_fetch_add(A.done + x, 1u, __ATOMIC_ACQ_REL,
+                                                        __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (old == 0) break;
+            treelet_node(A, x, mine);
+            x = A.parent[x];
+        }
+    }
+}
+
+// Pass 2, one workgroup: the nodes crossing window boundaries, as k_rt_top.
+__global__ __launch_bounds__(kTreeletBlock) void k_tl_top(RadixArrays A) {
+    __shared__ float C[kTreeletSets * kTreeletBlock];
+    __shared__ uint32_t nstart;
+    float* mine = C + threadIdx.x;
+    const uint32_t cnt = *A.top_count;
+    if (threadIdx.x == 0) nstart = 0;
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < cnt; i += kTreeletBlock) {
+        const int32_t x = (int32_t)A.top[i];
+        const int ready =
+            (ext_straddles(A, A.left[x]) ? 0 : 1) + (ext_straddles(A, A.right[x]) ? 0 : 1);
+        if (ready == 1)
+            A.done[x] = 1;
+        else if (ready == 2)
+            A.starts[atomicAdd(&nstart, 1u)] = (uint32_t)x;
+    }
+    __syncthreads();
+    const uint32_t ns = nstart;
+    for (uint32_t j = threadIdx.x; j < ns; j += kTreeletBlock) {
+        int32_t x = (int32_t)A.starts[j];
+        for (uint32_t steps = 0; steps <= cnt; ++steps) {  // a climb passes at most cnt nodes
+            treelet_node(A, x, mine);
+            x = A.parent[x];
+            if (x < 0) break;
+            if (__hip_atomic_fetch_add(A.done + x, 1u, __ATOMIC_ACQ_REL,
+                                       __HIP_MEMORY_SCOPE_WORKGROUP) == 0)
+                break;
+        }
+    }
+}
+
 // Internal node x with more than maxp triangles: search down from the root
 // through the candidates; if x is a Box4 node, its pre-order index is the
 // parent's index + 1 + the sizes of the inner candidates before it.
+// WALK (a restructured tree, whose subtrees are not ranges): the candidate on
+// the way to x is the one met first when walking up from x through parent
+// links; none before reaching cur means x was expanded into cur's node.
+template <bool WALK>
 __global__ __launch_bounds__(256) void k_rt_emit(RadixArrays A, rt_node4* __restrict__ nodes,
                                                  rt_build_info* info) {
     const uint32_t x = blockIdx.x * 256 + threadIdx.x;
@@ -1698,16 +1963,26 @@ __global__ __launch_bounds__(256) void k_rt_emit(RadixArrays A, rt_node4* __rest
         const int32_t cs[4] = {cv.x, cv.y, cv.z, cv.w};
         uint32_t acc = idx + 1;
         int32_t next = -1;
+        int on_path = -1;
+        if constexpr (WALK) {
+            int32_t u = (int32_t)x;
+            for (uint32_t steps = 0; u >= 0 && u != cur && on_path < 0 && steps <= A.m;
+                 ++steps, u = A.parent[u])
+                for (int q = 0; q < 4; ++q)
+                    if (cs[q] == u) on_path = q;
+        }
         for (int q = 0; q < 4 && cs[q] >= 0; ++q) {
             const int32_t c = cs[q];
             const bool inner = ninner(A, c);
-            if (nfirst(A, c) <= xf && xl <= nlast(A, c)) {
+            const bool hit = WALK ? q == on_path : (nfirst(A, c) <= xf && xl <= nlast(A, c));
+            if (hit) {
                 if (inner) next = c;
                 break;
             }
             if (inner) acc += A.size4[c];
         }
         if (next < 0) return;  // x was expanded into its parent's Box4 node
+        if (WALK && level > A.m) return;  // deeper than any tree of m inner nodes
         cur = next;
         idx = acc;
         ++level;
@@ -1786,7 +2061,8 @@ static_assert(kSmallMax == 4096 && kBoxPitch == 4 + 4 + 16 + 64 + 256 + 1024,
 struct ScratchPlan {
     size_t bounds = 0, keys0 = 0, vals0 = 0, keys1 = 0, vals1 = 0, hist = 0, totals = 0,
            boxes = 0, rbox = 0, rf = 0, rl = 0, left = 0, right = 0, parent = 0, done = 0,
-           top = 0, top_count = 0, starts = 0, cand = 0, size4 = 0, total = 0;
+           top = 0, top_count = 0, starts = 0, cand = 0, size4 = 0, cost = 0, emin = 0,
+           emax = 0, total = 0;
     uint32_t tiles = 0;
 };
 
@@ -1817,9 +2093,14 @@ ScratchPlan plan_scratch(uint32_t n, uint32_t algo) {
         p.done = off; off += al(m * 4);
         p.top = off; off += al(m * 4);
         p.top_count = off; off += al(4);
-        p.starts = off; off += al(((size_t)n / kWindow + 1) * 4);
+        // climb starts: one per window boundary in the radix tree, but a
+        // restructured tree's subtrees are no longer ranges: room for every node
+        p.starts = off; off += al(std::max(m, (size_t)n / kWindow + 1) * 4);
         p.cand = off; off += al(m * 16);
         p.size4 = off; off += al(m * 4);
+        p.cost = off; off += al(m * 4);
+        p.emin = off; off += al(m * 4);
+        p.emax = off; off += al(m * 4);
     }
     p.total = off;
     return p;
@@ -1906,7 +2187,8 @@ rt_status morton_sort(hipStream_t s, const float* pos, const uint32_t* idx, uint
 }
 
 rt_status build_radix(hipStream_t s, const float* pos, const uint32_t* idx, uint32_t n,
-                      uint32_t maxp, const ScratchPlan& p, char* base, const uint32_t* keys,
+                      uint32_t maxp, uint32_t rounds, const ScratchPlan& p, char* base,
+                      const uint32_t* keys,
                       const uint32_t* vals, rt_node4* nodes, rt_tri64* tris,
                       rt_build_info* info) {
     RadixArrays A;
@@ -1923,6 +2205,9 @@ rt_status build_radix(hipStream_t s, const float* pos, const uint32_t* idx, uint
     A.starts = reinterpret_cast<uint32_t*>(base + p.starts);
     A.cand = reinterpret_cast<int4*>(base + p.cand);
     A.size4 = reinterpret_cast<uint32_t*>(base + p.size4);
+    A.cost = reinterpret_cast<float*>(base + p.cost);
+    A.emin = reinterpret_cast<uint32_t*>(base + p.emin);
+    A.emax = reinterpret_cast<uint32_t*>(base + p.emax);
     A.n = n;
     A.m = n - 1;
     A.maxp = maxp;
@@ -1935,7 +2220,18 @@ rt_status build_radix(hipStream_t s, const float* pos, const uint32_t* idx, uint
     hipLaunchKernelGGL(k_rt_radix, dim3(ceil_div(A.m, 256)), dim3(256), 0, s, A, info);
     hipLaunchKernelGGL(k_rt_bottom_up, dim3(ceil_div(n, kWindow)), dim3(1024), 0, s, A);
     hipLaunchKernelGGL(k_rt_top, dim3(1), dim3(1024), 0, s, A);
-    hipLaunchKernelGGL(k_rt_emit, dim3(ceil_div(A.m, 256)), dim3(256), 0, s, A, nodes, info);
+    for (uint32_t r = 0; r < rounds; ++r) {
+        hipLaunchKernelGGL(k_tl_zero, dim3(1), dim3(64), 0, s, A);
+        hipLaunchKernelGGL(k_tl_prep, dim3(ceil_div(A.m, 256)), dim3(256), 0, s, A);
+        hipLaunchKernelGGL(k_tl_window, dim3(ceil_div(n, kWindow)), dim3(kTreeletBlock), 0, s, A);
+        hipLaunchKernelGGL(k_tl_top, dim3(1), dim3(kTreeletBlock), 0, s, A);
+    }
+    if (rounds)
+        hipLaunchKernelGGL(k_rt_emit<true>, dim3(ceil_div(A.m, 256)), dim3(256), 0, s, A, nodes,
+                           info);
+    else
+        hipLaunchKernelGGL(k_rt_emit<false>, dim3(ceil_div(A.m, 256)), dim3(256), 0, s, A, nodes,
+                           info);
     return launch_status("k_rt_emit");
 }
 
@@ -2016,9 +2312,11 @@ extern "C" rt_status rt_gpu_build(rt_stream stream, const float* d_pos, const ui
     const uint32_t algo = resolve_algo(cfg, n);
     const uint32_t maxp = resolve_maxp(cfg);
     const uint32_t bits = resolve_bits(cfg);
+    const uint32_t rounds = cfg ? cfg->treelet_rounds : 0u;
     if (algo != RT_BUILD_IMPLICIT && algo != RT_BUILD_RADIX) return RT_ERR_ARG;
     if (bits == 0) return RT_ERR_ARG;
     if (maxp < 1 || maxp > 16) return RT_ERR_ARG;
+    if (rounds > kMaxTreeletRounds || (rounds && algo != RT_BUILD_RADIX)) return RT_ERR_ARG;
     if (node_capacity < max_nodes(n, algo)) return RT_ERR_CAPACITY;
     const TreeLayout t = tree_layout(n);
     if (algo == RT_BUILD_IMPLICIT && n <= kSmallMax) {
@@ -2038,7 +2336,8 @@ extern "C" rt_status rt_gpu_build(rt_stream stream, const float* d_pos, const ui
     if (d_codes)
         RT_TRY_HIP(hipMemcpyAsync(d_codes, keys, (size_t)n * 4, hipMemcpyDeviceToDevice, s));
     if (algo == RT_BUILD_RADIX)
-        return build_radix(s, d_pos, d_idx, n, maxp, p, base, keys, vals, d_nodes, d_tris, d_info);
+        return build_radix(s, d_pos, d_idx, n, maxp, rounds, p, base, keys, vals, d_nodes, d_tris,
+                           d_info);
     float* gboxes = reinterpret_cast<float*>(base + p.boxes);
     st = emit_levels(s, d_pos, d_idx, n, vals, d_nodes, d_tris, gboxes, t, false);
     if (st != RT_OK) return st;

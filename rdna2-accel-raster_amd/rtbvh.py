@@ -99,7 +99,7 @@ class rt_build_item(C.Structure):
 
 class rt_build_cfg(C.Structure):
     _fields_ = [("algorithm", u32), ("max_leaf_prims", u32), ("sort_bits", u32),
-                ("reserved", u32)]
+                ("treelet_rounds", u32)]
 
 
 # Digit widths of the multi-launch Morton radix sort (same output for each);
@@ -258,9 +258,9 @@ def tree_layout(n):
     return list(counts[:L.value]), list(offs[:L.value]), tb.value, sz.value
 
 
-def build_cfg(algorithm="auto", max_leaf_prims=4, sort_bits=None):
+def build_cfg(algorithm="auto", max_leaf_prims=4, sort_bits=None, treelet_rounds=0):
     return rt_build_cfg(BUILD_ALGOS[algorithm], max_leaf_prims,
-                        sort_bits or DEFAULT_SORT_BITS, 0)
+                        sort_bits or DEFAULT_SORT_BITS, treelet_rounds)
 
 
 def build_requirements(n, cfg=None):
@@ -322,19 +322,21 @@ class GpuBVH4:
     # -- build ---------------------------------------------------------------
     @staticmethod
     def build(pos, idx=None, algorithm="auto", max_leaf_prims=4, stream=None, keep_order=False,
-              scratch=None, sync=True, sort_bits=None):
+              scratch=None, sync=True, sort_bits=None, treelet_rounds=0):
         """Device build (rt_gpu_build) of a Morton-sorted Box4 tree.
 
         pos: float32 (V, 3) device tensor; idx: uint32/int32 (T, 3) or None
         for 9 floats per triangle (pos then has shape (T, 9) or (3T, 3)).
         algorithm: "auto" | "radix" | "implicit" (see rt_bvh.h).  sort_bits:
         radix digit width of the multi-launch sort (SORT_BITS; default
-        DEFAULT_SORT_BITS).  With sync=True the node count and depth are read
+        DEFAULT_SORT_BITS).  treelet_rounds ("radix" only): rounds of SAH
+        treelet restructuring before the Box4 collapse (fewer node visits per
+        ray, the same leaves).  With sync=True the node count and depth are read
         back from the device; otherwise node_count is the capacity and depth 0
         (unknown)."""
         dev = pos.device
         n = idx.shape[0] if idx is not None else pos.numel() // 9
-        cfg = build_cfg(algorithm, max_leaf_prims, sort_bits)
+        cfg = build_cfg(algorithm, max_leaf_prims, sort_bits, treelet_rounds)
         cap, need = build_requirements(n, cfg)
         nodes = torch.empty(max(cap, 1) * NODE_BYTES, dtype=torch.uint8, device=dev)
         tris = torch.empty(max(n, 1) * TRI_BYTES, dtype=torch.uint8, device=dev)
