@@ -165,16 +165,19 @@ class GpuBVH4 {
 public:
     // SoftwareBVH::build(mesh, maxPrimsPerLeaf) + SoftwareBVH4::fromBVH2: a device build
     // of the same Box4 format (RT_BUILD_RADIX: Morton radix tree collapsed with
-    // fromBVH2's rule; RT_BUILD_IMPLICIT: the fast_bvh_builder tree).
+    // fromBVH2's rule; RT_BUILD_IMPLICIT: the fast_bvh_builder tree).  treeletRounds
+    // (RT_BUILD_RADIX only): rounds of SAH treelet restructuring before the collapse.
     template <class Mesh>
     static GpuBVH4 build(const Mesh& mesh, uint32_t maxPrimsPerLeaf = 4,
-                         uint32_t algorithm = RT_BUILD_RADIX, hipStream_t s = nullptr) {
+                         uint32_t algorithm = RT_BUILD_RADIX, hipStream_t s = nullptr,
+                         uint32_t treeletRounds = 0) {
         const std::vector<float> pos = packed_positions(mesh);
         GpuBVH4 b;
         b.s_ = s;
         b.pos_.upload(pos.data(), pos.size(), s);
         b.idx_.upload(mesh.indices.data(), mesh.indices.size(), s);
-        b.build_device((uint32_t)(mesh.indices.size() / 3), maxPrimsPerLeaf, algorithm);
+        b.build_device((uint32_t)(mesh.indices.size() / 3), maxPrimsPerLeaf, algorithm,
+                       treeletRounds);
         return b;
     }
 
@@ -263,8 +266,8 @@ public:
     hipStream_t stream() const { return s_; }
 
 private:
-    void build_device(uint32_t n, uint32_t maxp, uint32_t algorithm) {
-        rt_build_cfg cfg = {algorithm, maxp, 0u, 0u};
+    void build_device(uint32_t n, uint32_t maxp, uint32_t algorithm, uint32_t rounds) {
+        rt_build_cfg cfg = {algorithm, maxp, 0u, rounds};
         uint32_t cap = 0;
         size_t scratchBytes = 0;
         check(rt_build_requirements(n, &cfg, &cap, &scratchBytes), "rt_build_requirements");
