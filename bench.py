@@ -70,6 +70,7 @@ def build_time_4k(iters=200, warm=20):
 
 
 FRAME_TREELET_ROUNDS = (0, 4)  # treelet restructuring rounds the frame's tree is tuned over
+AO_TREELET_ROUNDS = (0, 8, 16)  # the same for the AO frame's terrain
 
 
 def same_frame(a, b):
@@ -549,7 +550,7 @@ def ao_frame_leg(frames=100, w=1920, h=1080, tuner=None):
                                                    (grid, gx, None, m_grid, rt.INST_TRANSLATE)])
             state["scene"], state["leaf"], state["rounds"] = scenes[(leaf, rounds)], leaf, rounds
 
-        cands = [(leaf, r) for r in FRAME_TREELET_ROUNDS for leaf in autotune.LEAF_SIZES]
+        cands = [(leaf, r) for r in AO_TREELET_ROUNDS for leaf in autotune.LEAF_SIZES]
         use(tuner.frame_leaf_size(use, lambda: frame(0), lambda: (t, prim, inst, ao),
                                   tag="ao-terrain", sizes=cands, same=same_ao_frame))
         scenes.clear()

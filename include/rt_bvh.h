@@ -147,7 +147,7 @@ typedef struct rt_build_cfg {
     uint32_t sort_bits;      /* digit width of the multi-launch Morton radix sort
                                 (n > 4096): 4, 6, 8 or 10 bits = 8, 5, 4 or 3
                                 passes; 0 = 8.  The output does not depend on it. */
-    uint32_t treelet_rounds; /* RADIX only: rounds of treelet restructuring (0..8, 0 =
+    uint32_t treelet_rounds; /* RADIX only: rounds of treelet restructuring (0..16, 0 =
                                 none; no reference counterpart): bottom-up, every inner
                                 node's treelet of up to 7 subtrees is rebuilt as the
                                 binary tree of least SAH cost over them (Karras & Aila

@@ -1705,7 +1705,7 @@ __global__ __launch_bounds__(1024) void k_rt_top(RadixArrays A) {
 // reach the nodes does not change the tree.
 // ---------------------------------------------------------------------------
 constexpr int kTreeletLeaves = 7;
-constexpr uint32_t kMaxTreeletRounds = 8;
+constexpr uint32_t kMaxTreeletRounds = 16;
 constexpr uint32_t kTreeletSets = 1u << kTreeletLeaves;
 constexpr uint32_t kTreeletBlock = 256;  // LDS: kTreeletSets floats per thread, 128 KiB
 constexpr float kCostInner = 1.2f;

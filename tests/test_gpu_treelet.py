@@ -77,4 +77,4 @@ def test_treelet_rejects_bad_config(rt):
     with pytest.raises(RuntimeError):
         rt.GpuBVH4.build(pos, None, algorithm="implicit", treelet_rounds=1)
     with pytest.raises(RuntimeError):
-        rt.GpuBVH4.build(pos, None, algorithm="radix", treelet_rounds=9)
+        rt.GpuBVH4.build(pos, None, algorithm="radix", treelet_rounds=17)
