@@ -5,7 +5,8 @@ Workload (one step = one frame of the shadow any-hit configuration):
          seeded random radial displacement +-2 % (generated on the device),
          Box4 BVH built on the device once (inputs resident before timing;
          radix tree, leaves of at most 1-4 triangles and 0 or 4 rounds of
-         treelet restructuring, as tuned on whole frames)
+         treelet restructuring, as tuned on whole frames; the tree is built
+         once, outside the timed frames)
   step   1920x1080 pinhole camera rays (60 deg vertical FOV, every pixel hits)
          -> closest-hit trace -> 16 shadow rays per pixel towards 16 lights
          uniform in the box 3x the mesh bounds (tmax = light distance)
@@ -496,9 +497,9 @@ def ao_frame_leg(frames=100, w=1920, h=1080, tuner=None):
     two-level scene.  Per frame: regenerate the grid and rebuild its tree
     (fused path, same buffers), 1920x1080 primary closest hit, 1 SPP cosine
     AO rays (tmax 0.5), any hit, AO resolve, 7x7 bilateral filter.  With a
-    tuner, the terrain tree's leaf size is timed on whole frames (a size that
-    changes the filtered image is left out) and both scene traces use its
-    measured launch shapes."""
+    tuner, the static terrain tree (leaf size x treelet rounds) is timed on
+    whole frames (a tree that changes the frame beyond ties, same_ao_frame,
+    is left out) and both scene traces use their measured launch shapes."""
     tp, ti = rt.gen_heightfield(707, 707, 4.0, 0.3, 31)
     terrain = rt.GpuBVH4.build(tp, ti, algorithm="radix")
     gx = rt.gen_dynamic_grid(4096, 0.0, 2.0)
