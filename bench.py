@@ -4,7 +4,7 @@ Workload (one step = one frame of the shadow any-hit configuration):
   mesh   UV sphere 1250 x 1000 quads = 2,500,000 FP32 triangles, radius 1,
          seeded random radial displacement +-2 % (generated on the device),
          Box4 BVH built on the device once (inputs resident before timing;
-         radix tree, leaves of at most 1-4 triangles and 0 or 4 rounds of
+         radix tree, leaves of at most 1-4 triangles and 0, 4 or 8 rounds of
          treelet restructuring, as tuned on whole frames; the tree is built
          once, outside the timed frames)
   step   1920x1080 pinhole camera rays (60 deg vertical FOV, every pixel hits)
@@ -71,6 +71,7 @@ def build_time_4k(iters=200, warm=20):
 
 
 FRAME_TREELET_ROUNDS = (0, 4)  # treelet restructuring rounds the frame's tree is tuned over
+FRAME_EXTRA_TREES = ((2, 8), (1, 8))  # (leaf size, rounds) also tried for the frame
 AO_TREELET_ROUNDS = (0, 8, 16)  # the same for the AO frame's terrain
 
 
@@ -174,6 +175,7 @@ class Frame:
             self.leaf, self.rounds = leaf, rounds
 
         cands = [(leaf, r) for r in FRAME_TREELET_ROUNDS for leaf in autotune.LEAF_SIZES]
+        cands += list(FRAME_EXTRA_TREES)
         use(tuner.frame_leaf_size(use, self.step, lambda: (self.t, self.prim, self.masks),
                                   tag="sphere-2p5m", sizes=cands, same=same_frame))
         trees.clear()
