@@ -240,6 +240,28 @@ def algorithmic_bytes_shadow(frame):
     return as_size + r * 32 + r / 8.0
 
 
+def cpu_model():
+    """The host CPU's model name (/proc/cpuinfo), or the platform string."""
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    import platform
+    return platform.processor() or platform.machine()
+
+
+# A GPU's share of the host: the CPU baseline runs on every CPU this process
+# may use, up to the 16 host CPUs allotted per GPU on a shared node.
+CPU_SHARE = 16
+
+
+def cpu_threads():
+    return max(1, min(CPU_SHARE, len(os.sched_getaffinity(0))))
+
+
 def cpu_baseline(frame, threads):
     """The CPU oracle's BVH4 traversal on the host cores over the same tree and
     the same rays: one whole frame (all primary rays closest-hit, all shadow
@@ -256,6 +278,7 @@ def cpu_baseline(frame, threads):
     dt = time.perf_counter() - t0
     n = len(prim_rays) + len(sh_rays)
     return {"value": round(n / dt / 1e6, 3), "unit": "Mrays/s", "cores": threads, "kind": "port",
+            "cpu": cpu_model(), "host_cpus_visible": len(os.sched_getaffinity(0)),
             "sample": f"one whole frame of the same workload and tree: {len(prim_rays)} primary "
                       f"closest-hit + {len(sh_rays)} shadow any-hit rays; {dt:.1f} s wall on "
                       f"{threads} threads",
@@ -493,10 +516,10 @@ def batched_2p5m_leg(frame, requests=64, seed=11, tuner=None):
             "speedup": round(plain_ms / batched_ms, 2), "identical_results": same}
 
 
-def ao_frame_leg(frames=100, w=1920, h=1080, tuner=None):
+def ao_frame_leg(frames=200, w=1920, h=1080, tuner=None):
     """Dynamic AO frame: a static 999,698-triangle seeded random-displaced
-    heightfield and a 4,096-triangle deforming grid (t_k = 0.05 k) as a
-    two-level scene.  Per frame: regenerate the grid and rebuild its tree
+    heightfield and a 4,096-triangle deforming grid (t_k = 0.05 k, the 200
+    time steps k = 0..199 of the configuration) as a two-level scene.  Per frame: regenerate the grid and rebuild its tree
     (fused path, same buffers), 1920x1080 primary closest hit, 1 SPP cosine
     AO rays (tmax 0.5), any hit, AO resolve, 7x7 bilateral filter.  With a
     tuner, the static terrain tree (leaf size x treelet rounds) is timed on
@@ -801,13 +824,17 @@ def main():
         out["serving"] = serving_leg(tuner=tuner)
         out["p99_request_latency_ms"] = out["serving"]["p99_ms"]
     if tuner is not None:
-        out["tuning"] = tuner.report()
+        # every tuned choice with its time and the default's: on stderr, so the
+        # result line stays short enough to be read whole from a log's tail
+        print("tuning " + json.dumps(tuner.report()), file=sys.stderr, flush=True)
     if not a.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
-        out["cpu_baseline"] = cpu_baseline(f, threads)
+        out["cpu_baseline"] = cpu_baseline(f, cpu_threads())
     else:
         out["cpu_baseline"] = None
-    print(json.dumps(out))
+    line = json.dumps(out)
+    if len(line) > 4000:
+        print(f"note: result line is {len(line)} bytes", file=sys.stderr)
+    print(line)
 
 
 if __name__ == "__main__":

@@ -73,7 +73,11 @@ typedef struct rt_tri64 {
 
 /* A device-resident BVH.  depth = node levels on the longest path (sizes the
  * per-lane traversal stack: at most 3 * depth + 1 entries are ever live);
- * 0 = unknown (any depth up to 64, the bound for device-built trees).      */
+ * trees deeper than 64 levels are rejected (RT_ERR_CAPACITY).  0 = unknown:
+ * the stack is sized for 64 levels and, should the tree be deeper, pushes
+ * beyond it are dropped (never written past the stack).  Device builds
+ * record the true depth in rt_build_info.depth (a radix tree is at most 62
+ * levels; treelet restructuring may deepen it).                            */
 typedef struct rt_bvh_dev {
     const rt_node4* nodes;
     const rt_tri64* tris;
@@ -184,7 +188,8 @@ rt_status rt_gpu_refit(rt_stream stream, const float* d_pos, const uint32_t* d_i
  * than their parents (pre-order or level order, as every builder here and
  * SoftwareBVH4::fromBVH2 number them).  node_count = the tree's node count
  * (rt_build_info.node_count, not the capacity: every node is visited),
- * depth = its node levels (rt_build_info.depth; 0 = 64);
+ * depth = its node levels (rt_build_info.depth, at most 64; 0 = 64: a
+ * deeper tree must pass its depth and is rejected, RT_ERR_ARG);
  * scratch >= 4 * node_count bytes.  No
  * reference counterpart (the reference rebuilds deforming meshes per frame,
  * src/main.cpp:871-885).                                                    */
@@ -512,6 +517,16 @@ rt_status rt_gen_ao_rays(rt_stream stream, const float* d_pos, const uint32_t* d
                          const rt_rays_soa* primary, const uint32_t* d_prim, const float* d_t,
                          uint32_t w, uint32_t h, uint32_t frame, float offset, float tmin,
                          float tmax, const rt_rays_out* out);
+/* benchmarks/exp0_probe/shaders/software_bvh_bvh4.comp:50-94 (the exp0
+ * probe's procedural rays, microbench_main.cpp:290-294 passes cam (0,50,-45),
+ * look (0,0,5)): coherent != 0, ray i is pixel (i % 1024, i / 1024) of a
+ * 1024-wide grid, dir = normalize(fwd + 0.55 u right + 0.55 v up); else
+ * origin (U*80-40, U*15+15, U*80-40) and dir normalize(U*0.8-0.4, -1-U*0.5,
+ * U*0.8-0.4) from pcg_hash(6 i + k) * (1 / (2^32 - 1)).  Every ray gets tmin /
+ * tmax; bit-identical to the shader's float arithmetic with one rounding per
+ * operator.                                                               */
+rt_status rt_gen_exp0_rays(rt_stream stream, uint32_t count, int coherent, const float cam[3],
+                           const float look[3], float tmin, float tmax, const rt_rays_out* out);
 rt_status rt_gen_random_rays(rt_stream stream, uint32_t count, uint32_t seed, const float lo[3],
                              const float hi[3], float tmax, const rt_rays_out* out);
 

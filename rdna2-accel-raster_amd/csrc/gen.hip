@@ -389,6 +389,48 @@ __global__ void k_random_rays(uint32_t count, uint32_t seed, float lx, float ly,
             r * cosf(phi), r * sinf(phi), z, tmax);
 }
 
+// software_bvh_bvh4.comp:50-94 procedural rays of the exp0 probe.
+// Coherent: ray i is pixel (i % 1024, i / 1024) of a 1024-wide grid seen from
+// cam towards look, dir = normalize(fwd + right u 0.55 + up v 0.55) with the
+// basis fwd = normalize(look - cam), right = normalize(cross(fwd, +y)), up =
+// cross(right, fwd).  Incoherent: origin and direction from six pcg_hash
+// values of i (hashFloat = float(pcg_hash(6 i + k)) * float(1 / (2^32 - 1))):
+// origin in [-40,40] x [15,30] x [-40,40], dir = normalize(U(-.4,.4),
+// -1 - U(0,.5), U(-.4,.4)).  Products are rounded one operator at a time in
+// the shader's left-to-right order (the basis is the same for every ray and
+// passed in, computed on the host the same way).
+struct Exp0Basis {
+    float F[3], R[3], U[3], cam[3];
+    float tmin, tmax;
+    uint32_t height;
+};
+
+__global__ void k_exp0_rays(uint32_t count, int coherent, Exp0Basis b, rt_rays_out o) {
+    const uint32_t i = blockIdx.x * kB + threadIdx.x;
+    if (i >= count) return;
+    float o0, o1, o2, d0, d1, d2;
+    if (coherent) {
+        const uint32_t x = i % 1024u, y = i / 1024u;
+        const float u = ((float)x + 0.5f) / 1024.0f * 2.0f - 1.0f;
+        const float v = ((float)y + 0.5f) / (float)b.height * 2.0f - 1.0f;
+        o0 = b.cam[0]; o1 = b.cam[1]; o2 = b.cam[2];
+        d0 = b.F[0] + b.R[0] * u * 0.55f + b.U[0] * v * 0.55f;
+        d1 = b.F[1] + b.R[1] * u * 0.55f + b.U[1] * v * 0.55f;
+        d2 = b.F[2] + b.R[2] * u * 0.55f + b.U[2] * v * 0.55f;
+    } else {
+        const float hs = 1.0f / 4294967295.0f;
+        const uint32_t s = i * 6u;
+        o0 = (float)pcg_hash(s + 0u) * hs * 80.0f - 40.0f;
+        o1 = (float)pcg_hash(s + 1u) * hs * 15.0f + 15.0f;
+        o2 = (float)pcg_hash(s + 2u) * hs * 80.0f - 40.0f;
+        d0 = (float)pcg_hash(s + 3u) * hs * 0.8f - 0.4f;
+        d1 = -1.0f - (float)pcg_hash(s + 4u) * hs * 0.5f;
+        d2 = (float)pcg_hash(s + 5u) * hs * 0.8f - 0.4f;
+    }
+    normalize3(d0, d1, d2);
+    put_ray(o, i, o0, o1, o2, b.tmin, d0, d1, d2, b.tmax);
+}
+
 __global__ void k_convert48(const float* __restrict__ t48, const uint32_t* __restrict__ order,
                             uint32_t n, rt_tri64* __restrict__ out) {
     const uint32_t i = blockIdx.x * kB + threadIdx.x;
@@ -602,6 +644,32 @@ extern "C" rt_status rt_gen_random_rays(rt_stream stream, uint32_t count, uint32
     hipLaunchKernelGGL(k_random_rays, dim3(ceil_div(count, kB)), dim3(kB), 0, as_stream(stream),
                        count, seed, lo[0], lo[1], lo[2], hi[0], hi[1], hi[2], tmax, *out);
     return launch_status("rt_gen_random_rays");
+}
+
+extern "C" rt_status rt_gen_exp0_rays(rt_stream stream, uint32_t count, int coherent,
+                                      const float cam[3], const float look[3], float tmin,
+                                      float tmax, const rt_rays_out* out) {
+    if (!cam || !look || !out) return RT_ERR_ARG;
+    if (count == 0) return RT_OK;
+    // the shader's basis (software_bvh_bvh4.comp:76-78), float operators in order
+    Exp0Basis b;
+    float fx = look[0] - cam[0], fy = look[1] - cam[1], fz = look[2] - cam[2];
+    float l = std::sqrt(fx * fx + fy * fy + fz * fz);
+    fx = fx / l; fy = fy / l; fz = fz / l;
+    // cross(fwd, (0, 1, 0))
+    float rx = fy * 0.0f - fz * 1.0f, ry = fz * 0.0f - fx * 0.0f, rz = fx * 1.0f - fy * 0.0f;
+    l = std::sqrt(rx * rx + ry * ry + rz * rz);
+    rx = rx / l; ry = ry / l; rz = rz / l;
+    b.F[0] = fx; b.F[1] = fy; b.F[2] = fz;
+    b.R[0] = rx; b.R[1] = ry; b.R[2] = rz;
+    b.U[0] = ry * fz - rz * fy; b.U[1] = rz * fx - rx * fz; b.U[2] = rx * fy - ry * fx;
+    for (int k = 0; k < 3; ++k) b.cam[k] = cam[k];
+    b.tmin = tmin;
+    b.tmax = tmax;
+    b.height = (uint32_t)(((uint64_t)count + 1023u) / 1024u);
+    hipLaunchKernelGGL(k_exp0_rays, dim3(ceil_div(count, kB)), dim3(kB), 0, as_stream(stream),
+                       count, coherent, b, *out);
+    return launch_status("rt_gen_exp0_rays");
 }
 
 extern "C" rt_status rt_convert_tris48(rt_stream stream, const float* d_tris48,

@@ -33,9 +33,13 @@ inline hipStream_t as_stream(rt_stream s) { return reinterpret_cast<hipStream_t>
 
 inline uint32_t ceil_div(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
 
-// Deepest tree the traversal kernels accept (node levels on a root-to-leaf
-// path).  A radix tree over 30-bit codes plus 32-bit position tie-breaks is
-// at most 62 binary levels deep, so every device-built tree fits.
+// Deepest tree the traversal and refit entry points accept (node levels on a
+// root-to-leaf path).  A radix tree over 30-bit codes plus 32-bit position
+// tie-breaks is at most 62 binary levels deep, so its Box4 collapse fits;
+// treelet restructuring can deepen a tree, and the build records the true
+// depth in rt_build_info.depth (a deeper tree is rejected by the entry points
+// given its depth).  A trace given depth 0 (unknown) sizes its stack for this
+// bound and drops pushes beyond it instead of writing past the stack.
 constexpr uint32_t kMaxDepth = 64;
 
 // include/radv_bvh.hpp:193-215 compute_tree_layout (no 8-level cap).

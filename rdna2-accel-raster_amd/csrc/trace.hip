@@ -118,9 +118,14 @@ struct Stack {
 
 // sp counts the entries; spo = sp * bs is the LDS slot offset of entry sp,
 // kept by additions (a multiply per push / pop is a quarter-rate VALU op);
-// lso = (ls - 1) * bs is the last LDS slot's.
+// lso = (ls - 1) * bs is the last LDS slot's.  The large private size serves
+// trees of unknown depth (sized for kMaxDepth levels): there a push beyond
+// the stack's capacity is dropped rather than written past it (only a tree
+// deeper than kMaxDepth, which the entry points reject given its depth, can
+// reach it).
 #define RT_SPUSH(v)                                                      \
     do {                                                                 \
+        if (PRIV == kPrivLarge && sp >= ls + PRIV) break;                \
         if (sp < ls) col[spo] = (v);                                     \
         else st.priv[sp - ls] = (v);                                     \
         ++sp;                                                            \

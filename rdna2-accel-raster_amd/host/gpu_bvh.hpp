@@ -142,10 +142,25 @@ public:
         for (int k = 0; k < 8; ++k) p[k] = f[k].data();
         return Rays(p, in.size(), s);
     }
+    // The exp0 probe's procedural rays generated on the device
+    // (software_bvh_bvh4.comp:50-94, rt_gen_exp0_rays): a 1024-wide camera grid
+    // from cam towards look (coherent) or pcg_hash origins / directions.
+    static Rays exp0(size_t n, bool coherent, const float cam[3], const float look[3],
+                     float tmin, float tmax, hipStream_t s) {
+        Rays r;
+        for (auto& a : r.a_) a.resize(n);
+        r.n_ = n;
+        const rt_rays_out o = {r.a_[0].data(), r.a_[1].data(), r.a_[2].data(), r.a_[3].data(),
+                               r.a_[4].data(), r.a_[5].data(), r.a_[6].data(), r.a_[7].data()};
+        check(rt_gen_exp0_rays(s, (uint32_t)n, coherent ? 1 : 0, cam, look, tmin, tmax, &o),
+              "rt_gen_exp0_rays");
+        return r;
+    }
     rt_rays_soa view() const {
         return {a_[0].data(), a_[1].data(), a_[2].data(), a_[3].data(),
                 a_[4].data(), a_[5].data(), a_[6].data(), a_[7].data()};
     }
+    std::vector<float> download(int field, hipStream_t s) const { return a_[field].download(s); }
     size_t size() const { return n_; }
 
 private:

@@ -26,6 +26,7 @@ _STATUS = {0: "RT_OK", 1: "RT_ERR_ARG", 2: "RT_ERR_CAPACITY", 3: "RT_ERR_HIP",
            4: "RT_ERR_NOMEM", 5: "RT_ERR_UNSUPPORTED"}
 
 NODE_BYTES = 128  # rt_node4 == GpuBvh4Node
+MAX_DEPTH = 64    # deepest tree the trace / refit entry points take
 TRI_BYTES = 64    # rt_tri64 == RadvBvhTriangleNode
 
 
@@ -174,6 +175,7 @@ _SIGS = {
     "rt_gen_shadow_rays": [vp, vp, vp, vp, vp, vp, u32, vp, u32, f32, vp],
     "rt_gen_ao_rays": [vp, vp, vp, vp, vp, vp, u32, u32, u32, f32, f32, f32, vp],
     "rt_gen_random_rays": [vp, u32, u32, vp, vp, f32, vp],
+    "rt_gen_exp0_rays": [vp, u32, C.c_int, vp, vp, f32, f32, vp],
     "rt_pack_light_masks": [vp, vp, u32, u32, vp],
     "rt_gpu_build_radv": [vp, vp, vp, u32, u32, vp, C.c_uint64, vp],
     "rt_gpu_build_batch_table_bytes": [u32, vp],
@@ -248,6 +250,49 @@ def _f3(v):
     return (C.c_float * 3)(*[float(x) for x in v])
 
 
+_INDEX_DTYPES = (torch.int32, getattr(torch, "uint32", torch.int32))
+
+
+def check_mesh(pos, idx):
+    """Raise ValueError unless pos / idx are what the C ABI reads: pos a
+    contiguous float32 device tensor (3 floats per vertex, or 9 per triangle
+    when idx is None), idx None or a contiguous int32 / uint32 device tensor
+    of shape (T, 3)."""
+    if not isinstance(pos, torch.Tensor) or not pos.is_cuda:
+        raise ValueError("pos: a device tensor expected")
+    if pos.dtype != torch.float32 or not pos.is_contiguous():
+        raise ValueError(f"pos: contiguous float32 expected (got {pos.dtype}, "
+                         f"contiguous={pos.is_contiguous()})")
+    if idx is None:
+        if pos.numel() % 9:
+            raise ValueError("pos without idx: 9 floats per triangle expected")
+        return
+    if not isinstance(idx, torch.Tensor) or not idx.is_cuda:
+        raise ValueError("idx: a device tensor expected")
+    if idx.dtype not in _INDEX_DTYPES or not idx.is_contiguous():
+        raise ValueError(f"idx: contiguous int32 / uint32 expected (got {idx.dtype}, "
+                         f"contiguous={idx.is_contiguous()})")
+    if idx.dim() != 2 or idx.shape[1] != 3:
+        raise ValueError(f"idx: shape (T, 3) expected (got {tuple(idx.shape)})")
+    if pos.numel() % 3:
+        raise ValueError("pos: 3 floats per vertex expected")
+
+
+def check_out(t, count, dtypes, what):
+    """Raise ValueError unless t is a contiguous device tensor of one of
+    `dtypes` holding at least `count` elements."""
+    if not isinstance(t, torch.Tensor) or not t.is_cuda:
+        raise ValueError(f"{what}: a device tensor expected")
+    if t.dtype not in dtypes or not t.is_contiguous():
+        raise ValueError(f"{what}: contiguous {dtypes} expected (got {t.dtype})")
+    if t.numel() < count:
+        raise ValueError(f"{what}: {t.numel()} elements < {count} needed")
+
+
+_I32 = _INDEX_DTYPES
+_F32 = (torch.float32,)
+
+
 # ------------------------------------------------------------------ layout --
 def tree_layout(n):
     counts = (u32 * 32)()
@@ -279,7 +324,8 @@ class Rays:
     FIELDS = ("ox", "oy", "oz", "tmin", "dx", "dy", "dz", "tmax")
 
     def __init__(self, data):
-        assert data.dim() == 2 and data.shape[0] == 8 and data.dtype == torch.float32
+        if data.dim() != 2 or data.shape[0] != 8 or data.dtype != torch.float32:
+            raise ValueError("Rays: a float32 (8, n) tensor expected")
         self.data = data.contiguous()
 
     @staticmethod
@@ -334,6 +380,7 @@ class GpuBVH4:
         ray, the same leaves).  With sync=True the node count and depth are read
         back from the device; otherwise node_count is the capacity and depth 0
         (unknown)."""
+        check_mesh(pos, idx)
         dev = pos.device
         n = idx.shape[0] if idx is not None else pos.numel() // 9
         cfg = build_cfg(algorithm, max_leaf_prims, sort_bits, treelet_rounds)
@@ -356,6 +403,8 @@ class GpuBVH4:
         bvh.info = info
         if sync:
             bvh.node_count, bvh.depth = [int(v) for v in info[:2].cpu().tolist()]
+            if bvh.depth > MAX_DEPTH:
+                raise RtError("rt_gpu_build", 2, f"tree depth {bvh.depth} > {MAX_DEPTH}")
         bvh.sorted_ids, bvh.codes = ids, codes
         bvh.pos, bvh.idx = pos, idx
         return bvh
@@ -365,6 +414,10 @@ class GpuBVH4:
         implicit refit (rt_gpu_refit); any other tree (radix, uploaded
         SoftwareBVH4), or generic=True, the generic one (rt_gpu_refit_bvh4),
         which needs the node count and depth (sync=True builds have them)."""
+        check_mesh(pos, idx)
+        if _tri_count(pos, idx) != self.tri_count:
+            raise ValueError(f"refit: {_tri_count(pos, idx)} triangles, the tree has "
+                             f"{self.tri_count}")
         if generic is None:
             generic = getattr(self, "algorithm", None) != "implicit"
         if generic:
@@ -431,6 +484,9 @@ class GpuBVH4:
         dev = rays.data.device
         prim = out[0] if out else torch.empty(n, dtype=torch.int32, device=dev)
         t = out[1] if out else torch.empty(n, dtype=torch.float32, device=dev)
+        if out:
+            check_out(prim, n, _I32, "out[0] (prim)")
+            check_out(t, n, _F32, "out[1] (t)")
         box = tri = None
         flags = RT_CLOSEST
         if counters:
@@ -451,6 +507,7 @@ class GpuBVH4:
         dev = rays.data.device
         bits = out if out is not None else torch.empty(max(1, (n + 31) // 32), dtype=torch.int32,
                                                        device=dev)
+        check_out(bits, (n + 31) // 32, _I32, "out (occlusion bits)")
         box = tri = None
         flags = RT_ANY
         if counters:
@@ -580,6 +637,7 @@ class BuildBatch:
         order = sorted(range(self.count), key=lambda i: -_tri_count(*meshes[i]))
         need = {}
         for i in order:
+            check_mesh(*meshes[i])
             n = _tri_count(*meshes[i])
             if n > BATCH_BUILD_MAX:
                 raise ValueError(f"mesh {i}: {n} triangles > BATCH_BUILD_MAX")
@@ -651,6 +709,7 @@ class TraceBatch:
             if mode == "any":
                 bits = out if out is not None else torch.empty(max(1, (n + 31) // 32),
                                                                dtype=torch.int32, device=dev)
+                check_out(bits, (n + 31) // 32, _I32, f"item {i} occlusion bits")
                 it.flags, it.occlusion_bits = RT_ANY, ptr(bits)
                 it.hits = rt_hits_soa(None, None)
                 self.outputs.append(bits)
@@ -658,6 +717,8 @@ class TraceBatch:
                 prim, t = out if out is not None else (
                     torch.empty(n, dtype=torch.int32, device=dev),
                     torch.empty(n, dtype=torch.float32, device=dev))
+                check_out(prim, n, _I32, f"item {i} prim")
+                check_out(t, n, _F32, f"item {i} t")
                 it.flags, it.occlusion_bits = RT_CLOSEST, None
                 it.hits = rt_hits_soa(ptr(prim), ptr(t))
                 self.outputs.append((prim, t))
@@ -1011,6 +1072,21 @@ def gen_random_rays(count, seed, lo, hi, tmax=1e30, stream=None, out=None):
     o = rays.out()
     call("rt_gen_random_rays", stream_handle(stream), u32(count), u32(seed), _f3(lo), _f3(hi),
          f32(tmax), C.byref(o))
+    return rays
+
+
+def gen_exp0_rays(count, coherent, cam=(0.0, 50.0, -45.0), look=(0.0, 0.0, 5.0), tmin=0.01,
+                  tmax=1000.0, stream=None, out=None):
+    """The exp0 probe's procedural rays on the device (rt_gen_exp0_rays,
+    software_bvh_bvh4.comp:50-94): a 1024-wide camera grid (coherent) or
+    pcg_hash origins and directions; cam / look default to the probe's
+    (microbench_main.cpp:290-294)."""
+    rays = out if out is not None else Rays.empty(count)
+    if len(rays) < count:
+        raise ValueError(f"out holds {len(rays)} rays < {count}")
+    o = rays.out()
+    call("rt_gen_exp0_rays", stream_handle(stream), u32(count), C.c_int(1 if coherent else 0),
+         _f3(cam), _f3(look), f32(tmin), f32(tmax), C.byref(o))
     return rays
 
 

@@ -5,7 +5,9 @@
 // its -O3 -march=native build evaluates it), the reference's SoftwareBVH4
 // built on the host (the oracle's byte-exact restatement, test
 // infrastructure) and uploaded with GpuBVH4::fromHost, the reference's
-// procedural coherent / incoherent rays (1 << 20, tmin 0.01, tmax 1000).
+// procedural coherent / incoherent rays (1 << 20, tmin 0.01, tmax 1000),
+// generated on the device (rt_gen_exp0_rays; checked bit for bit against the
+// oracle's restatement on the first terrain).
 // Prints the CSV's software-path columns plus this GPU's traversal rate:
 //   Triangles,RayPattern,HitRate_SW_pct,AvgBoxTests_PerRay,AvgBoxTests_HitOnly,
 //   AvgTriTests_PerRay,AvgTriTests_HitOnly,GPU_MRays_per_sec
@@ -64,6 +66,28 @@ std::string fmt(double v) {
     return b;
 }
 
+// The device-generated exp0 rays equal the oracle's restatement of the shader's
+// generator (software_bvh_bvh4.comp:50-94) bit for bit.  Returns false on a mismatch.
+bool rays_match_oracle(const rtgpu::Rays& rays, uint32_t n, int coherent, const float cam[3],
+                       const float look[3], hipStream_t s) {
+    std::vector<float> f[6];
+    for (auto& v : f) v.assign(n, 0.0f);
+    or_exp0_rays(n, coherent, cam, look, f[0].data(), f[1].data(), f[2].data(), f[3].data(),
+                 f[4].data(), f[5].data());
+    const int field[6] = {0, 1, 2, 4, 5, 6};
+    for (int k = 0; k < 6; ++k) {
+        const std::vector<float> d = rays.download(field[k], s);
+        if (std::memcmp(d.data(), f[k].data(), (size_t)n * 4) != 0) {
+            std::printf("# device exp0 rays (%s) differ from the oracle in field %d\n",
+                        coherent ? "coherent" : "incoherent", field[k]);
+            return false;
+        }
+    }
+    std::printf("# device exp0 rays (%s) equal the oracle's bit for bit\n",
+                coherent ? "coherent" : "incoherent");
+    return true;
+}
+
 // The RADV image of the mesh (Morton order) traced over its bytes equals the implicit
 // Box4 build of the same mesh; the image metrics of the two hit-distance images are
 // those of identical images.  Returns 1 on a mismatch.
@@ -72,18 +96,8 @@ int radv_check(const SceneMesh& mesh, hipStream_t s) {
     rtgpu::GpuBVH4 box = rtgpu::GpuBVH4::build(mesh, 4, RT_BUILD_IMPLICIT, s);
     const rt_radv_info info = img.describe();
     const uint32_t w = 256, h = 256, n = w * h;
-    std::vector<float> f[8];
-    for (auto& v : f) v.assign(n, 0.0f);
     const float cam[3] = {0.0f, 50.0f, -45.0f}, look[3] = {0.0f, 0.0f, 5.0f};
-    or_exp0_rays(n, 0, cam, look, f[0].data(), f[1].data(), f[2].data(), f[4].data(),
-                 f[5].data(), f[6].data());
-    for (uint32_t r = 0; r < n; ++r) {
-        f[3][r] = 0.01f;
-        f[7][r] = 1000.0f;
-    }
-    const float* p[8];
-    for (int k = 0; k < 8; ++k) p[k] = f[k].data();
-    rtgpu::Rays rays(p, n, s);
+    rtgpu::Rays rays = rtgpu::Rays::exp0(n, false, cam, look, 0.01f, 1000.0f, s);
     rtgpu::Hits a, b;
     img.intersect(rays, a, true);
     box.intersect(rays, b, true);
@@ -155,18 +169,11 @@ int main(int argc, char** argv) {
                                                        ntri, s);
         for (int coherent = 1; coherent >= 0; --coherent) {
             const uint32_t n = 1u << 20;
-            std::vector<float> f[8];
-            for (auto& v : f) v.assign(n, 0.0f);
             const float cam[3] = {0.0f, 50.0f, -45.0f}, look[3] = {0.0f, 0.0f, 5.0f};
-            or_exp0_rays(n, coherent, cam, look, f[0].data(), f[1].data(), f[2].data(),
-                         f[4].data(), f[5].data(), f[6].data());
-            for (uint32_t r = 0; r < n; ++r) {
-                f[3][r] = 0.01f;
-                f[7][r] = 1000.0f;
-            }
-            const float* p[8];
-            for (int k = 0; k < 8; ++k) p[k] = f[k].data();
-            rtgpu::Rays rays(p, n, s);
+            // the probe's procedural rays, generated on the device (rt_gen_exp0_rays)
+            rtgpu::Rays rays = rtgpu::Rays::exp0(n, coherent != 0, cam, look, 0.01f, 1000.0f, s);
+            if (target == targets.front() && !rays_match_oracle(rays, n, coherent, cam, look, s))
+                return 1;
             rtgpu::Hits hits;
             bvh.intersect(rays, hits, true);
             const std::vector<uint32_t> prim = hits.prim.download(s);
@@ -201,9 +208,8 @@ int main(int argc, char** argv) {
                         10.0 * n / (ms * 1e-3) / 1e6);
             // the BVH2 shaders on the same rays: tmin 0.01 (scratch stack) and 0.001 (LDS)
             for (int lds = 0; lds < 2; ++lds) {
-                for (uint32_t r = 0; r < n; ++r) f[3][r] = lds ? 0.001f : 0.01f;
-                for (int k = 0; k < 8; ++k) p[k] = f[k].data();
-                rtgpu::Rays r2(p, n, s);
+                rtgpu::Rays r2 = rtgpu::Rays::exp0(n, coherent != 0, cam, look,
+                                                   lds ? 0.001f : 0.01f, 1000.0f, s);
                 rtgpu::Hits h2;
                 bvh2.intersect(r2, h2, true);
                 const std::vector<uint32_t> p2 = h2.prim.download(s);

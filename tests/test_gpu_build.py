@@ -283,3 +283,39 @@ def test_terrain_generator(rt, O, target, fused):
     np.testing.assert_array_equal(p[:, 2], pc[:, 2])
     err = np.abs(p[:, 1].astype(np.float64) - pc[:, 1])
     assert err.max() <= 4e-6, err.max()
+
+
+def test_host_layer_rejects_wrong_arguments(rt, O):
+    """The Python host layer checks what the C ABI reads before calling it:
+    an int64 index tensor (torch's default integer type), a float64 or
+    non-contiguous position tensor and undersized outputs raise ValueError
+    instead of building from or writing to misread memory."""
+    pos, idx = O.wave_grid(8, 8)
+    dpos = torch.from_numpy(pos).cuda()
+    didx = torch.from_numpy(idx.view(np.int32)).cuda()
+    with pytest.raises(ValueError):
+        rt.GpuBVH4.build(dpos, didx.long())
+    with pytest.raises(ValueError):
+        rt.GpuBVH4.build(dpos.double(), didx)
+    with pytest.raises(ValueError):
+        rt.GpuBVH4.build(dpos.t().contiguous().t(), didx)
+    with pytest.raises(ValueError):
+        rt.GpuBVH4.build(dpos, didx.reshape(-1))
+    bvh = rt.GpuBVH4.build(dpos, didx)
+    rays = rt.Rays.from_numpy(O.random_rays(100, 1, (-1, -1, -1), (1, 1, 1)).packed())
+    with pytest.raises(ValueError):
+        bvh.intersect(rays, out=(torch.empty(99, dtype=torch.int32, device="cuda"),
+                                 torch.empty(100, dtype=torch.float32, device="cuda")))
+    with pytest.raises(ValueError):
+        bvh.intersect(rays, out=(torch.empty(100, dtype=torch.int32, device="cuda"),
+                                 torch.empty(100, dtype=torch.float64, device="cuda")))
+    with pytest.raises(ValueError):
+        bvh.occluded(rays, out=torch.empty(3, dtype=torch.int32, device="cuda"))
+    with pytest.raises(ValueError):
+        rt.TraceBatch([(bvh, rays, "any", torch.empty(3, dtype=torch.int32, device="cuda"))])
+    with pytest.raises(ValueError):
+        bvh.refit(dpos[:-3].contiguous(), didx[:-1].contiguous())
+    # the right shapes still work
+    bits = bvh.occluded(rays, out=torch.empty(4, dtype=torch.int32, device="cuda"))
+    torch.cuda.synchronize()
+    assert bits.numel() == 4

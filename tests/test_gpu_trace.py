@@ -137,13 +137,18 @@ def test_exp0_stats_against_published(rt, O, target, sched):
     (results/exp0_ray_stats.csv:2-5) reproduced by the HIP kernel on the
     reference terrain (as built by the reference's -march=native build) with
     the reference's procedural rays: mean box tests, triangle tests and hit
-    rate within float noise of the GPU the CSV came from."""
+    rate within float noise of the GPU the CSV came from.  The rays are made
+    on the device (rt_gen_exp0_rays) and equal the oracle's restatement of
+    the shader's generator bit for bit."""
     from test_oracle_known import EXP0
     pos, idx = O.terrain(target, fused=True)
     n4, tris, bvh = _upload_tree(rt, O, pos, idx)
     for coherent in (True, False):
         r = O.exp0_rays(1 << 20, coherent)
-        prim, t, box, tri = bvh.intersect(_rays_to_dev(rt, r), counters=True, **sched)
+        dr = rt.gen_exp0_rays(1 << 20, coherent)
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(u32(dr.data), r.packed().view(np.uint32))
+        prim, t, box, tri = bvh.intersect(dr, counters=True, **sched)
         torch.cuda.synchronize()
         prim, box, tri = u32(prim), u32(box), u32(tri)
         hit = prim != O.MISS
@@ -180,3 +185,16 @@ def test_zero_rays_and_empty_tree(rt, O, sched):
     torch.cuda.synchronize()
     assert (u32(prim) == O.MISS).all() and (to_np(t) == 3.0).all()
     assert not _bits(u32(bits), 100).any()
+
+
+@pytest.mark.parametrize("coherent", [True, False])
+@pytest.mark.parametrize("n", [1, 1000, 1024, 1025, 70000])
+def test_exp0_rays_device_equal_oracle(rt, O, coherent, n):
+    """rt_gen_exp0_rays (software_bvh_bvh4.comp:50-94) bit-identical to the
+    oracle's generator, ragged counts (the coherent grid's height is
+    ceil(n / 1024)), another camera, tmin / tmax filled."""
+    for cam, look in (((0.0, 50.0, -45.0), (0.0, 0.0, 5.0)), ((3.0, 7.0, 2.0), (-1.0, 0.5, 9.0))):
+        r = O.exp0_rays(n, coherent, cam=cam, look=look, tmin=0.02, tmax=77.0)
+        d = rt.gen_exp0_rays(n, coherent, cam=cam, look=look, tmin=0.02, tmax=77.0)
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(u32(d.data), r.packed().view(np.uint32))
