@@ -194,6 +194,12 @@ class Rays:
         return np.stack(self.arrays()).astype(np.float32)
 
 
+def default_threads():
+    """Worker threads of the parallel oracle routines: the CPUs this process
+    may run on, at most 16 (one GPU's share of a shared host)."""
+    return max(1, min(16, len(os.sched_getaffinity(0))))
+
+
 def trace_bvh4(nodes, tris, rays, mode=CLOSEST, stack_limit=0, threads=None, counters=False):
     nodes = np.ascontiguousarray(nodes)
     tris = np.ascontiguousarray(tris)
@@ -202,7 +208,7 @@ def trace_bvh4(nodes, tris, rays, mode=CLOSEST, stack_limit=0, threads=None, cou
     t = np.zeros(n, np.float32)
     box = np.zeros(n, np.uint32) if counters else None
     tri = np.zeros(n, np.uint32) if counters else None
-    threads = threads or os.cpu_count() or 1
+    threads = threads or default_threads()
     lib().or_trace_bvh4(_p(nodes), u32(len(nodes)), _p(tris), *[_p(a) for a in rays.arrays()],
                         u32(n), C.c_int(mode), u32(stack_limit), C.c_int(threads),
                         _p(prim), _p(t), _p(box), _p(tri))
@@ -225,7 +231,7 @@ def trace_bvh2(nodes2, tris48, rays, order=None, mode=CLOSEST, threads=None, cou
     lib().or_trace_bvh2(_p(nodes2), u32(len(nodes2)), _p(tris48), _p(o), _p(rays.ox),
                         _p(rays.oy), _p(rays.oz), _p(rays.tmin), _p(rays.dx), _p(rays.dy),
                         _p(rays.dz), _p(rays.tmax), u32(n), C.c_int(mode),
-                        C.c_int(threads or os.cpu_count() or 1), _p(prim), _p(t), _p(box),
+                        C.c_int(threads or default_threads()), _p(prim), _p(t), _p(box),
                         _p(tri))
     return (prim, t, box, tri) if counters else (prim, t)
 
@@ -235,7 +241,7 @@ def trace_brute(tris, rays, mode=CLOSEST, threads=None):
     n = len(rays)
     prim = np.zeros(n, np.uint32)
     t = np.zeros(n, np.float32)
-    threads = threads or os.cpu_count() or 1
+    threads = threads or default_threads()
     lib().or_trace_brute(_p(tris), u32(len(tris)), *[_p(a) for a in rays.arrays()], u32(n),
                          C.c_int(mode), C.c_int(threads), _p(prim), _p(t))
     return prim, t

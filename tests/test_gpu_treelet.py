@@ -3,7 +3,7 @@ treelet_rounds, Karras & Aila 2013; no reference counterpart) against the
 oracle's restructuring of the same radix tree: node and leaf bytes, node
 count and depth equal, for trees that need the one-workgroup pass over the
 nodes crossing 4096-position windows (n > 4096), equal Morton codes, leaf
-sizes 1 / 2 / 4 and 1-3 rounds.  The restructured tree traces to the same
+sizes 1 / 2 / 4 and 1, 3, 8 and 16 rounds.  The restructured tree traces to the same
 occlusion bits and closest-hit t as the plain radix tree (the hits do not
 depend on the tree).
 """
@@ -43,6 +43,25 @@ def test_treelet_sphere_matches_oracle(rt, O, n, maxp, rounds):
     nv = max(2, n // (2 * nu) + 1)
     pos, idx = O.sphere(nu, nv, 1.0, 0.02, 9)
     _cmp(rt, O, pos, np.ascontiguousarray(idx[:n]), maxp, rounds)
+
+
+@pytest.mark.parametrize("n", [20000, 150000])
+@pytest.mark.parametrize("maxp", [1, 2, 4])
+@pytest.mark.parametrize("rounds", [8, 16])
+def test_treelet_many_rounds_match_oracle(rt, O, n, maxp, rounds):
+    """The round counts the frames' tuner picks (8 for the shadow frame's
+    sphere, 16 for the AO frame's terrain), byte-exact against the oracle."""
+    nu = 50
+    nv = max(2, n // (2 * nu) + 1)
+    pos, idx = O.sphere(nu, nv, 1.0, 0.02, 9)
+    _cmp(rt, O, pos, np.ascontiguousarray(idx[:n]), maxp, rounds)
+
+
+def test_treelet_heightfield_16_rounds(rt, O):
+    """A seeded random-displaced heightfield (the AO frame's terrain kind,
+    at 80,000 triangles), leaves of 1, 16 rounds."""
+    pos, idx = O.heightfield(200, 200, 4.0, 0.3, 31)
+    _cmp(rt, O, pos, idx, 1, 16)
 
 
 def test_treelet_equal_codes_and_soup(rt, O):
