@@ -637,15 +637,17 @@ def ao_frame_leg(frames=200, w=1920, h=1080, tuner=None):
 
 
 def serving_leg(seed=7, tuner=None):
-    """Serving: 32 resident scenes (10k-2.5M tris, log-uniform), requests of
-    4k-2M rays (shadow, AO, closest) or 4k-64k-triangle deform-and-rebuilds.
-    Capacity (median of 3 alternating runs of each path): 300 requests all
-    waiting at t = 0, served (batched) and plain
-    (one launch per request, waiting for each; the library's default launch
-    shape, and again with the served path's shape).  Latency: 800 Poisson
-    arrivals at half the served capacity, p50 / p99 over all requests.  With
-    a tuner, the served trace schedule / shape is measured on a batch of 64
-    sample requests (not those served)."""
+    """Serving: 32 resident scenes (10k-2.5M tris, log-uniform) plus 8
+    deformable grid scenes; requests of 4k-2M rays (shadow, AO, closest) on
+    any scene or 4k-64k-triangle deform-and-rebuilds of a grid scene (later
+    requests on it trace the new version).  Capacity: 300 requests all
+    waiting at t = 0, served (batched, one submit call per tick and kind) and
+    plain (one launch per request, waiting for each; the library's default
+    launch shape, and again with the served path's shape), three alternating
+    runs of each, every run's ratio reported.  Latency: 800 Poisson arrivals
+    at half the served capacity, p50 / p99 over all requests.  With a tuner,
+    the served trace schedule / shape is measured on a batch of 64 sample
+    requests (not those served)."""
     import serve as S
     scenes = S.make_scenes(32)
     budget = 4 << 30
@@ -655,35 +657,39 @@ def serving_leg(seed=7, tuner=None):
                   if r.kind != "rebuild"][:64]
         items = []
         for r in sample:
-            sc = scenes[r.scene]
-            r.rays = S._rays_for(r, sc)
-            items.append((sc.bvh, r.rays, "any" if r.kind != "closest" else "closest", None))
+            rays = rt.gen_random_rays(r.n, r.seed, tuple(scenes.lo[r.scene]),
+                                      tuple(scenes.hi[r.scene]), S.TMAX[r.kind])
+            items.append((scenes.bvh(r.scene), rays,
+                          "any" if r.kind != "closest" else "closest", None))
         shape = tuner.trace_batch(rt.TraceBatch(items), tag="serving")
         torch.cuda.synchronize()
     S.run_plain(scenes, S.make_requests(40, 0, scenes, seed=seed + 200))  # warm every path
     S.Server(scenes, budget, trace_shape=shape).run(S.make_requests(40, 0, scenes,
                                                                     seed=seed + 300))
     reqs = S.make_requests(300, 0, scenes, seed=seed)
-    # three alternating repetitions of each path, the median of each reported (one
-    # pass of either is a fraction of a second, and the host loop's share varies)
+    # three alternating repetitions of each path (one pass of either is a
+    # fraction of a second, and the host loop's share varies)
     runs = {"sat": [], "plain": [], "shaped": []}
     for _ in range(3):
         runs["sat"].append(S.Server(scenes, budget, trace_shape=shape).run(
             S.clone_requests(reqs)))
         runs["plain"].append(S.run_plain(scenes, S.clone_requests(reqs)))
         runs["shaped"].append(S.run_plain(scenes, S.clone_requests(reqs), trace_shape=shape))
+    ratios = [round(a["mrays_per_s"] / b["mrays_per_s"], 2)
+              for a, b in zip(runs["sat"], runs["plain"])]
     sat, plain, plain_shaped = (sorted(runs[k], key=lambda r: r["mrays_per_s"])[1]
                                 for k in ("sat", "plain", "shaped"))
     qps = round(0.5 * len(reqs) / sat["span_s"])
     lat = S.Server(scenes, budget, trace_shape=shape).run(
         S.make_requests(800, qps, scenes, seed=seed + 1))
-    return {"workload": f"32 resident scenes ({sum(x.tris for x in scenes)} tris), 4k-2M-ray "
-                        "shadow/AO/closest requests and 4k-64k-tri rebuilds",
+    return {"workload": f"32 resident scenes ({sum(scenes.tris)} tris) + 8 deformable grids, "
+                        "4k-2M-ray shadow/AO/closest requests, 4k-64k-tri deform-and-rebuilds",
             "trace_shape": shape or "default",
             "capacity_mrays_per_s": sat["mrays_per_s"],
             "plain_mrays_per_s": plain["mrays_per_s"],
             "plain_same_shape_mrays_per_s": plain_shaped["mrays_per_s"],
             "speedup": round(sat["mrays_per_s"] / plain["mrays_per_s"], 2),
+            "speedup_runs": ratios,
             "speedup_vs_plain_same_shape": round(sat["mrays_per_s"] /
                                                  plain_shaped["mrays_per_s"], 2),
             "qps": qps, "p50_ms": lat["p50_ms"], "p99_ms": lat["p99_ms"],

@@ -270,6 +270,23 @@ rt_status rt_gpu_build_batch_plan(const rt_build_item* items, uint32_t count, vo
 rt_status rt_gpu_build_batch(rt_stream stream, const void* d_table, uint32_t count,
                              uint32_t workgroups);
 
+/* Upload-and-launch form of the batched build (plan, staging copy, launch in
+ * one call; no allocation, no host wait), for a serving loop's deform-and-
+ * rebuild requests.  gen (optional, one per item): the deform step, item i's
+ * pos receives generate_dynamic_grid_mesh(items[i].n, t, extent) (9 floats
+ * per triangle, include/radv_bvh.hpp:224-283, as rt_gen_dynamic_grid) in one
+ * launch before the build; items[i].idx must then be NULL.  Staging and
+ * device table as rt_trace_batch_submit (rt_gpu_build_batch_submit_bytes).  */
+typedef struct rt_grid_gen {
+    float t;
+    float extent;
+} rt_grid_gen;
+rt_status rt_gpu_build_batch_submit_bytes(uint32_t count, int with_gen, size_t* bytes);
+rt_status rt_gpu_build_batch_submit(rt_stream stream, const rt_build_item* items,
+                                    const rt_grid_gen* gen, uint32_t count, uint32_t workgroups,
+                                    void* pinned, size_t pinned_bytes, void* d_table,
+                                    size_t table_bytes);
+
 /* GpuTriangle records (48 B, software_bvh.hpp:18-22, leaf order) plus the
  * leaf-order original indices -> rt_tri64 leaves (software_bvh.hpp:66-84
  * drops the index; d_order restores it, NULL = identity).                  */
@@ -416,6 +433,30 @@ rt_status rt_trace_batch_ex(rt_stream stream, const rt_trace_item* d_items,
  * most 4096): item i owns chunks [chunk_start[i], chunk_start[i+1]).      */
 rt_status rt_trace_batch_plan_ex(const rt_trace_item* items, uint32_t count, uint32_t chunk_rays,
                                  uint32_t* chunk_start, uint32_t* depth);
+
+/* Upload-and-launch form of the batched trace, for a serving loop: ONE call
+ * per batch of requests, no allocation, no host wait.  No reference
+ * counterpart (the reference submits one dispatch per ray set and waits,
+ * microbench_main.cpp:344, vk_timer.cpp:88-89).  From the host items (and,
+ * if gen != NULL, one synthetic payload per item) it plans the chunk table
+ * (chunks of opts->rays_per_wave rays for RT_SCHED_REFILL, else 64), writes
+ * items, payloads and chunk table into the caller's pinned host staging
+ * (hipHostMalloc'd, rt_trace_batch_submit_bytes bytes), copies them to the
+ * caller's device table (same size, 256-B aligned) on the stream, generates
+ * the payloads (gen: item i's rays are rt_gen_random_rays(count, seed, lo,
+ * hi, tmax), written through items[i].rays) and launches rt_trace_batch_ex.
+ * The staging may be reused once the stream has passed this call (the copy
+ * reads it asynchronously).                                                  */
+typedef struct rt_ray_gen {
+    uint32_t seed;
+    float lo[3], hi[3]; /* origins uniform in [lo, hi), directions on the sphere */
+    float tmax;         /* tmin 0 */
+} rt_ray_gen;
+rt_status rt_trace_batch_submit_bytes(uint32_t count, int with_gen, size_t* bytes);
+rt_status rt_trace_batch_submit(rt_stream stream, const rt_trace_item* items,
+                                const rt_ray_gen* gen, uint32_t count, const rt_trace_opts* opts,
+                                void* pinned, size_t pinned_bytes, void* d_table,
+                                size_t table_bytes);
 
 /* ---- two-level scene (the reference traces through a TLAS of instances:
  * src/main.cpp:920-1000, fullframe_benchmark.cpp:207-219) ------------------

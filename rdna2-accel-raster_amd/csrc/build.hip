@@ -2436,6 +2436,69 @@ extern "C" rt_status rt_gpu_build_batch(rt_stream stream, const void* d_table, u
     return launch_status("k_build_batch");
 }
 
+namespace {
+// Byte layout of a batched-build submit table (staging and device copy
+// alike): the planned table, then the optional grid generators.
+struct BuildSubmitLayout {
+    size_t gen = 0, total = 0;
+};
+BuildSubmitLayout build_submit_layout(uint32_t count, bool with_gen) {
+    auto al = [](size_t x) { return (x + 255) & ~size_t(255); };
+    BuildSubmitLayout L;
+    size_t off = al((size_t)count * sizeof(BatchBuild));
+    if (with_gen) {
+        L.gen = off;
+        off += al((size_t)count * sizeof(GridGen));
+    }
+    L.total = off;
+    return L;
+}
+}  // namespace
+
+extern "C" rt_status rt_gpu_build_batch_submit_bytes(uint32_t count, int with_gen,
+                                                     size_t* bytes) {
+    if (!bytes) return RT_ERR_ARG;
+    *bytes = build_submit_layout(count, with_gen != 0).total;
+    return RT_OK;
+}
+
+extern "C" rt_status rt_gpu_build_batch_submit(rt_stream stream, const rt_build_item* items,
+                                               const rt_grid_gen* gen, uint32_t count,
+                                               uint32_t workgroups, void* pinned,
+                                               size_t pinned_bytes, void* d_table,
+                                               size_t table_bytes) {
+    if (count == 0) return RT_OK;
+    if (!items || !pinned || !d_table) return RT_ERR_ARG;
+    if (reinterpret_cast<uintptr_t>(d_table) % 256) return RT_ERR_ARG;
+    const BuildSubmitLayout L = build_submit_layout(count, gen != nullptr);
+    if (pinned_bytes < L.total || table_bytes < L.total) return RT_ERR_CAPACITY;
+    char* h = static_cast<char*>(pinned);
+    rt_status st = rt_gpu_build_batch_plan(items, count, h, (size_t)count * sizeof(BatchBuild));
+    if (st != RT_OK) return st;
+    uint32_t max_quads = 0;
+    if (gen) {
+        GridGen* hg = reinterpret_cast<GridGen*>(h + L.gen);
+        for (uint32_t k = 0; k < count; ++k) {
+            if (items[k].idx || items[k].n < 2) return RT_ERR_ARG;
+            GridGen g;
+            g.xyz = const_cast<float*>(items[k].pos);
+            dyn_grid_shape(items[k].n, &g.cols, &g.rows);
+            g.t = gen[k].t;
+            g.extent = gen[k].extent;
+            hg[k] = g;
+            max_quads = std::max(max_quads, g.cols * g.rows);
+        }
+    }
+    hipStream_t s = as_stream(stream);
+    RT_TRY_HIP(hipMemcpyAsync(d_table, pinned, L.total, hipMemcpyHostToDevice, s));
+    char* d = static_cast<char*>(d_table);
+    if (gen) {
+        st = launch_grid_gen(s, reinterpret_cast<const GridGen*>(d + L.gen), count, max_quads);
+        if (st != RT_OK) return st;
+    }
+    return rt_gpu_build_batch(stream, d, count, workgroups);
+}
+
 extern "C" rt_status rt_gpu_build_radv(rt_stream stream, const float* d_pos,
                                        const uint32_t* d_idx, uint32_t n, uint32_t order,
                                        void* d_image, uint64_t image_bytes,

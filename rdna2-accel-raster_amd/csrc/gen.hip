@@ -136,19 +136,15 @@ __global__ void k_dynamic_grid(uint32_t cols, uint32_t rows, float t, float exte
                                float* __restrict__ xyz) {
     const uint32_t q = blockIdx.x * kB + threadIdx.x;
     if (q >= cols * rows) return;
-    const uint32_t r = q / cols, c = q % cols;
-    const float dx = (2.0f * extent) / (float)cols, dy = (2.0f * extent) / (float)rows;
-    const float x0 = -extent + (float)c * dx, x1 = x0 + dx;
-    const float y0 = -extent + (float)r * dy, y1 = y0 + dy;
-    auto z = [t](float x, float y) {
-        return 5.0f + 0.2f * sinf(x * 2.0f + t) * cosf(y * 2.0f + t);
-    };
-    const float z00 = z(x0, y0), z10 = z(x1, y0), z01 = z(x0, y1), z11 = z(x1, y1);
-    float* o = xyz + 18 * (size_t)q;
-    const float tv[18] = {x0, y0, z00, x1, y0, z10, x0, y1, z01,
-                          x1, y0, z10, x1, y1, z11, x0, y1, z01};
-#pragma unroll
-    for (int k = 0; k < 18; ++k) o[k] = tv[k];
+    dyn_grid_quad(cols, rows, t, extent, q, xyz + 18 * (size_t)q);
+}
+
+// Many deforming grids in one launch: grid g's quads are blockIdx.y = g.
+__global__ void k_dynamic_grid_batch(const GridGen* __restrict__ gen) {
+    const GridGen g = gen[blockIdx.y];
+    const uint32_t q = blockIdx.x * kB + threadIdx.x;
+    if (q >= g.cols * g.rows) return;
+    dyn_grid_quad(g.cols, g.rows, g.t, g.extent, q, g.xyz + 18 * (size_t)q);
 }
 
 struct CamBasis {
@@ -379,14 +375,40 @@ __global__ void k_random_rays(uint32_t count, uint32_t seed, float lx, float ly,
                               float hx, float hy, float hz, float tmax, rt_rays_out o) {
     const uint32_t i = blockIdx.x * kB + threadIdx.x;
     if (i >= count) return;
-    const float u0 = hash01(seed, i * 5 + 0), u1 = hash01(seed, i * 5 + 1);
-    const float u2 = hash01(seed, i * 5 + 2), u3 = hash01(seed, i * 5 + 3);
-    const float u4 = hash01(seed, i * 5 + 4);
-    const float z = 1.0f - 2.0f * u3;
-    const float r = sqrtf(fmaxf(0.0f, 1.0f - z * z));
-    const float phi = 6.2831853f * u4;
-    put_ray(o, i, lx + u0 * (hx - lx), ly + u1 * (hy - ly), lz + u2 * (hz - lz), 0.0f,
-            r * cosf(phi), r * sinf(phi), z, tmax);
+    const float lo[3] = {lx, ly, lz}, hi[3] = {hx, hy, hz};
+    float org[3], dir[3];
+    random_ray(seed, i, lo, hi, org, dir);
+    put_ray(o, i, org[0], org[1], org[2], 0.0f, dir[0], dir[1], dir[2], tmax);
+}
+
+// The synthetic payloads of many ray requests in one launch: over the chunk
+// table of a batched trace (item i owns chunks [chunk[i], chunk[i + 1]) of
+// chunk_rays rays), ray r of item i is ray r of rt_gen_random_rays with that
+// item's generator (seed, box, tmax), written to the item's rays.
+__global__ void k_random_rays_batch(const rt_trace_item* __restrict__ items,
+                                    const rt_ray_gen* __restrict__ gen,
+                                    const uint32_t* __restrict__ chunk, uint32_t count,
+                                    uint32_t total_chunks, uint32_t chunk_rays) {
+    const uint64_t g = (uint64_t)blockIdx.x * kB + threadIdx.x;
+    const uint32_t c = (uint32_t)(g / chunk_rays);
+    if (c >= total_chunks) return;
+    uint32_t lo = 0, hi = count;  // last item with chunk[item] <= c
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (chunk[mid] <= c) lo = mid;
+        else hi = mid;
+    }
+    const rt_trace_item& it = items[lo];
+    const uint32_t r = (c - chunk[lo]) * chunk_rays + (uint32_t)(g % chunk_rays);
+    if (r >= it.count) return;
+    const rt_ray_gen& q = gen[lo];
+    float org[3], dir[3];
+    random_ray(q.seed, r, q.lo, q.hi, org, dir);
+    const rt_rays_out o = {const_cast<float*>(it.rays.ox), const_cast<float*>(it.rays.oy),
+                           const_cast<float*>(it.rays.oz), const_cast<float*>(it.rays.tmin),
+                           const_cast<float*>(it.rays.dx), const_cast<float*>(it.rays.dy),
+                           const_cast<float*>(it.rays.dz), const_cast<float*>(it.rays.tmax)};
+    put_ray(o, r, org[0], org[1], org[2], 0.0f, dir[0], dir[1], dir[2], q.tmax);
 }
 
 // software_bvh_bvh4.comp:50-94 procedural rays of the exp0 probe.
@@ -499,15 +521,32 @@ extern "C" rt_status rt_gen_heightfield(rt_stream stream, uint32_t nx, uint32_t 
 extern "C" rt_status rt_gen_dynamic_grid(rt_stream stream, uint32_t n, float t, float extent,
                                          float* d_xyz9) {
     if (!d_xyz9 || n < 2) return RT_ERR_ARG;
-    // include/radv_bvh.hpp:228-233 grid shape
-    const uint32_t quads = n / 2;
-    uint32_t cols = (uint32_t)std::sqrt((double)quads);
-    while (cols > 1 && quads % cols != 0) cols--;
-    const uint32_t rows = quads / cols;
-    hipLaunchKernelGGL(k_dynamic_grid, dim3(ceil_div(quads, kB)), dim3(kB), 0, as_stream(stream),
-                       cols, rows, t, extent, d_xyz9);
+    uint32_t cols, rows;
+    dyn_grid_shape(n, &cols, &rows);
+    hipLaunchKernelGGL(k_dynamic_grid, dim3(ceil_div(cols * rows, kB)), dim3(kB), 0,
+                       as_stream(stream), cols, rows, t, extent, d_xyz9);
     return launch_status("rt_gen_dynamic_grid");
 }
+
+namespace rt {
+rt_status launch_grid_gen(hipStream_t s, const GridGen* d_gen, uint32_t count,
+                          uint32_t max_quads) {
+    if (count == 0 || max_quads == 0) return RT_OK;
+    hipLaunchKernelGGL(k_dynamic_grid_batch, dim3(ceil_div(max_quads, kB), count), dim3(kB), 0, s,
+                       d_gen);
+    return launch_status("k_dynamic_grid_batch");
+}
+
+rt_status launch_ray_gen(hipStream_t s, const rt_trace_item* d_items, const rt_ray_gen* d_gen,
+                         const uint32_t* d_chunk_start, uint32_t count, uint32_t total_chunks,
+                         uint32_t chunk_rays) {
+    if (count == 0 || total_chunks == 0) return RT_OK;
+    hipLaunchKernelGGL(k_random_rays_batch,
+                       dim3(ceil_div((uint64_t)total_chunks * chunk_rays, kB)), dim3(kB), 0, s,
+                       d_items, d_gen, d_chunk_start, count, total_chunks, chunk_rays);
+    return launch_status("k_random_rays_batch");
+}
+}  // namespace rt
 
 // common/mesh_gen.hpp:31-34: res = floor(sqrt(target / 2)) + 1, at least 2.
 static uint32_t terrain_res(uint32_t target) {

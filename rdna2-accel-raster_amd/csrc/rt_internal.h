@@ -5,6 +5,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <cmath>
 
 #include <cfloat>
 #include <cstdint>
@@ -66,6 +67,66 @@ __host__ __device__ inline float hash01(uint32_t seed, uint32_t i) {
     uint32_t h = pcg_hash(i * 0x9E3779B9u + seed);
     return (float)(h >> 8) * (1.0f / 16777216.0f);
 }
+
+// Ray i of rt_gen_random_rays(seed, lo, hi, tmax): origin uniform in the box
+// [lo, hi), direction uniform on the unit sphere, tmin 0.  Shared by the
+// one-request generator and the batched one, so both give the same rays.
+__device__ __forceinline__ void random_ray(uint32_t seed, uint32_t i, const float lo[3],
+                                           const float hi[3], float o[3], float d[3]) {
+    const float u0 = hash01(seed, i * 5 + 0), u1 = hash01(seed, i * 5 + 1);
+    const float u2 = hash01(seed, i * 5 + 2), u3 = hash01(seed, i * 5 + 3);
+    const float u4 = hash01(seed, i * 5 + 4);
+    const float z = 1.0f - 2.0f * u3;
+    const float r = sqrtf(fmaxf(0.0f, 1.0f - z * z));
+    const float phi = 6.2831853f * u4;
+    o[0] = lo[0] + u0 * (hi[0] - lo[0]);
+    o[1] = lo[1] + u1 * (hi[1] - lo[1]);
+    o[2] = lo[2] + u2 * (hi[2] - lo[2]);
+    d[0] = r * cosf(phi);
+    d[1] = r * sinf(phi);
+    d[2] = z;
+}
+
+// Quad q (two triangles, 18 floats at o) of include/radv_bvh.hpp:224-283
+// generate_dynamic_grid_mesh over a cols x rows grid of [-extent, extent]^2,
+// z = 5 + 0.2 sin(2 x + t) cos(2 y + t).
+__device__ __forceinline__ void dyn_grid_quad(uint32_t cols, uint32_t rows, float t,
+                                              float extent, uint32_t q, float* o) {
+    const uint32_t r = q / cols, c = q % cols;
+    const float dx = (2.0f * extent) / (float)cols, dy = (2.0f * extent) / (float)rows;
+    const float x0 = -extent + (float)c * dx, x1 = x0 + dx;
+    const float y0 = -extent + (float)r * dy, y1 = y0 + dy;
+    auto z = [t](float x, float y) {
+        return 5.0f + 0.2f * sinf(x * 2.0f + t) * cosf(y * 2.0f + t);
+    };
+    const float z00 = z(x0, y0), z10 = z(x1, y0), z01 = z(x0, y1), z11 = z(x1, y1);
+    const float tv[18] = {x0, y0, z00, x1, y0, z10, x0, y1, z01,
+                          x1, y0, z10, x1, y1, z11, x0, y1, z01};
+#pragma unroll
+    for (int k = 0; k < 18; ++k) o[k] = tv[k];
+}
+
+// include/radv_bvh.hpp:228-233 grid shape of n triangles: quads = n / 2,
+// cols the largest divisor of quads not above sqrt(quads).
+inline void dyn_grid_shape(uint32_t n, uint32_t* cols, uint32_t* rows) {
+    const uint32_t quads = n / 2;
+    uint32_t c = (uint32_t)std::sqrt((double)quads);
+    while (c > 1 && quads % c != 0) c--;
+    *cols = c;
+    *rows = c ? quads / c : 0;
+}
+
+// Batched payload generators (gen.hip) for the submit entry points.
+struct GridGen {  // one deforming grid: 9 floats per triangle into xyz
+    float* xyz;
+    uint32_t cols, rows;
+    float t, extent;
+};
+rt_status launch_grid_gen(hipStream_t s, const GridGen* d_gen, uint32_t count,
+                          uint32_t max_quads);
+rt_status launch_ray_gen(hipStream_t s, const rt_trace_item* d_items, const rt_ray_gen* d_gen,
+                         const uint32_t* d_chunk_start, uint32_t count, uint32_t total_chunks,
+                         uint32_t chunk_rays);
 
 // Moller-Trumbore with the predicates of software_bvh_bvh4.comp:166-182:
 // |det| < 1e-7 rejects, u,v inclusive in [0,1], u+v <= 1.

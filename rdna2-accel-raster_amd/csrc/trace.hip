@@ -1328,6 +1328,67 @@ extern "C" rt_status rt_trace_batch_ex(rt_stream stream, const rt_trace_item* d_
     return launch_status("k_trace_batch");
 }
 
+namespace {
+// Byte layout of a submit table (staging and device copy alike): items, the
+// optional payload generators, the chunk table; each part 256-B aligned.
+struct SubmitLayout {
+    size_t items = 0, gen = 0, chunk = 0, total = 0;
+};
+SubmitLayout submit_layout(uint32_t count, bool with_gen) {
+    auto al = [](size_t x) { return (x + 255) & ~size_t(255); };
+    SubmitLayout L;
+    size_t off = al((size_t)count * sizeof(rt_trace_item));
+    if (with_gen) {
+        L.gen = off;
+        off += al((size_t)count * sizeof(rt_ray_gen));
+    }
+    L.chunk = off;
+    off += al(((size_t)count + 1) * 4);
+    L.total = off;
+    return L;
+}
+}  // namespace
+
+extern "C" rt_status rt_trace_batch_submit_bytes(uint32_t count, int with_gen, size_t* bytes) {
+    if (!bytes) return RT_ERR_ARG;
+    *bytes = submit_layout(count, with_gen != 0).total;
+    return RT_OK;
+}
+
+extern "C" rt_status rt_trace_batch_submit(rt_stream stream, const rt_trace_item* items,
+                                           const rt_ray_gen* gen, uint32_t count,
+                                           const rt_trace_opts* opts, void* pinned,
+                                           size_t pinned_bytes, void* d_table,
+                                           size_t table_bytes) {
+    if (count == 0) return RT_OK;
+    if (!items || !pinned || !d_table) return RT_ERR_ARG;
+    if (reinterpret_cast<uintptr_t>(d_table) % 256) return RT_ERR_ARG;
+    const SubmitLayout L = submit_layout(count, gen != nullptr);
+    if (pinned_bytes < L.total || table_bytes < L.total) return RT_ERR_CAPACITY;
+    uint32_t chunk_rays = 64u;
+    if (opts && opts->schedule == RT_SCHED_REFILL)
+        chunk_rays = opts->rays_per_wave ? opts->rays_per_wave : 1024u;
+    char* h = static_cast<char*>(pinned);
+    uint32_t* h_chunk = reinterpret_cast<uint32_t*>(h + L.chunk);
+    uint32_t depth = 0;
+    rt_status st = rt_trace_batch_plan_ex(items, count, chunk_rays, h_chunk, &depth);
+    if (st != RT_OK) return st;
+    std::memcpy(h, items, (size_t)count * sizeof(rt_trace_item));
+    if (gen) std::memcpy(h + L.gen, gen, (size_t)count * sizeof(rt_ray_gen));
+    const uint32_t total = h_chunk[count];
+    hipStream_t s = as_stream(stream);
+    RT_TRY_HIP(hipMemcpyAsync(d_table, pinned, L.total, hipMemcpyHostToDevice, s));
+    char* d = static_cast<char*>(d_table);
+    const auto d_items = reinterpret_cast<const rt_trace_item*>(d);
+    const auto d_chunk = reinterpret_cast<const uint32_t*>(d + L.chunk);
+    if (gen) {
+        st = launch_ray_gen(s, d_items, reinterpret_cast<const rt_ray_gen*>(d + L.gen), d_chunk,
+                            count, total, chunk_rays);
+        if (st != RT_OK) return st;
+    }
+    return rt_trace_batch_ex(stream, d_items, d_chunk, count, total, depth, opts);
+}
+
 extern "C" rt_status rt_trace_batch(rt_stream stream, const rt_trace_item* d_items,
                                     const uint32_t* d_chunk_start, uint32_t count,
                                     uint32_t total_chunks, uint32_t depth) {

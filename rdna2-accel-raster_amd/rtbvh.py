@@ -92,6 +92,14 @@ class rt_instance(C.Structure):
 INST_GENERAL, INST_TRANSLATE, INST_IDENTITY = 0, 1, 2
 
 
+class rt_ray_gen(C.Structure):
+    _fields_ = [("seed", u32), ("lo", f32 * 3), ("hi", f32 * 3), ("tmax", f32)]
+
+
+class rt_grid_gen(C.Structure):
+    _fields_ = [("t", f32), ("extent", f32)]
+
+
 class rt_build_item(C.Structure):
     _fields_ = [("pos", vp), ("idx", vp), ("nodes", vp), ("tris", vp), ("info", vp),
                 ("sorted_ids", vp), ("codes", vp), ("n", u32), ("node_capacity", u32),
@@ -193,6 +201,10 @@ _SIGS = {
     "rt_image_metrics_scratch_bytes": [u32, u32, vp],
     "rt_image_metrics": [vp, vp, vp, u32, u32, vp, vp, vp, C.c_size_t],
     "rt_device_sync": [],
+    "rt_trace_batch_submit_bytes": [u32, C.c_int, vp],
+    "rt_trace_batch_submit": [vp, vp, vp, u32, vp, vp, C.c_size_t, vp, C.c_size_t],
+    "rt_gpu_build_batch_submit_bytes": [u32, C.c_int, vp],
+    "rt_gpu_build_batch_submit": [vp, vp, vp, u32, u32, vp, C.c_size_t, vp, C.c_size_t],
 }
 
 
@@ -1115,3 +1127,43 @@ def _pcg(v):
     state = (v * 747796405 + 2891336453) & 0xFFFFFFFF
     word = (((state >> ((state >> 28) + 4)) ^ state) * 277803737) & 0xFFFFFFFF
     return ((word >> 22) ^ word) & 0xFFFFFFFF
+
+
+# ------------------------------------------------------ submit staging --
+def submit_bytes(kind, count, with_gen=True):
+    """Staging / device-table bytes of rt_trace_batch_submit ("trace") or
+    rt_gpu_build_batch_submit ("build") for count items."""
+    b = C.c_size_t(0)
+    fn = "rt_trace_batch_submit_bytes" if kind == "trace" else "rt_gpu_build_batch_submit_bytes"
+    call(fn, u32(count), C.c_int(1 if with_gen else 0), C.byref(b))
+    return b.value
+
+
+class SubmitRing:
+    """A ring of (pinned host staging, device table) slot pairs for the
+    submit entry points, allocated once.  A slot is handed out again only
+    after the stream work that read it has passed (its event)."""
+
+    def __init__(self, slots, slot_bytes, device=None):
+        dev = device or _dev()
+        self.slot_bytes = (slot_bytes + 255) // 256 * 256
+        self.pinned = torch.empty(slots * self.slot_bytes, dtype=torch.uint8, pin_memory=True)
+        self.table = torch.empty(slots * self.slot_bytes, dtype=torch.uint8, device=dev)
+        self.events = [None] * slots
+        self.next = 0
+
+    def take(self):
+        """(pinned pointer, device pointer, slot index) of the next slot."""
+        k = self.next
+        self.next = (k + 1) % len(self.events)
+        ev = self.events[k]
+        if ev is not None and not ev.query():
+            ev.synchronize()
+        off = k * self.slot_bytes
+        return (vp(self.pinned.data_ptr() + off), vp(self.table.data_ptr() + off), k)
+
+    def used(self, k, stream):
+        """Record that slot k is read by work on `stream` up to now."""
+        ev = torch.cuda.Event()
+        ev.record(stream)
+        self.events[k] = ev
