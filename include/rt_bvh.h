@@ -173,6 +173,21 @@ rt_status rt_gpu_build(rt_stream stream, const float* d_pos, const uint32_t* d_i
                        const rt_build_cfg* cfg, rt_node4* d_nodes, uint32_t node_capacity,
                        rt_tri64* d_tris, rt_build_info* d_info, uint32_t* d_sorted_ids,
                        uint32_t* d_codes, void* d_scratch, size_t scratch_bytes);
+/* rt_gpu_build that allocates nothing: no reference counterpart; for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  Same arguments, checks
+ * and results as rt_gpu_build, but the scratch comes from a fixed device
+ * array in the library (9 MiB, one per GPU, kept until the process ends), so
+ * n is at most RT_NOALLOC_BUILD_MAX (RT_ERR_CAPACITY beyond).  Every call on
+ * one GPU shares that array: calls that overlap on one GPU (other streams,
+ * other host threads) race and must be serialised by the caller.  The first
+ * call on a device looks up the array's address there.                      */
+#define RT_NOALLOC_BUILD_MAX 65536u
+rt_status rt_gpu_build_noalloc(rt_stream stream, const float* d_pos, const uint32_t* d_idx,
+                               uint32_t n, const rt_build_cfg* cfg, rt_node4* d_nodes,
+                               uint32_t node_capacity, rt_tri64* d_tris, rt_build_info* d_info,
+                               uint32_t* d_sorted_ids, uint32_t* d_codes);
+
 /* Refit of an RT_BUILD_IMPLICIT tree for a deforming mesh: same topology
  * (leaf order taken from d_tris[i].triangle_id), new positions.  Scratch as
  * rt_build_requirements reports for RT_BUILD_IMPLICIT.                      */

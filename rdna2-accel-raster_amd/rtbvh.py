@@ -201,6 +201,7 @@ _SIGS = {
     "rt_image_metrics_scratch_bytes": [u32, u32, vp],
     "rt_image_metrics": [vp, vp, vp, u32, u32, vp, vp, vp, C.c_size_t],
     "rt_device_sync": [],
+    "rt_gpu_build_noalloc": [vp, vp, vp, u32, vp, vp, u32, vp, vp, vp, vp],
     "rt_trace_batch_submit_bytes": [u32, C.c_int, vp],
     "rt_trace_batch_submit": [vp, vp, vp, u32, vp, vp, C.c_size_t, vp, C.c_size_t],
     "rt_gpu_build_batch_submit_bytes": [u32, C.c_int, vp],
