@@ -79,13 +79,38 @@ typedef struct rt_tri64 {
  * record the true depth in rt_build_info.depth (a radix tree is at most 62
  * levels; treelet restructuring may deepen it).                            */
 typedef struct rt_bvh_dev {
-    const rt_node4* nodes;
+    const rt_node4* nodes;  /* rt_node4[node_count], or rt_qnode4[] (format) */
     const rt_tri64* tris;
     uint32_t node_count;
     uint32_t tri_count;
     uint32_t depth;
-    uint32_t reserved;
+    uint32_t format;        /* RT_NODES_F32 (0) or RT_NODES_Q8 */
 } rt_bvh_dev;
+
+/* Node formats of rt_bvh_dev.format.  RT_NODES_Q8: `nodes` points at
+ * rt_qnode4 records (rt_gpu_quantize of the tree), traced by rt_trace /
+ * rt_trace_ex (the one-ray-per-thread schedule); other trace entry points
+ * return RT_ERR_UNSUPPORTED for it.                                         */
+#define RT_NODES_F32 0u
+#define RT_NODES_Q8 1u
+
+/* 64-B quantized Box4 node (no reference counterpart; the layout of
+ * compressed wide BVHs, Ylitie, Karras & Laine 2017, for four children):
+ * child k's box on axis a is [p_a + qlo[a][k] * 2^e_a, p_a + qhi[a][k] *
+ * 2^e_a] (one float rounding: the product is exact), rounded outward from
+ * the rt_node4 slot box it encodes, so it contains that box; child / count
+ * as rt_node4 (count as int16).  A trace over an RT_NODES_Q8 tree is the
+ * trace of the rt_node4 tree with these (decoded) boxes: the same
+ * triangles, and a superset of node visits of the exact tree.             */
+typedef struct rt_qnode4 {
+    float p[3];         /* per-axis origin: the min of the occupied slots */
+    int8_t e[3];        /* per-axis scale exponents, -126..127            */
+    uint8_t pad0;
+    uint8_t qlo[3][4];  /* [axis][child]                                  */
+    uint8_t qhi[3][4];
+    int16_t count[4];
+    int32_t child[4];
+} rt_qnode4;
 
 /* Structure-of-arrays rays: 8 float arrays of n entries (32 B per ray). */
 typedef struct rt_rays_soa {
@@ -301,6 +326,16 @@ rt_status rt_gpu_build_batch_submit(rt_stream stream, const rt_build_item* items
                                     const rt_grid_gen* gen, uint32_t count, uint32_t workgroups,
                                     void* pinned, size_t pinned_bytes, void* d_table,
                                     size_t table_bytes);
+
+/* The RT_NODES_Q8 copy of a Box4 tree: qnodes[i] encodes nodes[i] (node
+ * indices unchanged, same leaves), one thread per node.  Per axis, e is the
+ * least exponent with p + 255 * 2^e >= the node's max bound (float
+ * arithmetic), qlo the greatest code whose decoded bound is <= the slot's
+ * min, qhi the least whose decoded bound is >= its max.  Leaf counts must
+ * fit int16.  node_count = the tree's node count.  No reference counterpart.
+ */
+rt_status rt_gpu_quantize(rt_stream stream, const rt_node4* d_nodes, uint32_t node_count,
+                          rt_qnode4* d_qnodes);
 
 /* GpuTriangle records (48 B, software_bvh.hpp:18-22, leaf order) plus the
  * leaf-order original indices -> rt_tri64 leaves (software_bvh.hpp:66-84

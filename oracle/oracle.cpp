@@ -25,6 +25,17 @@ inline float hash01(uint32_t seed, uint32_t i) {
     return (float)(h >> 8) * (1.0f / 16777216.0f);
 }
 
+// least e in [lo, hi] with pred(e) true, pred monotone false..true; hi if none
+template <class P>
+static int least_true(int lo, int hi, P pred) {
+    while (lo < hi) {
+        const int mid = lo + (hi - lo) / 2;
+        if (pred(mid)) hi = mid;
+        else lo = mid + 1;
+    }
+    return lo;
+}
+
 template <class F>
 void parallel_chunks(uint32_t n, int nthreads, F fn) {
     const uint32_t chunk = 4096;  // rays handed to a thread at a time
@@ -596,6 +607,77 @@ uint32_t or_bvh4_depth(const or_bvh4_node* nodes, uint32_t count) {
             if (nodes[n].count[k] < 0) st.push_back({nodes[n].child[k], d + 1});
     }
     return best;
+}
+
+// ---------------------------------------------------------------------------
+// Quantized nodes (rt_qnode4): 8-bit child boxes on a per-node grid, rounded
+// outward (no reference counterpart; the compressed wide BVH of Ylitie,
+// Karras & Laine 2017 with four children)
+// ---------------------------------------------------------------------------
+
+static inline float q_dec(float p, int q, float s) { return p + (float)q * s; }
+
+void or_bvh4_quantize(const or_bvh4_node* nodes, uint32_t count, or_qnode4* out) {
+    for (uint32_t i = 0; i < count; ++i) {
+        const or_bvh4_node& n = nodes[i];
+        const float* lo[3] = {n.minX, n.minY, n.minZ};
+        const float* hi[3] = {n.maxX, n.maxY, n.maxZ};
+        or_qnode4 q;
+        std::memset(&q, 0, sizeof q);
+        for (int a = 0; a < 3; ++a) {
+            float p = INFINITY, m = -INFINITY;
+            for (int k = 0; k < 4; ++k)
+                if (n.count[k] != 0) {
+                    p = std::min(p, lo[a][k]);
+                    m = std::max(m, hi[a][k]);
+                }
+            if (!(p <= m)) p = m = 0.0f;
+            const int e = least_true(-126, 127, [&](int x) {
+                return q_dec(p, 255, std::ldexp(1.0f, x)) >= m;
+            });
+            const float s = std::ldexp(1.0f, e);
+            q.p[a] = p;
+            q.e[a] = (int8_t)e;
+            for (int k = 0; k < 4; ++k) {
+                if (n.count[k] == 0) continue;
+                // greatest code at or below the min: 255 - least code above it
+                const float l = lo[a][k], h = hi[a][k];
+                const int above = least_true(0, 256, [&](int c) {
+                    return c > 255 || q_dec(p, c, s) > l;
+                });
+                q.qlo[a][k] = (uint8_t)(above > 0 ? above - 1 : 0);
+                q.qhi[a][k] = (uint8_t)std::min(255, least_true(0, 255, [&](int c) {
+                    return q_dec(p, c, s) >= h;
+                }));
+            }
+        }
+        for (int k = 0; k < 4; ++k) {
+            q.count[k] = (int16_t)n.count[k];
+            q.child[k] = n.count[k] != 0 ? n.child[k] : 0;
+        }
+        out[i] = q;
+    }
+}
+
+void or_bvh4_dequantize(const or_qnode4* q, uint32_t count, or_bvh4_node* out) {
+    for (uint32_t i = 0; i < count; ++i) {
+        or_bvh4_node n;
+        std::memset(&n, 0, sizeof n);
+        float* lo[3] = {n.minX, n.minY, n.minZ};
+        float* hi[3] = {n.maxX, n.maxY, n.maxZ};
+        for (int a = 0; a < 3; ++a) {
+            const float s = std::ldexp(1.0f, q[i].e[a]);
+            for (int k = 0; k < 4; ++k) {
+                lo[a][k] = q_dec(q[i].p[a], q[i].qlo[a][k], s);
+                hi[a][k] = q_dec(q[i].p[a], q[i].qhi[a][k], s);
+            }
+        }
+        for (int k = 0; k < 4; ++k) {
+            n.child[k] = q[i].child[k];
+            n.count[k] = q[i].count[k];
+        }
+        out[i] = n;
+    }
 }
 
 // ---------------------------------------------------------------------------

@@ -41,6 +41,11 @@
  *   generators             -> closed forms (vertex on sphere radius bounds,
  *                             unit directions, hemisphere orientation).
  *   or_blur7               -> constant image is a fixed point; weights sum.
+ *   or_bvh4_quantize,
+ *   or_bvh4_dequantize     -> every decoded box contains its slot box; each
+ *                             code is optimal (one step tighter breaks the
+ *                             containment) and each exponent least, checked
+ *                             by brute force over all 256 codes / exponents.
  */
 #ifndef RT_ORACLE_H
 #define RT_ORACLE_H
@@ -96,6 +101,25 @@ int or_bvh4_collapse(const or_bvh2_node* n2, uint32_t count2, or_bvh4_node* n4,
                      uint32_t cap4, uint32_t* count4);
 void or_tris48_to_tri64(const float* tris48, const uint32_t* order, uint32_t n, or_tri64* out);
 uint32_t or_bvh4_depth(const or_bvh4_node* nodes, uint32_t count);
+
+/* ---- quantized nodes (rt_qnode4; no reference counterpart) --------------
+ * or_bvh4_quantize: per node and axis, e = least exponent in [-126, 127] with
+ * dec(255) >= the max bound of the occupied slots (dec(q) = p + q * 2^e in
+ * float, p = the min bound), qlo = greatest code with dec(q) <= the slot's
+ * min, qhi = least code with dec(q) >= its max (binary searches over the
+ * monotone dec).  or_bvh4_dequantize: the rt_node4 tree with the decoded
+ * boxes (the tree a quantized trace is the exact trace of).                */
+typedef struct {
+    float p[3];
+    int8_t e[3];
+    uint8_t pad0;
+    uint8_t qlo[3][4];
+    uint8_t qhi[3][4];
+    int16_t count[4];
+    int32_t child[4];
+} or_qnode4;
+void or_bvh4_quantize(const or_bvh4_node* nodes, uint32_t count, or_qnode4* out);
+void or_bvh4_dequantize(const or_qnode4* q, uint32_t count, or_bvh4_node* out);
 
 /* ---- traversal ---------------------------------------------------------- */
 enum { OR_CLOSEST = 0, OR_ANY = 1 };

@@ -21,7 +21,10 @@ NODE4 = np.dtype([("minX", "<f4", 4), ("maxX", "<f4", 4), ("minY", "<f4", 4),
                   ("child", "<i4", 4), ("count", "<i4", 4)])
 TRI64 = np.dtype([("v", "<f4", 9), ("reserved", "<u4", 3), ("triangle_id", "<u4"),
                   ("geometry_id_and_flags", "<u4"), ("reserved2", "<u4"), ("id", "<u4")])
+QNODE4 = np.dtype([("p", "<f4", 3), ("e", "i1", 3), ("pad0", "u1"), ("qlo", "u1", (3, 4)),
+                   ("qhi", "u1", (3, 4)), ("count", "<i2", 4), ("child", "<i4", 4)])
 assert NODE2.itemsize == 32 and NODE4.itemsize == 128 and TRI64.itemsize == 64
+assert QNODE4.itemsize == 64
 
 MISS = 0xFFFFFFFF
 CLOSEST, ANY = 0, 1
@@ -160,6 +163,22 @@ def software_bvh4(pos, idx, max_prims=4):
     n2, t48, order = bvh2_build(pos, idx, max_prims)
     n4 = bvh4_collapse(n2)
     return n4, tri64_from48(t48, order)
+
+
+def bvh4_quantize(nodes):
+    """rt_qnode4 records (QNODE4) of a Box4 tree: or_bvh4_quantize."""
+    nodes = np.ascontiguousarray(nodes)
+    out = np.zeros(len(nodes), QNODE4)
+    lib().or_bvh4_quantize(_p(nodes), u32(len(nodes)), _p(out))
+    return out
+
+
+def bvh4_dequantize(q):
+    """The Box4 tree (NODE4) with the decoded boxes of quantized nodes."""
+    q = np.ascontiguousarray(q)
+    out = np.zeros(len(q), NODE4)
+    lib().or_bvh4_dequantize(_p(q), u32(len(q)), _p(out))
+    return out
 
 
 def bvh4_depth(nodes):

@@ -39,7 +39,10 @@ struct Child {
 
 // Order constants of an any-hit query that returns only occlusion bits
 // (rt_trace_opts.any_order): near-first (the reference's), far-first, slot.
-constexpr int kOrderNear = 0, kOrderFar = 1, kOrderSlot = 2;
+// kOrderInner (internal, the default for bits-only queries): inner children
+// near-first, each placed on the stack at its rank (no sorted copy of the
+// node), leaf children tested in slot order.
+constexpr int kOrderNear = 0, kOrderFar = 1, kOrderSlot = 2, kOrderInner = 3;
 
 // Compare-exchange putting the smaller key first (the far-first order's
 // ascending sort; pushes then go nearest first, so the farthest pops first).
@@ -83,6 +86,31 @@ __device__ __forceinline__ const f4v* f4(const rt_tri64* p) {
     return reinterpret_cast<const f4v*>(p);
 }
 __device__ __forceinline__ const RT_GLOBAL f4v* f4(const RT_GLOBAL rt_tri64* p) {
+    return reinterpret_cast<const RT_GLOBAL f4v*>(p);
+}
+// Float k of a triangle record.
+__device__ __forceinline__ float f1(const rt_tri64* p, int k) {
+    return reinterpret_cast<const float*>(p)[k];
+}
+__device__ __forceinline__ float f1(const RT_GLOBAL rt_tri64* p, int k) {
+    return reinterpret_cast<const RT_GLOBAL float*>(p)[k];
+}
+__device__ __forceinline__ const char* bytes(const rt_node4* p) {
+    return reinterpret_cast<const char*>(p);
+}
+__device__ __forceinline__ const RT_GLOBAL char* bytes(const RT_GLOBAL rt_node4* p) {
+    return reinterpret_cast<const RT_GLOBAL char*>(p);
+}
+__device__ __forceinline__ const i4v* i4p(const char* p) {
+    return reinterpret_cast<const i4v*>(p);
+}
+__device__ __forceinline__ const RT_GLOBAL i4v* i4p(const RT_GLOBAL char* p) {
+    return reinterpret_cast<const RT_GLOBAL i4v*>(p);
+}
+__device__ __forceinline__ const f4v* f4p(const char* p) {
+    return reinterpret_cast<const f4v*>(p);
+}
+__device__ __forceinline__ const RT_GLOBAL f4v* f4p(const RT_GLOBAL char* p) {
     return reinterpret_cast<const RT_GLOBAL f4v*>(p);
 }
 __device__ __forceinline__ const i4v* i4(const rt_node4* p) {
@@ -173,10 +201,13 @@ __device__ __forceinline__ bool test_leaves(TP __restrict__ tris, const Child (&
             left = (uint32_t)n;
             if (COUNT) triTests += left;
         }
+        // v0, v1 and v2.xy in two 16-B loads, v2.z (the record's ninth float)
+        // in a 4-B one: the vector memory pipe is the traversal's bound
         const auto tp = f4(tris + first);
-        const f4v a = tp[0], b = tp[1], c = tp[2];
+        const f4v a = tp[0], b = tp[1];
+        const float c = f1(tris + first, 8);
         float t;
-        if (mt_hit(a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w, c.x, ox, oy, oz, dx, dy, dz, &t) &&
+        if (mt_hit(a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w, c, ox, oy, oz, dx, dy, dz, &t) &&
             t >= tmin && t < hitT) {
             hitT = t;
             hitPrim = tris[first].triangle_id;
@@ -244,7 +275,7 @@ __device__ __forceinline__ i4v radv_node(NP p, int32_t self, uint32_t boxes, uin
 // until an accepted hit ends the ray, every node whose box meets [tmin, tmax]
 // is visited, whatever the order.
 template <int PRIV, bool ANY, bool COUNT, typename NP, typename TP, bool RADV = false,
-          int ORDER = kOrderNear>
+          int ORDER = kOrderNear, bool Q8 = false>
 __device__ __forceinline__ void trace_ray(NP __restrict__ nodes, TP __restrict__ tris, bool start,
                                           float ox, float oy, float oz, float tmin, float dx,
                                           float dy, float dz, int32_t* col, int ls, int bs,
@@ -259,6 +290,16 @@ __device__ __forceinline__ void trace_ray(NP __restrict__ nodes, TP __restrict__
     // keyed -inf); clamping tmin once gives every key that bound, as
     // fmaxf(te, -FLT_MAX) per child would (max is exact)
     const float tminc = fmaxf(tmin, -FLT_MAX);
+    // Byte offsets in a node of each axis's near and far planes: the min
+    // plane is the near one for a positive inverse direction (ix is never 0).
+    // As b_min <= b_max, (b_min - o) * ix <= (b_max - o) * ix for ix > 0 and
+    // >= for ix < 0 under round-to-nearest, so the near plane's t is the
+    // min of the two (and the far plane's the max) the reference takes: the
+    // same values with no min / max per axis (a zero may differ in sign,
+    // which no comparison sees).
+    const uint32_t nxo = ix > 0.0f ? 0u : 16u, nyo = iy > 0.0f ? 32u : 48u;
+    const uint32_t nzo = iz > 0.0f ? 64u : 80u;
+    const uint32_t fxo = nxo ^ 16u, fyo = nyo ^ 16u, fzo = nzo ^ 16u;
     Stack<PRIV> st;
     int sp = 0, spo = 0;
     const int lso = (ls - 1) * bs;
@@ -269,44 +310,142 @@ __device__ __forceinline__ void trace_ray(NP __restrict__ nodes, TP __restrict__
         if (RADV && ++pops > rboxes) break;
         int32_t ni;
         RT_SPOP(ni);
-        float bminx[4], bmaxx[4], bminy[4], bmaxy[4], bminz[4], bmaxz[4];
         int32_t chv[4], cnv[4];
         i4v ch;
+        float te[4];
+        bool hit[4];
         if constexpr (RADV) {
+            float bminx[4], bmaxx[4], bminy[4], bmaxy[4], bminz[4], bmaxz[4];
             ch = radv_node(nodes + ni, ni, rboxes, rtris, bminx, bmaxx, bminy, bmaxy, bminz,
                            bmaxz, chv, cnv);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {  // :116-144 slab test + filter
+                float t0x = (bminx[k] - ox) * ix, t1x = (bmaxx[k] - ox) * ix;
+                float t0y = (bminy[k] - oy) * iy, t1y = (bmaxy[k] - oy) * iy;
+                float t0z = (bminz[k] - oz) * iz, t1z = (bmaxz[k] - oz) * iz;
+                float tminX = fminf(t0x, t1x), tmaxX = fmaxf(t0x, t1x);
+                float tminY = fminf(t0y, t1y), tmaxY = fmaxf(t0y, t1y);
+                float tminZ = fminf(t0z, t1z), tmaxZ = fmaxf(t0z, t1z);
+                te[k] = fmaxf(fmaxf(tminX, tminY), fmaxf(tminZ, tminc));
+                float tx = fminf(fminf(tmaxX, tmaxY), fminf(tmaxZ, hitT));
+                hit[k] = cnv[k] != 0 && tx >= te[k];
+            }
+        } else if constexpr (Q8) {
+            // rt_qnode4 (64 B, four 16-B loads): child boxes decoded on the
+            // node's grid, p + q * 2^e (the product is exact, so the fma
+            // rounds once, as the encoder's p + q * s does); the near / far
+            // codes of an axis are its qlo / qhi words as for rt_node4's planes
+            const auto qb = bytes(nodes) + (size_t)ni * 64u;
+            const i4v q0 = i4p(qb)[0], q1 = i4p(qb)[1], q2 = i4p(qb)[2];
+            ch = i4p(qb)[3];
+            const float px = __int_as_float(q0.x), py = __int_as_float(q0.y);
+            const float pz = __int_as_float(q0.z);
+            const uint32_t ew = (uint32_t)q0.w;
+            const float sx = __int_as_float((((int)(int8_t)(ew & 0xffu)) + 127) << 23);
+            const float sy = __int_as_float((((int)(int8_t)((ew >> 8) & 0xffu)) + 127) << 23);
+            const float sz = __int_as_float((((int)(int8_t)((ew >> 16) & 0xffu)) + 127) << 23);
+            const uint32_t nqx = nxo ? (uint32_t)q1.w : (uint32_t)q1.x;
+            const uint32_t fqx = nxo ? (uint32_t)q1.x : (uint32_t)q1.w;
+            const uint32_t nqy = nyo == 32u ? (uint32_t)q1.y : (uint32_t)q2.x;
+            const uint32_t fqy = nyo == 32u ? (uint32_t)q2.x : (uint32_t)q1.y;
+            const uint32_t nqz = nzo == 64u ? (uint32_t)q1.z : (uint32_t)q2.y;
+            const uint32_t fqz = nzo == 64u ? (uint32_t)q2.y : (uint32_t)q1.z;
+            chv[0] = ch.x; chv[1] = ch.y; chv[2] = ch.z; chv[3] = ch.w;
+            cnv[0] = (int32_t)(int16_t)(q2.z & 0xffff);
+            cnv[1] = (int32_t)q2.z >> 16;
+            cnv[2] = (int32_t)(int16_t)(q2.w & 0xffff);
+            cnv[3] = (int32_t)q2.w >> 16;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {  // :116-144 slab test + filter, decoded boxes
+                const int sh = 8 * k;
+                const float bnx = __builtin_fmaf((float)((nqx >> sh) & 0xffu), sx, px);
+                const float bfx = __builtin_fmaf((float)((fqx >> sh) & 0xffu), sx, px);
+                const float bny = __builtin_fmaf((float)((nqy >> sh) & 0xffu), sy, py);
+                const float bfy = __builtin_fmaf((float)((fqy >> sh) & 0xffu), sy, py);
+                const float bnz = __builtin_fmaf((float)((nqz >> sh) & 0xffu), sz, pz);
+                const float bfz = __builtin_fmaf((float)((fqz >> sh) & 0xffu), sz, pz);
+                const float tnx = (bnx - ox) * ix, tfx = (bfx - ox) * ix;
+                const float tny = (bny - oy) * iy, tfy = (bfy - oy) * iy;
+                const float tnz = (bnz - oz) * iz, tfz = (bfz - oz) * iz;
+                te[k] = fmaxf(fmaxf(tnx, tny), fmaxf(tnz, tminc));
+                const float tx = fminf(fminf(tfx, tfy), fminf(tfz, hitT));
+                hit[k] = cnv[k] != 0 && tx >= te[k];
+            }
         } else {
-            const auto np = f4(nodes + ni);
-            const f4v mnx = np[0], mxx = np[1], mny = np[2], mxy = np[3], mnz = np[4],
-                      mxz = np[5];
+            const auto nb = bytes(nodes + ni);
+            const f4v nx = *f4p(nb + nxo), fx = *f4p(nb + fxo), ny = *f4p(nb + nyo);
+            const f4v fy = *f4p(nb + fyo), nz = *f4p(nb + nzo), fz = *f4p(nb + fzo);
             ch = i4(nodes + ni)[6];
             const i4v cn = i4(nodes + ni)[7];
-            bminx[0] = mnx.x; bminx[1] = mnx.y; bminx[2] = mnx.z; bminx[3] = mnx.w;
-            bmaxx[0] = mxx.x; bmaxx[1] = mxx.y; bmaxx[2] = mxx.z; bmaxx[3] = mxx.w;
-            bminy[0] = mny.x; bminy[1] = mny.y; bminy[2] = mny.z; bminy[3] = mny.w;
-            bmaxy[0] = mxy.x; bmaxy[1] = mxy.y; bmaxy[2] = mxy.z; bmaxy[3] = mxy.w;
-            bminz[0] = mnz.x; bminz[1] = mnz.y; bminz[2] = mnz.z; bminz[3] = mnz.w;
-            bmaxz[0] = mxz.x; bmaxz[1] = mxz.y; bmaxz[2] = mxz.z; bmaxz[3] = mxz.w;
+            const float bnx[4] = {nx.x, nx.y, nx.z, nx.w}, bfx[4] = {fx.x, fx.y, fx.z, fx.w};
+            const float bny[4] = {ny.x, ny.y, ny.z, ny.w}, bfy[4] = {fy.x, fy.y, fy.z, fy.w};
+            const float bnz[4] = {nz.x, nz.y, nz.z, nz.w}, bfz[4] = {fz.x, fz.y, fz.z, fz.w};
             chv[0] = ch.x; chv[1] = ch.y; chv[2] = ch.z; chv[3] = ch.w;
             cnv[0] = cn.x; cnv[1] = cn.y; cnv[2] = cn.z; cnv[3] = cn.w;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {  // :116-144 slab test + filter
+                const float tnx = (bnx[k] - ox) * ix, tfx = (bfx[k] - ox) * ix;
+                const float tny = (bny[k] - oy) * iy, tfy = (bfy[k] - oy) * iy;
+                const float tnz = (bnz[k] - oz) * iz, tfz = (bfz[k] - oz) * iz;
+                te[k] = fmaxf(fmaxf(tnx, tny), fmaxf(tnz, tminc));
+                const float tx = fminf(fminf(tfx, tfy), fminf(tfz, hitT));
+                hit[k] = cnv[k] != 0 && tx >= te[k];
+            }
         }
         if (COUNT) boxTests += 4;
 
-        float te[4];
-        bool hit[4];
+        if constexpr (ORDER == kOrderInner) {
+            static_assert(ANY && !COUNT && !RADV, "for occlusion bits only");
+            // Inner children: rank r_k = inner hit children before k (larger
+            // entry t, ties by slot), pushed farthest first, so the nearest
+            // pops first -- the near-first order of the inner children, with
+            // each child written to stack entry sp + r_k instead of sorting.
+            bool inner[4];
+            uint32_t leafm = 0;
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {  // :116-144 slab test + filter
-            float t0x = (bminx[k] - ox) * ix, t1x = (bmaxx[k] - ox) * ix;
-            float t0y = (bminy[k] - oy) * iy, t1y = (bmaxy[k] - oy) * iy;
-            float t0z = (bminz[k] - oz) * iz, t1z = (bmaxz[k] - oz) * iz;
-            float tminX = fminf(t0x, t1x), tmaxX = fmaxf(t0x, t1x);
-            float tminY = fminf(t0y, t1y), tmaxY = fmaxf(t0y, t1y);
-            float tminZ = fminf(t0z, t1z), tmaxZ = fmaxf(t0z, t1z);
-            te[k] = fmaxf(fmaxf(tminX, tminY), fmaxf(tminZ, tminc));
-            float tx = fminf(fminf(tmaxX, tmaxY), fminf(tmaxZ, hitT));
-            hit[k] = cnv[k] != 0 && tx >= te[k];
+            for (int k = 0; k < 4; ++k) {
+                inner[k] = hit[k] && cnv[k] < 0;
+                leafm |= (hit[k] && cnv[k] > 0 ? 1u : 0u) << k;
+            }
+            keep_loaded(ch);
+            if (!(hit[0] || hit[1] || hit[2] || hit[3])) continue;
+            int rank[4] = {0, 0, 0, 0};
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+#pragma unroll
+                for (int k = j + 1; k < 4; ++k) {
+                    const bool jfirst = te[j] >= te[k];
+                    rank[k] += (inner[j] && jfirst) ? 1 : 0;
+                    rank[j] += (inner[k] && !jfirst) ? 1 : 0;
+                }
+            const int ni = (inner[0] ? 1 : 0) + (inner[1] ? 1 : 0) + (inner[2] ? 1 : 0) +
+                           (inner[3] ? 1 : 0);
+            if (sp + ni <= ls) {  // all four entries land in the LDS part
+#pragma unroll
+                for (int k = 0; k < 4; ++k)
+                    if (inner[k]) col[spo + rank[k] * bs] = chv[k];
+            } else {
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    const int q = sp + rank[k];
+                    // (large private part: sized for kMaxDepth levels; see RT_SPUSH)
+                    if (!inner[k] || (PRIV == kPrivLarge && q >= ls + PRIV)) continue;
+                    if (q < ls) col[q * bs] = chv[k];
+                    else st.priv[q - ls] = chv[k];
+                }
+            }
+            const int nst = PRIV == kPrivLarge ? min(ni, ls + PRIV - sp) : ni;
+            sp += nst;
+            spo += nst * bs;
+            if (leafm) {
+                const Child e[4] = {{te[0], chv[0], cnv[0]}, {te[1], chv[1], cnv[1]},
+                                    {te[2], chv[2], cnv[2]}, {te[3], chv[3], cnv[3]}};
+                if (test_leaves<ANY, COUNT>(tris, e, leafm, ox, oy, oz, dx, dy, dz, tmin, hitT,
+                                            hitPrim, triTests))
+                    break;  // the first accepted triangle ends the ray
+            }
+            continue;
         }
-
         // :149-156 exchange sort of the hit children, run in place over the 4
         // slots with missed children keyed -inf (keys of hit children are at
         // least -FLT_MAX): for every pattern of hits and ties this yields the
@@ -397,7 +536,8 @@ __device__ __forceinline__ void write_tile_bits(BP bits_out, uint32_t r, uint32_
     }
 }
 
-template <int PRIV, bool ANY, bool COUNT, int ORDER = kOrderNear, bool TILE = false>
+template <int PRIV, bool ANY, bool COUNT, int ORDER = kOrderNear, bool TILE = false,
+          bool Q8 = false>
 __global__ __launch_bounds__(kBlock) void k_trace(const rt_node4* __restrict__ nodes,
                                                   const rt_tri64* __restrict__ tris,
                                                   uint32_t node_count, rt_rays_soa rays,
@@ -428,7 +568,7 @@ __global__ __launch_bounds__(kBlock) void k_trace(const rt_node4* __restrict__ n
     float hitT = tmax;
     uint32_t hitPrim = RT_MISS;
     uint32_t boxTests = 0, triTests = 0;
-    trace_ray<PRIV, ANY, COUNT, const rt_node4*, const rt_tri64*, false, ORDER>(
+    trace_ray<PRIV, ANY, COUNT, const rt_node4*, const rt_tri64*, false, ORDER, Q8>(
         nodes, tris, valid && node_count > 0, ox, oy, oz, tmin, dx, dy, dz, lds_stack + tid,
         (int)ls, (int)bs, hitT, hitPrim, boxTests, triTests);
     if (ANY) {
@@ -1009,12 +1149,40 @@ rt_status launch_v(hipStream_t s, const rt_bvh_dev* b, const rt_rays_soa* rays, 
     }
     TileMap tm;
     dim3 grid(ceil_div(launch_threads(cfg, n, tm), cfg.block)), block(cfg.block);
+    // occlusion bits only (no counters, no hit ids): the order does not change
+    // the bits, so near-first is run as kOrderInner
+    const bool bits_only = any && !cnt && !prim;
+    if (b->format == RT_NODES_Q8) {
+#define RT_Q(A, C, O, T)                                                                      \
+    hipLaunchKernelGGL((k_trace<PRIV, A, C, O, T, true>), grid, block, shm, s, b->nodes,      \
+                       b->tris, b->node_count, *rays, n, cfg.lds, prim, t, bits, box, tri, tm)
+        if (cfg.tile_width) {
+            if (cfg.order == kOrderFar) RT_Q(true, false, kOrderFar, true);
+            else if (cfg.order == kOrderSlot) RT_Q(true, false, kOrderSlot, true);
+            else if (bits_only) RT_Q(true, false, kOrderInner, true);
+            else if (any && cnt) RT_Q(true, true, kOrderNear, true);
+            else if (any) RT_Q(true, false, kOrderNear, true);
+            else if (cnt) RT_Q(false, true, kOrderNear, true);
+            else RT_Q(false, false, kOrderNear, true);
+        } else {
+            if (cfg.order == kOrderFar) RT_Q(true, false, kOrderFar, false);
+            else if (cfg.order == kOrderSlot) RT_Q(true, false, kOrderSlot, false);
+            else if (bits_only) RT_Q(true, false, kOrderInner, false);
+            else if (any && cnt) RT_Q(true, true, kOrderNear, false);
+            else if (any) RT_Q(true, false, kOrderNear, false);
+            else if (cnt) RT_Q(false, true, kOrderNear, false);
+            else RT_Q(false, false, kOrderNear, false);
+        }
+#undef RT_Q
+        return launch_status("k_trace");
+    }
 #define RT_L(A, C, O, T)                                                                      \
     hipLaunchKernelGGL((k_trace<PRIV, A, C, O, T>), grid, block, shm, s, b->nodes, b->tris,   \
                        b->node_count, *rays, n, cfg.lds, prim, t, bits, box, tri, tm)
     if (cfg.tile_width) {
         if (cfg.order == kOrderFar) RT_L(true, false, kOrderFar, true);
         else if (cfg.order == kOrderSlot) RT_L(true, false, kOrderSlot, true);
+        else if (bits_only) RT_L(true, false, kOrderInner, true);
         else if (any && cnt) RT_L(true, true, kOrderNear, true);
         else if (any) RT_L(true, false, kOrderNear, true);
         else if (cnt) RT_L(false, true, kOrderNear, true);
@@ -1022,6 +1190,7 @@ rt_status launch_v(hipStream_t s, const rt_bvh_dev* b, const rt_rays_soa* rays, 
     } else {
         if (cfg.order == kOrderFar) RT_L(true, false, kOrderFar, false);
         else if (cfg.order == kOrderSlot) RT_L(true, false, kOrderSlot, false);
+        else if (bits_only) RT_L(true, false, kOrderInner, false);
         else if (any && cnt) RT_L(true, true, kOrderNear, false);
         else if (any) RT_L(true, false, kOrderNear, false);
         else if (cnt) RT_L(false, true, kOrderNear, false);
@@ -1068,6 +1237,8 @@ extern "C" rt_status rt_trace_ex(rt_stream stream, const rt_bvh_dev* bvh, const 
     if (!any && (!hits || !hits->prim || !hits->t)) return RT_ERR_ARG;
     if ((flags & RT_COUNTERS) && (!box_tests || !tri_tests)) return RT_ERR_ARG;
     if (bvh->node_count > 0 && (!bvh->nodes || !bvh->tris)) return RT_ERR_ARG;
+    if (bvh->format > RT_NODES_Q8) return RT_ERR_ARG;
+    if (bvh->format == RT_NODES_Q8 && cfg.refill) return RT_ERR_UNSUPPORTED;
     // depth 0 = unknown (<= kMaxDepth)
     const uint32_t depth = bvh->depth ? bvh->depth : kMaxDepth;
     if (depth > kMaxDepth) return RT_ERR_CAPACITY;
@@ -1271,6 +1442,7 @@ extern "C" rt_status rt_trace_batch_plan_ex(const rt_trace_item* items, uint32_t
                 return RT_ERR_ARG;
             if (it.bvh.node_count > 0 && (!it.bvh.nodes || !it.bvh.tris)) return RT_ERR_ARG;
         }
+        if (it.bvh.format != RT_NODES_F32) return RT_ERR_UNSUPPORTED;
         const uint32_t d = it.bvh.depth ? it.bvh.depth : kMaxDepth;
         if (d > kMaxDepth) return RT_ERR_CAPACITY;
         deep = d > deep ? d : deep;

@@ -43,7 +43,11 @@ f32 = C.c_float
 
 class rt_bvh_dev(C.Structure):
     _fields_ = [("nodes", vp), ("tris", vp), ("node_count", u32), ("tri_count", u32),
-                ("depth", u32), ("reserved", u32)]
+                ("depth", u32), ("format", u32)]
+
+
+NODES_F32, NODES_Q8 = 0, 1   # rt_bvh_dev.format
+QNODE_BYTES = 64             # rt_qnode4
 
 
 class rt_rays_soa(C.Structure):
@@ -202,6 +206,7 @@ _SIGS = {
     "rt_image_metrics": [vp, vp, vp, u32, u32, vp, vp, vp, C.c_size_t],
     "rt_device_sync": [],
     "rt_gpu_build_noalloc": [vp, vp, vp, u32, vp, vp, u32, vp, vp, vp, vp],
+    "rt_gpu_quantize": [vp, vp, u32, vp],
     "rt_trace_batch_submit_bytes": [u32, C.c_int, vp],
     "rt_trace_batch_submit": [vp, vp, vp, u32, vp, vp, C.c_size_t, vp, C.c_size_t],
     "rt_gpu_build_batch_submit_bytes": [u32, C.c_int, vp],
@@ -485,9 +490,31 @@ class GpuBVH4:
         return GpuBVH4(nodes, tris, ncount, n, depth)
 
     # -- trace -----------------------------------------------------------------
+    format = NODES_F32
+
     def dev(self):
         return rt_bvh_dev(ptr(self.nodes), ptr(self.tris), u32(self.node_count),
-                          u32(self.tri_count), u32(self.depth), u32(0))
+                          u32(self.tri_count), u32(self.depth), u32(self.format))
+
+    def quantize(self, stream=None, out=None):
+        """The RT_NODES_Q8 copy of this tree (rt_gpu_quantize): 64-B nodes
+        with 8-bit child boxes rounded outward, the same leaves (this tree's
+        triangle records are shared).  It traces the same triangles as this
+        tree with half the node bytes per visit (rt_trace / rt_trace_ex,
+        one ray per thread).  Needs the real node count (a sync build)."""
+        if self.format != NODES_F32:
+            raise ValueError("already quantized")
+        if getattr(self, "info", None) is not None and self.depth == 0:
+            self.node_count, self.depth = [int(v) for v in self.info[:2].cpu().tolist()]
+        q = out if out is not None else torch.empty(max(self.node_count, 1) * QNODE_BYTES,
+                                                    dtype=torch.uint8, device=self.nodes.device)
+        check_out(q, self.node_count * QNODE_BYTES, (torch.uint8,), "out (quantized nodes)")
+        call("rt_gpu_quantize", stream_handle(stream), ptr(self.nodes), u32(self.node_count),
+             ptr(q))
+        b = GpuBVH4(q, self.tris, self.node_count, self.tri_count, self.depth)
+        b.format = NODES_Q8
+        b.source = self
+        return b
 
     def intersect(self, rays, counters=False, stream=None, out=None, schedule=None, **tune):
         """Closest hit per ray: (prim uint32-as-int32, t float32[, box, tri]).
@@ -796,6 +823,8 @@ class Scene:
         arr = (rt_instance * max(1, len(self.instances)))()
         depth = 1
         for k, (bvh, pos, idx, m, kind) in enumerate(self.instances):
+            if bvh.format != NODES_F32:
+                raise ValueError("scene instances take rt_node4 trees (RT_NODES_F32)")
             arr[k].blas = bvh.dev()
             arr[k].pos, arr[k].idx = ptr(pos), ptr(idx)
             mm = [1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0] if m is None else \

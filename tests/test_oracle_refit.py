@@ -106,3 +106,56 @@ def test_refit_refuses_child_before_parent(O):
     bad[i]["child"][k] = i  # a child that is its own parent
     with pytest.raises(ValueError):
         O.bvh4_refit(pos, idx, bad, tris)
+
+
+def test_quantize_contains_and_is_tight(O):
+    """or_bvh4_quantize on a radix tree with leaves of 1 and 2 treelet rounds:
+    every decoded slot box contains the rt_node4 slot box; each code is the
+    tightest (one step inward breaks the containment, checked by brute force
+    over the 256 codes) and each exponent the least with dec(255) >= the
+    node's max bound (brute force over -126..127); child / count unchanged;
+    empty slots code 0."""
+    pos, idx = O.sphere(60, 40, 1.0, 0.05, 4)
+    nodes, tris, _, _ = O.rlbvh_build(pos, idx, max_prims=1, treelet_rounds=2)
+    q = O.bvh4_quantize(nodes)
+    d = O.bvh4_dequantize(q)
+    assert (d["child"] == np.where(nodes["count"] != 0, nodes["child"], 0)).all()
+    assert (d["count"] == nodes["count"]).all()
+    axes = (("minX", "maxX"), ("minY", "maxY"), ("minZ", "maxZ"))
+    rng = np.random.default_rng(1)
+    for i in rng.choice(len(nodes), 300, replace=False):
+        n, qi = nodes[i], q[i]
+        for a, (lo, hi) in enumerate(axes):
+            occ = n["count"] != 0
+            p = np.float32(n[lo][occ].min())
+            m = np.float32(n[hi][occ].max())
+            assert qi["p"][a] == p
+            dec = lambda c, e: np.float32(p + np.float32(c) * np.float32(2.0 ** e))
+            es = [e for e in range(-126, 128) if dec(255, e) >= m]
+            assert qi["e"][a] == es[0]
+            for k in range(4):
+                if not occ[k]:
+                    assert qi["qlo"][a][k] == 0 and qi["qhi"][a][k] == 0
+                    continue
+                codes = [dec(c, es[0]) for c in range(256)]
+                assert qi["qlo"][a][k] == max(c for c in range(256) if codes[c] <= n[lo][k])
+                assert qi["qhi"][a][k] == min(c for c in range(256) if codes[c] >= n[hi][k])
+                assert d[lo][i][k] <= n[lo][k] and d[hi][i][k] >= n[hi][k]
+
+
+def test_quantized_tree_traces_the_same_triangles(O):
+    """Tracing the dequantized tree (the boxes a quantized trace tests)
+    gives the exact tree's closest hits and occlusion bits on random rays
+    (the boxes only grow: a superset of node visits, the same triangles)."""
+    pos, idx = O.sphere(200, 150, 1.0, 0.05, 9)
+    nodes, tris, _, _ = O.rlbvh_build(pos, idx, max_prims=2)
+    d = O.bvh4_dequantize(O.bvh4_quantize(nodes))
+    r = O.random_rays(50_000, 3, (-1.3,) * 3, (1.3,) * 3, tmax=1e30)
+    pe, te = O.trace_bvh4(nodes, tris, r)
+    pq, tq = O.trace_bvh4(d, tris, r)
+    assert (te.view(np.uint32) == tq.view(np.uint32)).all()
+    assert (pe == pq).mean() > 0.9999
+    r.tmax[:] = 0.3
+    ae, _ = O.trace_bvh4(nodes, tris, r, mode=O.ANY)
+    aq, _ = O.trace_bvh4(d, tris, r, mode=O.ANY)
+    assert ((ae != O.MISS) == (aq != O.MISS)).all()
