@@ -1707,7 +1707,8 @@ __global__ __launch_bounds__(1024) void k_rt_top(RadixArrays A) {
 constexpr int kTreeletLeaves = 7;
 constexpr uint32_t kMaxTreeletRounds = 16;
 constexpr uint32_t kTreeletSets = 1u << kTreeletLeaves;
-constexpr uint32_t kTreeletBlock = 256;  // LDS: kTreeletSets floats per thread, 128 KiB
+constexpr uint32_t kTreeletBlock = 256;  // LDS: kTreeletSets floats per wave
+constexpr uint32_t kTreeletTopBlocks = 1024;  // pass 2 grid (walks the start list)
 constexpr float kCostInner = 1.2f;
 
 __device__ __forceinline__ uint32_t ext_lo(const RadixArrays& A, int32_t u) {
@@ -1727,10 +1728,11 @@ __device__ __forceinline__ float sah_cost(const RadixArrays& A, int32_t u) {
     return box_sa(A.box + 6 * (size_t)u) * (float)(nlast(A, u) - nfirst(A, u) + 1);
 }
 
-// Cheapest split of leaf subset s; C is this thread's LDS column.  The parts
-// holding s's lowest leaf, in descending order, are low | sub for the proper
-// submasks sub of rest = s ^ low, descending; four are loaded at a time (the
-// scan is bound by LDS latency at one wave per SIMD) and compared in order.
+// Cheapest split of leaf subset s over the cost table C (C[t * STRIDE] =
+// cost of subset t).  The parts holding s's lowest leaf, in descending order,
+// are low | sub for the proper submasks sub of rest = s ^ low, descending; four
+// are loaded at a time and compared in order, the first strict minimum kept.
+template <uint32_t STRIDE>
 __device__ __forceinline__ float best_split(const float* C, uint32_t s, uint32_t& bp) {
     const uint32_t low = s & (0u - s), rest = s ^ low;
     float best = INFINITY;
@@ -1754,8 +1756,8 @@ __device__ __forceinline__ float best_split(const float* C, uint32_t s, uint32_t
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             if (j >= k) continue;
-            a[j] = C[(low | q[j]) * kTreeletBlock];
-            b[j] = C[(rest ^ q[j]) * kTreeletBlock];
+            a[j] = C[(low | q[j]) * STRIDE];
+            b[j] = C[(rest ^ q[j]) * STRIDE];
         }
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
@@ -1770,8 +1772,29 @@ __device__ __forceinline__ float best_split(const float* C, uint32_t s, uint32_t
     return best;
 }
 
-__device__ void treelet_node(const RadixArrays& A, int32_t x, float* C) {
+// LDS ordering between the steps of a wave's dynamic program: the wave's own
+// LDS writes complete before its later reads (LDS instructions of one wave
+// execute in order); this keeps the compiler from moving them across.
+__device__ __forceinline__ void wave_lds_step() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// The treelet rebuild at inner node x (wave-uniform), by the whole wave
+// (Karras & Aila 2013 process one treelet per warp): the treelet's leaves
+// and boxes are formed by every lane alike; subset costs go to the wave's
+// 128-float LDS table C, the subsets of one size at a time, lane l holding
+// subsets l + 1 and l + 65 (its own scan of their splits, as best_split);
+// the full set's splits are spread over the lanes and reduced to their
+// least cost; lane 0 rebuilds the treelet when that is strictly cheaper and
+// finishes x.  The costs, the splits chosen and so the tree are those of
+// the sequential program (each subset's cost is a least value, which does
+// not depend on the order its candidates are compared in; the rebuild
+// re-derives every split with best_split's order).
+__device__ void treelet_node_wave(const RadixArrays& A, int32_t x, float* C) {
     if (!ninner(A, x)) return;  // a leaf range: its cost is computed where used
+    const uint32_t lane = threadIdx.x & 63u;
     int32_t lf[kTreeletLeaves], in[kTreeletLeaves - 1];
     lf[0] = A.left[x];
     lf[1] = A.right[x];
@@ -1803,27 +1826,50 @@ __device__ void treelet_node(const RadixArrays& A, int32_t x, float* C) {
             if (q >= nl) continue;
 #pragma unroll
             for (int a = 0; a < 6; ++a) lb[q][a] = A.box[6 * (size_t)lf[q] + a];
-            C[(1u << q) * kTreeletBlock] = sah_cost(A, lf[q]);
+            if (lane == (uint32_t)q) C[1u << q] = sah_cost(A, lf[q]);
         }
         const uint32_t full = (1u << nl) - 1;
-        for (uint32_t s = 3; s <= full; ++s) {
-            if (!(s & (s - 1))) continue;  // one leaf: set above
+        // surface areas of this lane's two subsets (as the sequential union)
+        const uint32_t sl[2] = {lane + 1u, lane + 65u};
+        float sa[2];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
             float b[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
 #pragma unroll
             for (int q = 0; q < kTreeletLeaves; ++q) {
-                if (!((s >> q) & 1u)) continue;
+                if (!((sl[h] >> q) & 1u)) continue;
 #pragma unroll
                 for (int a = 0; a < 3; ++a) {
                     b[a] = fminf(b[a], lb[q][a]);
                     b[3 + a] = fmaxf(b[3 + a], lb[q][3 + a]);
                 }
             }
-            uint32_t bp;
-            const float split = best_split(C, s, bp);
-            C[s * kTreeletBlock] = kCostInner * box_sa(b) + split;
+            sa[h] = box_sa(b);
         }
-        const float cf = C[full * kTreeletBlock];
-        if (cf < cx) {
+        wave_lds_step();
+        for (int k = 2; k < nl; ++k) {
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                if (sl[h] > full || __popc(sl[h]) != k) continue;
+                uint32_t bp;
+                const float split = best_split<1>(C, sl[h], bp);
+                C[sl[h]] = kCostInner * sa[h] + split;
+            }
+            wave_lds_step();
+        }
+        // the full set: its 2^(nl-1) - 1 splits over the lanes (low = leaf 0)
+        const uint32_t rest = full ^ 1u, nsplit = (1u << (nl - 1)) - 1u;
+        float c = INFINITY;
+        if (lane < nsplit) {
+            const uint32_t sub = (nsplit - 1u - lane) << 1;
+            c = C[1u | sub] + C[rest ^ sub];
+        }
+        c = wave_min(c);
+        const float sa_full = __shfl(sa[(full - 1u) >> 6], (int)((full - 1u) & 63u));
+        const float cf = kCostInner * sa_full + c;
+        if (lane == 0) C[full] = cf;
+        wave_lds_step();
+        if (lane == 0 && cf < cx) {
             // plan the rebuild first (no stores): breadth-first over the subsets
             int32_t qn[kTreeletLeaves - 1], ql[kTreeletLeaves - 1], qr[kTreeletLeaves - 1];
             uint32_t qs[kTreeletLeaves - 1];
@@ -1833,7 +1879,7 @@ __device__ void treelet_node(const RadixArrays& A, int32_t x, float* C) {
             bool ok = true;
             for (int i = 0; i < nq && ok; ++i) {
                 uint32_t p;
-                best_split(C, qs[i], p);
+                best_split<1>(C, qs[i], p);
                 if (p == 0) {
                     ok = false;
                     break;
@@ -1864,21 +1910,25 @@ __device__ void treelet_node(const RadixArrays& A, int32_t x, float* C) {
                     const int32_t y = qn[i];
                     A.emin[y] = min(ext_lo(A, ql[i]), ext_lo(A, qr[i]));
                     A.emax[y] = max(ext_hi(A, ql[i]), ext_hi(A, qr[i]));
-                    A.cost[y] = C[qs[i] * kTreeletBlock];
+                    A.cost[y] = C[qs[i]];
                     finish_node(A, y);
                 }
                 cx = cf;
             }
         }
+        cx = __shfl(cx, 0);
     }
-    A.cost[x] = cx;
-    finish_node(A, x);
+    if (lane == 0) {
+        A.cost[x] = cx;
+        finish_node(A, x);
+    }
 }
 
 // Round set-up: arrival counters cleared, the nodes whose positions cross a
-// window boundary listed (top_count zeroed by k_tl_zero before).
+// window boundary listed (top_count[0], zeroed with the start count
+// top_count[1] by k_tl_zero before).
 __global__ void k_tl_zero(RadixArrays A) {
-    if (threadIdx.x == 0) *A.top_count = 0;
+    if (threadIdx.x < 2) A.top_count[threadIdx.x] = 0;
 }
 __global__ __launch_bounds__(256) void k_tl_prep(RadixArrays A) {
     const uint32_t i = blockIdx.x * 256 + threadIdx.x;
@@ -1887,59 +1937,88 @@ __global__ __launch_bounds__(256) void k_tl_prep(RadixArrays A) {
     if (A.emin[i] / kWindow != A.emax[i] / kWindow) A.top[atomicAdd(A.top_count, 1u)] = i;
 }
 
-// One restructuring round, pass 1: as k_rt_bottom_up over kWindow sorted
-// positions, with a node inside the window when all positions below it are;
-// the arrival counters are global (every node inside the window is reached
-// only by this workgroup, so workgroup scope suffices).
-__global__ __launch_bounds__(kTreeletBlock) void k_tl_window(RadixArrays A) {
-    __shared__ float C[kTreeletSets * kTreeletBlock];
-    float* mine = C + threadIdx.x;
-    const uint32_t lo = blockIdx.x * kWindow;
-    const uint32_t hi = min(lo + kWindow, A.n) - 1;
-    for (uint32_t p = lo + threadIdx.x; p <= hi; p += kTreeletBlock) {
-        int32_t x = A.parent[A.m + p];
-        // a climb inside the window passes fewer than kWindow nodes (bound kept
-        // so that no malformed link can hold a wave)
-        for (uint32_t steps = 0; x >= 0 && steps < kWindow; ++steps) {
-            if (!(A.emin[x] >= lo && A.emax[x] <= hi)) break;
-            const uint32_t old = __hip_atomic_fetch_add(A.done + x, 1u, __ATOMIC_ACQ_REL,
-                                                        __HIP_MEMORY_SCOPE_WORKGROUP);
-            if (old == 0) break;
-            treelet_node(A, x, mine);
-            x = A.parent[x];
+// Bottom-up climbs of one block, the treelet rebuilds done by whole waves:
+// lane climbs start at `start` (or none: -1); at every step each lane
+// checks its node (window test for pass 1) and counts its arrival; the
+// second arrival at a node owns it, and the wave rebuilds its lanes' owned
+// nodes one after another.  SCOPE: workgroup for the windows (every node
+// inside a window is reached only from its block), agent for the nodes
+// crossing windows (reached from any block).
+template <bool WINDOW>
+__device__ void tl_climbs(const RadixArrays& A, int32_t x, uint32_t lo, uint32_t hi, float* C) {
+    const uint32_t lane = threadIdx.x & 63u;
+    for (uint32_t steps = 0;; ++steps) {
+        bool own = false;
+        if (x >= 0 && steps <= A.m) {
+            if constexpr (WINDOW) {
+                if (A.emin[x] >= lo && A.emax[x] <= hi)
+                    own = __hip_atomic_fetch_add(A.done + x, 1u, __ATOMIC_ACQ_REL,
+                                                 __HIP_MEMORY_SCOPE_WORKGROUP) != 0;
+            } else {
+                // the climb's first node (steps 0) is a start: both children
+                // finished before this pass
+                own = steps == 0 || __hip_atomic_fetch_add(A.done + x, 1u, __ATOMIC_ACQ_REL,
+                                                            __HIP_MEMORY_SCOPE_AGENT) != 0;
+            }
         }
+        if (!own) x = -1;
+        uint64_t m = __ballot(own);
+        if (m == 0) break;
+        // the wave reads what other lanes' (and, for agent scope, other
+        // blocks') rebuilds wrote before their arrivals
+        if constexpr (WINDOW) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+        else __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        while (m) {
+            const int l = __builtin_ctzll(m);
+            m &= m - 1;
+            treelet_node_wave(A, __shfl(x, l), C);
+        }
+        if constexpr (WINDOW) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        else __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        if (own) x = A.parent[x];
+        (void)lane;
     }
 }
 
-// Pass 2, one workgroup: the nodes crossing window boundaries, as k_rt_top.
-__global__ __launch_bounds__(kTreeletBlock) void k_tl_top(RadixArrays A) {
-    __shared__ float C[kTreeletSets * kTreeletBlock];
-    __shared__ uint32_t nstart;
-    float* mine = C + threadIdx.x;
-    const uint32_t cnt = *A.top_count;
-    if (threadIdx.x == 0) nstart = 0;
-    __syncthreads();
-    for (uint32_t i = threadIdx.x; i < cnt; i += kTreeletBlock) {
-        const int32_t x = (int32_t)A.top[i];
-        const int ready =
-            (ext_straddles(A, A.left[x]) ? 0 : 1) + (ext_straddles(A, A.right[x]) ? 0 : 1);
-        if (ready == 1)
-            A.done[x] = 1;
-        else if (ready == 2)
-            A.starts[atomicAdd(&nstart, 1u)] = (uint32_t)x;
+constexpr uint32_t kTreeletWaves = kTreeletBlock / 64;
+
+// One restructuring round, pass 1: as k_rt_bottom_up over kWindow sorted
+// positions, with a node inside the window when all positions below it are.
+__global__ __launch_bounds__(kTreeletBlock) void k_tl_window(RadixArrays A) {
+    __shared__ float C[kTreeletWaves][kTreeletSets];
+    float* mine = C[threadIdx.x >> 6];
+    const uint32_t lo = blockIdx.x * kWindow;
+    const uint32_t hi = min(lo + kWindow, A.n) - 1;
+    for (uint32_t base = lo; base <= hi; base += kTreeletBlock) {
+        const uint32_t p = base + threadIdx.x;
+        tl_climbs<true>(A, p <= hi ? A.parent[A.m + p] : -1, lo, hi, mine);
     }
-    __syncthreads();
-    const uint32_t ns = nstart;
-    for (uint32_t j = threadIdx.x; j < ns; j += kTreeletBlock) {
-        int32_t x = (int32_t)A.starts[j];
-        for (uint32_t steps = 0; steps <= cnt; ++steps) {  // a climb passes at most cnt nodes
-            treelet_node(A, x, mine);
-            x = A.parent[x];
-            if (x < 0) break;
-            if (__hip_atomic_fetch_add(A.done + x, 1u, __ATOMIC_ACQ_REL,
-                                       __HIP_MEMORY_SCOPE_WORKGROUP) == 0)
-                break;
-        }
+}
+
+// Pass 2 set-up: the crossing nodes whose children both lie inside windows
+// (finished by pass 1) start climbs; one whose single such child counts as
+// an arrival up front.
+__global__ __launch_bounds__(256) void k_tl_top_prep(RadixArrays A) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= *A.top_count) return;
+    const int32_t x = (int32_t)A.top[i];
+    const int ready =
+        (ext_straddles(A, A.left[x]) ? 0 : 1) + (ext_straddles(A, A.right[x]) ? 0 : 1);
+    if (ready == 1)
+        A.done[x] = 1;
+    else if (ready == 2)
+        A.starts[atomicAdd(A.top_count + 1, 1u)] = (uint32_t)x;
+}
+
+// Pass 2: the crossing nodes, climbed from the starts by every block (agent
+// scope arrivals); the grid walks the start list in block-sized steps.
+__global__ __launch_bounds__(kTreeletBlock) void k_tl_top(RadixArrays A) {
+    __shared__ float C[kTreeletWaves][kTreeletSets];
+    float* mine = C[threadIdx.x >> 6];
+    const uint32_t ns = A.top_count[1];
+    for (uint32_t base = blockIdx.x * kTreeletBlock; base < ns; base += gridDim.x * kTreeletBlock) {
+        const uint32_t j = base + threadIdx.x;
+        tl_climbs<false>(A, j < ns ? (int32_t)A.starts[j] : -1, 0, 0, mine);
     }
 }
 
@@ -2224,7 +2303,8 @@ rt_status build_radix(hipStream_t s, const float* pos, const uint32_t* idx, uint
         hipLaunchKernelGGL(k_tl_zero, dim3(1), dim3(64), 0, s, A);
         hipLaunchKernelGGL(k_tl_prep, dim3(ceil_div(A.m, 256)), dim3(256), 0, s, A);
         hipLaunchKernelGGL(k_tl_window, dim3(ceil_div(n, kWindow)), dim3(kTreeletBlock), 0, s, A);
-        hipLaunchKernelGGL(k_tl_top, dim3(1), dim3(kTreeletBlock), 0, s, A);
+        hipLaunchKernelGGL(k_tl_top_prep, dim3(ceil_div(A.m, 256)), dim3(256), 0, s, A);
+        hipLaunchKernelGGL(k_tl_top, dim3(kTreeletTopBlocks), dim3(kTreeletBlock), 0, s, A);
     }
     if (rounds)
         hipLaunchKernelGGL(k_rt_emit<true>, dim3(ceil_div(A.m, 256)), dim3(256), 0, s, A, nodes,
