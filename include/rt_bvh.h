@@ -202,7 +202,7 @@ rt_status rt_gpu_build(rt_stream stream, const float* d_pos, const uint32_t* d_i
  * that cannot allocate device memory (code under HIP graph capture, tight
  * serving loops, harnesses that forbid hipMalloc).  Same arguments, checks
  * and results as rt_gpu_build, but the scratch comes from a fixed device
- * array in the library (9 MiB, one per GPU, kept until the process ends), so
+ * array in the library (10 MiB, one per GPU, kept until the process ends), so
  * n is at most RT_NOALLOC_BUILD_MAX (RT_ERR_CAPACITY beyond).  Every call on
  * one GPU shares that array: calls that overlap on one GPU (other streams,
  * other host threads) race and must be serialised by the caller.  The first

@@ -1485,8 +1485,13 @@ struct RadixArrays {
     int4* cand;            // m, -1 = empty
     uint32_t* size4;       // m: Box4 nodes in the subtree rooted here
     float* cost;           // m: SAH cost of the subtree (treelet restructuring)
-    uint32_t* emin;        // m: lowest sorted position below the node
-    uint32_t* emax;        // m: highest sorted position below the node
+    uint32_t* emin;        // m: lowest leaf rank below the node (see k_tl_rank_jump)
+    uint32_t* emax;        // m: highest leaf rank below the node
+    // treelet restructuring only (null otherwise)
+    uint32_t* cnt;         // m: triangles below the node
+    uint64_t* jump;        // m: (ancestor, rank offset) pairs of the rank pass
+    uint32_t* rank;        // n: leaf rank of sorted position p (tree order)
+    uint32_t* order;       // n: sorted position of leaf rank r
     uint32_t n, m, maxp;
 };
 
@@ -1580,20 +1585,11 @@ __device__ __forceinline__ float box_sa(const float* b) {
     return 2.0f * (dx * dy + dy * dz + dz * dx);
 }
 
-// Finish internal node x once both subtrees are: box = union of the
-// children, the greedy Box4 candidates and the Box4 subtree size.
-__device__ void finish_node(const RadixArrays& A, int32_t x) {
-    const int32_t L = A.left[x], R = A.right[x];
-    float* bx = A.box + 6 * (size_t)x;
-    const float* bl = A.box + 6 * (size_t)L;
-    const float* br = A.box + 6 * (size_t)R;
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        bx[a] = fminf(bl[a], br[a]);
-        bx[3 + a] = fmaxf(bl[3 + a], br[3 + a]);
-    }
+// The greedy Box4 candidates and the Box4 subtree size of internal node x,
+// once the boxes of its subtree and the sizes below it are final.
+__device__ void finish_box4(const RadixArrays& A, int32_t x) {
     if (A.rl[x] - A.rf[x] + 1 > A.maxp) {
-        int32_t c[4] = {L, R, -1, -1};
+        int32_t c[4] = {A.left[x], A.right[x], -1, -1};
         int nc = 2;
         while (nc < 4) {  // software_bvh.hpp:225-247
             int best = -1;
@@ -1617,6 +1613,21 @@ __device__ void finish_node(const RadixArrays& A, int32_t x) {
         A.cand[x] = make_int4(c[0], c[1], c[2], c[3]);
         A.size4[x] = sz;
     }
+}
+
+// Finish internal node x once both subtrees are: box = union of the
+// children, then the Box4 candidates and size.
+__device__ void finish_node(const RadixArrays& A, int32_t x) {
+    const int32_t L = A.left[x], R = A.right[x];
+    float* bx = A.box + 6 * (size_t)x;
+    const float* bl = A.box + 6 * (size_t)L;
+    const float* br = A.box + 6 * (size_t)R;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        bx[a] = fminf(bl[a], br[a]);
+        bx[3 + a] = fmaxf(bl[3 + a], br[3 + a]);
+    }
+    finish_box4(A, x);
 }
 
 __device__ __forceinline__ bool straddles(const RadixArrays& A, int32_t x) {
@@ -1707,15 +1718,19 @@ __global__ __launch_bounds__(1024) void k_rt_top(RadixArrays A) {
 constexpr int kTreeletLeaves = 7;
 constexpr uint32_t kMaxTreeletRounds = 16;
 constexpr uint32_t kTreeletSets = 1u << kTreeletLeaves;
-constexpr uint32_t kTreeletBlock = 256;  // LDS: kTreeletSets floats per wave
+constexpr uint32_t kTreeletBlock = 256;  // 4 waves; 3 blocks fit a CU (LDS, registers)
 constexpr uint32_t kTreeletTopBlocks = 1024;  // pass 2 grid (walks the start list)
 constexpr float kCostInner = 1.2f;
 
 __device__ __forceinline__ uint32_t ext_lo(const RadixArrays& A, int32_t u) {
-    return (uint32_t)u < A.m ? A.emin[u] : (uint32_t)u - A.m;
+    return (uint32_t)u < A.m ? A.emin[u] : A.rank[(uint32_t)u - A.m];
 }
 __device__ __forceinline__ uint32_t ext_hi(const RadixArrays& A, int32_t u) {
-    return (uint32_t)u < A.m ? A.emax[u] : (uint32_t)u - A.m;
+    return (uint32_t)u < A.m ? A.emax[u] : A.rank[(uint32_t)u - A.m];
+}
+// Triangles below u.
+__device__ __forceinline__ uint32_t ncount(const RadixArrays& A, int32_t u) {
+    return (uint32_t)u < A.m ? A.cnt[u] : 1u;
 }
 __device__ __forceinline__ bool ext_straddles(const RadixArrays& A, int32_t x) {
     return (uint32_t)x < A.m && A.emin[x] / kWindow != A.emax[x] / kWindow;
@@ -1781,53 +1796,128 @@ __device__ __forceinline__ void wave_lds_step() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// The treelet rebuild at inner node x (wave-uniform), by the whole wave
-// (Karras & Aila 2013 process one treelet per warp): the treelet's leaves
-// and boxes are formed by every lane alike; subset costs go to the wave's
-// 128-float LDS table C, the subsets of one size at a time, lane l holding
-// subsets l + 1 and l + 65 (its own scan of their splits, as best_split);
-// the full set's splits are spread over the lanes and reduced to their
-// least cost; lane 0 rebuilds the treelet when that is strictly cheaper and
-// finishes x.  The costs, the splits chosen and so the tree are those of
-// the sequential program (each subset's cost is a least value, which does
-// not depend on the order its candidates are compared in; the rebuild
-// re-derives every split with best_split's order).
-__device__ void treelet_node_wave(const RadixArrays& A, int32_t x, float* C) {
-    if (!ninner(A, x)) return;  // a leaf range: its cost is computed where used
-    const uint32_t lane = threadIdx.x & 63u;
-    int32_t lf[kTreeletLeaves], in[kTreeletLeaves - 1];
-    lf[0] = A.left[x];
-    lf[1] = A.right[x];
-    in[0] = x;
-    int nl = 2, ni = 1;
-    float cx = kCostInner * box_sa(A.box + 6 * (size_t)x) +
-               (sah_cost(A, lf[0]) + sah_cost(A, lf[1]));
-    while (nl < kTreeletLeaves) {
+// One wave's LDS: the subset cost and split tables of the treelet in
+// progress, and a row per lane with the leaf boxes (7 x 6 floats) and leaf costs (7) of the
+// treelet that lane formed.
+constexpr uint32_t kRowFloats = 50;
+struct TlShared {
+    float C[kTreeletSets];
+    uint8_t P[kTreeletSets];  // the split best_split chooses for each subset (0: none)
+    float row[64 * kRowFloats];
+};
+
+// A treelet as formed by one lane: leaves lf[0..nl), inner nodes
+// in[0..nl-1) (in[0] = the root x, then in the order they were expanded),
+// the root's current cost.  nl = 0: the root is a leaf range (no step);
+// nl = 2: neither child is inner (nothing to restructure).
+struct TlForm {
+    int32_t lf[kTreeletLeaves];
+    int32_t in[kTreeletLeaves - 1];
+    int32_t nl;
+    float cx;
+};
+
+// Surface area of u when it is an inner node (a candidate to expand), else -1.
+__device__ __forceinline__ float expand_sa(const RadixArrays& A, int32_t u) {
+    return ninner(A, u) ? box_sa(A.box + 6 * (size_t)u) : -1.0f;
+}
+
+// Treelet formation at x by one lane (Karras & Aila 2013 grow the treelet
+// from x's children, expanding the inner leaf of largest surface area, the
+// first in slot order on ties), with the leaves' boxes and costs written to
+// the lane's LDS row.  Lanes form their own nodes' treelets side by side, so
+// the dependent loads of one formation overlap those of the others.
+__device__ void treelet_form(const RadixArrays& A, int32_t x, TlForm& f, float* row) {
+    f.nl = 0;
+    if (!ninner(A, x)) return;
+    f.lf[0] = A.left[x];
+    f.lf[1] = A.right[x];
+    f.in[0] = x;
+    float sa[kTreeletLeaves];
+    sa[0] = expand_sa(A, f.lf[0]);
+    sa[1] = expand_sa(A, f.lf[1]);
+    f.cx = kCostInner * box_sa(A.box + 6 * (size_t)x) +
+           (sah_cost(A, f.lf[0]) + sah_cost(A, f.lf[1]));
+    int nl = 2;
+    // unrolled: expansion `it` has it + 2 leaves (static slots, so the
+    // arrays stay in registers)
+#pragma unroll
+    for (int it = 0; it < kTreeletLeaves - 2; ++it) {
+        const int have = it + 2;
         int best = -1;
         float bs = -1.0f;
-        for (int q = 0; q < nl; ++q) {
-            if (!ninner(A, lf[q])) continue;
-            const float a = box_sa(A.box + 6 * (size_t)lf[q]);
-            if (a > bs) {
-                bs = a;
+#pragma unroll
+        for (int q = 0; q < have; ++q)
+            if (sa[q] > bs) {  // -1 (a leaf range) never is
+                bs = sa[q];
                 best = q;
             }
-        }
         if (best < 0) break;
-        const int32_t e = lf[best];
-        in[ni++] = e;
-        lf[best] = A.left[e];
-        lf[nl++] = A.right[e];
+        int32_t e = 0;
+#pragma unroll
+        for (int q = 0; q < have; ++q)
+            if (q == best) e = f.lf[q];
+        f.in[it + 1] = e;
+        const int32_t l = A.left[e], r = A.right[e];
+        const float sl = expand_sa(A, l), sr = expand_sa(A, r);
+#pragma unroll
+        for (int q = 0; q < have; ++q)
+            if (q == best) {
+                f.lf[q] = l;
+                sa[q] = sl;
+            }
+        f.lf[have] = r;
+        sa[have] = sr;
+        nl = have + 1;
     }
+    f.nl = nl;
+    if (nl < 3) return;
+#pragma unroll
+    for (int q = 0; q < kTreeletLeaves; ++q) {
+        if (q >= nl) continue;
+        const float* b = A.box + 6 * (size_t)f.lf[q];
+#pragma unroll
+        for (int a = 0; a < 6; ++a) row[6 * q + a] = b[a];
+        row[6 * kTreeletLeaves + q] = sah_cost(A, f.lf[q]);
+    }
+}
+
+// The dynamic program and rebuild of one formed treelet (root x, wave-
+// uniform arguments: lf, in, nl, cx read from the forming lane; row = that
+// lane's LDS row), by the whole wave (Karras & Aila 2013 process one treelet
+// per warp): subset costs go to the wave's table C, the subsets of one size
+// at a time, lane l holding subsets l + 1 and l + 65 (its own scan of their
+// splits, as best_split, its choice kept in P); the full set's splits are
+// spread over the lanes, in best_split's order, and reduced to their least
+// cost and the first lane reaching it; lane 0 rebuilds the treelet from P
+// when that is strictly cheaper and stores x's cost.  The costs, the splits
+// chosen and so the tree are those of the sequential program (each subset's
+// cost is a least value, which does not depend on the order its candidates
+// are compared in; each choice is the first strict minimum in best_split's
+// order).
+// A rebuilt inner node's box is the union of its subset's leaf boxes (exact:
+// min and max do not round), its rank range and count those of the leaves;
+// the root's box, range and count do not change (same leaves).
+__device__ void treelet_step(const RadixArrays& A, int32_t x, int nl, float cx,
+                             const int32_t* lf, const int32_t* in, const float* row,
+                             TlShared* sh) {
+    const uint32_t lane = threadIdx.x & 63u;
+    float* C = sh->C;
+    uint8_t* P = sh->P;
     if (nl >= 3) {
+        wave_lds_step();  // the previous treelet's reads of C are done
         float lb[kTreeletLeaves][6];
 #pragma unroll
         for (int q = 0; q < kTreeletLeaves; ++q) {
             if (q >= nl) continue;
 #pragma unroll
-            for (int a = 0; a < 6; ++a) lb[q][a] = A.box[6 * (size_t)lf[q] + a];
-            if (lane == (uint32_t)q) C[1u << q] = sah_cost(A, lf[q]);
+            for (int a = 0; a < 6; a += 2) {
+                const float2 v = *reinterpret_cast<const float2*>(row + 6 * q + a);
+                lb[q][a] = v.x;
+                lb[q][a + 1] = v.y;
+            }
         }
+        if (lane < (uint32_t)nl) C[1u << lane] = row[6 * kTreeletLeaves + lane];
         const uint32_t full = (1u << nl) - 1;
         // surface areas of this lane's two subsets (as the sequential union)
         const uint32_t sl[2] = {lane + 1u, lane + 65u};
@@ -1854,6 +1944,7 @@ __device__ void treelet_node_wave(const RadixArrays& A, int32_t x, float* C) {
                 uint32_t bp;
                 const float split = best_split<1>(C, sl[h], bp);
                 C[sl[h]] = kCostInner * sa[h] + split;
+                P[sl[h]] = (uint8_t)bp;
             }
             wave_lds_step();
         }
@@ -1864,10 +1955,17 @@ __device__ void treelet_node_wave(const RadixArrays& A, int32_t x, float* C) {
             const uint32_t sub = (nsplit - 1u - lane) << 1;
             c = C[1u | sub] + C[rest ^ sub];
         }
-        c = wave_min(c);
+        const float cmin = wave_min(c);
+        // best_split's choice: its order is the lanes' order, so the first
+        // lane at the least cost, none when no cost is below infinity
+        const uint64_t at = __ballot(lane < nsplit && c == cmin && cmin < INFINITY);
+        const uint32_t pfull = at ? 1u | ((nsplit - 1u - (uint32_t)__builtin_ctzll(at)) << 1) : 0u;
         const float sa_full = __shfl(sa[(full - 1u) >> 6], (int)((full - 1u) & 63u));
-        const float cf = kCostInner * sa_full + c;
-        if (lane == 0) C[full] = cf;
+        const float cf = kCostInner * sa_full + cmin;
+        if (lane == 0) {
+            C[full] = cf;
+            P[full] = (uint8_t)pfull;
+        }
         wave_lds_step();
         if (lane == 0 && cf < cx) {
             // plan the rebuild first (no stores): breadth-first over the subsets
@@ -1878,8 +1976,7 @@ __device__ void treelet_node_wave(const RadixArrays& A, int32_t x, float* C) {
             int nq = 1, next = 1;
             bool ok = true;
             for (int i = 0; i < nq && ok; ++i) {
-                uint32_t p;
-                best_split<1>(C, qs[i], p);
+                const uint32_t p = P[qs[i]];
                 if (p == 0) {
                     ok = false;
                     break;
@@ -1900,52 +1997,127 @@ __device__ void treelet_node_wave(const RadixArrays& A, int32_t x, float* C) {
                 qr[i] = ch[1];
             }
             if (ok) {
+                uint32_t elo[kTreeletLeaves], ehi[kTreeletLeaves], ec[kTreeletLeaves];
+#pragma unroll
+                for (int q = 0; q < kTreeletLeaves; ++q) {
+                    if (q >= nl) continue;
+                    elo[q] = ext_lo(A, lf[q]);
+                    ehi[q] = ext_hi(A, lf[q]);
+                    ec[q] = ncount(A, lf[q]);
+                }
                 for (int i = 0; i < nq; ++i) {
                     A.left[qn[i]] = ql[i];
                     A.right[qn[i]] = qr[i];
                     A.parent[ql[i]] = qn[i];
                     A.parent[qr[i]] = qn[i];
                 }
-                for (int i = nq - 1; i >= 1; --i) {  // children before parents
+                for (int i = 1; i < nq; ++i) {
                     const int32_t y = qn[i];
-                    A.emin[y] = min(ext_lo(A, ql[i]), ext_lo(A, qr[i]));
-                    A.emax[y] = max(ext_hi(A, ql[i]), ext_hi(A, qr[i]));
+                    float b[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+                    uint32_t lo = 0xffffffffu, hi = 0, c = 0;
+#pragma unroll
+                    for (int q = 0; q < kTreeletLeaves; ++q) {
+                        if (!((qs[i] >> q) & 1u)) continue;
+#pragma unroll
+                        for (int a = 0; a < 3; ++a) {
+                            b[a] = fminf(b[a], lb[q][a]);
+                            b[3 + a] = fmaxf(b[3 + a], lb[q][3 + a]);
+                        }
+                        lo = min(lo, elo[q]);
+                        hi = max(hi, ehi[q]);
+                        c += ec[q];
+                    }
+                    float* by = A.box + 6 * (size_t)y;
+#pragma unroll
+                    for (int a = 0; a < 6; ++a) by[a] = b[a];
+                    A.emin[y] = lo;
+                    A.emax[y] = hi;
+                    A.cnt[y] = c;
                     A.cost[y] = C[qs[i]];
-                    finish_node(A, y);
                 }
                 cx = cf;
             }
         }
-        cx = __shfl(cx, 0);
     }
-    if (lane == 0) {
-        A.cost[x] = cx;
-        finish_node(A, x);
-    }
+    if (lane == 0) A.cost[x] = cx;
 }
 
-// Round set-up: arrival counters cleared, the nodes whose positions cross a
-// window boundary listed (top_count[0], zeroed with the start count
-// top_count[1] by k_tl_zero before).
+// Windows of a restructured tree.  Restructuring regroups subtrees, so a
+// node's triangles stop being a range of sorted positions, and with windows
+// over positions most nodes of a terrain came to straddle window boundaries
+// after a few rounds (most of a round's nodes then went to the agent-scope
+// pass).  Each pass therefore first ranks the leaves in the current tree's
+// order (left subtree before right): every subtree is a range of ranks
+// [emin, emax], and the windows are over ranks.  rank_start(x) = the sum,
+// over the ancestors a of x reached from a's right child, of the triangles
+// in a's left child: pointer jumping over (ancestor, partial sum) pairs,
+// each a single 64-bit word, so any pair read is a consistent one and every
+// thread's chain of reads ends at the root whatever it sees of the others'
+// progress.
+
+// The triangle counts of the radix tree (before the first round).
+__global__ __launch_bounds__(256) void k_tl_count(RadixArrays A) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i < A.m) A.cnt[i] = A.rl[i] - A.rf[i] + 1;
+}
 __global__ void k_tl_zero(RadixArrays A) {
     if (threadIdx.x < 2) A.top_count[threadIdx.x] = 0;
 }
-__global__ __launch_bounds__(256) void k_tl_prep(RadixArrays A) {
-    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= A.m) return;
-    A.done[i] = 0;
-    if (A.emin[i] / kWindow != A.emax[i] / kWindow) A.top[atomicAdd(A.top_count, 1u)] = i;
+__device__ __forceinline__ uint64_t rank_pair(int32_t anc, uint32_t sum) {
+    return ((uint64_t)(uint32_t)anc << 32) | sum;
+}
+__global__ __launch_bounds__(256) void k_tl_rank_init(RadixArrays A) {
+    const uint32_t x = blockIdx.x * 256 + threadIdx.x;
+    if (x >= A.m) return;
+    const int32_t p = A.parent[x];
+    const uint32_t off = (p >= 0 && A.right[p] == (int32_t)x) ? ncount(A, A.left[p]) : 0u;
+    A.jump[x] = rank_pair(p, off);
+}
+// Ranks of the internal nodes, then arrival counters cleared and the nodes
+// whose rank ranges cross a window boundary listed (top_count[0]; zeroed
+// with the start count top_count[1] by k_tl_zero before).
+__global__ __launch_bounds__(256) void k_tl_rank_jump(RadixArrays A) {
+    const uint32_t x = blockIdx.x * 256 + threadIdx.x;
+    if (x >= A.m) return;
+    uint64_t v = A.jump[x];
+    for (uint32_t steps = 0; (int32_t)(v >> 32) >= 0 && steps <= A.m; ++steps) {
+        const uint64_t w = __hip_atomic_load(A.jump + (uint32_t)(v >> 32), __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_AGENT);
+        v = (w & 0xffffffff00000000ull) | (uint32_t)((uint32_t)v + (uint32_t)w);
+        __hip_atomic_store(A.jump + x, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    const uint32_t r = (uint32_t)v, last = r + A.cnt[x] - 1;
+    A.emin[x] = r;
+    A.emax[x] = last;
+    A.done[x] = 0;
+    if (r / kWindow != last / kWindow) A.top[atomicAdd(A.top_count, 1u)] = x;
+}
+// Ranks of the sorted positions, and the position of each rank.
+__global__ __launch_bounds__(256) void k_tl_rank_leaves(RadixArrays A) {
+    const uint32_t p = blockIdx.x * 256 + threadIdx.x;
+    if (p >= A.n) return;
+    const int32_t u = (int32_t)(A.m + p), x = A.parent[u];
+    const uint32_t r = A.emin[x] + (A.right[x] == u ? ncount(A, A.left[x]) : 0u);
+    A.rank[p] = r;
+    if (r < A.n) A.order[r] = p;  // always: the ranks are a permutation of [0, n)
 }
 
-// Bottom-up climbs of one block, the treelet rebuilds done by whole waves:
-// lane climbs start at `start` (or none: -1); at every step each lane
-// checks its node (window test for pass 1) and counts its arrival; the
-// second arrival at a node owns it, and the wave rebuilds its lanes' owned
-// nodes one after another.  SCOPE: workgroup for the windows (every node
-// inside a window is reached only from its block), agent for the nodes
-// crossing windows (reached from any block).
-template <bool WINDOW>
-__device__ void tl_climbs(const RadixArrays& A, int32_t x, uint32_t lo, uint32_t hi, float* C) {
+// What a bottom-up pass over the restructured tree does at a node: a
+// treelet step (the rounds), or the Box4 candidates and sizes (once, after
+// the last round: treelet steps change no box they do not write, so the
+// final boxes are there).
+enum TlMode { kTlTreelet, kTlBox4 };
+
+// Bottom-up climbs of one block: lane climbs start at `start` (or none: -1);
+// at every step each lane checks its node (window test for pass 1) and counts
+// its arrival; the second arrival at a node owns it.  kTlTreelet: the lanes
+// form their owned nodes' treelets, then the wave runs the treelet steps one
+// after another; kTlBox4: each lane finishes its own node.  Arrival scope:
+// workgroup for the windows (every node inside a window is reached only from
+// its block), agent for the nodes crossing windows (reached from any block).
+template <bool WINDOW, int MODE>
+__device__ void tl_climbs(const RadixArrays& A, int32_t x, uint32_t lo, uint32_t hi,
+                          TlShared* sh) {
     const uint32_t lane = threadIdx.x & 63u;
     for (uint32_t steps = 0;; ++steps) {
         bool own = false;
@@ -1965,13 +2137,33 @@ __device__ void tl_climbs(const RadixArrays& A, int32_t x, uint32_t lo, uint32_t
         uint64_t m = __ballot(own);
         if (m == 0) break;
         // the wave reads what other lanes' (and, for agent scope, other
-        // blocks') rebuilds wrote before their arrivals
+        // blocks') steps wrote before their arrivals
         if constexpr (WINDOW) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
         else __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-        while (m) {
-            const int l = __builtin_ctzll(m);
-            m &= m - 1;
-            treelet_node_wave(A, __shfl(x, l), C);
+        if constexpr (MODE == kTlBox4) {
+            if (own) finish_box4(A, x);
+        } else {
+            TlForm f;
+            f.nl = 0;
+            float* rows = sh->row;
+            if (own) treelet_form(A, x, f, rows + lane * kRowFloats);
+            while (m) {
+                const int l = __builtin_ctzll(m);
+                m &= m - 1;
+                const int nl = __builtin_amdgcn_readlane(f.nl, l);
+                if (nl == 0) continue;
+                int32_t lf[kTreeletLeaves], in[kTreeletLeaves - 1];
+#pragma unroll
+                for (int q = 0; q < kTreeletLeaves; ++q)
+                    lf[q] = __builtin_amdgcn_readlane(f.lf[q], l);
+#pragma unroll
+                for (int q = 0; q < kTreeletLeaves - 1; ++q)
+                    in[q] = __builtin_amdgcn_readlane(f.in[q], l);
+                const float cx = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(f.cx), l));
+                wave_lds_step();  // the rows written above
+                treelet_step(A, __builtin_amdgcn_readlane(x, l), nl, cx, lf, in,
+                             rows + l * kRowFloats, sh);
+            }
         }
         if constexpr (WINDOW) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
         else __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
@@ -1982,16 +2174,18 @@ __device__ void tl_climbs(const RadixArrays& A, int32_t x, uint32_t lo, uint32_t
 
 constexpr uint32_t kTreeletWaves = kTreeletBlock / 64;
 
-// One restructuring round, pass 1: as k_rt_bottom_up over kWindow sorted
-// positions, with a node inside the window when all positions below it are.
-__global__ __launch_bounds__(kTreeletBlock) void k_tl_window(RadixArrays A) {
-    __shared__ float C[kTreeletWaves][kTreeletSets];
-    float* mine = C[threadIdx.x >> 6];
+// One restructuring round (or the Box4 pass), pass 1: as k_rt_bottom_up over
+// kWindow leaf ranks, with a node inside the window when all ranks below it
+// are.
+template <int MODE>
+__global__ __launch_bounds__(kTreeletBlock, 3) void k_tl_window(RadixArrays A) {
+    __shared__ TlShared sh[MODE == kTlTreelet ? kTreeletWaves : 1];
+    TlShared* mine = sh + (MODE == kTlTreelet ? threadIdx.x >> 6 : 0);
     const uint32_t lo = blockIdx.x * kWindow;
     const uint32_t hi = min(lo + kWindow, A.n) - 1;
     for (uint32_t base = lo; base <= hi; base += kTreeletBlock) {
-        const uint32_t p = base + threadIdx.x;
-        tl_climbs<true>(A, p <= hi ? A.parent[A.m + p] : -1, lo, hi, mine);
+        const uint32_t r = base + threadIdx.x;
+        tl_climbs<true, MODE>(A, r <= hi ? A.parent[A.m + A.order[r]] : -1, lo, hi, mine);
     }
 }
 
@@ -2010,15 +2204,24 @@ __global__ __launch_bounds__(256) void k_tl_top_prep(RadixArrays A) {
         A.starts[atomicAdd(A.top_count + 1, 1u)] = (uint32_t)x;
 }
 
-// Pass 2: the crossing nodes, climbed from the starts by every block (agent
-// scope arrivals); the grid walks the start list in block-sized steps.
-__global__ __launch_bounds__(kTreeletBlock) void k_tl_top(RadixArrays A) {
-    __shared__ float C[kTreeletWaves][kTreeletSets];
-    float* mine = C[threadIdx.x >> 6];
+// Pass 2: the crossing nodes, climbed from the starts (agent scope
+// arrivals).  kTlTreelet: one climb per wave (its steps are the wave's
+// work, and a wave's climbs run one after another); kTlBox4: one per lane.
+template <int MODE>
+__global__ __launch_bounds__(kTreeletBlock, 3) void k_tl_top(RadixArrays A) {
+    __shared__ TlShared sh[MODE == kTlTreelet ? kTreeletWaves : 1];
     const uint32_t ns = A.top_count[1];
-    for (uint32_t base = blockIdx.x * kTreeletBlock; base < ns; base += gridDim.x * kTreeletBlock) {
-        const uint32_t j = base + threadIdx.x;
-        tl_climbs<false>(A, j < ns ? (int32_t)A.starts[j] : -1, 0, 0, mine);
+    if constexpr (MODE == kTlTreelet) {
+        const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+        for (uint32_t j = blockIdx.x * kTreeletWaves + wave; j < ns;
+             j += gridDim.x * kTreeletWaves)
+            tl_climbs<false, MODE>(A, lane == 0 ? (int32_t)A.starts[j] : -1, 0, 0, sh + wave);
+    } else {
+        for (uint32_t base = blockIdx.x * kTreeletBlock; base < ns;
+             base += gridDim.x * kTreeletBlock) {
+            const uint32_t j = base + threadIdx.x;
+            tl_climbs<false, MODE>(A, j < ns ? (int32_t)A.starts[j] : -1, 0, 0, sh);
+        }
     }
 }
 
@@ -2141,13 +2344,13 @@ struct ScratchPlan {
     size_t bounds = 0, keys0 = 0, vals0 = 0, keys1 = 0, vals1 = 0, hist = 0, totals = 0,
            boxes = 0, rbox = 0, rf = 0, rl = 0, left = 0, right = 0, parent = 0, done = 0,
            top = 0, top_count = 0, starts = 0, cand = 0, size4 = 0, cost = 0, emin = 0,
-           emax = 0, total = 0;
+           emax = 0, cnt = 0, jump = 0, rank = 0, order = 0, total = 0;
     uint32_t tiles = 0;
 };
 
 // Scratch of one build: sort buffers, then the implicit level boxes or the
 // radix-tree arrays.
-ScratchPlan plan_scratch(uint32_t n, uint32_t algo) {
+ScratchPlan plan_scratch(uint32_t n, uint32_t algo, uint32_t rounds) {
     ScratchPlan p;
     auto al = [](size_t x) { return (x + 255) & ~size_t(255); };
     size_t off = 0;
@@ -2180,6 +2383,12 @@ ScratchPlan plan_scratch(uint32_t n, uint32_t algo) {
         p.cost = off; off += al(m * 4);
         p.emin = off; off += al(m * 4);
         p.emax = off; off += al(m * 4);
+        if (rounds) {
+            p.cnt = off; off += al(m * 4);
+            p.jump = off; off += al(m * 8);
+            p.rank = off; off += al((size_t)n * 4);
+            p.order = off; off += al((size_t)n * 4);
+        }
     }
     p.total = off;
     return p;
@@ -2287,6 +2496,10 @@ rt_status build_radix(hipStream_t s, const float* pos, const uint32_t* idx, uint
     A.cost = reinterpret_cast<float*>(base + p.cost);
     A.emin = reinterpret_cast<uint32_t*>(base + p.emin);
     A.emax = reinterpret_cast<uint32_t*>(base + p.emax);
+    A.cnt = rounds ? reinterpret_cast<uint32_t*>(base + p.cnt) : nullptr;
+    A.jump = rounds ? reinterpret_cast<uint64_t*>(base + p.jump) : nullptr;
+    A.rank = rounds ? reinterpret_cast<uint32_t*>(base + p.rank) : nullptr;
+    A.order = rounds ? reinterpret_cast<uint32_t*>(base + p.order) : nullptr;
     A.n = n;
     A.m = n - 1;
     A.maxp = maxp;
@@ -2299,12 +2512,27 @@ rt_status build_radix(hipStream_t s, const float* pos, const uint32_t* idx, uint
     hipLaunchKernelGGL(k_rt_radix, dim3(ceil_div(A.m, 256)), dim3(256), 0, s, A, info);
     hipLaunchKernelGGL(k_rt_bottom_up, dim3(ceil_div(n, kWindow)), dim3(1024), 0, s, A);
     hipLaunchKernelGGL(k_rt_top, dim3(1), dim3(1024), 0, s, A);
-    for (uint32_t r = 0; r < rounds; ++r) {
+    // restructuring rounds, then the Box4 candidates of the final tree; each
+    // pass ranks the leaves of the current tree first
+    if (rounds) hipLaunchKernelGGL(k_tl_count, dim3(ceil_div(A.m, 256)), dim3(256), 0, s, A);
+    for (uint32_t r = 0; r <= rounds && rounds; ++r) {
         hipLaunchKernelGGL(k_tl_zero, dim3(1), dim3(64), 0, s, A);
-        hipLaunchKernelGGL(k_tl_prep, dim3(ceil_div(A.m, 256)), dim3(256), 0, s, A);
-        hipLaunchKernelGGL(k_tl_window, dim3(ceil_div(n, kWindow)), dim3(kTreeletBlock), 0, s, A);
+        hipLaunchKernelGGL(k_tl_rank_init, dim3(ceil_div(A.m, 256)), dim3(256), 0, s, A);
+        hipLaunchKernelGGL(k_tl_rank_jump, dim3(ceil_div(A.m, 256)), dim3(256), 0, s, A);
+        hipLaunchKernelGGL(k_tl_rank_leaves, dim3(ceil_div(n, 256)), dim3(256), 0, s, A);
+        if (r < rounds)
+            hipLaunchKernelGGL(k_tl_window<kTlTreelet>, dim3(ceil_div(n, kWindow)),
+                               dim3(kTreeletBlock), 0, s, A);
+        else
+            hipLaunchKernelGGL(k_tl_window<kTlBox4>, dim3(ceil_div(n, kWindow)),
+                               dim3(kTreeletBlock), 0, s, A);
         hipLaunchKernelGGL(k_tl_top_prep, dim3(ceil_div(A.m, 256)), dim3(256), 0, s, A);
-        hipLaunchKernelGGL(k_tl_top, dim3(kTreeletTopBlocks), dim3(kTreeletBlock), 0, s, A);
+        if (r < rounds)
+            hipLaunchKernelGGL(k_tl_top<kTlTreelet>, dim3(kTreeletTopBlocks), dim3(kTreeletBlock),
+                               0, s, A);
+        else
+            hipLaunchKernelGGL(k_tl_top<kTlBox4>, dim3(kTreeletTopBlocks), dim3(kTreeletBlock), 0,
+                               s, A);
     }
     if (rounds)
         hipLaunchKernelGGL(k_rt_emit<true>, dim3(ceil_div(A.m, 256)), dim3(256), 0, s, A, nodes,
@@ -2371,7 +2599,7 @@ extern "C" rt_status rt_build_requirements(uint32_t n, const rt_build_cfg* cfg,
     if (node_capacity) *node_capacity = max_nodes(n, algo);
     if (scratch_bytes) {
         const bool fused = algo == RT_BUILD_IMPLICIT && n <= kSmallMax;
-        *scratch_bytes = fused ? 0 : plan_scratch(n, algo).total;
+        *scratch_bytes = fused ? 0 : plan_scratch(n, algo, cfg ? cfg->treelet_rounds : 0u).total;
     }
     return RT_OK;
 }
@@ -2405,7 +2633,7 @@ extern "C" rt_status rt_gpu_build(rt_stream stream, const float* d_pos, const ui
                            small_tree(t), d_info);
         return launch_status("k_build_small");
     }
-    const ScratchPlan p = plan_scratch(n, algo);
+    const ScratchPlan p = plan_scratch(n, algo, rounds);
     if (!d_scratch || scratch_bytes < p.total) return RT_ERR_CAPACITY;
     char* base = static_cast<char*>(d_scratch);
     uint32_t *keys = nullptr, *vals = nullptr;
@@ -2438,7 +2666,7 @@ extern "C" rt_status rt_gpu_refit(rt_stream stream, const float* d_pos, const ui
                            d_pos, d_idx, n, d_nodes, d_tris, small_tree(t), nullptr);
         return launch_status("k_refit_small");
     }
-    const ScratchPlan p = plan_scratch(n, RT_BUILD_IMPLICIT);
+    const ScratchPlan p = plan_scratch(n, RT_BUILD_IMPLICIT, 0);
     if (!d_scratch || scratch_bytes < p.total) return RT_ERR_CAPACITY;
     float* gboxes = reinterpret_cast<float*>(static_cast<char*>(d_scratch) + p.boxes);
     return emit_levels(s, d_pos, d_idx, n, nullptr, d_nodes, d_tris, gboxes, t, true);
@@ -2486,7 +2714,7 @@ extern "C" rt_status rt_gpu_build_batch_plan(const rt_build_item* items, uint32_
             e.tree = small_tree(t);
         } else {
             // the multi-launch implicit build's scratch layout
-            const ScratchPlan p = plan_scratch(it.n, RT_BUILD_IMPLICIT);
+            const ScratchPlan p = plan_scratch(it.n, RT_BUILD_IMPLICIT, 0);
             if (!it.scratch || it.scratch_bytes < p.total) return RT_ERR_CAPACITY;
             MedTree& m = e.med;
             m.levels = t.levels;

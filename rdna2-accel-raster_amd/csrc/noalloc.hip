@@ -14,10 +14,10 @@
 namespace {
 
 // Room for the largest build this entry point accepts: a RT_BUILD_RADIX tree
-// of RT_NOALLOC_BUILD_MAX triangles needs 8,720,896 bytes (the implicit
-// layout 1,642,752); the size is checked against the build's own plan on
-// every call.
-constexpr size_t kScratchBytes = 9u << 20;
+// of RT_NOALLOC_BUILD_MAX triangles needs 8,720,896 bytes, 10,031,616 with
+// treelet restructuring (the implicit layout 1,642,752); the size is checked
+// against the build's own plan on every call.
+constexpr size_t kScratchBytes = 10u << 20;
 constexpr int kMaxDevices = 64;
 
 static __device__ __align__(256) unsigned char scratch[kScratchBytes];
