@@ -1718,7 +1718,9 @@ __global__ __launch_bounds__(1024) void k_rt_top(RadixArrays A) {
 constexpr int kTreeletLeaves = 7;
 constexpr uint32_t kMaxTreeletRounds = 16;
 constexpr uint32_t kTreeletSets = 1u << kTreeletLeaves;
-constexpr uint32_t kTreeletBlock = 256;  // 4 waves; 3 blocks fit a CU (LDS, registers)
+// 4 waves; 3 blocks fit a CU (registers, and LDS: 52,864 bytes a block; at
+// 53,888 only 2 fit, which made a round over 2.5 M triangles 1.6x slower)
+constexpr uint32_t kTreeletBlock = 256;
 constexpr uint32_t kTreeletTopBlocks = 1024;  // pass 2 grid (walks the start list)
 constexpr float kCostInner = 1.2f;
 
@@ -1799,7 +1801,7 @@ __device__ __forceinline__ void wave_lds_step() {
 // One wave's LDS: the subset cost and split tables of the treelet in
 // progress, and a row per lane with the leaf boxes (7 x 6 floats) and leaf costs (7) of the
 // treelet that lane formed.
-constexpr uint32_t kRowFloats = 50;
+constexpr uint32_t kRowFloats = 6 * kTreeletLeaves + kTreeletLeaves;  // 49, unpadded
 struct TlShared {
     float C[kTreeletSets];
     uint8_t P[kTreeletSets];  // the split best_split chooses for each subset (0: none)
@@ -1816,6 +1818,33 @@ struct TlForm {
     int32_t nl;
     float cx;
 };
+
+// The subsets of 7 leaves by size, ascending within a size (so the size-k
+// subsets of nl < 7 leaves are the first C(nl, k) of size k), where each
+// size starts, and the binomial coefficients C(nl, k).  The treelet kernels
+// copy the subset list to LDS (lanes index it by their group); the other two
+// are read with wave-uniform indices (scalar loads: 32-bit entries).
+__constant__ uint8_t kSubsetsBySize[kTreeletSets] = {
+    0,  1,  2,  4,  8,  16, 32, 64, 3,  5,  6,  9,  10, 12, 17, 18, 20, 24, 33, 34, 36, 40,
+    48, 65, 66, 68, 72, 80, 96, 7,  11, 13, 14, 19, 21, 22, 25, 26, 28, 35, 37, 38, 41, 42,
+    44, 49, 50, 52, 56, 67, 69, 70, 73, 74, 76, 81, 82, 84, 88, 97, 98, 100, 104, 112, 15, 23,
+    27, 29, 30, 39, 43, 45, 46, 51, 53, 54, 57, 58, 60, 71, 75, 77, 78, 83, 85, 86, 89, 90,
+    92, 99, 101, 102, 105, 106, 108, 113, 114, 116, 120, 31, 47, 55, 59, 61, 62, 79, 87, 91, 93,
+    94, 103, 107, 109, 110, 115, 117, 118, 121, 122, 124, 63, 95, 111, 119, 123, 125, 126, 127};
+__constant__ uint32_t kSubsetStart[kTreeletLeaves + 1] = {0, 1, 8, 29, 64, 99, 120, 127};
+__constant__ uint32_t kBinom[kTreeletLeaves + 1][kTreeletLeaves + 1] = {
+    {1, 0, 0, 0, 0, 0, 0, 0},     {1, 1, 0, 0, 0, 0, 0, 0},      {1, 2, 1, 0, 0, 0, 0, 0},
+    {1, 3, 3, 1, 0, 0, 0, 0},     {1, 4, 6, 4, 1, 0, 0, 0},      {1, 5, 10, 10, 5, 1, 0, 0},
+    {1, 6, 15, 20, 15, 6, 1, 0},  {1, 7, 21, 35, 35, 21, 7, 1}};
+
+// The bits of v deposited, lowest first, at the set bits of mask: the
+// proper submasks of mask in descending order are v = 2^popc(mask) - 2 ... 0.
+__device__ __forceinline__ uint32_t deposit_bits(uint32_t v, uint32_t mask) {
+    uint32_t r = 0;
+    for (uint32_t b = mask; b; b &= b - 1u, v >>= 1)
+        if (v & 1u) r |= b & (0u - b);
+    return r;
+}
 
 // Surface area of u when it is an inner node (a candidate to expand), else -1.
 __device__ __forceinline__ float expand_sa(const RadixArrays& A, int32_t u) {
@@ -1900,51 +1929,91 @@ __device__ void treelet_form(const RadixArrays& A, int32_t x, TlForm& f, float* 
 // the root's box, range and count do not change (same leaves).
 __device__ void treelet_step(const RadixArrays& A, int32_t x, int nl, float cx,
                              const int32_t* lf, const int32_t* in, const float* row,
-                             TlShared* sh) {
+                             TlShared* sh, const uint8_t* subsets) {
     const uint32_t lane = threadIdx.x & 63u;
     float* C = sh->C;
     uint8_t* P = sh->P;
     if (nl >= 3) {
+        // lane q < nl fetches leaf q's rank range and count now, for a
+        // rebuild at the end (read there by lane 0)
+        uint32_t mlo = 0, mhi = 0, mc = 0;
+        if (lane < (uint32_t)nl) {
+            int32_t u = lf[0];  // lf[lane], by masks (no indexed private array)
+#pragma unroll
+            for (int q = 1; q < kTreeletLeaves; ++q)
+                u ^= (u ^ lf[q]) & -(int32_t)(lane == (uint32_t)q);
+            mlo = ext_lo(A, u);
+            mhi = ext_hi(A, u);
+            mc = ncount(A, u);
+        }
         wave_lds_step();  // the previous treelet's reads of C are done
         float lb[kTreeletLeaves][6];
 #pragma unroll
         for (int q = 0; q < kTreeletLeaves; ++q) {
             if (q >= nl) continue;
 #pragma unroll
-            for (int a = 0; a < 6; a += 2) {
-                const float2 v = *reinterpret_cast<const float2*>(row + 6 * q + a);
-                lb[q][a] = v.x;
-                lb[q][a + 1] = v.y;
-            }
+            for (int a = 0; a < 6; ++a) lb[q][a] = row[6 * q + a];
         }
         if (lane < (uint32_t)nl) C[1u << lane] = row[6 * kTreeletLeaves + lane];
         const uint32_t full = (1u << nl) - 1;
-        // surface areas of this lane's two subsets (as the sequential union)
-        const uint32_t sl[2] = {lane + 1u, lane + 65u};
-        float sa[2];
+        // surface areas of the subsets of two leaves or more (as the
+        // sequential union), parked in C until the subset's cost replaces it
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
+            const uint32_t sl = lane + 1u + 64u * h;
             float b[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
 #pragma unroll
             for (int q = 0; q < kTreeletLeaves; ++q) {
-                if (!((sl[h] >> q) & 1u)) continue;
+                if (!((sl >> q) & 1u)) continue;
 #pragma unroll
                 for (int a = 0; a < 3; ++a) {
                     b[a] = fminf(b[a], lb[q][a]);
                     b[3 + a] = fmaxf(b[3 + a], lb[q][3 + a]);
                 }
             }
-            sa[h] = box_sa(b);
+            if (sl <= full && (sl & (sl - 1))) C[sl] = box_sa(b);
         }
         wave_lds_step();
+        // subsets by size k: the S size-k subsets (kSubsetsBySize) each get a
+        // group of G lanes (G the largest power of two with G S <= 64), which
+        // share its 2^(k-1) - 1 splits in best_split's order, and reduce
+        // their first strict minima to the least cost, then the first split
         for (int k = 2; k < nl; ++k) {
-#pragma unroll
-            for (int h = 0; h < 2; ++h) {
-                if (sl[h] > full || __popc(sl[h]) != k) continue;
-                uint32_t bp;
-                const float split = best_split<1>(C, sl[h], bp);
-                C[sl[h]] = kCostInner * sa[h] + split;
-                P[sl[h]] = (uint8_t)bp;
+            const uint32_t S = kBinom[nl][k];  // 3 <= S <= 35
+            const uint32_t lg = __clz(S - 1u) - 26u;  // 6 - ceil(log2 S)
+            const uint32_t t = lane >> lg, g = lane & ((1u << lg) - 1u);
+            const uint32_t ncand = (1u << (k - 1)) - 1u;
+            const uint32_t per = (ncand + (1u << lg) - 1u) >> lg;
+            uint32_t s = 0, low = 0, rest = 0;
+            float best = INFINITY;
+            uint32_t bj = 0xffffffffu;
+            if (t < S) {
+                s = subsets[kSubsetStart[k] + t];
+                low = s & (0u - s);
+                rest = s ^ low;
+                const uint32_t j1 = min(g * per + per, ncand);
+                uint32_t sub = deposit_bits(ncand - 1u - g * per, rest);
+                for (uint32_t j = g * per; j < j1; ++j) {
+                    const float c = C[low | sub] + C[rest ^ sub];
+                    if (c < best) {
+                        best = c;
+                        bj = j;
+                    }
+                    sub = (sub - 1u) & rest;
+                }
+            }
+            for (uint32_t o = 1; o < (1u << lg); o <<= 1) {
+                const float ob = __shfl_xor(best, (int)o);
+                const uint32_t oj = (uint32_t)__shfl_xor((int)bj, (int)o);
+                if (ob < best || (ob == best && oj < bj)) {
+                    best = ob;
+                    bj = oj;
+                }
+            }
+            if (t < S && g == 0) {
+                C[s] = kCostInner * C[s] + best;
+                P[s] = (uint8_t)(bj == 0xffffffffu ? 0u
+                                                   : low | deposit_bits(ncand - 1u - bj, rest));
             }
             wave_lds_step();
         }
@@ -1960,8 +2029,8 @@ __device__ void treelet_step(const RadixArrays& A, int32_t x, int nl, float cx,
         // lane at the least cost, none when no cost is below infinity
         const uint64_t at = __ballot(lane < nsplit && c == cmin && cmin < INFINITY);
         const uint32_t pfull = at ? 1u | ((nsplit - 1u - (uint32_t)__builtin_ctzll(at)) << 1) : 0u;
-        const float sa_full = __shfl(sa[(full - 1u) >> 6], (int)((full - 1u) & 63u));
-        const float cf = kCostInner * sa_full + cmin;
+        const float cf = kCostInner * C[full] + cmin;  // C[full]: its surface area
+        wave_lds_step();  // every lane has read C[full] before lane 0 replaces it
         if (lane == 0) {
             C[full] = cf;
             P[full] = (uint8_t)pfull;
@@ -2000,10 +2069,9 @@ __device__ void treelet_step(const RadixArrays& A, int32_t x, int nl, float cx,
                 uint32_t elo[kTreeletLeaves], ehi[kTreeletLeaves], ec[kTreeletLeaves];
 #pragma unroll
                 for (int q = 0; q < kTreeletLeaves; ++q) {
-                    if (q >= nl) continue;
-                    elo[q] = ext_lo(A, lf[q]);
-                    ehi[q] = ext_hi(A, lf[q]);
-                    ec[q] = ncount(A, lf[q]);
+                    elo[q] = __builtin_amdgcn_readlane(mlo, q);
+                    ehi[q] = __builtin_amdgcn_readlane(mhi, q);
+                    ec[q] = __builtin_amdgcn_readlane(mc, q);
                 }
                 for (int i = 0; i < nq; ++i) {
                     A.left[qn[i]] = ql[i];
@@ -2117,7 +2185,7 @@ enum TlMode { kTlTreelet, kTlBox4 };
 // its block), agent for the nodes crossing windows (reached from any block).
 template <bool WINDOW, int MODE>
 __device__ void tl_climbs(const RadixArrays& A, int32_t x, uint32_t lo, uint32_t hi,
-                          TlShared* sh) {
+                          TlShared* sh, const uint8_t* subsets) {
     const uint32_t lane = threadIdx.x & 63u;
     for (uint32_t steps = 0;; ++steps) {
         bool own = false;
@@ -2162,7 +2230,7 @@ __device__ void tl_climbs(const RadixArrays& A, int32_t x, uint32_t lo, uint32_t
                 const float cx = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(f.cx), l));
                 wave_lds_step();  // the rows written above
                 treelet_step(A, __builtin_amdgcn_readlane(x, l), nl, cx, lf, in,
-                             rows + l * kRowFloats, sh);
+                             rows + l * kRowFloats, sh, subsets);
             }
         }
         if constexpr (WINDOW) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
@@ -2173,6 +2241,8 @@ __device__ void tl_climbs(const RadixArrays& A, int32_t x, uint32_t lo, uint32_t
 }
 
 constexpr uint32_t kTreeletWaves = kTreeletBlock / 64;
+static_assert(kTreeletWaves * sizeof(TlShared) + kTreeletSets <= 53760,
+              "treelet blocks: 3 to a CU (LDS)");
 
 // One restructuring round (or the Box4 pass), pass 1: as k_rt_bottom_up over
 // kWindow leaf ranks, with a node inside the window when all ranks below it
@@ -2180,12 +2250,17 @@ constexpr uint32_t kTreeletWaves = kTreeletBlock / 64;
 template <int MODE>
 __global__ __launch_bounds__(kTreeletBlock, 3) void k_tl_window(RadixArrays A) {
     __shared__ TlShared sh[MODE == kTlTreelet ? kTreeletWaves : 1];
+    __shared__ uint8_t subsets[kTreeletSets];
+    if (MODE == kTlTreelet && threadIdx.x < kTreeletSets)
+        subsets[threadIdx.x] = kSubsetsBySize[threadIdx.x];
+    __syncthreads();
     TlShared* mine = sh + (MODE == kTlTreelet ? threadIdx.x >> 6 : 0);
     const uint32_t lo = blockIdx.x * kWindow;
     const uint32_t hi = min(lo + kWindow, A.n) - 1;
     for (uint32_t base = lo; base <= hi; base += kTreeletBlock) {
         const uint32_t r = base + threadIdx.x;
-        tl_climbs<true, MODE>(A, r <= hi ? A.parent[A.m + A.order[r]] : -1, lo, hi, mine);
+        tl_climbs<true, MODE>(A, r <= hi ? A.parent[A.m + A.order[r]] : -1, lo, hi, mine,
+                              subsets);
     }
 }
 
@@ -2210,17 +2285,23 @@ __global__ __launch_bounds__(256) void k_tl_top_prep(RadixArrays A) {
 template <int MODE>
 __global__ __launch_bounds__(kTreeletBlock, 3) void k_tl_top(RadixArrays A) {
     __shared__ TlShared sh[MODE == kTlTreelet ? kTreeletWaves : 1];
+    __shared__ uint8_t subsets[kTreeletSets];
     const uint32_t ns = A.top_count[1];
+    if (blockIdx.x * kTreeletWaves >= ns) return;  // no climb for this block (uniform)
+    if (MODE == kTlTreelet && threadIdx.x < kTreeletSets)
+        subsets[threadIdx.x] = kSubsetsBySize[threadIdx.x];
+    __syncthreads();
     if constexpr (MODE == kTlTreelet) {
         const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
         for (uint32_t j = blockIdx.x * kTreeletWaves + wave; j < ns;
              j += gridDim.x * kTreeletWaves)
-            tl_climbs<false, MODE>(A, lane == 0 ? (int32_t)A.starts[j] : -1, 0, 0, sh + wave);
+            tl_climbs<false, MODE>(A, lane == 0 ? (int32_t)A.starts[j] : -1, 0, 0, sh + wave,
+                                   subsets);
     } else {
         for (uint32_t base = blockIdx.x * kTreeletBlock; base < ns;
              base += gridDim.x * kTreeletBlock) {
             const uint32_t j = base + threadIdx.x;
-            tl_climbs<false, MODE>(A, j < ns ? (int32_t)A.starts[j] : -1, 0, 0, sh);
+            tl_climbs<false, MODE>(A, j < ns ? (int32_t)A.starts[j] : -1, 0, 0, sh, subsets);
         }
     }
 }
