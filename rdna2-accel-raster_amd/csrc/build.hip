@@ -1493,6 +1493,7 @@ struct RadixArrays {
     uint32_t* rank;        // n: leaf rank of sorted position p (tree order)
     uint32_t* order;       // n: sorted position of leaf rank r
     uint32_t n, m, maxp;
+    uint32_t wshift;       // treelet passes: windows of 1 << wshift leaf ranks
 };
 
 // Highest differing bit of sorted positions i and j: key bits, then 32 +
@@ -1735,7 +1736,7 @@ __device__ __forceinline__ uint32_t ncount(const RadixArrays& A, int32_t u) {
     return (uint32_t)u < A.m ? A.cnt[u] : 1u;
 }
 __device__ __forceinline__ bool ext_straddles(const RadixArrays& A, int32_t x) {
-    return (uint32_t)x < A.m && A.emin[x] / kWindow != A.emax[x] / kWindow;
+    return (uint32_t)x < A.m && A.emin[x] >> A.wshift != A.emax[x] >> A.wshift;
 }
 
 // SAH cost of the subtree at u: a leaf range SA * triangles, an inner node its
@@ -2158,7 +2159,7 @@ __global__ __launch_bounds__(256) void k_tl_rank_jump(RadixArrays A) {
     A.emin[x] = r;
     A.emax[x] = last;
     A.done[x] = 0;
-    if (r / kWindow != last / kWindow) A.top[atomicAdd(A.top_count, 1u)] = x;
+    if (r >> A.wshift != last >> A.wshift) A.top[atomicAdd(A.top_count, 1u)] = x;
 }
 // Ranks of the sorted positions, and the position of each rank.
 __global__ __launch_bounds__(256) void k_tl_rank_leaves(RadixArrays A) {
@@ -2245,8 +2246,8 @@ static_assert(kTreeletWaves * sizeof(TlShared) + kTreeletSets <= 53760,
               "treelet blocks: 3 to a CU (LDS)");
 
 // One restructuring round (or the Box4 pass), pass 1: as k_rt_bottom_up over
-// kWindow leaf ranks, with a node inside the window when all ranks below it
-// are.
+// windows of 1 << wshift leaf ranks, with a node inside the window when all
+// ranks below it are.
 template <int MODE>
 __global__ __launch_bounds__(kTreeletBlock, 3) void k_tl_window(RadixArrays A) {
     __shared__ TlShared sh[MODE == kTlTreelet ? kTreeletWaves : 1];
@@ -2255,8 +2256,8 @@ __global__ __launch_bounds__(kTreeletBlock, 3) void k_tl_window(RadixArrays A) {
         subsets[threadIdx.x] = kSubsetsBySize[threadIdx.x];
     __syncthreads();
     TlShared* mine = sh + (MODE == kTlTreelet ? threadIdx.x >> 6 : 0);
-    const uint32_t lo = blockIdx.x * kWindow;
-    const uint32_t hi = min(lo + kWindow, A.n) - 1;
+    const uint32_t lo = blockIdx.x << A.wshift;
+    const uint32_t hi = min(lo + (1u << A.wshift), A.n) - 1;
     for (uint32_t base = lo; base <= hi; base += kTreeletBlock) {
         const uint32_t r = base + threadIdx.x;
         tl_climbs<true, MODE>(A, r <= hi ? A.parent[A.m + A.order[r]] : -1, lo, hi, mine,
@@ -2429,6 +2430,17 @@ struct ScratchPlan {
     uint32_t tiles = 0;
 };
 
+// Treelet windows: the window pass runs one block per window with 3 blocks
+// to a CU, its waves taking their nodes' treelets one after another, so
+// windows small enough for every CU to hold blocks, and for blocks to
+// outnumber the resident slots (balance); larger windows leave fewer nodes
+// crossing window boundaries to the agent-scope pass.
+uint32_t treelet_window_shift(uint32_t n) {
+    uint32_t sh = 12;  // 4096
+    while (sh > 9 && ((size_t)n >> sh) < 1536) --sh;
+    return sh;
+}
+
 // Scratch of one build: sort buffers, then the implicit level boxes or the
 // radix-tree arrays.
 ScratchPlan plan_scratch(uint32_t n, uint32_t algo, uint32_t rounds) {
@@ -2584,6 +2596,7 @@ rt_status build_radix(hipStream_t s, const float* pos, const uint32_t* idx, uint
     A.n = n;
     A.m = n - 1;
     A.maxp = maxp;
+    A.wshift = treelet_window_shift(n);
     hipLaunchKernelGGL(k_rt_leaves, dim3(ceil_div(n, 256)), dim3(256), 0, s, pos, idx, n, vals,
                        tris, A.box, A.m, A.top_count);
     if (n <= maxp) {
@@ -2602,10 +2615,10 @@ rt_status build_radix(hipStream_t s, const float* pos, const uint32_t* idx, uint
         hipLaunchKernelGGL(k_tl_rank_jump, dim3(ceil_div(A.m, 256)), dim3(256), 0, s, A);
         hipLaunchKernelGGL(k_tl_rank_leaves, dim3(ceil_div(n, 256)), dim3(256), 0, s, A);
         if (r < rounds)
-            hipLaunchKernelGGL(k_tl_window<kTlTreelet>, dim3(ceil_div(n, kWindow)),
+            hipLaunchKernelGGL(k_tl_window<kTlTreelet>, dim3(ceil_div(n, 1u << A.wshift)),
                                dim3(kTreeletBlock), 0, s, A);
         else
-            hipLaunchKernelGGL(k_tl_window<kTlBox4>, dim3(ceil_div(n, kWindow)),
+            hipLaunchKernelGGL(k_tl_window<kTlBox4>, dim3(ceil_div(n, 1u << A.wshift)),
                                dim3(kTreeletBlock), 0, s, A);
         hipLaunchKernelGGL(k_tl_top_prep, dim3(ceil_div(A.m, 256)), dim3(256), 0, s, A);
         if (r < rounds)
