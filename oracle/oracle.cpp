@@ -1090,11 +1090,14 @@ void or_shadow_rays(const float* pos, const uint32_t* idx,
     }
 }
 
-// shaders/rq_ao.comp:18-45 hash / rand / cosine hemisphere sample
+// shaders/rq_ao.comp:18-45 hash / rand / cosine hemisphere sample.  The
+// hash is Jenkins' one-at-a-time hash of the single key value x (one mixing
+// step, then the final avalanche).
 static inline uint32_t ao_hash(uint32_t x) {
     x += (x << 10u); x ^= (x >> 6u); x += (x << 3u); x ^= (x >> 11u); x += (x << 15u);
     return x;
 }
+uint32_t or_ao_hash(uint32_t x) { return ao_hash(x); }
 
 // One cosine-weighted AO ray for pixel p about the unit normal n
 // (rq_ao.comp:32-97, 70-87): seed y*W + x + 5544 + frame*W*H, origin

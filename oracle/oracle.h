@@ -140,6 +140,7 @@ void or_trace_brute(const or_tri64* tris, uint32_t ntris,
 
 /* ---- ray generators ----------------------------------------------------- */
 uint32_t or_pcg_hash(uint32_t v);
+uint32_t or_ao_hash(uint32_t x);
 void or_exp0_rays(uint32_t count, int coherent, const float cam[3], const float look[3],
                   float* ox, float* oy, float* oz, float* dx, float* dy, float* dz);
 void or_camera_rays(uint32_t w, uint32_t h, const float eye[3], const float look[3],

@@ -45,6 +45,7 @@ def lib():
         _lib = C.CDLL(LIB_PATH)
         _lib.or_terrain_res.restype = C.c_uint32
         _lib.or_pcg_hash.restype = C.c_uint32
+        _lib.or_ao_hash.restype = C.c_uint32
         _lib.or_expand_bits10.restype = C.c_uint32
         _lib.or_bvh4_depth.restype = C.c_uint32
         _lib.or_rms.restype = C.c_double
@@ -269,6 +270,11 @@ def trace_brute(tris, rays, mode=CLOSEST, threads=None):
 # ---------------------------------------------------------------- rays -------
 def pcg_hash(v):
     return int(lib().or_pcg_hash(u32(v)))
+
+
+def ao_hash(x):
+    """rq_ao.comp's hash (the AO sampler's random numbers)."""
+    return int(lib().or_ao_hash(u32(x)))
 
 
 def exp0_rays(count, coherent, cam=(0.0, 50.0, -45.0), look=(0.0, 0.0, 5.0),

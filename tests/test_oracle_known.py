@@ -394,6 +394,42 @@ def test_generators_closed_forms(O):
     assert abs(d.mean(0)).max() < 0.05 and rr.ox.min() >= -1 and rr.oz.max() < 3
 
 
+def _one_at_a_time(data):
+    """Jenkins' one-at-a-time hash of a byte string, as published (Dr. Dobb's
+    1997): per byte h += b; h += h << 10; h ^= h >> 6; then h += h << 3;
+    h ^= h >> 11; h += h << 15 (mod 2^32)."""
+    h = 0
+    for b in data:
+        h = (h + b) & 0xFFFFFFFF
+        h = (h + (h << 10)) & 0xFFFFFFFF
+        h ^= h >> 6
+    h = (h + (h << 3)) & 0xFFFFFFFF
+    h ^= h >> 11
+    return (h + (h << 15)) & 0xFFFFFFFF
+
+
+def test_ao_hash_is_one_at_a_time(O):
+    """The AO sampler's hash (rq_ao.comp:18-25) is the one-at-a-time hash of
+    its one key value: equal to the published algorithm, itself checked on
+    its published vectors, on every one-byte key (each shift and add shows)."""
+    assert _one_at_a_time(b"a") == 0xCA2E9442
+    assert _one_at_a_time(b"The quick brown fox jumps over the lazy dog") == 0x519E91F5
+    for x in range(256):
+        assert O.ao_hash(x) == _one_at_a_time(bytes([x])), x
+    # a wrong shift in the oracle's copy would fail the loop above: the
+    # planted mistake, checked on the Python model
+    assert any(_one_at_a_time(bytes([x])) != _oaat_slip(x) for x in range(256))
+
+
+def _oaat_slip(x):
+    h = x
+    h = (h + (h << 10)) & 0xFFFFFFFF
+    h ^= h >> 5  # 6 in the algorithm
+    h = (h + (h << 3)) & 0xFFFFFFFF
+    h ^= h >> 11
+    return (h + (h << 15)) & 0xFFFFFFFF
+
+
 def test_shadow_and_ao_rays(O):
     pos, idx = O.sphere(64, 48, 1.0, 0.02, 7)
     cam = O.camera_rays(40, 30, (0, 0, 1.25), (0, 0, 0), (0, 1, 0), 60.0)
