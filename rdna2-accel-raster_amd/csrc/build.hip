@@ -600,7 +600,6 @@ constexpr uint32_t kBuckets = 1u << kBucketBits;
 constexpr uint32_t kBucketShift = 30 - kBucketBits;
 constexpr uint32_t kBucketCap = 256;
 constexpr uint32_t kPartRecs = 256;   // sorted positions written per workgroup
-constexpr uint32_t kStageWords = 11;  // 9 vertex floats, triangle index, code
 
 // Level table of a fused-path tree (n <= 4096: at most 6 levels).  Read only
 // with compile-time indices (pick), so it stays in scalar registers; a
@@ -648,7 +647,6 @@ struct SmallLds {
             uint32_t cnt[(kSmallBlock / 64) * 256];
         } lsd;
         float tbox[6][kSmallMax];                  // triangle bounds at their ranks
-        uint32_t stage[kPartRecs * kStageWords];  // this workgroup's records
     } u;
     float box[6][kBoxPitch];  // node bounds: level lv from slot pad[lv]
     uint32_t spread[1024];    // expand_bits10 of every 10-bit value
@@ -871,38 +869,23 @@ __device__ void small_emit(SmallLds& S, uint32_t n,
         nd[g] = RADV ? radv_chunk(n, g >> 3, g & 7u, S) : node_chunk(n, g >> 3, g & 7u, S);
     if (RADV && part == 0 && tid == 0)  // the image starts 192 B before the first box
         radv_header(reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(nodes) - 192), n, S);
-    lds_barrier();  // triangle bounds dead: the union stages records
-    // sorted positions [base, base + kPartRecs) for base = part * kPartRecs,
-    // stepping by parts * kPartRecs (one slice per workgroup when parts
-    // workgroups share the mesh, all of them when one builds it alone)
-    for (uint32_t base = part * kPartRecs; base < n; base += parts * kPartRecs) {
+    // records: the thread holding a triangle writes its 64-B record (4 x 16 B,
+    // store_tri's layout) when its sorted position lies in this workgroup's
+    // share: slices of kPartRecs positions, every parts-th one (all of them
+    // when one workgroup builds the mesh alone)
 #pragma unroll
-        for (int i = 0; i < kSmallItems; ++i) {
-            if (!ok[i] || r[i] < base || r[i] - base >= kPartRecs) continue;
-            uint32_t* st = S.u.stage + (r[i] - base) * kStageWords;
+    for (int i = 0; i < kSmallItems; ++i) {
+        if (!ok[i] || (r[i] / kPartRecs) % parts != part) continue;
+        uint4* td = reinterpret_cast<uint4*>(tris + r[i]);
+        uint32_t u[9];
 #pragma unroll
-            for (int a = 0; a < 9; ++a) st[a] = __float_as_uint(v[i][a]);
-            st[9] = id[i];
-            st[10] = key[i];
-        }
-        lds_barrier();
-        const uint32_t cnt = min(kPartRecs, n - base);
-        uint4* td = reinterpret_cast<uint4*>(tris + base);
-        for (uint32_t g = tid; g < 4 * cnt; g += kSmallBlock) {
-            const uint32_t* st = S.u.stage + (g >> 2) * kStageWords;
-            const uint32_t c = g & 3u;
-            uint4 o;
-            if (c == 0) o = make_uint4(st[0], st[1], st[2], st[3]);
-            else if (c == 1) o = make_uint4(st[4], st[5], st[6], st[7]);
-            else if (c == 2) o = make_uint4(st[8], 0u, 0u, 0u);
-            else o = make_uint4(st[9], 0x80000000u, 0u, 9u);  // store_tri's last 16 B
-            td[g] = o;
-        }
-        if (tid < cnt) {
-            if (ids_out) ids_out[base + tid] = S.u.stage[tid * kStageWords + 9];
-            if (codes_out) codes_out[base + tid] = S.u.stage[tid * kStageWords + 10];
-        }
-        lds_barrier();  // staging reused by the next slice
+        for (int a = 0; a < 9; ++a) u[a] = __float_as_uint(v[i][a]);
+        td[0] = make_uint4(u[0], u[1], u[2], u[3]);
+        td[1] = make_uint4(u[4], u[5], u[6], u[7]);
+        td[2] = make_uint4(u[8], 0u, 0u, 0u);
+        td[3] = make_uint4(id[i], 0x80000000u, 0u, 9u);
+        if (ids_out) ids_out[r[i]] = id[i];
+        if (codes_out) codes_out[r[i]] = key[i];
     }
 }
 
