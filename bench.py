@@ -844,11 +844,18 @@ def main():
     kern_avg = sum(kern_ms) / len(kern_ms)
     ab = algorithmic_bytes_shadow(f)
     achieved = ab / (kern_avg * 1e-3) / 1e9
+    # HBM bytes per launch (FETCH_SIZE + WRITE_SIZE, calibrated on a copy) of
+    # the shadow kernel measured for this run's tree and wave tile; null when
+    # the profile holds no measurement of that configuration
     traffic = None
     tp = os.path.join(ROOT, "profiles", "traffic_shadow_anyhit_2p5m.json")
+    tile = f.shadow_opts.get("tile")
     if os.path.exists(tp):
         with open(tp) as fh:
-            traffic = json.load(fh).get("hbm_bytes_per_launch")
+            for e in json.load(fh).get("entries", []):
+                if (e["max_leaf_prims"], e["treelet_rounds"]) == (f.leaf, f.rounds) and \
+                        tile is not None and e["tile_rows"] == tile[1]:
+                    traffic = e["hbm_bytes_per_launch"]
 
     out = {
         "metric": METRIC,

@@ -35,7 +35,13 @@ TRACE_BLOCKS = (64, 128, 256)
 TRACE_LDS = (8, 12, 16, 20, 24, 32)
 REFILL_RAYS_PER_WAVE = (256, 1024, 4096)
 REFILL_LANES = (8, 16, 32)
-TILE_WIDTHS = (8, 16, 32)  # wave tiles of a ray grid: 8x8, 16x4, 32x2 rays
+# wave tiles of a ray grid: 16x4, 8x8, 32x2 rays.  16x4 first: on the shadow
+# frame 8x8 tiles take the same time but miss L2 about twice as often
+# (7.1 vs 3.8 GB of L2 misses per launch), so a tie keeps 16x4
+TILE_WIDTHS = (16, 8, 32)
+# a later candidate must be faster than the best so far by more than this
+# fraction (the spread of repeated timings of one candidate) to replace it
+TIE_MARGIN = 0.01
 LEAF_SIZES = (4, 3, 2, 1)  # radix-tree leaf sizes of a frame's tree, the default first
 SAMPLE_ROW_BLOCK = 8       # rows per block of a grid sample (a multiple of every tile height)
 
@@ -124,9 +130,10 @@ class Tuner:
     def _pick(self, key, candidates, run, check, skip_changed=False, same=None):
         """Times run(c) for every candidate c, checks check(c) == the first
         candidate's output (or same(out, first) when given), caches and
-        returns the fastest.  A candidate whose output differs raises, or with
-        skip_changed (knobs that may change results, such as the tree's leaf
-        size) is left out."""
+        returns the fastest (a later candidate wins only when faster by more
+        than TIE_MARGIN: within timing noise the earlier one stays).  A
+        candidate whose output differs raises, or with skip_changed (knobs
+        that may change results, such as the tree's leaf size) is left out."""
         if key in self.cache:
             return self.cache[key]["choice"]
         t0 = time.perf_counter()
@@ -162,7 +169,7 @@ class Tuner:
             ms = time_ms(lambda: run(c), reps=self.reps)
             table.append((c, ms))
             self.history.append((key, c, ms))
-            if ms < best_ms:
+            if ms < best_ms * (1.0 - TIE_MARGIN) or best is None:
                 best, best_ms = c, ms
         return best, best_ms, table
 
