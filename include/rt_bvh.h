@@ -360,6 +360,14 @@ rt_status rt_trace(rt_stream stream, const rt_bvh_dev* bvh, const rt_rays_soa* r
  *   RT_SCHED_REFILL  each wave walks a chunk of rays_per_wave consecutive rays
  *                    and hands new rays to its finished lanes whenever at
  *                    least refill_lanes of 64 are idle.
+ *   RT_SCHED_PACKET  (rt_trace_ex, RT_ANY without RT_COUNTERS or hit ids,
+ *                    RT_NODES_F32 trees) a wave walks one node order for
+ *                    its 64 rays: each stack entry carries the lanes whose
+ *                    own box tests reached it, so every ray tests exactly the
+ *                    triangles its own traversal reaches and its occlusion
+ *                    bit is the same; node records are read once per wave.
+ *                    For coherent rays (a light's shadow rays in wave tiles).
+ *                    lds_stack does not apply; any_order must be 0.
  *   block            threads per workgroup: 64, 128 or 256 (0 = 256).
  *   lds_stack        traversal-stack entries per lane kept in LDS, 4..48
  *                    (0 = 20); deeper entries spill to private memory.  More
@@ -383,6 +391,7 @@ rt_status rt_trace(rt_stream stream, const rt_bvh_dev* bvh, const rt_rays_soa* r
  * opts = NULL: RT_SCHED_SIMPLE (RT_TRACE_PERSISTENT in flags: RT_SCHED_REFILL). */
 #define RT_SCHED_SIMPLE 0u
 #define RT_SCHED_REFILL 1u
+#define RT_SCHED_PACKET 2u
 #define RT_ORDER_NEAR 0u
 #define RT_ORDER_FAR 1u
 #define RT_ORDER_SLOT 2u

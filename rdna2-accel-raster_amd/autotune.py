@@ -39,6 +39,7 @@ REFILL_LANES = (8, 16, 32)
 # frame 8x8 tiles take the same time but miss L2 about twice as often
 # (7.1 vs 3.8 GB of L2 misses per launch), so a tie keeps 16x4
 TILE_WIDTHS = (16, 8, 32)
+PACKET_TILE_WIDTHS = (16, 8)  # packet traversal (occlusion bits) of 16x4 / 8x8 tiles
 # a later candidate must be faster than the best so far by more than this
 # fraction (the spread of repeated timings of one candidate) to replace it
 TIE_MARGIN = 0.01
@@ -212,7 +213,12 @@ class Tuner:
             dict(best, any_order=o) for o in rt.ANY_ORDERS if o != "near"]
         tiles = [dict(best, tile=(grid_width, tw)) for tw in TILE_WIDTHS
                  if grid_width and grid_width % tw == 0]
-        return self._pick(key, [best] + refill + orders + tiles, run, check)
+        # occlusion bits of a ray grid: packet traversal of wave tiles too
+        packets = [] if mode == "closest" else [
+            {"schedule": "packet", "block": b, "tile": (grid_width, tw)}
+            for tw in PACKET_TILE_WIDTHS for b in (64, 256)
+            if grid_width and grid_width % tw == 0]
+        return self._pick(key, [best] + refill + orders + tiles + packets, run, check)
 
     def trace_scene(self, scene, rays, mode, tag="", grid_width=None):
         """Launch shape (block, lds_stack, and wave tiles of the ray grid when

@@ -587,6 +587,154 @@ __global__ __launch_bounds__(kBlock) void k_trace(const rt_node4* __restrict__ n
     }
 }
 
+// Packet traversal (RT_SCHED_PACKET), occlusion bits only: a wave walks one
+// node order for its 64 rays (Garanzha & Loop 2010 trace coherent packets
+// breadth-first; here depth-first with a per-entry lane mask).  Each stack
+// entry carries the lanes whose own slab tests reached it, so a ray tests
+// exactly the triangles its single-ray traversal reaches (same predicates),
+// and its bit equals the single-ray kernel's whatever the order.  Node and
+// leaf addresses are wave-uniform: node records come in through scalar
+// loads, and a visit costs one slab test per lane instead of a vector load
+// per lane of possibly 64 different nodes (the single-ray kernel's bound on
+// shadow rays is the vector memory pipe: its loads are issued for the whole
+// wave while about a third of the lanes are active).  Inner children are
+// pushed nearest last by the entry t of the first lane that reached each.
+constexpr int kPacketStack = 3 * (int)kMaxDepth + 1;  // live entries of a depth-64 tree
+
+// A float read at a wave-uniform address, as a wave-uniform value.
+__device__ __forceinline__ float uniform_f(const float& x) {
+    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(x)));
+}
+
+template <bool TILE>
+__global__ __launch_bounds__(kBlock) void k_trace_packet(const rt_node4* __restrict__ nodes,
+                                                         const rt_tri64* __restrict__ tris,
+                                                         uint32_t node_count, rt_rays_soa rays,
+                                                         uint32_t n,
+                                                         uint32_t* __restrict__ bits_out,
+                                                         TileMap tmap) {
+    extern __shared__ uint4 pstack[];  // dynamic: kPacketStack entries per wave
+    const uint32_t tid = threadIdx.x, lane = tid & 63u;
+    uint4* S = pstack + (tid >> 6) * kPacketStack;
+    bool valid;
+    uint32_t r;
+    if constexpr (TILE) {
+        r = tile_ray(blockIdx.x * blockDim.x + tid, tmap, valid);
+    } else {
+        r = blockIdx.x * blockDim.x + tid;
+        valid = r < n;
+    }
+    float ox = 0.0f, oy = 0.0f, oz = 0.0f, tmin = 1.0f;
+    float dx = 0.0f, dy = 0.0f, dz = 1.0f, tmax = 0.0f;
+    if (valid) {
+        ox = rays.ox[r]; oy = rays.oy[r]; oz = rays.oz[r]; tmin = rays.tmin[r];
+        dx = rays.dx[r]; dy = rays.dy[r]; dz = rays.dz[r]; tmax = rays.tmax[r];
+    }
+    // the single-ray kernel's setup (software_bvh_bvh4.comp:96-100)
+    const float ix = fabsf(dx) > 1e-6f ? 1.0f / dx : 1e6f;
+    const float iy = fabsf(dy) > 1e-6f ? 1.0f / dy : 1e6f;
+    const float iz = fabsf(dz) > 1e-6f ? 1.0f / dz : 1e6f;
+    const float tminc = fmaxf(tmin, -FLT_MAX);
+    const bool pxs = ix > 0.0f, pys = iy > 0.0f, pzs = iz > 0.0f;
+    uint64_t alive = __ballot(valid && node_count > 0);  // lanes not yet occluded
+    bool occl = false;
+    int sp = 0;
+    if (alive) {
+        if (lane == 0) S[0] = make_uint4(0u, (uint32_t)alive, (uint32_t)(alive >> 32), 0u);
+        sp = 1;
+    }
+    for (uint32_t pops = 0; sp > 0 && alive && pops < node_count; ++pops) {
+        --sp;
+        const uint4 e = S[sp];
+        const uint32_t node = __builtin_amdgcn_readfirstlane(e.x);
+        const uint64_t m = ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane(e.z) << 32 |
+                            (uint32_t)__builtin_amdgcn_readfirstlane(e.y)) & alive;
+        if (m == 0) continue;
+        const bool in = (m >> lane) & 1u;
+        const rt_node4* nd = nodes + node;
+        float te[4];
+        uint64_t hm[4];
+        int32_t chv[4], cnv[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {  // :116-144 slab test + filter, per lane
+            // the 24 bounds as wave-uniform values first (a per-lane choice
+            // of the load address would make them vector loads)
+            chv[k] = (int32_t)__builtin_amdgcn_readfirstlane(nd->child[k]);
+            cnv[k] = (int32_t)__builtin_amdgcn_readfirstlane(nd->count[k]);
+            const float x0 = uniform_f(nd->minX[k]), x1 = uniform_f(nd->maxX[k]);
+            const float y0 = uniform_f(nd->minY[k]), y1 = uniform_f(nd->maxY[k]);
+            const float z0 = uniform_f(nd->minZ[k]), z1 = uniform_f(nd->maxZ[k]);
+            const float bnx = pxs ? x0 : x1, bfx = pxs ? x1 : x0;
+            const float bny = pys ? y0 : y1, bfy = pys ? y1 : y0;
+            const float bnz = pzs ? z0 : z1, bfz = pzs ? z1 : z0;
+            const float tnx = (bnx - ox) * ix, tfx = (bfx - ox) * ix;
+            const float tny = (bny - oy) * iy, tfy = (bfy - oy) * iy;
+            const float tnz = (bnz - oz) * iz, tfz = (bfz - oz) * iz;
+            te[k] = fmaxf(fmaxf(tnx, tny), fmaxf(tnz, tminc));
+            const float tx = fminf(fminf(tfx, tfy), fminf(tfz, tmax));
+            hm[k] = __ballot(in && cnv[k] != 0 && tx >= te[k]);
+        }
+        // leaf children: each triangle by the lanes that reached it and are
+        // not yet occluded (test_leaves' predicate: t >= tmin and t < tmax)
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            if (cnv[k] <= 0 || !(hm[k] & alive)) continue;
+            const uint32_t first = (uint32_t)~chv[k];
+            for (int32_t j = 0; j < cnv[k]; ++j) {
+                const uint64_t tm = hm[k] & alive;
+                if (!tm) break;
+                // every lane computes (uniform control flow: the record is a
+                // scalar load); only the lanes of tm take the result
+                const rt_tri64* tp = tris + first + j;
+                float t;
+                const bool h =
+                    mt_hit(uniform_f(tp->v[0]), uniform_f(tp->v[1]), uniform_f(tp->v[2]),
+                           uniform_f(tp->v[3]), uniform_f(tp->v[4]), uniform_f(tp->v[5]),
+                           uniform_f(tp->v[6]), uniform_f(tp->v[7]), uniform_f(tp->v[8]), ox,
+                           oy, oz, dx, dy, dz, &t) &&
+                    t >= tmin && t < tmax;
+                occl = occl || (h && ((tm >> lane) & 1u));
+                alive &= ~__ballot(occl);
+            }
+        }
+        // inner children reached by a live lane, nearest last: rank = the
+        // children before it (a larger key, ties by slot) from the entry t
+        // of the first lane reaching each
+        bool inner[4];
+        float key[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            inner[k] = cnv[k] < 0 && (hm[k] & alive) != 0;
+            const int rep = inner[k] ? __builtin_ctzll(hm[k] & alive) : 0;
+            key[k] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(te[k]), rep));
+        }
+        int rank[4] = {0, 0, 0, 0};
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int k = j + 1; k < 4; ++k) {
+                const bool jfirst = key[j] >= key[k];
+                rank[k] += (inner[j] && jfirst) ? 1 : 0;
+                rank[j] += (inner[k] && !jfirst) ? 1 : 0;
+            }
+        int ni = 0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            if (!inner[k]) continue;
+            ++ni;
+            const int q = sp + rank[k];
+            const uint64_t cm = hm[k] & alive;
+            if (lane == 0 && q < kPacketStack)  // (deeper than 64 levels: dropped)
+                S[q] = make_uint4((uint32_t)chv[k], (uint32_t)cm, (uint32_t)(cm >> 32), 0u);
+        }
+        sp = min(sp + ni, kPacketStack);
+    }
+    if constexpr (TILE)
+        write_tile_bits(bits_out, r, n, valid, occl);
+    else
+        write_wave_bits(bits_out, r - (tid & 63u), n, valid && occl);
+}
+
 // k_trace over a RADV image: nodes = the Box32 nodes (image + bvh_offset),
 // tris = the triangle nodes (image + leaf_nodes_offset).
 template <int PRIV, bool ANY, bool COUNT>
@@ -1073,6 +1221,7 @@ __global__ __launch_bounds__(kBlock) void k_trace_bvh2(
 
 struct TraceCfg {
     bool refill;
+    bool packet;            // RT_SCHED_PACKET (occlusion bits only)
     uint32_t per_wave;      // multiple of 64, <= kMaxPerWave
     uint32_t refill_lanes;  // 1..64
     uint32_t block;         // 64, 128 or 256
@@ -1121,6 +1270,20 @@ uint64_t launch_threads(const TraceCfg& cfg, uint32_t n, TileMap& tm) {
 bool small_priv(uint32_t depth, uint32_t lds) {
     const uint32_t need = 3u * depth + 1u;
     return need <= lds + (uint32_t)kPrivSmall;
+}
+
+rt_status launch_packet(hipStream_t s, const rt_bvh_dev* b, const rt_rays_soa* rays,
+                        uint32_t n, const TraceCfg& cfg, uint32_t* bits) {
+    TileMap tm;
+    const dim3 grid(ceil_div(launch_threads(cfg, n, tm), cfg.block)), block(cfg.block);
+    const size_t shm = (size_t)(cfg.block / 64) * kPacketStack * sizeof(uint4);
+    if (cfg.tile_width)
+        hipLaunchKernelGGL(k_trace_packet<true>, grid, block, shm, s, b->nodes, b->tris,
+                           b->node_count, *rays, n, bits, tm);
+    else
+        hipLaunchKernelGGL(k_trace_packet<false>, grid, block, shm, s, b->nodes, b->tris,
+                           b->node_count, *rays, n, bits, tm);
+    return launch_status("k_trace_packet");
 }
 
 template <int PRIV>
@@ -1211,10 +1374,12 @@ extern "C" rt_status rt_trace_ex(rt_stream stream, const rt_bvh_dev* bvh, const 
                                  uint32_t* tri_tests) {
     if (!bvh || !rays) return RT_ERR_ARG;
     if (flags & ~(RT_ANY | RT_COUNTERS | RT_TRACE_PERSISTENT)) return RT_ERR_ARG;
-    TraceCfg cfg{(flags & RT_TRACE_PERSISTENT) != 0, 1024u, 16u, (uint32_t)kBlock, kLdsStack};
+    TraceCfg cfg{(flags & RT_TRACE_PERSISTENT) != 0, false, 1024u, 16u, (uint32_t)kBlock,
+                 kLdsStack};
     if (opts) {
-        if (opts->schedule > RT_SCHED_REFILL) return RT_ERR_ARG;
+        if (opts->schedule > RT_SCHED_PACKET) return RT_ERR_ARG;
         cfg.refill = cfg.refill || opts->schedule == RT_SCHED_REFILL;
+        cfg.packet = opts->schedule == RT_SCHED_PACKET;
         if (opts->rays_per_wave) cfg.per_wave = opts->rays_per_wave;
         if (opts->refill_lanes) cfg.refill_lanes = opts->refill_lanes;
     }
@@ -1225,6 +1390,10 @@ extern "C" rt_status rt_trace_ex(rt_stream stream, const rt_bvh_dev* bvh, const 
     // ray per thread
     if (cfg.order != kOrderNear &&
         (!(flags & RT_ANY) || (flags & RT_COUNTERS) || (hits && hits->prim) || cfg.refill))
+        return RT_ERR_ARG;
+    // packets: occlusion bits only, their own order
+    if (cfg.packet && (!(flags & RT_ANY) || (flags & RT_COUNTERS) || (hits && hits->prim) ||
+                       cfg.refill || cfg.order != kOrderNear))
         return RT_ERR_ARG;
     // wave tiles: one ray per thread, whole rows
     if (cfg.tile_width && (cfg.refill || n % cfg.grid_width)) return RT_ERR_ARG;
@@ -1238,11 +1407,12 @@ extern "C" rt_status rt_trace_ex(rt_stream stream, const rt_bvh_dev* bvh, const 
     if ((flags & RT_COUNTERS) && (!box_tests || !tri_tests)) return RT_ERR_ARG;
     if (bvh->node_count > 0 && (!bvh->nodes || !bvh->tris)) return RT_ERR_ARG;
     if (bvh->format > RT_NODES_Q8) return RT_ERR_ARG;
-    if (bvh->format == RT_NODES_Q8 && cfg.refill) return RT_ERR_UNSUPPORTED;
+    if (bvh->format == RT_NODES_Q8 && (cfg.refill || cfg.packet)) return RT_ERR_UNSUPPORTED;
     // depth 0 = unknown (<= kMaxDepth)
     const uint32_t depth = bvh->depth ? bvh->depth : kMaxDepth;
     if (depth > kMaxDepth) return RT_ERR_CAPACITY;
     hipStream_t s = as_stream(stream);
+    if (cfg.packet) return launch_packet(s, bvh, rays, n, cfg, occl_bits);
     if (small_priv(depth, cfg.lds))
         return launch_v<kPrivSmall>(s, bvh, rays, n, flags, cfg, hits, occl_bits, box_tests,
                                     tri_tests);
@@ -1295,7 +1465,7 @@ extern "C" rt_status rt_trace_radv(rt_stream stream, const rt_radv_dev* img,
                                    uint32_t* tri_tests) {
     if (!img || !rays) return RT_ERR_ARG;
     if (flags & ~(RT_ANY | RT_COUNTERS)) return RT_ERR_ARG;
-    TraceCfg cfg{false, 0u, 0u, (uint32_t)kBlock, kLdsStack};
+    TraceCfg cfg{false, false, 0u, 0u, (uint32_t)kBlock, kLdsStack};
     if (opts && opts->schedule != RT_SCHED_SIMPLE) return RT_ERR_ARG;
     if (!shape_opts(opts, cfg)) return RT_ERR_ARG;
     // the sections must lie inside the image, 16-byte aligned for the loads
@@ -1375,7 +1545,7 @@ extern "C" rt_status rt_trace_bvh2(rt_stream stream, const rt_bvh2_dev* bvh,
                                    uint32_t* tri_tests) {
     if (!bvh || !rays) return RT_ERR_ARG;
     if (flags & ~(RT_ANY | RT_COUNTERS)) return RT_ERR_ARG;
-    TraceCfg cfg{false, 0u, 0u, (uint32_t)kBlock, kLdsStack};
+    TraceCfg cfg{false, false, 0u, 0u, (uint32_t)kBlock, kLdsStack};
     if (opts && opts->schedule != RT_SCHED_SIMPLE) return RT_ERR_ARG;
     if (!shape_opts(opts, cfg)) return RT_ERR_ARG;
     if (n == 0) return RT_OK;
@@ -1464,7 +1634,7 @@ extern "C" rt_status rt_trace_batch_ex(rt_stream stream, const rt_trace_item* d_
                                        const uint32_t* d_chunk_start, uint32_t count,
                                        uint32_t total_chunks, uint32_t depth,
                                        const rt_trace_opts* opts) {
-    TraceCfg cfg{false, 1024u, 16u, (uint32_t)kBlock, kLdsStack};
+    TraceCfg cfg{false, false, 1024u, 16u, (uint32_t)kBlock, kLdsStack};
     if (opts) {
         if (opts->schedule > RT_SCHED_REFILL) return RT_ERR_ARG;
         cfg.refill = opts->schedule == RT_SCHED_REFILL;
@@ -1572,7 +1742,7 @@ extern "C" rt_status rt_trace_scene_ex(rt_stream stream, const rt_instance* d_in
                                        const rt_rays_soa* rays, uint32_t n, uint32_t flags,
                                        const rt_trace_opts* opts, rt_hits_soa* hits,
                                        uint32_t* d_inst, uint32_t* occl_bits) {
-    TraceCfg cfg{false, 0u, 0u, (uint32_t)kBlock, kLdsStack};
+    TraceCfg cfg{false, false, 0u, 0u, (uint32_t)kBlock, kLdsStack};
     if (opts && opts->schedule != RT_SCHED_SIMPLE) return RT_ERR_ARG;
     if (!shape_opts(opts, cfg, kWithTiles)) return RT_ERR_ARG;
     if (!rays || (flags & ~RT_ANY)) return RT_ERR_ARG;

@@ -138,8 +138,10 @@ ANY_ORDERS = {"near": 0, "far": 1, "slot": 2}
 # Traversal schedule used when a call does not choose one: "simple" (one ray
 # per thread) or "refill" (waves walk ray chunks, refilling finished lanes),
 # with the chunk size, refill threshold, block size and LDS stack depth of
-# TRACE_TUNING.  Every choice gives the same results (autotune.py measures
-# them).
+# TRACE_TUNING; "packet" (a wave walks one node order for its rays) for
+# occlusion bits only.  Every choice gives the same results (autotune.py
+# measures them).
+SCHEDULES = {"simple": 0, "refill": 1, "packet": 2}
 DEFAULT_SCHEDULE = "simple"
 TRACE_TUNING = {"rays_per_wave": 1024, "refill_lanes": 16, "block": 256, "lds_stack": 20}
 TRACE_BLOCKS = (64, 128, 256)
@@ -150,10 +152,10 @@ def trace_opts(schedule=None, rays_per_wave=None, refill_lanes=None, block=None,
                lds_stack=None, any_order=None, tile=None):
     """tile: (grid_width, tile_width) wave tiles of a ray grid, or None."""
     s = schedule or DEFAULT_SCHEDULE
-    if s not in ("simple", "refill"):
+    if s not in SCHEDULES:
         raise ValueError(f"schedule {s!r}")
     gw, tw = tile if tile else (0, 0)
-    return rt_trace_opts(1 if s == "refill" else 0,
+    return rt_trace_opts(SCHEDULES[s],
                          rays_per_wave or TRACE_TUNING["rays_per_wave"],
                          refill_lanes or TRACE_TUNING["refill_lanes"],
                          block or TRACE_TUNING["block"],
