@@ -359,7 +359,9 @@ rt_status rt_trace(rt_stream stream, const rt_bvh_dev* bvh, const rt_rays_soa* r
  *   RT_SCHED_SIMPLE  one ray per thread, the hardware dispatcher balances.
  *   RT_SCHED_REFILL  each wave walks a chunk of rays_per_wave consecutive rays
  *                    and hands new rays to its finished lanes whenever at
- *                    least refill_lanes of 64 are idle.
+ *                    least refill_lanes of 64 are idle.  With wave tiles
+ *                    (occlusion bits only) the chunk is rays_per_wave / 64
+ *                    consecutive tiles, refilled tile by tile.
  *   RT_SCHED_PACKET  (rt_trace_ex, RT_ANY without RT_COUNTERS or hit ids,
  *                    RT_NODES_F32 trees) a wave walks one node order for
  *                    its 64 rays: each stack entry carries the lanes whose
@@ -380,14 +382,16 @@ rt_status rt_trace(rt_stream stream, const rt_bvh_dev* bvh, const rt_rays_soa* r
  *                    for every order (until an accepted hit ends a ray, every
  *                    node whose box meets [tmin, tmax] is visited); only the
  *                    work differs.  Other entry points take 0 only.
- *   grid_width,      (rt_trace_ex, RT_SCHED_SIMPLE) wave tiles: the n rays are
- *   tile_width       rows of grid_width consecutive rays (an image, or
- *                    several stacked: n a multiple of grid_width, grid_width
- *                    a multiple of tile_width); each wave traces a tile of
- *                    tile_width (8, 16, 32) rays across and 64 / tile_width
- *                    rows down instead of 64 consecutive rays.  Results are
- *                    written per ray as before (same values); only which rays
- *                    share a wave changes.  0 / 0 = linear waves.
+ *   grid_width,      (rt_trace_ex: RT_SCHED_SIMPLE, RT_SCHED_PACKET, and
+ *   tile_width       RT_SCHED_REFILL for occlusion bits only) wave tiles:
+ *                    the n rays are rows of grid_width consecutive rays (an
+ *                    image, or several stacked: n a multiple of grid_width,
+ *                    grid_width a multiple of tile_width); each wave traces
+ *                    a tile of tile_width (8, 16, 32) rays across and
+ *                    64 / tile_width rows down instead of 64 consecutive
+ *                    rays.  Results are written per ray as before (same
+ *                    values); only which rays share a wave changes.  0 / 0 =
+ *                    linear waves.
  * opts = NULL: RT_SCHED_SIMPLE (RT_TRACE_PERSISTENT in flags: RT_SCHED_REFILL). */
 #define RT_SCHED_SIMPLE 0u
 #define RT_SCHED_REFILL 1u

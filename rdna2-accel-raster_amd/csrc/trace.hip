@@ -262,6 +262,97 @@ __device__ __forceinline__ i4v radv_node(NP p, int32_t self, uint32_t boxes, uin
     return ids;
 }
 
+// Loads rt_node4 ni and tests its four slots (software_bvh_bvh4.comp:116-144
+// slab test + filter): entry t in te, hit slots in hit, child words and
+// counts in chv / cnv; returns the child-index vector (kept live by the
+// caller, see keep_loaded).  n?o / f?o: byte offsets of each axis's near and
+// far planes in the node (min plane near for a positive inverse direction).
+template <typename NP>
+__device__ __forceinline__ i4v slab_f32(NP __restrict__ nodes, int32_t ni, uint32_t nxo,
+                                        uint32_t nyo, uint32_t nzo, float ox, float oy,
+                                        float oz, float ix, float iy, float iz, float tminc,
+                                        float hitT, float (&te)[4], bool (&hit)[4],
+                                        int32_t (&chv)[4], int32_t (&cnv)[4]) {
+    const uint32_t fxo = nxo ^ 16u, fyo = nyo ^ 16u, fzo = nzo ^ 16u;
+    const auto nb = bytes(nodes + ni);
+    const f4v nx = *f4p(nb + nxo), fx = *f4p(nb + fxo), ny = *f4p(nb + nyo);
+    const f4v fy = *f4p(nb + fyo), nz = *f4p(nb + nzo), fz = *f4p(nb + fzo);
+    const i4v ch = i4(nodes + ni)[6];
+    const i4v cn = i4(nodes + ni)[7];
+    const float bnx[4] = {nx.x, nx.y, nx.z, nx.w}, bfx[4] = {fx.x, fx.y, fx.z, fx.w};
+    const float bny[4] = {ny.x, ny.y, ny.z, ny.w}, bfy[4] = {fy.x, fy.y, fy.z, fy.w};
+    const float bnz[4] = {nz.x, nz.y, nz.z, nz.w}, bfz[4] = {fz.x, fz.y, fz.z, fz.w};
+    chv[0] = ch.x; chv[1] = ch.y; chv[2] = ch.z; chv[3] = ch.w;
+    cnv[0] = cn.x; cnv[1] = cn.y; cnv[2] = cn.z; cnv[3] = cn.w;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const float tnx = (bnx[k] - ox) * ix, tfx = (bfx[k] - ox) * ix;
+        const float tny = (bny[k] - oy) * iy, tfy = (bfy[k] - oy) * iy;
+        const float tnz = (bnz[k] - oz) * iz, tfz = (bfz[k] - oz) * iz;
+        te[k] = fmaxf(fmaxf(tnx, tny), fmaxf(tnz, tminc));
+        const float tx = fminf(fminf(tfx, tfy), fminf(tfz, hitT));
+        hit[k] = cnv[k] != 0 && tx >= te[k];
+    }
+    return ch;
+}
+
+// After a node's slab tests, occlusion bits only (kOrderInner): inner hit
+// children pushed nearest on top, leaf hit children tested in slot order;
+// true when an accepted triangle ends the ray.  Rank r_k = inner hit
+// children before k (larger entry t, ties by slot), so child k goes to stack
+// entry sp + r_k: the near-first order of the inner children without
+// sorting.  col / st / sp / spo / ls / bs: the ray's stack as in RT_SPUSH.
+template <int PRIV, typename TP>
+__device__ __forceinline__ bool inner_push_leaves(
+    TP __restrict__ tris, const float (&te)[4], const bool (&hit)[4], const int32_t (&chv)[4],
+    const int32_t (&cnv)[4], const i4v& ch, int32_t* col, Stack<PRIV>& st, int& sp, int& spo,
+    int ls, int bs, float ox, float oy, float oz, float dx, float dy, float dz, float tmin,
+    float& hitT, uint32_t& hitPrim) {
+    bool inner[4];
+    uint32_t leafm = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        inner[k] = hit[k] && cnv[k] < 0;
+        leafm |= (hit[k] && cnv[k] > 0 ? 1u : 0u) << k;
+    }
+    keep_loaded(ch);
+    if (!(hit[0] || hit[1] || hit[2] || hit[3])) return false;
+    int rank[4] = {0, 0, 0, 0};
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+        for (int k = j + 1; k < 4; ++k) {
+            const bool jfirst = te[j] >= te[k];
+            rank[k] += (inner[j] && jfirst) ? 1 : 0;
+            rank[j] += (inner[k] && !jfirst) ? 1 : 0;
+        }
+    const int ni = (inner[0] ? 1 : 0) + (inner[1] ? 1 : 0) + (inner[2] ? 1 : 0) +
+                   (inner[3] ? 1 : 0);
+    if (sp + ni <= ls) {  // all four entries land in the LDS part
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            if (inner[k]) col[spo + rank[k] * bs] = chv[k];
+    } else {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int q = sp + rank[k];
+            // (large private part: sized for kMaxDepth levels; see RT_SPUSH)
+            if (!inner[k] || (PRIV == kPrivLarge && q >= ls + PRIV)) continue;
+            if (q < ls) col[q * bs] = chv[k];
+            else st.priv[q - ls] = chv[k];
+        }
+    }
+    const int nst = PRIV == kPrivLarge ? min(ni, ls + PRIV - sp) : ni;
+    sp += nst;
+    spo += nst * bs;
+    if (!leafm) return false;
+    const Child e[4] = {{te[0], chv[0], cnv[0]}, {te[1], chv[1], cnv[1]},
+                        {te[2], chv[2], cnv[2]}, {te[3], chv[3], cnv[3]}};
+    uint32_t triTests = 0;
+    return test_leaves<true, false>(tris, e, leafm, ox, oy, oz, dx, dy, dz, tmin, hitT, hitPrim,
+                                    triTests);
+}
+
 // One ray's traversal (the body shared by k_trace, k_trace_batch,
 // k_trace_scene and k_trace_radv).  col is the thread's LDS stack column (ls
 // entries, stride bs).  Returns with hitT / hitPrim (closest: nearest
@@ -299,7 +390,6 @@ __device__ __forceinline__ void trace_ray(NP __restrict__ nodes, TP __restrict__
     // which no comparison sees).
     const uint32_t nxo = ix > 0.0f ? 0u : 16u, nyo = iy > 0.0f ? 32u : 48u;
     const uint32_t nzo = iz > 0.0f ? 64u : 80u;
-    const uint32_t fxo = nxo ^ 16u, fyo = nyo ^ 16u, fzo = nzo ^ 16u;
     Stack<PRIV> st;
     int sp = 0, spo = 0;
     const int lso = (ls - 1) * bs;
@@ -372,78 +462,16 @@ __device__ __forceinline__ void trace_ray(NP __restrict__ nodes, TP __restrict__
                 hit[k] = cnv[k] != 0 && tx >= te[k];
             }
         } else {
-            const auto nb = bytes(nodes + ni);
-            const f4v nx = *f4p(nb + nxo), fx = *f4p(nb + fxo), ny = *f4p(nb + nyo);
-            const f4v fy = *f4p(nb + fyo), nz = *f4p(nb + nzo), fz = *f4p(nb + fzo);
-            ch = i4(nodes + ni)[6];
-            const i4v cn = i4(nodes + ni)[7];
-            const float bnx[4] = {nx.x, nx.y, nx.z, nx.w}, bfx[4] = {fx.x, fx.y, fx.z, fx.w};
-            const float bny[4] = {ny.x, ny.y, ny.z, ny.w}, bfy[4] = {fy.x, fy.y, fy.z, fy.w};
-            const float bnz[4] = {nz.x, nz.y, nz.z, nz.w}, bfz[4] = {fz.x, fz.y, fz.z, fz.w};
-            chv[0] = ch.x; chv[1] = ch.y; chv[2] = ch.z; chv[3] = ch.w;
-            cnv[0] = cn.x; cnv[1] = cn.y; cnv[2] = cn.z; cnv[3] = cn.w;
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {  // :116-144 slab test + filter
-                const float tnx = (bnx[k] - ox) * ix, tfx = (bfx[k] - ox) * ix;
-                const float tny = (bny[k] - oy) * iy, tfy = (bfy[k] - oy) * iy;
-                const float tnz = (bnz[k] - oz) * iz, tfz = (bfz[k] - oz) * iz;
-                te[k] = fmaxf(fmaxf(tnx, tny), fmaxf(tnz, tminc));
-                const float tx = fminf(fminf(tfx, tfy), fminf(tfz, hitT));
-                hit[k] = cnv[k] != 0 && tx >= te[k];
-            }
+            ch = slab_f32(nodes, ni, nxo, nyo, nzo, ox, oy, oz, ix, iy, iz, tminc, hitT, te, hit,
+                          chv, cnv);
         }
         if (COUNT) boxTests += 4;
 
         if constexpr (ORDER == kOrderInner) {
             static_assert(ANY && !COUNT && !RADV, "for occlusion bits only");
-            // Inner children: rank r_k = inner hit children before k (larger
-            // entry t, ties by slot), pushed farthest first, so the nearest
-            // pops first -- the near-first order of the inner children, with
-            // each child written to stack entry sp + r_k instead of sorting.
-            bool inner[4];
-            uint32_t leafm = 0;
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                inner[k] = hit[k] && cnv[k] < 0;
-                leafm |= (hit[k] && cnv[k] > 0 ? 1u : 0u) << k;
-            }
-            keep_loaded(ch);
-            if (!(hit[0] || hit[1] || hit[2] || hit[3])) continue;
-            int rank[4] = {0, 0, 0, 0};
-#pragma unroll
-            for (int j = 0; j < 4; ++j)
-#pragma unroll
-                for (int k = j + 1; k < 4; ++k) {
-                    const bool jfirst = te[j] >= te[k];
-                    rank[k] += (inner[j] && jfirst) ? 1 : 0;
-                    rank[j] += (inner[k] && !jfirst) ? 1 : 0;
-                }
-            const int ni = (inner[0] ? 1 : 0) + (inner[1] ? 1 : 0) + (inner[2] ? 1 : 0) +
-                           (inner[3] ? 1 : 0);
-            if (sp + ni <= ls) {  // all four entries land in the LDS part
-#pragma unroll
-                for (int k = 0; k < 4; ++k)
-                    if (inner[k]) col[spo + rank[k] * bs] = chv[k];
-            } else {
-#pragma unroll
-                for (int k = 0; k < 4; ++k) {
-                    const int q = sp + rank[k];
-                    // (large private part: sized for kMaxDepth levels; see RT_SPUSH)
-                    if (!inner[k] || (PRIV == kPrivLarge && q >= ls + PRIV)) continue;
-                    if (q < ls) col[q * bs] = chv[k];
-                    else st.priv[q - ls] = chv[k];
-                }
-            }
-            const int nst = PRIV == kPrivLarge ? min(ni, ls + PRIV - sp) : ni;
-            sp += nst;
-            spo += nst * bs;
-            if (leafm) {
-                const Child e[4] = {{te[0], chv[0], cnv[0]}, {te[1], chv[1], cnv[1]},
-                                    {te[2], chv[2], cnv[2]}, {te[3], chv[3], cnv[3]}};
-                if (test_leaves<ANY, COUNT>(tris, e, leafm, ox, oy, oz, dx, dy, dz, tmin, hitT,
-                                            hitPrim, triTests))
-                    break;  // the first accepted triangle ends the ray
-            }
+            if (inner_push_leaves<PRIV>(tris, te, hit, chv, cnv, ch, col, st, sp, spo, ls, bs, ox,
+                                        oy, oz, dx, dy, dz, tmin, hitT, hitPrim))
+                break;  // the first accepted triangle ends the ray
             continue;
         }
         // :149-156 exchange sort of the hit children, run in place over the 4
@@ -1072,6 +1100,116 @@ __global__ __launch_bounds__(kBlock) void k_trace_persist(
                                   prim_out, t_out, bits_out, box_out, tri_out);
 }
 
+// Refill form for occlusion bits only (no counters, no hit ids), with or
+// without wave tiles: the node step of k_trace's bits-only path (near / far
+// planes by offset, inner children at their ranks, leaf children in slot
+// order) inside refill_wave's loop.  Wave w owns the launch's thread slots
+// [w * per_wave, (w + 1) * per_wave) in k_trace's order (with tiles: tile
+// by tile, each tile's 64 slots row by row; slots of a partial tile row
+// below the grid hold no ray), so the rays a wave refills from are its next
+// tiles.  Each finished ray's bit goes to the wave's LDS words at its slot;
+// at the end every 8 slots (8 consecutive rays of one row: tile widths and
+// the row width are multiples of 8) become one byte of the bit array, as in
+// write_tile_bits.  A ray's bit does not depend on when it is traced.
+template <int PRIV, bool TILE>
+__global__ __launch_bounds__(kBlock) void k_trace_refill_bits(
+    const rt_node4* __restrict__ nodes, const rt_tri64* __restrict__ tris, uint32_t node_count,
+    rt_rays_soa rays, uint32_t n, uint32_t slots, uint32_t per_wave, uint32_t refill,
+    uint32_t lsu, uint32_t* __restrict__ bits_out, TileMap tmap) {
+    // dynamic: lsu * blockDim.x stack entries, then per_wave / 32 words per wave
+    extern __shared__ int32_t lds_stack[];
+    const uint32_t tid = threadIdx.x, lane = tid & 63u;
+    const int bs = (int)blockDim.x, ls = (int)lsu;
+    int32_t* col = lds_stack + tid;
+    uint32_t* wbits = reinterpret_cast<uint32_t*>(lds_stack + ls * bs) + (tid >> 6) * (per_wave >> 5);
+    const uint64_t w0 = (uint64_t)((blockIdx.x * blockDim.x + tid) >> 6) * per_wave;
+    const uint32_t beg = (uint32_t)min<uint64_t>(w0, slots);  // a multiple of 64
+    const uint32_t end = (uint32_t)min<uint64_t>(w0 + per_wave, slots);
+    for (uint32_t i = lane; i < (per_wave >> 5); i += 64) wbits[i] = 0u;
+
+    bool active = false;
+    uint32_t slot = 0;  // the lane's ray: slot - beg in the wave's words
+    float ox = 0.f, oy = 0.f, oz = 0.f, tmin = 1.f, dx = 0.f, dy = 0.f, dz = 1.f;
+    float ix = 0.f, iy = 0.f, iz = 0.f, hitT = 0.f, tminc = -FLT_MAX;
+    uint32_t nxo = 0u, nyo = 32u, nzo = 64u, hitPrim = RT_MISS;
+    Stack<PRIV> st;
+    int sp = 0, spo = 0;
+    const int lso = (ls - 1) * bs;
+    uint32_t cur = beg;
+
+    for (;;) {
+        const uint64_t idle = __ballot(!active);
+        if (idle == ~0ull && cur >= end) break;
+        if (cur < end && ((uint32_t)__popcll(idle) >= refill || idle == ~0ull)) {
+            if (!active) {
+                // idle lanes below this one: its place among the new rays
+                const uint32_t below = __builtin_amdgcn_mbcnt_hi(
+                    (uint32_t)(idle >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)idle, 0u));
+                const uint32_t g = cur + below;
+                bool valid = g < end;
+                uint32_t r = g;
+                if constexpr (TILE) {
+                    if (valid) r = tile_ray(g, tmap, valid);
+                }
+                if (valid && node_count > 0) {
+                    slot = g;
+                    ox = rays.ox[r]; oy = rays.oy[r]; oz = rays.oz[r]; tmin = rays.tmin[r];
+                    dx = rays.dx[r]; dy = rays.dy[r]; dz = rays.dz[r]; hitT = rays.tmax[r];
+                    // software_bvh_bvh4.comp:96-100, planes as in trace_ray
+                    ix = fabsf(dx) > 1e-6f ? 1.0f / dx : 1e6f;
+                    iy = fabsf(dy) > 1e-6f ? 1.0f / dy : 1e6f;
+                    iz = fabsf(dz) > 1e-6f ? 1.0f / dz : 1e6f;
+                    tminc = fmaxf(tmin, -FLT_MAX);
+                    nxo = ix > 0.0f ? 0u : 16u;
+                    nyo = iy > 0.0f ? 32u : 48u;
+                    nzo = iz > 0.0f ? 64u : 80u;
+                    hitPrim = RT_MISS;
+                    sp = 0;
+                    spo = 0;
+                    RT_SPUSH(0);
+                    active = true;
+                }
+            }
+            cur = (uint32_t)min<uint64_t>((uint64_t)cur + __popcll(idle), end);
+        }
+        if (!active) continue;
+
+        bool ended = false;
+        if (sp > 0) {
+            int32_t ni;
+            RT_SPOP(ni);
+            float te[4];
+            bool hit[4];
+            int32_t chv[4], cnv[4];
+            const i4v ch = slab_f32(nodes, ni, nxo, nyo, nzo, ox, oy, oz, ix, iy, iz, tminc,
+                                    hitT, te, hit, chv, cnv);
+            ended = inner_push_leaves<PRIV>(tris, te, hit, chv, cnv, ch, col, st, sp, spo, ls, bs,
+                                            ox, oy, oz, dx, dy, dz, tmin, hitT, hitPrim);
+        }
+        if (ended || sp == 0) {
+            if (ended) atomicOr(wbits + ((slot - beg) >> 5), 1u << ((slot - beg) & 31u));
+            active = false;
+        }
+    }
+    wave_lds_sync();  // the wave's atomics before its flush
+    if constexpr (TILE) {
+        auto bytes_out = reinterpret_cast<uint8_t*>(bits_out);
+        for (uint32_t m = lane; m < ((end - beg) >> 3); m += 64) {
+            bool valid;
+            const uint32_t r0 = tile_ray(beg + 8u * m, tmap, valid);
+            if (!valid) continue;
+            bytes_out[r0 >> 3] = (uint8_t)(wbits[m >> 2] >> ((m & 3u) * 8u));
+            // the grid's last byte: clear the bytes after it up to the end of
+            // the last 32-bit word (as the linear kernels leave them zero)
+            if (r0 + 8 == n)
+                for (uint32_t b = n >> 3; b < ((n + 31) >> 5) * 4; ++b) bytes_out[b] = 0;
+        }
+    } else {  // whole words (beg is a multiple of 64; bits past n stay zero)
+        for (uint32_t i = lane; i < ((end - beg + 31) >> 5); i += 64)
+            bits_out[(beg >> 5) + i] = wbits[i];
+    }
+}
+
 // Batched trace, refill form: item i owns the chunks [chunk[i], chunk[i+1])
 // of per_wave rays each (rt_trace_batch_plan_ex with chunk_rays = per_wave);
 // each wave takes one chunk (tree, rays and mode wave-uniform) and walks it
@@ -1295,6 +1433,24 @@ rt_status launch_v(hipStream_t s, const rt_bvh_dev* b, const rt_rays_soa* rays, 
     uint32_t* prim = hits ? hits->prim : nullptr;
     float* t = hits ? hits->t : nullptr;
     const size_t shm = lds_bytes(cfg);
+    if (cfg.refill && any && !cnt && !prim) {
+        // occlusion bits only: the bits-only node step; waves x per_wave
+        // covers the launch's slots (n, or 64 per wave tile)
+        TileMap tm;
+        const uint64_t slots = launch_threads(cfg, n, tm);
+        if (slots > 0xFFFFFFFFull) return RT_ERR_CAPACITY;
+        const uint32_t blocks = ceil_div(slots, (uint64_t)cfg.per_wave * (cfg.block / 64));
+        const size_t shm_bits = shm + (size_t)(cfg.block / 64) * (cfg.per_wave / 32) * 4;
+        if (cfg.tile_width)
+            hipLaunchKernelGGL((k_trace_refill_bits<PRIV, true>), dim3(blocks), dim3(cfg.block),
+                               shm_bits, s, b->nodes, b->tris, b->node_count, *rays, n,
+                               (uint32_t)slots, cfg.per_wave, cfg.refill_lanes, cfg.lds, bits, tm);
+        else
+            hipLaunchKernelGGL((k_trace_refill_bits<PRIV, false>), dim3(blocks), dim3(cfg.block),
+                               shm_bits, s, b->nodes, b->tris, b->node_count, *rays, n,
+                               (uint32_t)slots, cfg.per_wave, cfg.refill_lanes, cfg.lds, bits, tm);
+        return launch_status("k_trace_refill_bits");
+    }
     if (cfg.refill) {
         // waves x per_wave covers [0, n); per_wave is a multiple of 64 and at
         // most kMaxPerWave, so each wave owns whole occlusion words
@@ -1395,8 +1551,9 @@ extern "C" rt_status rt_trace_ex(rt_stream stream, const rt_bvh_dev* bvh, const 
     if (cfg.packet && (!(flags & RT_ANY) || (flags & RT_COUNTERS) || (hits && hits->prim) ||
                        cfg.refill || cfg.order != kOrderNear))
         return RT_ERR_ARG;
-    // wave tiles: one ray per thread, whole rows
-    if (cfg.tile_width && (cfg.refill || n % cfg.grid_width)) return RT_ERR_ARG;
+    // wave tiles: whole rows; one ray per thread, or refill for occlusion bits only
+    const bool bits_only = (flags & RT_ANY) && !(flags & RT_COUNTERS) && !(hits && hits->prim);
+    if (cfg.tile_width && ((cfg.refill && !bits_only) || n % cfg.grid_width)) return RT_ERR_ARG;
     if (n == 0) return RT_OK;
     if (!rays->ox || !rays->oy || !rays->oz || !rays->tmin || !rays->dx || !rays->dy ||
         !rays->dz || !rays->tmax)

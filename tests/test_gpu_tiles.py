@@ -3,7 +3,8 @@ tile width gives per-ray results identical to the linear launch (closest-hit
 ids and t bits, any-hit bits including the zeroed padding of the last word,
 box / triangle counters) on a frame's camera rays and its 16-light shadow
 rays, on a grid whose rows are not a multiple of the tile height, and with
-the any-hit child orders; bad grids are refused."""
+the any-hit child orders; bad grids, and refill tiles for queries other
+than occlusion bits, are refused."""
 import numpy as np
 import pytest
 import torch
@@ -78,5 +79,8 @@ def test_bad_grids_are_refused(rt, frame):
         bvh.intersect(prim_rays, tile=(W, 64))  # tile width not 8, 16 or 32
     with pytest.raises(rt.RtError):
         bvh.intersect(prim_rays, tile=(40, 16))  # grid width not a multiple of the tile
+    # refill with wave tiles: occlusion bits only
     with pytest.raises(rt.RtError):
-        bvh.occluded(prim_rays, tile=(W, 8), schedule="refill")
+        bvh.intersect(prim_rays, tile=(W, 8), schedule="refill")
+    with pytest.raises(rt.RtError):
+        bvh.occluded(prim_rays, tile=(W, 8), schedule="refill", counters=True)
