@@ -235,7 +235,8 @@ class Frame:
                 self.primary_opts = tuner.trace(self.bvh, self.primary, "closest",
                                                 tag=f"primary-{tag}", grid_width=W)
                 self.shadow_opts = tuner.trace(self.bvh, self.shadow, "any",
-                                               tag=f"shadow-{tag}", grid_width=W)
+                                               tag=f"shadow-{tag}", grid_width=W,
+                                               hint=(self.prim, self.npix))
                 trees[(leaf, rounds)] = (self.bvh, self.build_ms, self.primary_opts,
                                          self.shadow_opts)
             self.bvh, self.build_ms, self.primary_opts, self.shadow_opts = trees[(leaf, rounds)]
@@ -255,7 +256,8 @@ class Frame:
                            out=self.shadow)
         if events:
             events[0].record()
-        self.bvh.occluded(self.shadow, out=self.bits, **self.shadow_opts)
+        self.bvh.occluded(self.shadow, out=self.bits,
+                          **autotune.hinted_kwargs(self.shadow_opts, (self.prim, self.npix)))
         if events:
             events[1].record()
         rt.pack_light_masks(self.bits, self.npix, NLIGHTS, out=self.masks)
@@ -850,12 +852,20 @@ def main():
     traffic = None
     tp = os.path.join(ROOT, "profiles", "traffic_shadow_anyhit_2p5m.json")
     tile = f.shadow_opts.get("tile")
+    hinted = bool(f.shadow_opts.get("hinted"))
     if os.path.exists(tp):
         with open(tp) as fh:
             for e in json.load(fh).get("entries", []):
                 if (e["max_leaf_prims"], e["treelet_rounds"]) == (f.leaf, f.rounds) and \
-                        tile is not None and e["tile_rows"] == tile[1]:
+                        tile is not None and e["tile_rows"] == tile[1] and \
+                        bool(e.get("hinted", False)) == hinted:
                     traffic = e["hbm_bytes_per_launch"]
+    if hinted:
+        kernel = "k_hint_pass + k_hint_trace (hinted occlusion of the shadow rays)"
+    elif f.shadow_opts.get("schedule", a.schedule) == "refill":
+        kernel = "k_trace_refill_bits any-hit (shadow)"
+    else:
+        kernel = "k_trace any-hit (shadow)"
 
     out = {
         "metric": METRIC,
@@ -875,16 +885,14 @@ def main():
                    "tris": f.bvh.tri_count, "width": W, "height": H, "lights": NLIGHTS,
                    "rays_per_step": f.rays_per_step, "builder": f.builder,
                    "sort_bits": f.sort_bits,
-                   "schedule": f.shadow_opts.get("schedule", a.schedule),
+                   "schedule": "hinted" if hinted else f.shadow_opts.get("schedule", a.schedule),
                    "primary_trace": f.primary_opts or "default",
                    "shadow_trace": f.shadow_opts or "default",
                    "bvh_nodes": f.bvh.node_count, "bvh_depth": f.bvh.depth,
                    "max_leaf_prims": f.leaf, "treelet_rounds": f.rounds},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": traffic, "kernel": "k_trace_persist any-hit (shadow)"
-                     if f.shadow_opts.get("schedule", a.schedule) == "refill"
-                     else "k_trace any-hit (shadow)",
+                     "traffic": traffic, "kernel": kernel,
                      "kernel_ms": round(kern_avg, 4), "algorithmic_bytes": int(ab)},
         "build_us_4k_p50": round(b4k_p50, 2),
         "build_us_4k_p99": round(b4k_p99, 2),

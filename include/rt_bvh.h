@@ -413,6 +413,53 @@ rt_status rt_trace_ex(rt_stream stream, const rt_bvh_dev* bvh, const rt_rays_soa
                       uint32_t n, uint32_t flags, const rt_trace_opts* opts, rt_hits_soa* hits,
                       uint32_t* occl_bits, uint32_t* box_tests, uint32_t* tri_tests);
 
+/* Occlusion bits with occluder hints: the shadow ray queries of
+ * benchmarks/expB_shadows/shaders/lighting_rayquery_tiled.comp:110-129 and
+ * shaders/tiled_lighting_ra.comp (TerminateOnFirstHit queries from surface
+ * points), given per ray a triangle likely to occlude it -- for shadow rays
+ * the triangle the primary ray hit (a shadow cache: Haines & Greenberg 1986,
+ * "The light buffer").  The bits equal rt_trace_ex(RT_ANY) on the same tree
+ * and rays, for any hints: a hint settles a ray only when the triangle passes
+ * the hit predicate in [tmin, tmax) AND its own box (min / max of its
+ * vertices) passes the traversal's slab test, which implies that the
+ * traversal reaches it (the slab test is monotone in the box and every slot
+ * box contains the boxes of the triangles under it: true of every tree this
+ * library builds or refits and of a SoftwareBVH4 upload); rays no hint
+ * settles are traversed.
+ *   prim    hint slot s: a triangle id (triangle_id of the leaf records), or
+ *           RT_MISS / any id >= ids for none; ray r uses slot r % period
+ *           (period 0 = n), e.g. the primary hits of a frame for its shadow
+ *           rays laid out light by light (r = light * pixels + pixel).
+ *   record  triangle id -> leaf record index, ids entries (rt_prim_records).
+ *   flags   RT_HINT_WAVE: a ray also tries the hints of the other rays of its
+ *           wave of 64 (wave tiles: its tile), one triangle for the wave at a
+ *           time.
+ * opts: block, lds_stack, grid_width / tile_width as rt_trace_ex; the rays
+ * the hints leave open are traced with the refill schedule (refill_lanes) by
+ * persistent waves that take segments of rays_per_wave ray slots (a
+ * multiple of 64, at most 4096) one after another.  Scratch: device memory
+ * of rt_trace_occluded_hinted_scratch_bytes bytes (256-B aligned), reusable
+ * once the stream has passed the call.  RT_NODES_F32 trees only.           */
+#define RT_HINT_WAVE 1u
+typedef struct rt_occluder_hint {
+    const uint32_t* prim;
+    const uint32_t* record;
+    uint32_t period;
+    uint32_t ids;
+    uint32_t flags;
+} rt_occluder_hint;
+rt_status rt_trace_occluded_hinted_scratch_bytes(uint32_t n, const rt_trace_opts* opts,
+                                                 const rt_occluder_hint* hint, size_t* bytes);
+rt_status rt_trace_occluded_hinted(rt_stream stream, const rt_bvh_dev* bvh,
+                                   const rt_rays_soa* rays, uint32_t n, const rt_trace_opts* opts,
+                                   const rt_occluder_hint* hint, uint32_t* occl_bits,
+                                   void* d_scratch, size_t scratch_bytes);
+/* record[id] = the leaf record holding triangle id (RT_MISS where none), for
+ * ids ids, from the tree's leaf records (no reference counterpart: the
+ * reference's GpuTriangle records carry no id).                            */
+rt_status rt_prim_records(rt_stream stream, const rt_bvh_dev* bvh, uint32_t ids,
+                          uint32_t* record);
+
 /* Software traversal of a RADV image in device memory: the consumer the
  * reference's hardware ray queries are for (rayQueryEXT over the
  * fast_bvh_builder output, src/main.cpp:871-885 then the a17 shaders), with

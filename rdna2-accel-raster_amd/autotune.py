@@ -13,6 +13,9 @@ configuration's bit for bit (a mismatch raises: it would be a kernel bug).
                                (the wave's ray tile) and refill_lanes
   batched trace (rt_trace_batch_ex)   the same knobs (the refill chunk is a
                                wave's share of one item)
+  hinted occlusion (rt_trace_occluded_hinted)  block, lds_stack, wave
+                               tiles, segment slots, refill_lanes (when the
+                               caller has occluder hints for the rays)
   two-level scene (rt_trace_scene_ex)  block, lds_stack
   device build (rt_gpu_build)  sort_bits: digit width of the Morton radix
                                sort, 4 / 6 / 8 / 10 bits = 8 / 5 / 4 / 3 passes
@@ -40,11 +43,22 @@ REFILL_LANES = (8, 16, 32)
 # (7.1 vs 3.8 GB of L2 misses per launch), so a tie keeps 16x4
 TILE_WIDTHS = (16, 8, 32)
 PACKET_TILE_WIDTHS = (16, 8)  # packet traversal (occlusion bits) of 16x4 / 8x8 tiles
+HINT_SEGMENTS = (256, 512, 1024)  # slots per segment of the hinted trace
+HINT_REFILL_LANES = (16, 32)
 # a later candidate must be faster than the best so far by more than this
 # fraction (the spread of repeated timings of one candidate) to replace it
 TIE_MARGIN = 0.01
 LEAF_SIZES = (4, 3, 2, 1)  # radix-tree leaf sizes of a frame's tree, the default first
 SAMPLE_ROW_BLOCK = 8       # rows per block of a grid sample (a multiple of every tile height)
+
+
+def hinted_kwargs(choice, hint):
+    """The keyword arguments of bvh.occluded for a tuned choice: a hinted
+    choice ("hinted": True) gets the hints hint = (triangle ids, period)."""
+    kw = dict(choice)
+    if kw.pop("hinted", False):
+        kw.update(hint=hint[0], hint_period=hint[1])
+    return kw
 
 
 def _grid_sample(rays, width, m):
@@ -175,12 +189,17 @@ class Tuner:
         return best, best_ms, table
 
     # -- trace -----------------------------------------------------------------
-    def trace(self, bvh, rays, mode, tag="", grid_width=None):
+    def trace(self, bvh, rays, mode, tag="", grid_width=None, hint=None):
         """Schedule and launch shape for bvh.intersect / bvh.occluded on rays
         like these (a leading slice of at most sample_rays is measured).
         grid_width: the rays are rows of that many rays (an image), which
         adds wave tiles of the grid to the candidates (the slice is then
-        whole rows).  Returns the keyword arguments for intersect / occluded."""
+        whole rows).  hint = (triangle ids, period): occluder hints of the
+        rays (mode "any", measured on the whole ray set only), which adds the
+        hinted trace (rt_trace_occluded_hinted) to the candidates; a hinted
+        choice carries "hinted": True, for which the caller passes
+        hint= / hint_period= (hinted_kwargs).  Returns the keyword arguments
+        for intersect / occluded."""
         n = len(rays)
         m = min(n, self.sample_rays)
         sample = rays if m == n else (_grid_sample(rays, grid_width, m) if grid_width
@@ -190,10 +209,11 @@ class Tuner:
                f"|r{_size_class(m)}" + (f"|g{grid_width}" if grid_width else ""))
         fn = bvh.intersect if mode == "closest" else bvh.occluded
         out = None
+        hinted = hint is not None and mode == "any" and m == n
 
         def run(c):
             nonlocal out
-            out = fn(sample, out=out, **c)
+            out = fn(sample, out=out, **hinted_kwargs(c, hint))
             return out
 
         def check(c):
@@ -218,7 +238,17 @@ class Tuner:
             {"schedule": "packet", "block": b, "tile": (grid_width, tw)}
             for tw in PACKET_TILE_WIDTHS for b in (64, 256)
             if grid_width and grid_width % tw == 0]
-        return self._pick(key, [best] + refill + orders + tiles + packets, run, check)
+        # occluder hints: the hinted trace over segments of HINT_SEGMENTS
+        # slots, linear waves and the grid's wave tiles
+        hints = [] if not hinted else [
+            {"hinted": True, "block": best["block"], "lds_stack": best["lds_stack"],
+             "rays_per_wave": w, "refill_lanes": r, **({"tile": t} if t else {})}
+            for t in [None] + [(grid_width, tw) for tw in TILE_WIDTHS[:1]
+                               if grid_width and grid_width % tw == 0]
+            for w in HINT_SEGMENTS for r in HINT_REFILL_LANES]
+        if hinted:
+            key += "|hint"
+        return self._pick(key, [best] + refill + orders + tiles + packets + hints, run, check)
 
     def trace_scene(self, scene, rays, mode, tag="", grid_width=None):
         """Launch shape (block, lds_stack, and wave tiles of the ray grid when

@@ -181,7 +181,10 @@ struct Stack {
 // triangle); per wave the loop runs max over lanes of the lane's triangle
 // count instead of, position by position, the max over lanes of that
 // position's count (divergent lanes share the one loop body).
-template <bool ANY, bool COUNT, typename TP>
+// REC: hitPrim receives the accepted triangle's leaf record index instead of
+// its triangle_id (for callers that use the hit only as an occlusion bit, or
+// want the record).
+template <bool ANY, bool COUNT, typename TP, bool REC = false>
 __device__ __forceinline__ bool test_leaves(TP __restrict__ tris, const Child (&e)[4],
                                             uint32_t pend, float ox, float oy, float oz,
                                             float dx, float dy, float dz, float tmin,
@@ -210,7 +213,7 @@ __device__ __forceinline__ bool test_leaves(TP __restrict__ tris, const Child (&
         if (mt_hit(a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w, c, ox, oy, oz, dx, dy, dz, &t) &&
             t >= tmin && t < hitT) {
             hitT = t;
-            hitPrim = tris[first].triangle_id;
+            hitPrim = REC ? first : tris[first].triangle_id;
             if (ANY) return true;
         }
         ++first;
@@ -298,7 +301,7 @@ __device__ __forceinline__ i4v slab_f32(NP __restrict__ nodes, int32_t ni, uint3
 
 // After a node's slab tests, occlusion bits only (kOrderInner): inner hit
 // children pushed nearest on top, leaf hit children tested in slot order;
-// true when an accepted triangle ends the ray.  Rank r_k = inner hit
+// true when an accepted triangle ends the ray (hitPrim = its leaf record).  Rank r_k = inner hit
 // children before k (larger entry t, ties by slot), so child k goes to stack
 // entry sp + r_k: the near-first order of the inner children without
 // sorting.  col / st / sp / spo / ls / bs: the ray's stack as in RT_SPUSH.
@@ -349,8 +352,8 @@ __device__ __forceinline__ bool inner_push_leaves(
     const Child e[4] = {{te[0], chv[0], cnv[0]}, {te[1], chv[1], cnv[1]},
                         {te[2], chv[2], cnv[2]}, {te[3], chv[3], cnv[3]}};
     uint32_t triTests = 0;
-    return test_leaves<true, false>(tris, e, leafm, ox, oy, oz, dx, dy, dz, tmin, hitT, hitPrim,
-                                    triTests);
+    return test_leaves<true, false, TP, true>(tris, e, leafm, ox, oy, oz, dx, dy, dz, tmin, hitT,
+                                              hitPrim, triTests);
 }
 
 // One ray's traversal (the body shared by k_trace, k_trace_batch,
@@ -1210,6 +1213,259 @@ __global__ __launch_bounds__(kBlock) void k_trace_refill_bits(
     }
 }
 
+// Occluder hints (rt_trace_occluded_hinted): occlusion bits only.
+//
+// For a tree whose slot boxes contain the boxes of the triangles under them
+// (every tree this library builds or refits, and a SoftwareBVH4 upload), the
+// traversal's occlusion bit of a ray is 1 exactly when some triangle T passes
+// the predicate (t in [tmin, tmax)) while the box of T's leaf slot passes the
+// slab test.  The slab test is monotone in the box: under round-to-nearest a
+// box containing another has a near t no later and a far t no earlier, so
+// every slot box on the way down to T's slot passes as well; T is then
+// reached unless another accepted triangle ends the ray first, and either way
+// the bit is 1.  T's own box (the min / max of its vertices) lies in its slot
+// box, so a triangle that passes the predicate and whose own box passes the
+// slab test proves the bit without a traversal: the hint test.  A ray whose
+// hints all fail is traced by the bits-only node step of k_trace_refill_bits,
+// so its bit is the traversal's.
+struct HintArgs {
+    const uint32_t* prim;    // hint triangle id per slot, RT_MISS for none
+    const uint32_t* record;  // triangle id -> leaf record, RT_MISS for none
+    uint32_t period;         // ray r's hint: prim[r % period]
+    uint32_t ids;            // entries of record
+    uint32_t wave;           // also try the hints of the other rays of the 64
+};
+
+// The hint test: leaf record rec through the leaf predicate of test_leaves,
+// then its own box through the slab test of slab_f32 (min / max of t0, t1
+// per axis: the same values as its near / far plane selection).
+template <typename TP>
+__device__ __forceinline__ bool hint_occludes(TP __restrict__ tris, uint32_t rec, float ox,
+                                              float oy, float oz, float dx, float dy, float dz,
+                                              float ix, float iy, float iz, float tmin,
+                                              float tminc, float tmax) {
+    const auto tp = f4(tris + rec);
+    const f4v a = tp[0], b = tp[1];
+    const float c = f1(tris + rec, 8);
+    float t;
+    if (!mt_hit(a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w, c, ox, oy, oz, dx, dy, dz, &t) ||
+        !(t >= tmin && t < tmax))
+        return false;
+    // vertices v0 = a.xyz, v1 = (a.w, b.x, b.y), v2 = (b.z, b.w, c)
+    const float mnx = fminf(fminf(a.x, a.w), b.z), mxx = fmaxf(fmaxf(a.x, a.w), b.z);
+    const float mny = fminf(fminf(a.y, b.x), b.w), mxy = fmaxf(fmaxf(a.y, b.x), b.w);
+    const float mnz = fminf(fminf(a.z, b.y), c), mxz = fmaxf(fmaxf(a.z, b.y), c);
+    const float t0x = (mnx - ox) * ix, t1x = (mxx - ox) * ix;
+    const float t0y = (mny - oy) * iy, t1y = (mxy - oy) * iy;
+    const float t0z = (mnz - oz) * iz, t1z = (mxz - oz) * iz;
+    const float te = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), tminc));
+    const float tx = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fminf(fmaxf(t0z, t1z), tmax));
+    return tx >= te;
+}
+
+// One ray of the launch: slot g's ray (the slot itself, or its wave-tile ray).
+struct HRay {
+    float ox, oy, oz, tmin, dx, dy, dz, tmax, ix, iy, iz, tminc;
+};
+template <bool TILE>
+__device__ __forceinline__ HRay load_hray(const rt_rays_soa& rays, uint32_t g,
+                                          const TileMap& tmap) {
+    bool valid = true;
+    uint32_t r = g;
+    if constexpr (TILE) r = tile_ray(g, tmap, valid);
+    HRay q;
+    q.ox = rays.ox[r]; q.oy = rays.oy[r]; q.oz = rays.oz[r]; q.tmin = rays.tmin[r];
+    q.dx = rays.dx[r]; q.dy = rays.dy[r]; q.dz = rays.dz[r]; q.tmax = rays.tmax[r];
+    // software_bvh_bvh4.comp:96-100
+    q.ix = fabsf(q.dx) > 1e-6f ? 1.0f / q.dx : 1e6f;
+    q.iy = fabsf(q.dy) > 1e-6f ? 1.0f / q.dy : 1e6f;
+    q.iz = fabsf(q.dz) > 1e-6f ? 1.0f / q.dz : 1e6f;
+    q.tminc = fmaxf(q.tmin, -FLT_MAX);
+    return q;
+}
+
+// The passes of rt_trace_occluded_hinted share an open mask over the launch's
+// thread slots (k_trace's order; with wave tiles, tile by tile): bit g set =
+// slot g's ray is still open.  The slots are cut into segments of
+// seg_words * 32 slots; the trace pass streams the open rays segment after
+// segment.
+struct HintPlan {
+    uint32_t* open;      // nseg * seg_words words
+    uint32_t* counter;   // segments taken by the trace pass
+    uint32_t nseg;       // segments
+    uint32_t seg_words;  // open-mask words per segment (even)
+};
+
+// Pass 1, one wave per 64 slots: each ray tries its own hint and, with
+// h.wave, the distinct hints of the other 63 rays (one triangle for the whole
+// wave at a time).  Writes every bit of the wave's rays (1 where a hint
+// settles the ray) and the open mask of its 64 slots; block 0 zeroes the
+// trace pass's segment counter.
+template <bool TILE>
+__global__ __launch_bounds__(kBlock) void k_hint_pass(const rt_tri64* __restrict__ tris,
+                                                      uint32_t node_count, uint32_t tri_count,
+                                                      rt_rays_soa rays, uint32_t n, uint32_t slots,
+                                                      uint32_t* __restrict__ bits_out,
+                                                      TileMap tmap, HintArgs h, HintPlan pl) {
+    const uint32_t lane = threadIdx.x & 63u;
+    if (blockIdx.x == 0 && threadIdx.x == 0) *pl.counter = 0u;
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t g0 = g - lane;
+    if (g0 >= slots) {  // the last segment's padding (whole waves): closed
+        if (lane == 0 || lane == 32) pl.open[(g0 >> 5) + (lane >> 5)] = 0u;
+        return;
+    }
+    bool valid;
+    uint32_t r = g;
+    if constexpr (TILE) r = tile_ray(g, tmap, valid);
+    else valid = g < n;
+    const bool live = valid && node_count > 0;
+    HRay q{0.f, 0.f, 0.f, 1.f, 0.f, 0.f, 1.f, 0.f, 1e6f, 1e6f, 1.f, 1.f};
+    uint32_t rec = RT_MISS;
+    if (live) {
+        q = load_hray<TILE>(rays, g, tmap);
+        const uint32_t pid = h.prim[r % h.period];
+        if (pid < h.ids) {
+            const uint32_t c = h.record[pid];
+            if (c < tri_count) rec = c;
+        }
+    }
+    bool occl = rec != RT_MISS && hint_occludes(tris, rec, q.ox, q.oy, q.oz, q.dx, q.dy, q.dz,
+                                                q.ix, q.iy, q.iz, q.tmin, q.tminc, q.tmax);
+    if (h.wave) {
+        // the wave's distinct hints in lane order, each tried by the rays
+        // still open (a ray's own hint is not tried twice)
+        uint64_t cand = __ballot(rec != RT_MISS);
+        while (cand != 0ull && __ballot(live && !occl) != 0ull) {
+            const uint32_t c = __builtin_amdgcn_readlane(rec, (int)__builtin_ctzll(cand));
+            cand &= ~__ballot(rec == c);
+            if (live && !occl && rec != c)
+                occl = hint_occludes(tris, c, q.ox, q.oy, q.oz, q.dx, q.dy, q.dz, q.ix, q.iy, q.iz,
+                                     q.tmin, q.tminc, q.tmax);
+        }
+    }
+    if constexpr (TILE) write_tile_bits(bits_out, r, n, valid, occl);
+    else write_wave_bits(bits_out, g0, n, valid && occl);
+    const uint64_t om = __ballot(live && !occl);
+    if (lane == 0) pl.open[g0 >> 5] = (uint32_t)om;
+    if (lane == 32) pl.open[(g0 >> 5) + 1] = (uint32_t)(om >> 32);
+}
+
+constexpr uint32_t kRing = 128;  // staged rays per wave (a power of two)
+
+// The trace pass, persistent: each wave takes segment after segment (an
+// atomic counter) and stages the segment's open rays in slot order, 64 slots
+// at a time, into its ring of kRing entries; whenever `refill` lanes are idle
+// they take the next staged rays (the refill loop of k_trace_refill_bits), so
+// a wave stays filled across segment boundaries until no segment is left.  A
+// ray that ends on an accepted triangle ORs its bit into bits_out.
+template <int PRIV, bool TILE>
+__global__ __launch_bounds__(kBlock) void k_hint_trace(const rt_node4* __restrict__ nodes,
+                                                       const rt_tri64* __restrict__ tris,
+                                                       rt_rays_soa rays, TileMap tmap,
+                                                       HintPlan pl, uint32_t refill, uint32_t lsu,
+                                                       uint32_t* __restrict__ bits_out) {
+    // dynamic: lsu * blockDim.x stack entries, then per wave kRing slots
+    extern __shared__ int32_t lds_stack[];
+    const uint32_t tid = threadIdx.x, lane = tid & 63u;
+    const int bs = (int)blockDim.x, ls = (int)lsu;
+    int32_t* col = lds_stack + tid;
+    uint32_t* ring = reinterpret_cast<uint32_t*>(lds_stack + ls * bs) + (tid >> 6) * kRing;
+    uint32_t* const counter = pl.counter;
+    const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64u - lane));
+    bool active = false;
+    uint32_t r = 0;  // the lane's ray
+    HRay q{0.f, 0.f, 0.f, 1.f, 0.f, 0.f, 1.f, 0.f, 0.f, 0.f, 0.f, -FLT_MAX};
+    float hitT = 0.f;
+    uint32_t nxo = 0u, nyo = 32u, nzo = 64u, hitPrim = RT_MISS;
+    Stack<PRIV> st;
+    int sp = 0, spo = 0;
+    const int lso = (ls - 1) * bs;
+    // wave-uniform: current segment, next word in it, staged range
+    // [head, tail), whether segments may remain
+    uint32_t seg = 0, wcur = pl.seg_words, head = 0, tail = 0;
+    bool more = true;
+    for (;;) {
+        while (tail - head < 64u && more) {
+            if (wcur >= pl.seg_words) {
+                uint32_t sg = 0;
+                if (lane == 0) sg = atomicAdd(counter, 1u);
+                sg = __builtin_amdgcn_readfirstlane(sg);
+                if (sg >= pl.nseg) {
+                    more = false;
+                    break;
+                }
+                seg = sg;
+                wcur = 0;
+            }
+            const uint32_t w = seg * pl.seg_words + wcur;
+            const uint64_t m = (uint64_t)pl.open[w] | (uint64_t)pl.open[w + 1] << 32;
+            if ((m >> lane) & 1ull)
+                ring[(tail + (uint32_t)__popcll(m & lt)) & (kRing - 1)] = w * 32u + lane;
+            tail += (uint32_t)__popcll(m);
+            wcur += 2;
+        }
+        const uint64_t idle = __ballot(!active);
+        if (idle == ~0ull && head == tail && !more) break;
+        if (head < tail && ((uint32_t)__popcll(idle) >= refill || idle == ~0ull || !more)) {
+            if (!active) {
+                const uint32_t below = __builtin_amdgcn_mbcnt_hi(
+                    (uint32_t)(idle >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)idle, 0u));
+                const uint32_t k = head + below;
+                if (k < tail) {
+                    const uint32_t g = ring[k & (kRing - 1)];
+                    bool valid = true;
+                    r = g;
+                    if constexpr (TILE) r = tile_ray(g, tmap, valid);
+                    q = load_hray<TILE>(rays, g, tmap);
+                    hitT = q.tmax;
+                    nxo = q.ix > 0.0f ? 0u : 16u;
+                    nyo = q.iy > 0.0f ? 32u : 48u;
+                    nzo = q.iz > 0.0f ? 64u : 80u;
+                    hitPrim = RT_MISS;
+                    sp = 0;
+                    spo = 0;
+                    RT_SPUSH(0);
+                    active = true;
+                }
+            }
+            head = min(head + (uint32_t)__popcll(idle), tail);
+        }
+        if (!active) continue;
+
+        bool ended = false;
+        if (sp > 0) {
+            int32_t ni;
+            RT_SPOP(ni);
+            float te[4];
+            bool hit[4];
+            int32_t chv[4], cnv[4];
+            const i4v ch = slab_f32(nodes, ni, nxo, nyo, nzo, q.ox, q.oy, q.oz, q.ix, q.iy, q.iz,
+                                    q.tminc, hitT, te, hit, chv, cnv);
+            ended = inner_push_leaves<PRIV>(tris, te, hit, chv, cnv, ch, col, st, sp, spo, ls, bs,
+                                            q.ox, q.oy, q.oz, q.dx, q.dy, q.dz, q.tmin, hitT,
+                                            hitPrim);
+        }
+        if (ended || sp == 0) {
+            if (ended) atomicOr(bits_out + (r >> 5), 1u << (r & 31u));
+            active = false;
+        }
+    }
+}
+
+// record[id] = RT_MISS for every id, then record[triangle_id of leaf i] = i.
+__global__ void k_fill_u32(uint32_t* __restrict__ p, uint32_t count, uint32_t v) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) p[i] = v;
+}
+__global__ void k_prim_records(const rt_tri64* __restrict__ tris, uint32_t tri_count,
+                               uint32_t ids, uint32_t* __restrict__ record) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= tri_count) return;
+    const uint32_t id = tris[i].triangle_id;
+    if (id < ids) record[id] = i;
+}
+
 // Batched trace, refill form: item i owns the chunks [chunk[i], chunk[i+1])
 // of per_wave rays each (rt_trace_batch_plan_ex with chunk_rays = per_wave);
 // each wave takes one chunk (tree, rays and mode wave-uniform) and walks it
@@ -1575,6 +1831,132 @@ extern "C" rt_status rt_trace_ex(rt_stream stream, const rt_bvh_dev* bvh, const 
                                     tri_tests);
     return launch_v<kPrivLarge>(s, bvh, rays, n, flags, cfg, hits, occl_bits, box_tests,
                                 tri_tests);
+}
+
+extern "C" rt_status rt_prim_records(rt_stream stream, const rt_bvh_dev* bvh, uint32_t ids,
+                                     uint32_t* d_record) {
+    if (!bvh) return RT_ERR_ARG;
+    if (ids == 0) return RT_OK;
+    if (!d_record || (bvh->tri_count > 0 && !bvh->tris)) return RT_ERR_ARG;
+    hipStream_t s = as_stream(stream);
+    hipLaunchKernelGGL(k_fill_u32, dim3(ceil_div(ids, 256)), dim3(256), 0, s, d_record, ids,
+                       RT_MISS);
+    if (bvh->tri_count > 0)
+        hipLaunchKernelGGL(k_prim_records, dim3(ceil_div(bvh->tri_count, 256)), dim3(256), 0, s,
+                           bvh->tris, bvh->tri_count, ids, d_record);
+    return launch_status("rt_prim_records");
+}
+
+namespace rt {
+namespace {
+// The hinted trace's launch shape and scratch layout; false on bad arguments.
+struct HintLayout {
+    TraceCfg cfg;
+    TileMap tm;
+    uint64_t slots;
+    uint32_t nseg, seg_words;
+    size_t open_bytes, total;
+};
+bool hint_layout(uint32_t n, const rt_trace_opts* opts, const rt_occluder_hint* hint,
+                 HintLayout& L) {
+    L.cfg = TraceCfg{true, false, 1024u, 16u, (uint32_t)kBlock, kLdsStack};
+    if (opts) {
+        if (opts->schedule != RT_SCHED_SIMPLE && opts->schedule != RT_SCHED_REFILL) return false;
+        if (opts->rays_per_wave) L.cfg.per_wave = opts->rays_per_wave;
+        if (opts->refill_lanes) L.cfg.refill_lanes = opts->refill_lanes;
+    }
+    if (!shape_opts(opts, L.cfg, kWithTiles)) return false;
+    if (L.cfg.per_wave == 0 || L.cfg.per_wave % 64 || L.cfg.per_wave > kMaxPerWave ||
+        L.cfg.refill_lanes == 0 || L.cfg.refill_lanes > 64)
+        return false;
+    if (L.cfg.tile_width && n % L.cfg.grid_width) return false;
+    if (hint->flags & ~RT_HINT_WAVE) return false;
+    L.slots = launch_threads(L.cfg, n, L.tm);
+    if (L.slots > 0xFFFFFFFFull - kMaxPerWave) return false;
+    L.seg_words = L.cfg.per_wave / 32;
+    L.nseg = ceil_div(L.slots, L.cfg.per_wave);
+    // the open mask covers whole segments (the last one padded with closed
+    // slots), then the segment counter
+    L.open_bytes = ((size_t)L.nseg * L.seg_words * 4 + 255) & ~(size_t)255;
+    L.total = L.open_bytes + 256;
+    return true;
+}
+}  // namespace
+}  // namespace rt
+
+extern "C" rt_status rt_trace_occluded_hinted_scratch_bytes(uint32_t n,
+                                                            const rt_trace_opts* opts,
+                                                            const rt_occluder_hint* hint,
+                                                            size_t* bytes) {
+    if (!hint || !bytes) return RT_ERR_ARG;
+    HintLayout L;
+    if (!hint_layout(n, opts, hint, L)) return RT_ERR_ARG;
+    *bytes = L.total;
+    return RT_OK;
+}
+
+extern "C" rt_status rt_trace_occluded_hinted(rt_stream stream, const rt_bvh_dev* bvh,
+                                              const rt_rays_soa* rays, uint32_t n,
+                                              const rt_trace_opts* opts,
+                                              const rt_occluder_hint* hint, uint32_t* occl_bits,
+                                              void* d_scratch, size_t scratch_bytes) {
+    if (!bvh || !rays || !hint) return RT_ERR_ARG;
+    HintLayout L;
+    if (!hint_layout(n, opts, hint, L)) return RT_ERR_ARG;
+    if (n == 0) return RT_OK;
+    if (!rays->ox || !rays->oy || !rays->oz || !rays->tmin || !rays->dx || !rays->dy ||
+        !rays->dz || !rays->tmax || !occl_bits || !hint->prim || (hint->ids && !hint->record))
+        return RT_ERR_ARG;
+    if (bvh->node_count > 0 && (!bvh->nodes || !bvh->tris)) return RT_ERR_ARG;
+    if (bvh->format != RT_NODES_F32) return RT_ERR_UNSUPPORTED;
+    const uint32_t depth = bvh->depth ? bvh->depth : kMaxDepth;
+    if (depth > kMaxDepth) return RT_ERR_CAPACITY;
+    if (!d_scratch || scratch_bytes < L.total) return RT_ERR_CAPACITY;
+    const TraceCfg& cfg = L.cfg;
+    char* sc = static_cast<char*>(d_scratch);
+    const HintPlan pl{reinterpret_cast<uint32_t*>(sc), reinterpret_cast<uint32_t*>(sc + L.open_bytes),
+                      L.nseg, L.seg_words};
+    const HintArgs h{hint->prim, hint->record, hint->period ? hint->period : n, hint->ids,
+                     (hint->flags & RT_HINT_WAVE) ? 1u : 0u};
+    hipStream_t s = as_stream(stream);
+    const TileMap tm = L.tm;
+    // pass 1 over every slot of whole segments (k_hint_pass closes the slots
+    // past the launch's)
+    const uint64_t pass_slots = (uint64_t)L.nseg * cfg.per_wave;
+    const dim3 g1(ceil_div(pass_slots, kBlock));
+    if (cfg.tile_width)
+        hipLaunchKernelGGL(k_hint_pass<true>, g1, dim3(kBlock), 0, s, bvh->tris, bvh->node_count,
+                           bvh->tri_count, *rays, n, (uint32_t)L.slots, occl_bits, tm, h, pl);
+    else
+        hipLaunchKernelGGL(k_hint_pass<false>, g1, dim3(kBlock), 0, s, bvh->tris,
+                           bvh->node_count, bvh->tri_count, *rays, n, (uint32_t)L.slots,
+                           occl_bits, tm, h, pl);
+    RT_TRY_HIP(hipGetLastError());
+    // the trace pass, persistent: as many workgroups as fit on the device at once
+    const size_t shm = lds_bytes(cfg) + (size_t)(cfg.block / 64) * kRing * 4;
+    const void* fn;
+    if (small_priv(depth, cfg.lds))
+        fn = cfg.tile_width ? reinterpret_cast<const void*>(&k_hint_trace<kPrivSmall, true>)
+                            : reinterpret_cast<const void*>(&k_hint_trace<kPrivSmall, false>);
+    else
+        fn = cfg.tile_width ? reinterpret_cast<const void*>(&k_hint_trace<kPrivLarge, true>)
+                            : reinterpret_cast<const void*>(&k_hint_trace<kPrivLarge, false>);
+    int dev = 0, cus = 0, per_cu = 0;
+    RT_TRY_HIP(hipGetDevice(&dev));
+    RT_TRY_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    RT_TRY_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, (int)cfg.block, shm));
+    const uint64_t fit = (uint64_t)(per_cu > 0 ? per_cu : 1) * (uint64_t)(cus > 0 ? cus : 1);
+    const uint32_t blocks = (uint32_t)(fit < L.nseg ? fit : L.nseg);
+    const rt_node4* nodes = bvh->nodes;
+    const rt_tri64* tris = bvh->tris;
+    rt_rays_soa rs = *rays;
+    TileMap tmv = tm;
+    HintPlan plv = pl;
+    uint32_t refill = cfg.refill_lanes, lsu = cfg.lds;
+    uint32_t* bits = occl_bits;
+    void* args[] = {&nodes, &tris, &rs, &tmv, &plv, &refill, &lsu, &bits};
+    RT_TRY_HIP(hipLaunchKernel(fn, dim3(blocks), dim3(cfg.block), args, shm, s));
+    return launch_status("rt_trace_occluded_hinted");
 }
 
 extern "C" rt_status rt_radv_describe(const void* host_image, uint64_t bytes,

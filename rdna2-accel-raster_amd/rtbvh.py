@@ -130,6 +130,13 @@ class rt_trace_opts(C.Structure):
                 ("tile_width", u32)]
 
 
+class rt_occluder_hint(C.Structure):
+    _fields_ = [("prim", vp), ("record", vp), ("period", u32), ("ids", u32), ("flags", u32)]
+
+
+RT_HINT_WAVE = 1
+
+
 # Child order of an any-hit query returning occlusion bits only (same bits
 # for every order; the work differs).
 ANY_ORDERS = {"near": 0, "far": 1, "slot": 2}
@@ -213,6 +220,9 @@ _SIGS = {
     "rt_trace_batch_submit": [vp, vp, vp, u32, vp, vp, C.c_size_t, vp, C.c_size_t],
     "rt_gpu_build_batch_submit_bytes": [u32, C.c_int, vp],
     "rt_gpu_build_batch_submit": [vp, vp, vp, u32, u32, vp, C.c_size_t, vp, C.c_size_t],
+    "rt_prim_records": [vp, vp, u32, vp],
+    "rt_trace_occluded_hinted_scratch_bytes": [u32, vp, vp, vp],
+    "rt_trace_occluded_hinted": [vp, vp, vp, u32, vp, vp, vp, vp, C.c_size_t],
 }
 
 
@@ -543,13 +553,55 @@ class GpuBVH4:
              C.byref(o), C.byref(hits), None, ptr(box), ptr(tri))
         return (prim, t, box, tri) if counters else (prim, t)
 
-    def occluded(self, rays, stream=None, out=None, counters=False, schedule=None, **tune):
-        """Any-hit occlusion bits: int32 tensor of ceil(n/32) words."""
+    def prim_records(self, stream=None):
+        """Device map triangle id -> leaf record index (rt_prim_records) over
+        ids 0 .. tri_count - 1, made once per tree (the hint table of
+        occluded(hint=...)); rebuild the tree object after a rebuild."""
+        rec = getattr(self, "_prim_records", None)
+        if rec is None:
+            rec = torch.empty(max(1, self.tri_count), dtype=torch.int32, device=self.tris.device)
+            b = self.dev()
+            call("rt_prim_records", stream_handle(stream), C.byref(b), u32(self.tri_count),
+                 ptr(rec))
+            self._prim_records = rec
+        return rec
+
+    def occluded(self, rays, stream=None, out=None, counters=False, schedule=None, hint=None,
+                 hint_period=0, hint_wave=True, **tune):
+        """Any-hit occlusion bits: int32 tensor of ceil(n/32) words.
+
+        hint: int32 device tensor of triangle ids (RT_MISS = none), ray r's
+        hint being hint[r % hint_period] (0 = n), e.g. the primary hits of a
+        frame for its shadow rays; hint_wave also tries the hints of the
+        other rays of each wave.  The bits are the same with or without hints
+        (rt_trace_occluded_hinted); tune: block, lds_stack, tile,
+        rays_per_wave (slots of a segment), refill_lanes."""
         n = len(rays)
         dev = rays.data.device
         bits = out if out is not None else torch.empty(max(1, (n + 31) // 32), dtype=torch.int32,
                                                        device=dev)
         check_out(bits, (n + 31) // 32, _I32, "out (occlusion bits)")
+        if hint is not None:
+            if counters or schedule not in (None, "refill"):
+                raise ValueError("hints: occlusion bits only, refill schedule")
+            period = hint_period or n
+            check_out(hint, period, _I32, "hint (triangle ids)")
+            rec = self.prim_records(stream)
+            h = rt_occluder_hint(ptr(hint), ptr(rec), u32(period), u32(self.tri_count),
+                                 u32(RT_HINT_WAVE if hint_wave else 0))
+            b = self.dev()
+            r = rays.soa()
+            o = trace_opts("refill", **tune)
+            need = C.c_size_t(0)
+            call("rt_trace_occluded_hinted_scratch_bytes", u32(n), C.byref(o), C.byref(h),
+                 C.byref(need))
+            sc = getattr(self, "_hint_scratch", None)
+            if sc is None or sc.numel() < need.value:
+                sc = torch.empty(max(256, need.value), dtype=torch.uint8, device=dev)
+                self._hint_scratch = sc
+            call("rt_trace_occluded_hinted", stream_handle(stream), C.byref(b), C.byref(r),
+                 u32(n), C.byref(o), C.byref(h), ptr(bits), ptr(sc), C.c_size_t(sc.numel()))
+            return bits
         box = tri = None
         flags = RT_ANY
         if counters:

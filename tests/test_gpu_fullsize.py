@@ -138,10 +138,11 @@ def test_headline_frame_with_bench_shapes_matches_oracle(rt, O, headline):
     """The whole frame traced as the bench traces it -- primary rays with
     64-thread blocks (linear waves and 16 x 4 tiles), the 33.2M shadow rays
     with 64-thread blocks, 20 LDS entries and 16 x 4 wave tiles of the
-    1920-wide grid -- gives the same ids, t and occlusion bits as the
-    default linear launch; 100,000 sampled camera rays and 100,000 sampled
-    shadow rays of that frame equal the oracle's traversal of the same tree
-    (ids, t bits, occlusion bits)."""
+    1920-wide grid, and with the pixels' primary hits as occluder hints --
+    gives the same ids, t and occlusion bits as the default linear launch;
+    100,000 sampled camera rays and 100,000 sampled shadow rays of that frame
+    equal the oracle's traversal of the same tree (ids, t bits, occlusion
+    bits)."""
     pos, idx, bvh = headline
     prim_rays = rt.gen_camera_rays(W, H, (0.0, 0.0, 1.25), (0.0, 0.0, 0.0), (0.0, 1.0, 0.0),
                                    60.0, 0.0, 1e30)
@@ -162,6 +163,17 @@ def test_headline_frame_with_bench_shapes_matches_oracle(rt, O, headline):
     masks = rt.pack_light_masks(bits, W * H, NL)
     torch.cuda.synchronize()
     assert torch.equal(bits, bits_lin)
+    # the hinted trace as the bench's tuner runs it: the pixels' primary hits
+    # as occluder hints (with the wave's other hints), 16 x 4 tiles and linear
+    # waves, segments of 256-1024 slots
+    for tile in ((W, 16), None):
+        for seg, refill in ((512, 32), (256, 16), (1024, 16)):
+            print(f"headline frame hinted tile={tile} segment={seg} refill={refill}", flush=True)
+            hb = torch.full_like(bits, -1)
+            bvh.occluded(shadow, out=hb, hint=prim, hint_period=W * H, block=64, lds_stack=16,
+                         rays_per_wave=seg, refill_lanes=refill, tile=tile)
+            torch.cuda.synchronize()
+            assert torch.equal(hb, bits)
     nodes = to_np(bvh.nodes)[: bvh.node_count * 128].view(O.NODE4)
     tris = to_np(bvh.tris)[: bvh.tri_count * 64].view(O.TRI64)
     rng = np.random.default_rng(5)
