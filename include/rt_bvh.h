@@ -289,8 +289,14 @@ rt_status rt_radv_describe(const void* host_image, uint64_t bytes, rt_radv_info*
  *   2. the caller copies the table to device memory;
  *   3. rt_gpu_build_batch launches over it (no host work, no allocation:
  *      capturable in a graph).  workgroups = 0: min(count, 256).
+ *      workgroups | RT_BUILD_BATCH_SHARED: the meshes above 4096 triangles
+ *      are built by 256-thread workgroups with a few KB of LDS (a second
+ *      launch), which fit on a compute unit beside other kernels' waves;
+ *      the fused path's 1024-thread workgroup needs an idle compute unit.
+ *      Same outputs; for builds issued while traces run (a serving loop).
  * Items must not share outputs or scratch.                                  */
 #define RT_BATCH_BUILD_MAX 65536u
+#define RT_BUILD_BATCH_SHARED 0x80000000u
 typedef struct rt_build_item {
     const float* pos;       /* as rt_gpu_build */
     const uint32_t* idx;    /* NULL: 9 floats per triangle */

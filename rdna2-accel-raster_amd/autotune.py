@@ -44,7 +44,7 @@ REFILL_LANES = (8, 16, 32)
 TILE_WIDTHS = (16, 8, 32)
 PACKET_TILE_WIDTHS = (16, 8)  # packet traversal (occlusion bits) of 16x4 / 8x8 tiles
 HINT_SEGMENTS = (256, 512, 1024)  # slots per segment of the hinted trace
-HINT_REFILL_LANES = (16, 32)
+HINT_REFILL_LANES = (32, 48, 16)
 # a later candidate must be faster than the best so far by more than this
 # fraction (the spread of repeated timings of one candidate) to replace it
 TIE_MARGIN = 0.01

@@ -1149,7 +1149,6 @@ __global__ __launch_bounds__(kSmallBlock) void k_build_radv(
 constexpr uint32_t kMedMax = 65536;
 constexpr int kMedLevels = 8;  // levels of a 65536-triangle implicit tree
 constexpr int kMedItems = 4;   // keys per thread and tile
-constexpr uint32_t kMedTile = kSmallBlock * kMedItems;
 static_assert(RT_BATCH_BUILD_MAX == kMedMax, "the batched build covers the medium path");
 
 struct MedTree {
@@ -1159,19 +1158,48 @@ struct MedTree {
     uint64_t keys0, vals0, keys1, vals1, boxes;  // byte offsets in the item's scratch
 };
 
-__device__ void build_medium(SmallLds& S, const MedTree& T, const float* __restrict__ pos,
+// LDS of the medium path: digit counts, running digit starts, the centroid
+// bounds, block_rank's per-wave counters, digit starts and wave sums, the
+// per-wave partial bounds.
+struct MedLdsRefs {
+    uint32_t* hist;    // 256
+    uint32_t* run;     // 256
+    float* bnd;        // 6
+    uint32_t* cnt;     // (BLOCK / 64) * 256
+    uint32_t* dstart;  // 256
+    uint32_t* wsum;    // 16
+    float (*red)[6];   // BLOCK / 64 rows
+};
+
+// The medium path as its own kernel's LDS (BLOCK = 256: a workgroup that
+// shares a compute unit with other kernels' waves).
+template <int BLOCK>
+struct MedLds {
+    uint32_t hist[256];
+    uint32_t run[256];
+    float bnd[8];
+    uint32_t cnt[(BLOCK / 64) * 256];
+    uint32_t dstart[256];
+    uint32_t wsum[16];
+    float red[BLOCK / 64][6];
+    __device__ MedLdsRefs refs() { return {hist, run, bnd, cnt, dstart, wsum, red}; }
+};
+
+template <int BLOCK>
+__device__ void build_medium(const MedLdsRefs& S, const MedTree& T, const float* __restrict__ pos,
                              const uint32_t* __restrict__ idx, uint32_t n,
                              rt_node4* __restrict__ nodes, rt_tri64* __restrict__ tris,
                              uint32_t* __restrict__ ids_out, uint32_t* __restrict__ codes_out,
                              rt_build_info* __restrict__ info, char* __restrict__ scratch) {
+    constexpr uint32_t kTile = BLOCK * kMedItems;
     const uint32_t tid = threadIdx.x, lane = tid & 63u, w = tid >> 6;
-    uint32_t* hist = S.u.lsd.key[0];        // 256 digit counts
-    uint32_t* run = S.u.lsd.key[0] + 256;   // running global start of each digit
-    float* bnd = reinterpret_cast<float*>(S.u.lsd.val[0]);  // lo[3], hi[3]
+    uint32_t* hist = S.hist;  // 256 digit counts
+    uint32_t* run = S.run;    // running global start of each digit
+    float* bnd = S.bnd;       // lo[3], hi[3]
 
     // 1. centroid bounds
     float mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
-    for (uint32_t t = tid; t < n; t += kSmallBlock) {
+    for (uint32_t t = tid; t < n; t += BLOCK) {
         float v[9], c[3];
         tri_verts(pos, idx, t, v);
         centroid(v, c);
@@ -1192,7 +1220,7 @@ __device__ void build_medium(SmallLds& S, const MedTree& T, const float* __restr
     lds_barrier();
     if (tid < 6) {
         uint32_t o = f2o(S.red[0][tid]);
-        for (int ww = 1; ww < kSmallBlock / 64; ++ww) {
+        for (int ww = 1; ww < BLOCK / 64; ++ww) {
             const uint32_t q = f2o(S.red[ww][tid]);
             o = tid < 3 ? min(o, q) : max(o, q);
         }
@@ -1207,7 +1235,7 @@ __device__ void build_medium(SmallLds& S, const MedTree& T, const float* __restr
     uint32_t* vi = reinterpret_cast<uint32_t*>(scratch + T.vals0);
     uint32_t* ko = reinterpret_cast<uint32_t*>(scratch + T.keys1);
     uint32_t* vo = reinterpret_cast<uint32_t*>(scratch + T.vals1);
-    for (uint32_t t = tid; t < n; t += kSmallBlock) {
+    for (uint32_t t = tid; t < n; t += BLOCK) {
         float v[9], c[3];
         tri_verts(pos, idx, t, v);
         centroid(v, c);
@@ -1220,15 +1248,15 @@ __device__ void build_medium(SmallLds& S, const MedTree& T, const float* __restr
     for (uint32_t shift = 0; shift < 32; shift += 8) {
         if (tid < 256) hist[tid] = 0;
         lds_barrier();
-        for (uint32_t t = tid; t < n; t += kSmallBlock)
+        for (uint32_t t = tid; t < n; t += BLOCK)
             atomicAdd(&hist[(ki[t] >> shift) & 0xFFu], 1u);
         lds_barrier();
         uint32_t hv[1] = {tid < 256 ? hist[tid] : 0u};
         digit_excl_scan<256>(hv, S.wsum);
         if (tid < 256) run[tid] = hv[0];
         lds_barrier();
-        for (uint32_t base = 0; base < n; base += kMedTile) {
-            const uint32_t valid = min(kMedTile, n - base);
+        for (uint32_t base = 0; base < n; base += kTile) {
+            const uint32_t valid = min(kTile, n - base);
             uint32_t key[kMedItems], val[kMedItems], dig[kMedItems], rk[kMedItems];
 #pragma unroll
             for (int i = 0; i < kMedItems; ++i) {
@@ -1237,7 +1265,7 @@ __device__ void build_medium(SmallLds& S, const MedTree& T, const float* __restr
                 val[i] = lp < valid ? vi[base + lp] : 0u;
                 dig[i] = (key[i] >> shift) & 0xFFu;
             }
-            block_rank<kSmallBlock, kMedItems>(dig, rk, S.u.lsd.cnt, S.dstart, S.wsum);
+            block_rank<BLOCK, kMedItems>(dig, rk, S.cnt, S.dstart, S.wsum);
 #pragma unroll
             for (int i = 0; i < kMedItems; ++i) {
                 const uint32_t lp = w * 64 * kMedItems + i * 64 + lane;
@@ -1262,13 +1290,13 @@ __device__ void build_medium(SmallLds& S, const MedTree& T, const float* __restr
     }
 
     // 4. outputs
-    for (uint32_t q = tid; q < n; q += kSmallBlock) {
+    for (uint32_t q = tid; q < n; q += BLOCK) {
         if (ids_out) ids_out[q] = vi[q];
         if (codes_out) codes_out[q] = ki[q];
     }
     float* gb = reinterpret_cast<float*>(scratch + T.boxes);
     const uint32_t lb = T.levels - 1;
-    for (uint32_t b = tid; b < T.counts[lb]; b += kSmallBlock) {
+    for (uint32_t b = tid; b < T.counts[lb]; b += BLOCK) {
         float box[6];
         emit_leaf_box<false>(pos, idx, n, vi, nodes, tris, T.offsets[lb], b, box);
 #pragma unroll
@@ -1276,7 +1304,7 @@ __device__ void build_medium(SmallLds& S, const MedTree& T, const float* __restr
     }
     __syncthreads();
     for (int lv = (int)lb - 1; lv >= 0; --lv) {
-        for (uint32_t i = tid; i < T.counts[lv]; i += kSmallBlock) {
+        for (uint32_t i = tid; i < T.counts[lv]; i += BLOCK) {
             float box[6];
             emit_inner(nodes, T, (uint32_t)lv, i, gb + 6 * (size_t)T.offsets[lv + 1], 0u, box);
             if (lv > 0) {
@@ -1395,18 +1423,45 @@ struct BatchBuild {
 
 // Persistent batched build: workgroup b builds items b, b + grid, b + 2 grid,
 // ... (the host orders them largest first), each alone (parts = 1).
+// skip_medium: items above kSmallMax are left to k_build_batch_shared.
 __global__ __launch_bounds__(kSmallBlock) void k_build_batch(const BatchBuild* __restrict__ table,
-                                                              uint32_t count) {
+                                                              uint32_t count, uint32_t skip_medium) {
     __shared__ SmallLds S;
+    const MedLdsRefs M{S.u.lsd.key[0], S.u.lsd.key[0] + 256,
+                       reinterpret_cast<float*>(S.u.lsd.val[0]), S.u.lsd.cnt, S.dstart, S.wsum,
+                       S.red};
     for (uint32_t k = blockIdx.x; k < count; k += gridDim.x) {
         const BatchBuild& b = table[k];
         const rt_build_item& it = b.item;
         if (it.n <= kSmallMax)
             build_small<false>(S, b.tree, it.pos, it.idx, it.n, it.nodes, it.tris, it.sorted_ids,
                                it.codes, it.info, 0u, 1u);
-        else
-            build_medium(S, b.med, it.pos, it.idx, it.n, it.nodes, it.tris, it.sorted_ids,
-                         it.codes, it.info, static_cast<char*>(it.scratch));
+        else if (!skip_medium)
+            build_medium<kSmallBlock>(M, b.med, it.pos, it.idx, it.n, it.nodes, it.tris,
+                                      it.sorted_ids, it.codes, it.info,
+                                      static_cast<char*>(it.scratch));
+        __syncthreads();  // LDS and scratch reused by the next mesh
+    }
+}
+
+// The medium items of a batch (RT_BUILD_BATCH_SHARED): the same medium path
+// by 256-thread workgroups with about 7 KB of LDS, which fit on a compute
+// unit beside other kernels' waves (a 1024-thread workgroup with the fused
+// path's LDS needs an idle compute unit, and waits for one while traces
+// run).  Workgroup b builds items b, b + grid, ...; items of kSmallMax
+// triangles or fewer are skipped (k_build_batch builds them).
+constexpr int kSharedBlock = 256;
+__global__ __launch_bounds__(kSharedBlock) void k_build_batch_shared(
+    const BatchBuild* __restrict__ table, uint32_t count) {
+    __shared__ MedLds<kSharedBlock> S;
+    const MedLdsRefs M = S.refs();
+    for (uint32_t k = blockIdx.x; k < count; k += gridDim.x) {
+        const BatchBuild& b = table[k];
+        const rt_build_item& it = b.item;
+        if (it.n <= kSmallMax) continue;
+        build_medium<kSharedBlock>(M, b.med, it.pos, it.idx, it.n, it.nodes, it.tris,
+                                   it.sorted_ids, it.codes, it.info,
+                                   static_cast<char*>(it.scratch));
         __syncthreads();  // LDS and scratch reused by the next mesh
     }
 }
@@ -2811,14 +2866,29 @@ extern "C" rt_status rt_gpu_build_batch_plan(const rt_build_item* items, uint32_
     return RT_OK;
 }
 
+namespace {
+// small / medium: whether the batch may hold items of each path (both when
+// the host does not know the items).
+rt_status launch_build_batch(hipStream_t s, const void* d_table, uint32_t count,
+                             uint32_t workgroups, bool small, bool medium) {
+    const bool shared = (workgroups & RT_BUILD_BATCH_SHARED) != 0;
+    workgroups &= ~RT_BUILD_BATCH_SHARED;
+    const uint32_t g = workgroups ? min(workgroups, count) : min(count, 256u);
+    const auto* t = static_cast<const BatchBuild*>(d_table);
+    if (!shared || small)
+        hipLaunchKernelGGL(k_build_batch, dim3(g), dim3(kSmallBlock), 0, s, t, count,
+                           shared ? 1u : 0u);
+    if (shared && medium)
+        hipLaunchKernelGGL(k_build_batch_shared, dim3(count), dim3(kSharedBlock), 0, s, t, count);
+    return launch_status("k_build_batch");
+}
+}  // namespace
+
 extern "C" rt_status rt_gpu_build_batch(rt_stream stream, const void* d_table, uint32_t count,
                                         uint32_t workgroups) {
     if (count == 0) return RT_OK;
     if (!d_table) return RT_ERR_ARG;
-    const uint32_t g = workgroups ? min(workgroups, count) : min(count, 256u);
-    hipLaunchKernelGGL(k_build_batch, dim3(g), dim3(kSmallBlock), 0, as_stream(stream),
-                       static_cast<const BatchBuild*>(d_table), count);
-    return launch_status("k_build_batch");
+    return launch_build_batch(as_stream(stream), d_table, count, workgroups, true, true);
 }
 
 namespace {
@@ -2881,7 +2951,9 @@ extern "C" rt_status rt_gpu_build_batch_submit(rt_stream stream, const rt_build_
         st = launch_grid_gen(s, reinterpret_cast<const GridGen*>(d + L.gen), count, max_quads);
         if (st != RT_OK) return st;
     }
-    return rt_gpu_build_batch(stream, d, count, workgroups);
+    bool small = false, medium = false;
+    for (uint32_t k = 0; k < count; ++k) (items[k].n <= kSmallMax ? small : medium) = true;
+    return launch_build_batch(s, d, count, workgroups, small, medium);
 }
 
 extern "C" rt_status rt_gpu_build_radv(rt_stream stream, const float* d_pos,

@@ -951,7 +951,10 @@ constexpr uint32_t kMaxPerWave = 4096;  // rays per wave: 128 occlusion words in
 // compaction); each lane advances its ray by one node per iteration.  Any-hit
 // bits collect in wbits (the wave's LDS words, zeroed here) and are flushed
 // at the end.  Pointers are used as global (address space 1) views.
-template <int PRIV, bool ANY, bool COUNT>
+// BITS (any hit, no counters, no hit ids): the node step is the bits-only
+// one of k_trace_refill_bits (near / far planes by offset, inner children at
+// their ranks, leaf children in slot order); the bits do not depend on it.
+template <int PRIV, bool ANY, bool COUNT, bool BITS = false>
 __device__ __forceinline__ void refill_wave(
     const rt_node4* __restrict__ nodes_, const rt_tri64* __restrict__ tris_, uint32_t node_count,
     const rt_rays_soa& rays_, uint32_t wbeg32, uint32_t end, uint32_t refill, int32_t* col,
@@ -979,6 +982,8 @@ __device__ __forceinline__ void refill_wave(
     float ix = 0.f, iy = 0.f, iz = 0.f, hitT = 0.f;
     uint32_t hitPrim = RT_MISS, boxTests = 0, triTests = 0;
     float tminc = -FLT_MAX;  // fmaxf(tmin, -FLT_MAX), as in trace_ray
+    uint32_t nxo = 0u, nyo = 32u, nzo = 64u;  // BITS: near planes by offset
+    static_assert(!BITS || (ANY && !COUNT), "bits-only step: any hit without counters");
     Stack<PRIV> st;
     int sp = 0, spo = 0;
     const int lso = (ls - 1) * bs;
@@ -997,6 +1002,11 @@ __device__ __forceinline__ void refill_wave(
                     iy = fabsf(dy) > 1e-6f ? 1.0f / dy : 1e6f;
                     iz = fabsf(dz) > 1e-6f ? 1.0f / dz : 1e6f;
                     tminc = fmaxf(tmin, -FLT_MAX);
+                    if (BITS) {
+                        nxo = ix > 0.0f ? 0u : 16u;
+                        nyo = iy > 0.0f ? 32u : 48u;
+                        nzo = iz > 0.0f ? 64u : 80u;
+                    }
                     hitPrim = RT_MISS;
                     boxTests = triTests = 0;
                     sp = 0;
@@ -1010,9 +1020,20 @@ __device__ __forceinline__ void refill_wave(
         if (!active) continue;
 
         bool finished = sp == 0;
-        if (!finished) {
+        if (BITS && !finished) {
             int32_t ni;
-        RT_SPOP(ni);
+            RT_SPOP(ni);
+            float te[4];
+            bool hit[4];
+            int32_t chv[4], cnv[4];
+            const i4v ch = slab_f32(nodes, ni, nxo, nyo, nzo, ox, oy, oz, ix, iy, iz, tminc,
+                                    hitT, te, hit, chv, cnv);
+            finished = inner_push_leaves<PRIV>(tris, te, hit, chv, cnv, ch, col, st, sp, spo, ls,
+                                               bs, ox, oy, oz, dx, dy, dz, tmin, hitT, hitPrim) ||
+                       sp == 0;
+        } else if (!finished) {
+            int32_t ni;
+            RT_SPOP(ni);
             const auto np = f4(nodes + ni);
             const f4v mnx = np[0], mxx = np[1], mny = np[2], mxy = np[3], mnz = np[4],
                       mxz = np[5];
@@ -1490,7 +1511,12 @@ __global__ __launch_bounds__(kBlock) void k_trace_batch_refill(
     const uint32_t beg = (uint32_t)min<uint64_t>(b0, it.count);
     const uint32_t end = (uint32_t)min<uint64_t>(b0 + per_wave, it.count);
     uint32_t* wbits = lds_bits + (tid >> 6) * (kMaxPerWave / 32);
-    if (it.flags & RT_ANY)
+    if ((it.flags & RT_ANY) && !it.hits.prim)  // occlusion bits only
+        refill_wave<PRIV, true, false, true>(it.bvh.nodes, it.bvh.tris, it.bvh.node_count,
+                                             it.rays, beg, end, refill, lds_stack + tid, (int)ls,
+                                             (int)bs, wbits, nullptr, nullptr, it.occlusion_bits,
+                                             nullptr, nullptr);
+    else if (it.flags & RT_ANY)
         refill_wave<PRIV, true, false>(it.bvh.nodes, it.bvh.tris, it.bvh.node_count, it.rays,
                                        beg, end, refill, lds_stack + tid, (int)ls, (int)bs,
                                        wbits, it.hits.prim, nullptr, it.occlusion_bits, nullptr,

@@ -135,6 +135,9 @@ class rt_occluder_hint(C.Structure):
 
 
 RT_HINT_WAVE = 1
+# rt_gpu_build_batch workgroups flag: meshes above 4096 triangles built by
+# 256-thread workgroups that share compute units with running traces
+BUILD_BATCH_SHARED = 0x80000000
 
 
 # Child order of an any-hit query returning occlusion bits only (same bits

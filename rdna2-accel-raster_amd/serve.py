@@ -26,8 +26,7 @@ Scheduling (one host thread):
     next tick's batch (continuous batching); a tick started on an idle
     device takes at most `idle_batch` requests (its first launch is not held
     back by a large batch's setup), others up to `max_batch` requests and
-    about `max_tick_rays` rays (a tick's trace that runs for long keeps the
-    small rebuild launches of other ticks waiting for compute units);
+    about `max_tick_rays` rays;
   * admission is FIFO for memory: it stops at the first request whose
     buffers do not fit in the budget (a first-fit arena over one device
     allocation of `budget_bytes`: rays 32 B per ray, results 8 B or 1/8 B
@@ -39,15 +38,19 @@ Scheduling (one host thread):
     the ring is long enough that a rebuild seldom overwrites a set a tick
     still on the device reads;
   * a tick's rebuilds go out as ONE rt_gpu_build_batch_submit call (deform
-    step and batched build, one workgroup per grid) on a rebuild stream,
-    after the traces still reading the set it overwrites and the rebuild
-    that wrote it last (stream waits on their events); its ray requests as
-    ONE rt_trace_batch_submit call (rays
-    generated on the device, one batched trace) on the next of `streams`
-    streams; requests on versions whose rebuild may still run go out in a
-    second submit on the rebuild stream, after the tick's builds.  The item tables
-    are filled with numpy and staged through pinned rings allocated once:
-    no device allocation, no host wait per tick;
+    step and batched build, one workgroup per grid; grids above 4096
+    triangles by 256-thread workgroups, RT_BUILD_BATCH_SHARED, which run
+    beside the traces of other ticks instead of waiting for an idle compute
+    unit), after the traces still reading the set it overwrites and the
+    rebuild that wrote it last (stream waits on their events), on the tick's
+    own stream ahead of its traces (rebuild_streams = 0, the default: then
+    ONE rt_trace_batch_submit call per tick, rays generated on the device,
+    one batched trace, after the builds) or on one of `rebuild_streams`
+    streams, the tick's requests on versions whose rebuild may still run
+    then going out in a second submit on that stream.  Ticks go to the next
+    of `streams` streams (4: one hardware queue each).  The item tables are
+    filled with numpy and staged through pinned rings allocated once: no
+    device allocation, no host wait per tick;
   * completion is polled through events; a finished request releases its
     arena blocks and its latency (arrival to observed completion) is
     recorded.
@@ -362,9 +365,9 @@ def _live(events):
 
 # --------------------------------------------------------------- server ----
 class Server:
-    def __init__(self, scenes, budget_bytes=2 << 30, streams=3, rebuild_streams=2,
+    def __init__(self, scenes, budget_bytes=2 << 30, streams=4, rebuild_streams=0,
                  max_inflight=256, max_batch=128, keep_results=False, trace_shape=None,
-                 max_ticks=4, idle_batch=16, max_tick_rays=8 << 20):
+                 max_ticks=3, idle_batch=16, max_tick_rays=16 << 20):
         self.scenes = scenes
         self.max_ticks = max_ticks or streams
         self.trace_shape = dict(trace_shape or {})
@@ -374,8 +377,6 @@ class Server:
         self.max_inflight = max_inflight
         self.max_batch = max_batch
         self.idle_batch = min(idle_batch, max_batch) if idle_batch else max_batch
-        # rays per tick: a tick whose trace runs for long keeps the next tick's
-        # (small) rebuild launches waiting for free compute units
         self.max_tick_rays = max_tick_rays
         dev = rt._dev()
         self.arena = Arena(budget_bytes, dev)
@@ -508,9 +509,11 @@ class Server:
             gen[j] = (GRID_STEP * r.step, GRID_EXTENT)
         pin, tab, slot = self.bring.take()
         nb = self.bring.slot_bytes
+        # medium grids by 256-thread workgroups: they run beside the traces
+        # instead of waiting for an idle compute unit
         rt.call("rt_gpu_build_batch_submit", rt.stream_handle(bs), items.ctypes.data,
-                gen.ctypes.data, rt.u32(len(reb)), rt.u32(0), pin, C.c_size_t(nb), tab,
-                C.c_size_t(nb))
+                gen.ctypes.data, rt.u32(len(reb)), rt.u32(rt.BUILD_BATCH_SHARED), pin,
+                C.c_size_t(nb), tab, C.c_size_t(nb))
         self.build_launches += 1
         self.bring.used(slot, bs)
         if self.keep_results:

@@ -131,7 +131,7 @@ def test_build_batch_equals_alone(rt, O):
     not): every tree, leaf record, sorted index and code equals the fused
     build of that mesh alone; one also against the oracle."""
     meshes = _meshes(rt, O)
-    for wg in (0, 3):
+    for wg in (0, 3, rt.BUILD_BATCH_SHARED):
         bb = rt.BuildBatch(meshes, keep_order=True, workgroups=wg)
         trees = bb.launch()
         torch.cuda.synchronize()
@@ -196,14 +196,18 @@ def _medium_meshes(rt, O):
 
 def test_build_batch_medium_equals_alone(rt, O):
     """Meshes of 4097..65536 triangles built by one workgroup each inside the
-    persistent launch: nodes, leaf records, sorted ids and codes equal the
-    multi-launch implicit build of the same mesh alone, and the oracle's."""
+    persistent launch (and, with RT_BUILD_BATCH_SHARED, by 256-thread
+    workgroups of a second launch, also mixed with fused-path meshes): nodes,
+    leaf records, sorted ids and codes equal the multi-launch implicit build
+    of the same mesh alone, and the oracle's."""
     meshes = _medium_meshes(rt, O)
-    for wg in (0, 2):
-        bb = rt.BuildBatch(meshes, keep_order=True, workgroups=wg)
+    mixed = meshes + _meshes(rt, O)[:4]
+    for wg, ms in ((2 | rt.BUILD_BATCH_SHARED, mixed), (rt.BUILD_BATCH_SHARED, meshes),
+                   (2, meshes), (0, meshes)):
+        bb = rt.BuildBatch(ms, keep_order=True, workgroups=wg)
         trees = bb.launch()
         torch.cuda.synchronize()
-        for (pos, idx), got in zip(meshes, trees):
+        for (pos, idx), got in zip(ms, trees):
             want = rt.GpuBVH4.build(pos, idx, algorithm="implicit", keep_order=True)
             torch.cuda.synchronize()
             n = got.tri_count
