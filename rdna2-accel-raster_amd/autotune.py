@@ -44,12 +44,13 @@ REFILL_LANES = (8, 16, 32)
 TILE_WIDTHS = (16, 8, 32)
 PACKET_TILE_WIDTHS = (16, 8)  # packet traversal (occlusion bits) of 16x4 / 8x8 tiles
 HINT_SEGMENTS = (256, 512, 1024)  # slots per segment of the hinted trace
-HINT_REFILL_LANES = (32, 48, 16)
+HINT_REFILL_LANES = (32, 48)
 # a later candidate must be faster than the best so far by more than this
 # fraction (the spread of repeated timings of one candidate) to replace it
 TIE_MARGIN = 0.01
 LEAF_SIZES = (4, 3, 2, 1)  # radix-tree leaf sizes of a frame's tree, the default first
 SAMPLE_ROW_BLOCK = 8       # rows per block of a grid sample (a multiple of every tile height)
+FRAME_REPS, FRAME_TARGET_MS = 7, 40.0  # timing of a frame-tree candidate
 
 
 def hinted_kwargs(choice, hint):
@@ -142,7 +143,8 @@ class Tuner:
                 json.dump(self.cache, f, indent=1, sort_keys=True)
             os.replace(tmp, self.cache_path)
 
-    def _pick(self, key, candidates, run, check, skip_changed=False, same=None):
+    def _pick(self, key, candidates, run, check, skip_changed=False, same=None, reps=None,
+              target_ms=2.0):
         """Times run(c) for every candidate c, checks check(c) == the first
         candidate's output (or same(out, first) when given), caches and
         returns the fastest (a later candidate wins only when faster by more
@@ -155,7 +157,7 @@ class Tuner:
         self._depth += 1  # a candidate's run may tune other knobs: time counted once
         try:
             best, best_ms, table = self._measure(key, candidates, run, check, skip_changed,
-                                                 same or _same)
+                                                 same or _same, reps or self.reps, target_ms)
         finally:
             self._depth -= 1
         if self._depth == 0:
@@ -169,7 +171,7 @@ class Tuner:
         self._save()
         return best
 
-    def _measure(self, key, candidates, run, check, skip_changed, same):
+    def _measure(self, key, candidates, run, check, skip_changed, same, reps, target_ms):
         ref = _clone(check(candidates[0]))
         torch.cuda.synchronize()
         best, best_ms, table = None, float("inf"), []
@@ -181,7 +183,7 @@ class Tuner:
                     self.history.append((key, c, None))
                     continue
                 raise AssertionError(f"tuner: candidate {c} changed the result of {key}")
-            ms = time_ms(lambda: run(c), reps=self.reps)
+            ms = time_ms(lambda: run(c), reps=reps, target_ms=target_ms)
             table.append((c, ms))
             self.history.append((key, c, ms))
             if ms < best_ms * (1.0 - TIE_MARGIN) or best is None:
@@ -240,12 +242,13 @@ class Tuner:
             if grid_width and grid_width % tw == 0]
         # occluder hints: the hinted trace over segments of HINT_SEGMENTS
         # slots, linear waves and the grid's wave tiles
+        tile = [(grid_width, tw) for tw in TILE_WIDTHS[:1] if grid_width and grid_width % tw == 0]
         hints = [] if not hinted else [
-            {"hinted": True, "block": best["block"], "lds_stack": best["lds_stack"],
-             "rays_per_wave": w, "refill_lanes": r, **({"tile": t} if t else {})}
-            for t in [None] + [(grid_width, tw) for tw in TILE_WIDTHS[:1]
-                               if grid_width and grid_width % tw == 0]
-            for w in HINT_SEGMENTS for r in HINT_REFILL_LANES]
+            {"hinted": True, "block": b, "lds_stack": best["lds_stack"], "rays_per_wave": w,
+             "refill_lanes": r, **({"tile": t} if t else {})}
+            for t in tile + [None] for b in dict.fromkeys((best["block"], 128))
+            for w in HINT_SEGMENTS for r in HINT_REFILL_LANES
+            if t or (b == best["block"] and r == HINT_REFILL_LANES[0])]
         if hinted:
             key += "|hint"
         return self._pick(key, [best] + refill + orders + tiles + packets + hints, run, check)
@@ -342,7 +345,10 @@ class Tuner:
             step()
             return _clone(output())
 
-        return self._pick(f"leaf|{tag}", list(sizes), run, check, skip_changed=True, same=same)
+        # whole frames differ by a few per cent between trees: each candidate
+        # is the median of FRAME_REPS trials of about FRAME_TARGET_MS of frames
+        return self._pick(f"leaf|{tag}", list(sizes), run, check, skip_changed=True, same=same,
+                          reps=FRAME_REPS, target_ms=FRAME_TARGET_MS)
 
     def build_batch(self, batch, tag=""):
         """Persistent-grid size for BuildBatch.launch."""

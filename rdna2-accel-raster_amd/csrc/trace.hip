@@ -1312,10 +1312,33 @@ __device__ __forceinline__ HRay load_hray(const rt_rays_soa& rays, uint32_t g,
 // segment.
 struct HintPlan {
     uint32_t* open;      // nseg * seg_words words
-    uint32_t* counter;   // segments taken by the trace pass
+    uint32_t* counter;   // kXcds segment counters of the trace pass
     uint32_t nseg;       // segments
     uint32_t seg_words;  // open-mask words per segment (even)
+    uint32_t layer_segs; // segments per layer of `period` rays (0: one queue)
+    uint32_t layers;     // nseg / layer_segs
 };
+
+// Compute dies of an MI355X: workgroups of a launch go to them round-robin
+// (workgroup b to die b % kXcds), each die with its own L2.
+constexpr uint32_t kXcds = 8;
+
+// Segment k of die x's queue.  With layers (the rays are layers of `period`
+// rays, e.g. a frame's shadow rays light by light, each layer an image in
+// slot order) die x takes band x of every layer: the segments
+// [ceil(x B / 8), ceil((x + 1) B / 8)) of each layer of B segments, layer by
+// layer, so a die's L2 holds the geometry of one image band.  Without
+// layers there is one queue (die 0's), in slot order.  RT_MISS: k is past
+// the queue's end.
+__device__ __forceinline__ uint32_t queue_segment(const HintPlan& pl, uint32_t x, uint32_t k) {
+    if (pl.layer_segs == 0) return x == 0 && k < pl.nseg ? k : RT_MISS;
+    const uint32_t B = pl.layer_segs;
+    const uint32_t b0 = (x * B + kXcds - 1) / kXcds, b1 = ((x + 1) * B + kXcds - 1) / kXcds;
+    const uint32_t c = b1 - b0;
+    if (c == 0 || k >= c * pl.layers) return RT_MISS;
+    const uint32_t layer = k / c;
+    return layer * B + b0 + (k - layer * c);
+}
 
 // Pass 1, one wave per 64 slots: each ray tries its own hint and, with
 // h.wave, the distinct hints of the other 63 rays (one triangle for the whole
@@ -1329,7 +1352,7 @@ __global__ __launch_bounds__(kBlock) void k_hint_pass(const rt_tri64* __restrict
                                                       uint32_t* __restrict__ bits_out,
                                                       TileMap tmap, HintArgs h, HintPlan pl) {
     const uint32_t lane = threadIdx.x & 63u;
-    if (blockIdx.x == 0 && threadIdx.x == 0) *pl.counter = 0u;
+    if (blockIdx.x == 0 && threadIdx.x < kXcds) pl.counter[threadIdx.x] = 0u;
     const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t g0 = g - lane;
     if (g0 >= slots) {  // the last segment's padding (whole waves): closed
@@ -1375,7 +1398,8 @@ __global__ __launch_bounds__(kBlock) void k_hint_pass(const rt_tri64* __restrict
 constexpr uint32_t kRing = 128;  // staged rays per wave (a power of two)
 
 // The trace pass, persistent: each wave takes segment after segment (an
-// atomic counter) and stages the segment's open rays in slot order, 64 slots
+// atomic counter per die, queue_segment; a die whose queue is empty takes
+// from the others) and stages the segment's open rays in slot order, 64 slots
 // at a time, into its ring of kRing entries; whenever `refill` lanes are idle
 // they take the next staged rays (the refill loop of k_trace_refill_bits), so
 // a wave stays filled across segment boundaries until no segment is left.  A
@@ -1409,10 +1433,16 @@ __global__ __launch_bounds__(kBlock) void k_hint_trace(const rt_node4* __restric
     for (;;) {
         while (tail - head < 64u && more) {
             if (wcur >= pl.seg_words) {
-                uint32_t sg = 0;
-                if (lane == 0) sg = atomicAdd(counter, 1u);
-                sg = __builtin_amdgcn_readfirstlane(sg);
-                if (sg >= pl.nseg) {
+                // the next segment of this die's queue, then of the others'
+                uint32_t sg = RT_MISS;
+                for (uint32_t j = 0; j < kXcds && sg == RT_MISS; ++j) {
+                    const uint32_t x = (blockIdx.x + j) % kXcds;
+                    if (pl.layer_segs == 0 && x != 0) continue;
+                    uint32_t k = 0;
+                    if (lane == 0) k = atomicAdd(counter + x, 1u);
+                    sg = queue_segment(pl, x, __builtin_amdgcn_readfirstlane(k));
+                }
+                if (sg == RT_MISS) {
                     more = false;
                     break;
                 }
@@ -1940,9 +1970,17 @@ extern "C" rt_status rt_trace_occluded_hinted(rt_stream stream, const rt_bvh_dev
     if (!d_scratch || scratch_bytes < L.total) return RT_ERR_CAPACITY;
     const TraceCfg& cfg = L.cfg;
     char* sc = static_cast<char*>(d_scratch);
+    // layers of `period` rays: the die bands of queue_segment when whole
+    // segments (and, with wave tiles, whole tile rows) make up each layer
+    const uint32_t period = hint->period ? hint->period : n;
+    const uint32_t tile_rows = cfg.tile_width ? 64u / cfg.tile_width : 1u;
+    const bool layered = n % period == 0 && period % cfg.per_wave == 0 &&
+                         (!cfg.tile_width || period % (cfg.grid_width * tile_rows) == 0) &&
+                         (uint64_t)n == L.slots;
     const HintPlan pl{reinterpret_cast<uint32_t*>(sc), reinterpret_cast<uint32_t*>(sc + L.open_bytes),
-                      L.nseg, L.seg_words};
-    const HintArgs h{hint->prim, hint->record, hint->period ? hint->period : n, hint->ids,
+                      L.nseg, L.seg_words, layered ? period / cfg.per_wave : 0u,
+                      layered ? n / period : 0u};
+    const HintArgs h{hint->prim, hint->record, period, hint->ids,
                      (hint->flags & RT_HINT_WAVE) ? 1u : 0u};
     hipStream_t s = as_stream(stream);
     const TileMap tm = L.tm;
