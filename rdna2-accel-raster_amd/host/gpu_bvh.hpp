@@ -7,6 +7,7 @@
 //   a host-built SoftwareBVH4 uploaded as SSBOs -> rtgpu::GpuBVH4::fromHost(...)
 //   software_bvh_bvh4.comp closest hit          -> GpuBVH4::intersect
 //   TerminateOnFirstHit ray queries (rq_ao.comp) -> GpuBVH4::occluded
+//   the same with occluder hints (expB_shadows)   -> GpuBVH4::occludedHinted
 //   (no reference counterpart)                  -> GpuBVH4::refit
 //   a host-built SoftwareBVH + software_bvh_procedural.comp (BVH2 loop)
 //                                               -> rtgpu::GpuBVH2::fromHost, intersect
@@ -256,6 +257,39 @@ public:
               "rt_trace_ex (any)");
     }
 
+    // Any hit with occluder hints (rt_trace_occluded_hinted): ray r first tries
+    // triangle hintPrim[r % period] (period 0 = every ray its own slot; RT_MISS = no
+    // hint), and with waveHints the hints of the other rays of its wave; the bits equal
+    // occluded() for any hints.  Shadow rays laid out light by light take the primary
+    // hits of the frame as hints with period = pixels.  The triangle-id -> leaf-record
+    // map (which a refit leaves as it is) and the scratch are kept with the object.
+    void occludedHinted(const Rays& rays, const DeviceArray<uint32_t>& hintPrim,
+                        uint32_t period, DeviceArray<uint32_t>& bits, bool waveHints = true,
+                        const rt_trace_opts* opts = nullptr) {
+        const size_t n = rays.size();
+        const size_t slots = period ? period : n;
+        if (hintPrim.size() < slots) throw Error("occludedHinted: fewer hints than the period");
+        bits.resize(n ? (n + 31) / 32 : 1);
+        const rt_bvh_dev v = view();
+        if (records_.size() != (size_t)std::max<uint32_t>(triCount_, 1u)) {
+            records_.resize(std::max<uint32_t>(triCount_, 1u));
+            check(rt_prim_records(s_, &v, triCount_, records_.data()), "rt_prim_records");
+        }
+        const rt_occluder_hint h = {hintPrim.data(), records_.data(), period, triCount_,
+                                    waveHints ? RT_HINT_WAVE : 0u};
+        size_t bytes = 0;
+        check(rt_trace_occluded_hinted_scratch_bytes((uint32_t)n, opts, &h, &bytes),
+              "rt_trace_occluded_hinted_scratch_bytes");
+        if (hintScratch_.size() < bytes) {
+            hip_check(hipStreamSynchronize(s_), "hipStreamSynchronize");  // old scratch in use
+            hintScratch_.resize(bytes);
+        }
+        const rt_rays_soa r = rays.view();
+        check(rt_trace_occluded_hinted(s_, &v, &r, (uint32_t)n, opts, &h, bits.data(),
+                                       hintScratch_.data(), hintScratch_.size()),
+              "rt_trace_occluded_hinted");
+    }
+
     // Same topology, new positions (a deforming mesh with the same indices).
     template <class Mesh>
     void refit(const Mesh& mesh) {
@@ -304,6 +338,8 @@ private:
     DeviceArray<uint32_t> idx_;
     DeviceArray<rt_node4> nodes_;
     DeviceArray<rt_tri64> tris_;
+    DeviceArray<uint32_t> records_;     // triangle id -> leaf record (occludedHinted)
+    DeviceArray<uint8_t> hintScratch_;  // open masks and queue counters (occludedHinted)
     uint32_t nodeCount_ = 0, triCount_ = 0, depth_ = 0;
 };
 

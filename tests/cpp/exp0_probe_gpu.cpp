@@ -189,6 +189,26 @@ int main(int argc, char** argv) {
                     sth += tri[r];
                 }
             }
+            // occluder hints: each ray's own closest hit (period 0) and the hits of the
+            // first quarter of the rays for every quarter (period n / 4); the bits equal
+            // the plain any-hit traversal's either way
+            {
+                rtgpu::DeviceArray<uint32_t> plain, hinted;
+                bvh.occluded(rays, plain);
+                const std::vector<uint32_t> want = plain.download(s);
+                for (uint32_t period : {0u, n / 4}) {
+                    for (int wave = 0; wave < 2; ++wave) {
+                        bvh.occludedHinted(rays, hits.prim, period, hinted, wave != 0);
+                        if (hinted.download(s) != want) {
+                            std::printf("# hinted occlusion differs: %u triangles, period %u, "
+                                        "wave %d\n", ntri, period, wave);
+                            return 1;
+                        }
+                    }
+                }
+                std::printf("# hinted occlusion equals any hit: %u triangles, %s\n", ntri,
+                            coherent ? "Coherent" : "Incoherent");
+            }
             // traversal rate: the plain closest-hit launch, events around 10 launches
             hipEvent_t e0, e1;
             rtgpu::hip_check(hipEventCreate(&e0), "hipEventCreate");

@@ -47,6 +47,9 @@ def test_cpp_probe_reproduces_published_exp0():
         assert abs(tri - ref["tri"]) / ref["tri"] < 2e-3
         assert mrays > 0
     assert r.stdout.count("refit at the same positions unchanged") == 2
+    # occluder hints (own closest hits, and a quarter of them for every quarter) leave
+    # the any-hit bits unchanged, with and without the wave's hints
+    assert r.stdout.count("hinted occlusion equals any hit") == 4
     # BVH2 shaders on the same terrains and rays (GpuBVH2): hit rates within 0.05 points
     # of the BVH4 path (the shaders' predicates differ only at |a| < 1e-7 vs 1e-8 and the
     # t >= / > tmin edge), both stack variants measured
