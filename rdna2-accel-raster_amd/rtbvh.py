@@ -558,8 +558,9 @@ class GpuBVH4:
 
     def prim_records(self, stream=None):
         """Device map triangle id -> leaf record index (rt_prim_records) over
-        ids 0 .. tri_count - 1, made once per tree (the hint table of
-        occluded(hint=...)); rebuild the tree object after a rebuild."""
+        ids 0 .. tri_count - 1, made once per tree object (the hint table of
+        occluded(hint=...)); a refit keeps it valid (leaf records keep their
+        triangles), a build makes a new tree object."""
         rec = getattr(self, "_prim_records", None)
         if rec is None:
             rec = torch.empty(max(1, self.tri_count), dtype=torch.int32, device=self.tris.device)
@@ -578,7 +579,10 @@ class GpuBVH4:
         frame for its shadow rays; hint_wave also tries the hints of the
         other rays of each wave.  The bits are the same with or without hints
         (rt_trace_occluded_hinted); tune: block, lds_stack, tile,
-        rays_per_wave (slots of a segment), refill_lanes."""
+        rays_per_wave (slots of a segment), refill_lanes.  The triangle-id ->
+        leaf-record map and the scratch of hinted calls are kept with the
+        tree, so hinted calls on one tree from several streams must be
+        ordered by the caller (one stream, or events between them)."""
         n = len(rays)
         dev = rays.data.device
         bits = out if out is not None else torch.empty(max(1, (n + 31) // 32), dtype=torch.int32,
