@@ -751,7 +751,8 @@ def serving_leg(seed=7, tuner=None):
     qps = round(0.5 * len(reqs) / sat["span_s"])
     lat = S.Server(scenes, budget, trace_shape=shape).run(
         S.make_requests(2400, qps, scenes, seed=seed + 1))
-    return {"workload": f"32 resident scenes ({sum(scenes.tris)} tris) + 8 deformable grids, "
+    return {"workload": f"32 resident scenes ({sum(scenes.tris)} tris, SAH treelet trees) "
+                        "+ 8 deformable grids, "
                         "4k-2M-ray shadow/AO/closest requests, 4k-64k-tri deform-and-rebuilds",
             "capacity_mrays_per_s": sat["mrays_per_s"],
             "plain_mrays_per_s": plain["mrays_per_s"],
