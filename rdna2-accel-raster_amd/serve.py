@@ -221,7 +221,7 @@ class Scenes:
 
 
 def make_scenes(count=32, lo=10_000, hi=2_500_000, seed=5, grids=8, grid_tris=(4_096, 65_536),
-                max_leaf_prims=1, treelet_rounds=8):
+                max_leaf_prims=1, treelet_rounds=12):
     """`count` static scenes, triangle counts log-uniform in [lo, hi]
     (seeded): displaced UV spheres and random heightfields in turn, each
     built once on the device (radix builder with `treelet_rounds` rounds of
