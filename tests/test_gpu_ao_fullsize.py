@@ -12,8 +12,8 @@
     1 SPP cosine AO rays (shaders/rq_ao.comp:32-97) within 2e-6 (device sin /
     cos against the C library's), and the AO image after the 7x7 bilateral
     filter (shaders/bilateral_blur.comp:28-49) within 1e-3 RMS of the
-    oracle's frame (the bar of the north star; AO rays that differ in a last
-    bit can flip a pixel's visibility).
+    oracle's frame (the stated bar for AO images; AO rays that differ in a
+    last bit can flip a pixel's visibility).
 """
 import numpy as np
 import pytest
