@@ -137,8 +137,8 @@ def build_time_4k(iters=2000, warm=50):
     return median(device), percentile(device, 0.99), percentile(launch, 0.99), back_to_back
 
 
-FRAME_TREELET_ROUNDS = (0, 4)  # treelet restructuring rounds the frame's tree is tuned over
-FRAME_EXTRA_TREES = ((2, 8), (1, 8))  # (leaf size, rounds) also tried for the frame
+FRAME_TREELET_ROUNDS = (0, 8)  # treelet restructuring rounds the frame's tree is tuned over
+FRAME_EXTRA_TREES = ((2, 4), (1, 4))  # (leaf size, rounds) also tried for the frame
 AO_TREELET_ROUNDS = (0, 8, 16)  # the same for the AO frame's terrain
 
 
